@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X EAM step on the large single-instance workload.
+
+Workload (one GPU): three-element Ni-Al-Co fcc random solution, Ni 75 / Al 20 /
+Co 5 at.% placed by a seeded permutation, 30x30x30 cells = 108,000 atoms,
+a = 3.56 A, Mishin_updated-Ni-Al-Co-2013 EAM tables (quintic), NVT at 600 K,
+dt = 1 fs, Nose-Hoover chain of 10, virial and the 21-component Born tensor
+accumulated every step, neighbour list r_skin 1.2 rebuilt every 20 steps
+(plus whenever an atom pair could have crossed into the cut-off).
+
+A step is one full MD step (md_driver.cpp:563-601 of the reference): NHC,
+kick, drift, list check/build, density pass, force pass, kick, NHC and
+thermodynamics. value = atoms * steps / wall time of the timed steps, inputs
+resident in HBM. The line also carries the force kernel's roofline position
+(algorithmic bytes / measured kernel time, HIP events on the driver's stream)
+and the CPU oracle timed on a bounded sample of the same workload on one core.
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--precision fp64]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from mdelast_amd import md_driver_module as mdm  # noqa: E402
+from mdelast_amd import setfl, units as U  # noqa: E402
+
+POTENTIAL = "Mishin_updated-Ni-Al-Co-2013.eam.alloy"
+HBM_PEAK_GBS = 8000.0
+
+
+def build_workload(nside=30, a=3.56, fractions=(0.75, 0.20, 0.05), seed=42, T=600.0, M=10,
+                   tau_T=5e-14, vel_seed=0):
+    """Positions (CGS), types, box, and MolecularDynamics-style velocities and chain."""
+    data = setfl.read_setfl(POTENTIAL)
+    basis = np.array([[0, 0, 0], [.5, .5, 0], [.5, 0, .5], [0, .5, .5]])
+    idx = np.indices((nside, nside, nside)).reshape(3, -1).T
+    pos = ((idx[:, None, :] + 0.25 + basis[None, :, :]) * a).reshape(-1, 3) - nside * a / 2.0
+    N = len(pos)
+    counts = [int(round(f * N)) for f in fractions]
+    counts[0] = N - sum(counts[1:])
+    perm = np.random.RandomState(seed).permutation(N)
+    types = np.zeros(N, dtype=np.uint32)
+    k = 0
+    for t, c in enumerate(counts):
+        types[perm[k:k + c]] = t
+        k += c
+    mass_t = np.array([e.atomic_weight / U.Navogadro for e in data.elements])
+    m = mass_t[types]
+    rs = np.random.RandomState(vel_seed)
+    sig = np.sqrt(U.k_boltz * T / m)
+    v = np.stack([rs.normal(0.0, sig), rs.normal(0.0, sig), rs.normal(0.0, sig)], axis=1)
+    v -= (v * m[:, None]).sum(0) / m.sum()
+    v *= np.sqrt((3.0 * N - 3.0) * U.k_boltz * T / 2.0 / (0.5 * np.sum(m[:, None] * v ** 2)))
+    Nf = 3.0 * (N - 1.0)
+    kT = U.k_boltz * T
+    q = np.ones(M) * tau_T ** 2 * kT
+    q[0] *= Nf
+    p_eta = rs.normal(0.0, np.sqrt(q * kT))
+    L = nside * a * U.angstrom
+    return dict(data=data, pos=pos * U.angstrom, vel=v, types=types, mass=mass_t, box=L,
+                Nf=Nf, T=T, q=q, p_eta=p_eta, N=N, counts=counts)
+
+
+def make_driver(w, precision="fp64", lanes_per_atom=0):
+    from mdelast_amd import get_potential
+    pots, eams = get_potential.tables_from_setfl(w["data"], "quintic")
+    th = mdm.Thermostat(w["T"], w["q"], w["p_eta"], w["Nf"])
+    ba = mdm.Barostat(0.0, 1.0, 0.0, mdm.Thermostat(w["T"], [1.0], [0.0], 1.0))
+    box = mdm.Box(np.diag([w["box"]] * 3))
+    return mdm.MDDriver(w["pos"], w["vel"], mdm.PotentialVector(pots), mdm.EAMVector(eams),
+                        mdm.MDDriver.NVT, mdm.MDDriver.THERMOSTAT_KINETIC, th, ba, box, [1e-10],
+                        w["Nf"], w["mass"], w["types"], 1e-15, True, 1.2, 1e3,
+                        precision=precision, lanes_per_atom=lanes_per_atom), (pots, eams)
+
+
+def force_pass_bytes(N, entries, precision):
+    """Compulsory HBM bytes of one force-pass launch (k_force).
+
+    Per atom: own position record 32 B (double4), F'/F'' 16 B, F(rho) 8 B, list
+    count 4 B and list row 4 B per entry (read); force 24 B and energy 8 B
+    (written). Neighbour positions, F' of neighbours and the tables are gathers
+    of data other atoms also read: re-use, not compulsory traffic.
+    """
+    per_atom = 32 + 16 + 8 + 4 + 24 + 8
+    return N * per_atom + 4.0 * entries
+
+
+def read_pmc_profile():
+    """HBM bytes per k_force dispatch from the committed rocprofv3 --pmc summary."""
+    path = os.path.join(ROOT, "profiles", "k_force_nialco108k_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["hbm_bytes_per_dispatch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def cpu_baseline(w, nsteps=2, rebuild_every=20):
+    """The oracle (single-threaded FP64 C restatement of the reference) on one core:
+    one construction (O(N^2) list build + forces) and nsteps steps of the same
+    workload; per-step cost = step time + build time / rebuild_every."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    try:
+        cpus = sorted(os.sched_getaffinity(0))
+        os.sched_setaffinity(0, {cpus[-1]})
+    except (AttributeError, OSError):
+        cpus = None
+    try:
+        th = (w["T"], w["q"], w["p_eta"], w["Nf"])
+        t0 = time.perf_counter()
+        o = O.MD(positions=w["pos"], velocities=w["vel"], types=w["types"], mass=w["mass"],
+                 box=[w["box"]] * 3, data=w["data"], ensemble=O.NVT, thermostat=th,
+                 barostat=(0.0, 1.0, 0.0), baro_thermostat=(w["T"], [1.0], [0.0], 1.0),
+                 Nf=w["Nf"], dt=1e-15, neighbors_list=True, r_skin=1.2)
+        t1 = time.perf_counter()
+        o.integrate(nsteps)
+        t2 = time.perf_counter()
+    finally:
+        if cpus:
+            os.sched_setaffinity(0, set(cpus))
+    t_step = (t2 - t1) / nsteps
+    t_build = max(0.0, (t1 - t0) - t_step)
+    per_step = t_step + t_build / rebuild_every
+    return {"value": w["N"] / per_step, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle (FP64 C restatement, gcc -O2) on the same {w['N']}-atom "
+                       f"workload: 1 construction with the O(N^2) list build "
+                       f"({t1 - t0:.1f} s) + {nsteps} NVT steps ({t2 - t1:.1f} s); "
+                       f"per step = step + build/{rebuild_every}"),
+            "ms_per_step": 1e3 * per_step}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--nside", type=int, default=30)
+    ap.add_argument("--lanes-per-atom", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only; the workload is a single "
+              "instance on one MI355X (multi-GPU sharding is design-only).", file=sys.stderr)
+        sys.exit(2)
+    mdm.VERBOSE = False
+    w = build_workload(nside=args.nside)
+    d, keep = make_driver(w, args.precision, args.lanes_per_atom)
+    d.run(args.warmup, 1, 20, records=False)
+    t0 = time.perf_counter()
+    rec = d.run(args.steps, 1, 20, records=True)   # synchronises before returning
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    kt = d.kernel_times()
+    N = w["N"]
+    value = N * args.steps / wall
+    entries = d.neighbor_list_entries
+    fbytes = force_pass_bytes(N, entries, args.precision)
+    achieved = fbytes / (kt["force_ms"] * 1e-3) / 1e9
+    traffic = read_pmc_profile()
+    out = {
+        "metric": "atom-steps/s",
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"NiAlCo fcc {args.nside}^3 cells, {N} atoms "
+                        f"({w['counts'][0]} Ni/{w['counts'][1]} Al/{w['counts'][2]} Co), "
+                        "a=3.56 A, NVT 600 K, dt 1 fs, NHC M=10, Born every step, "
+                        "list r_skin 1.2 rebuilt every 20 steps",
+            "atoms": N,
+            "ns_per_day": 1e-6 * (args.steps / wall) * 86400.0,
+            "precision": args.precision,
+            "neighbor_entries_per_atom": entries / N,
+            "list_builds": d.neighbor_list_builds,
+            "kernel_ms": {k: round(v, 5) for k, v in kt.items()},
+            "final_T_kin": float(rec[-1][11]),
+            "final_etot_cons_err": float(rec[-1][7]),
+        },
+        "roofline": {
+            "kernel": "k_force (force/virial/Born pass)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": fbytes,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
+        out["cpu_baseline"] = cpu_baseline(w)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

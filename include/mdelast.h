@@ -1,0 +1,187 @@
+/*
+ * mdelast-mi355x: C-ABI of the MI355X-native EAM step (libmdelast_hip.so).
+ *
+ * This is the drop-in boundary that replaces the pybind11 module
+ * `md_driver_module` of mdelast v1.2 (src/md_driver_module.cpp:16-224) for the
+ * per-step EAM path. Plain C types only: pointers, sizes, doubles, ints. All
+ * quantities are CGS, as at the reference boundary (src/units.hpp:6-24).
+ *
+ * Every function returns an mdl_status; on failure mdl_last_error() holds a
+ * message for the calling thread. No function calls exit(): the FATAL paths
+ * of the reference (printf + exit(1)) become MDL_ERR_* codes.
+ *
+ * Threading: one driver handle must not be used from two threads at once.
+ * Each driver owns one HIP stream on the current device at creation.
+ */
+#ifndef MDELAST_H_
+#define MDELAST_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t mdl_status;
+
+enum {
+    MDL_OK = 0,
+    MDL_ERR_ARG = 1,          /* invalid argument */
+    MDL_ERR_POSITION = 2,     /* initial |x_a| >= L_a/2 (md_driver.cpp:91-97) */
+    MDL_ERR_TYPE = 3,         /* type >= ntypes (md_driver.cpp:101-104) */
+    MDL_ERR_ENSEMBLE = 4,     /* unknown/unsupported ensemble (md_driver.cpp:141-144) */
+    MDL_ERR_RHOMAX = 5,       /* rho > rho_max in F(rho) (eam.hpp:48-70) */
+    MDL_ERR_VOLUME = 6,       /* NPT volume change > 5 % in one step (md_driver.cpp:493-497) */
+    MDL_ERR_MOMENTUM = 7,     /* |P|^2 / (2 m_0 K) > 1e-14 (md_driver.cpp:620-627) */
+    MDL_ERR_PRESSURE = 8,     /* |P / (tr sigma / 3) - 1| > 1e-5 (md_driver.cpp:693-698) */
+    MDL_ERR_CONSERVATION = 9, /* E_cons error > fatal_conservation (md_driver.cpp:700-725) */
+    MDL_ERR_STATE = 10,       /* the driver failed before; it accepts no more steps */
+    MDL_ERR_DEVICE = 11,      /* HIP runtime error (message holds the HIP error string) */
+    MDL_ERR_UNSUPPORTED = 12, /* a configuration the GPU path does not implement */
+    MDL_ERR_NEIGHBORS = 13,   /* neighbour list or cell grid overflow */
+    MDL_ERR_TABLE = 14        /* invalid table (poly_table.hpp:39-44) */
+};
+
+/* Table interpolation, poly_table.hpp:140-152 */
+enum { MDL_QUINTIC = 0, MDL_BSPLINE = 1 };
+/* MDDriver::Ensamble and ThermostatCoupling, md_driver.hpp:20-21 */
+enum { MDL_NVE = 0, MDL_NVT = 1, MDL_NPT = 2, MDL_NPT_FLEX = 3 };
+enum { MDL_KINETIC = 0, MDL_CONFIGURATIONAL = 1, MDL_AVERAGE = 2 };
+/* Arithmetic of the pair/table passes */
+enum { MDL_FP64 = 0, MDL_MIXED = 1 };
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *mdl_last_error(void);
+/* Name and compute units of the current HIP device; MDL_ERR_DEVICE without one. */
+mdl_status mdl_device_info(char *name, int32_t name_len, int32_t *compute_units);
+
+/* ---------------------------------------------------------------------------
+ * Tables. Replaces potential::Table / QuinticTable / BSplineTable
+ * (poly_table.hpp:33-176), the tabulation inside TablePotential
+ * (potential.hpp:77-104) and EAM (eam.hpp:10-71).
+ * ------------------------------------------------------------------------- */
+typedef struct mdl_table mdl_table;
+
+/* Nodes x_k = k h, k = 0..n_nodes-1, with f, f', f'' at the nodes.
+ * MDL_QUINTIC: per interval the quintic Hermite polynomial (poly_table.hpp:35-63),
+ * coefficients formed in long double. MDL_BSPLINE: three cardinal cubic
+ * B-splines through f, f', f'' (poly_table.hpp:120-135), n_nodes >= 5.
+ * Host-only: no device memory is touched until a driver or eval uses it. */
+mdl_status mdl_table_create(int32_t kind, double h, int64_t n_nodes, const double *f,
+                            const double *f1, const double *f2, mdl_table **out);
+mdl_status mdl_table_destroy(mdl_table *t);
+/* Host copy of the per-interval polynomial coefficients: quintic [n][6]
+ * (value polynomial in t); B-spline [n][12] (cubics of f, f', f'' in t, 4 each).
+ * Pass out = NULL to query *n_intervals and *stride. */
+mdl_status mdl_table_coefficients(const mdl_table *t, double *out, int64_t *n_intervals,
+                                  int32_t *stride);
+/* Evaluate value, d1, d2 at n points on the GPU (poly_table.hpp:71-97, 164-175);
+ * any output pointer may be NULL. Host arrays in and out. */
+mdl_status mdl_table_eval(mdl_table *t, int64_t n, const double *x, double *value, double *d1,
+                          double *d2);
+
+/* ---------------------------------------------------------------------------
+ * Driver. Replaces MDDriver (md_driver.hpp:18-168): ctor md_driver.cpp:16-160,
+ * integrate() :563-601, calculate_neighbors() :279-298, and the attribute
+ * getters bound at md_driver_module.cpp:103-140.
+ * ------------------------------------------------------------------------- */
+typedef struct mdl_driver mdl_driver;
+
+typedef struct {
+    int64_t N;                /* atoms */
+    int32_t ntypes;           /* species = len(mass) */
+    const double *positions;  /* N*3, |x_a| < L_a/2 */
+    const double *velocities; /* N*3 */
+    const uint32_t *types;    /* N */
+    const double *mass;       /* ntypes, per type */
+    double box[3];            /* diagonal box (md_driver.cpp:60) */
+    int32_t ensemble;         /* MDL_NVE / NVT / NPT */
+    int32_t coupling;         /* MDL_KINETIC / CONFIGURATIONAL / AVERAGE */
+    /* Thermostat(T, q, p_eta, Nf) (thermostat.hpp:17-18) */
+    double th_T, th_Nf;
+    int32_t th_M;
+    const double *th_q, *th_p_eta;
+    /* Barostat(P, w, p_epsilon, Thermostat) (barostat.hpp:12-13) */
+    double bar_P, bar_W, bar_p_eps;
+    double bth_T, bth_Nf;
+    int32_t bth_M;
+    const double *bth_q, *bth_p_eta;
+    double Nf, dt;
+    int32_t neighbors_list;   /* reference list on/off; the GPU always keeps an exact list */
+    double r_skin;            /* list radius factor (>= 1) */
+    double fatal_conservation;
+    /* pair potentials, ntypes(ntypes+1)/2 in slot j(j+1)/2 + i, i <= j */
+    const mdl_table *const *pair;
+    const double *pair_rcut, *pair_rmin;
+    int32_t has_eam;
+    /* EAM per type: rho(r) and F(rho) tables, cut-off and rho_max */
+    const mdl_table *const *rho_tab;
+    const mdl_table *const *F_tab;
+    const double *eam_rcut, *eam_rhomax;
+    int32_t precision;        /* MDL_FP64 or MDL_MIXED */
+    int32_t lanes_per_atom;   /* 0 = autotuned; else 4, 8, 16, 32 or 64 */
+} mdl_config;
+
+/* Copies the state to the device, builds the list, computes forces and the
+ * thermodynamics of the initial state (md_driver.cpp:16-160). */
+mdl_status mdl_driver_create(const mdl_config *cfg, mdl_driver **out);
+mdl_status mdl_driver_destroy(mdl_driver *d);
+
+/* One call = nsteps consecutive integrate() (md_driver.cpp:563-601) with the
+ * current sample_observables flag, then a synchronisation and error check. */
+mdl_status mdl_driver_integrate(mdl_driver *d, int64_t nsteps);
+/* Production loop of MolecularDynamics.run (molecular_dynamics.py:1240-1255)
+ * without the Python: nsteps steps; step k samples iff (cycle+1) % sample_stride
+ * == 0; a list rebuild is forced before a step whose cycle % rebuild_every == 0
+ * (0 = never forced). If records != NULL, each sampled step appends
+ * MDL_NRECORD doubles (the MDL_S_* layout below, first MDL_NRECORD entries).
+ * *n_records receives the count. */
+mdl_status mdl_driver_run(mdl_driver *d, int64_t nsteps, int64_t sample_stride,
+                          int64_t rebuild_every, double *records, int64_t *n_records);
+/* Force a rebuild before the next force evaluation (md_driver.cpp:279-298). */
+mdl_status mdl_driver_calculate_neighbors(mdl_driver *d);
+/* MDDriver::sample_observables (md_driver.hpp:163-167) */
+mdl_status mdl_driver_set_sample_observables(mdl_driver *d, int32_t flag);
+mdl_status mdl_driver_get_sample_observables(mdl_driver *d, int32_t *flag);
+
+/* Scalar attributes, MDL_NSCALARS doubles in this order. */
+enum {
+    MDL_S_EPOT_EAM_ATOM = 0, MDL_S_EPOT_ATOM, MDL_S_EKIN_ATOM, MDL_S_ETOT_ATOM, MDL_S_EKIN_TOT,
+    MDL_S_ETOT_CONS_INITIAL, MDL_S_ETOT_CONS, MDL_S_ETOT_CONS_ERR, MDL_S_VIRIAL,
+    MDL_S_LAPLACIAN, MDL_S_FORCE2, MDL_S_TKIN, MDL_S_TCONF_INV, MDL_S_PRESSURE, MDL_S_TIME,
+    MDL_S_CYCLE, MDL_S_BOX0, MDL_S_BOX1, MDL_S_BOX2, MDL_S_BAR_EPS, MDL_S_BAR_PEPS,
+    MDL_S_MOM0, MDL_S_MOM1, MDL_S_MOM2,
+    MDL_S_BORN_STRESS = 24,   /* 6, Voigt xx yy zz yz xz xy, divided by V */
+    MDL_S_STRESS = 30,        /* 6 */
+    MDL_S_BORN_ELAST = 36,    /* 36, row-major 6x6, upper triangle filled, lower 0 */
+    MDL_NRECORD = 72,
+    MDL_S_BOX_INITIAL0 = 72, MDL_S_BOX_INITIAL1, MDL_S_BOX_INITIAL2,
+    MDL_S_NLIST_BUILDS = 75,  /* list builds since creation */
+    MDL_S_WARN_VIRIAL = 76,   /* steps whose virial self-check failed (md_driver.cpp:422-429) */
+    MDL_S_WARN_RMIN = 77,     /* pair distances below r_min seen (potential.hpp:93-95) */
+    MDL_S_NLIST_ENTRIES = 78, /* stored (full) list entries */
+    MDL_NSCALARS = 80
+};
+mdl_status mdl_driver_get_scalars(mdl_driver *d, double *out);
+
+/* Per-atom arrays (N*3 or N doubles). POSITIONS are wrapped real positions as
+ * the reference keeps them (md_driver.cpp:485), COORDINATES the unwrapped
+ * scaled ones. */
+enum {
+    MDL_A_POSITIONS = 0, MDL_A_COORDINATES, MDL_A_VELOCITIES, MDL_A_FORCES, MDL_A_RHO,
+    MDL_A_EPOT, MDL_A_EKIN
+};
+mdl_status mdl_driver_get_atoms(mdl_driver *d, int32_t what, double *out);
+/* p_eta and eta of the thermostat (which = 0) or barostat chain (which = 1). */
+mdl_status mdl_driver_get_chain(mdl_driver *d, int32_t which, double *p_eta, double *eta);
+
+/* Average device time per launch of the two pair kernels over the last
+ * mdl_driver_run, measured with HIP events on the driver's stream:
+ * out[0] density pass, out[1] force pass, out[2] whole step (ms). */
+mdl_status mdl_driver_kernel_times(mdl_driver *d, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MDELAST_H_ */

@@ -1,0 +1,773 @@
+// MDDriver on the GPU: creation (md_driver.cpp:16-160), integrate()
+// (md_driver.cpp:563-601), calculate_neighbors() (md_driver.cpp:279-298) and
+// the attribute getters of md_driver_module.cpp:103-140, as the mdl_driver_*
+// C-ABI of include/mdelast.h.
+#include "mdl_kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <tuple>
+
+namespace mdl {
+
+constexpr int GRID_MAX = 2048;      // blocks of the pair passes (multiple of 8)
+constexpr int GRID_ATOM_MAX = 1024; // blocks of the kick kernels
+
+using DensFn = void (*)(AtomArrays, ListArrays, RTables, FTables, int, int, int, double, double,
+                        DevState *, double *);
+using ForceFn = void (*)(AtomArrays, ListArrays, RTables, int, int, double, double, double,
+                         double, DevState *, double *);
+
+template <int KIND, bool S> static DensFn dens_fn(int lpa) {
+    switch (lpa) {
+    case 4: return k_density<KIND, 4, S>;
+    case 8: return k_density<KIND, 8, S>;
+    case 16: return k_density<KIND, 16, S>;
+    case 32: return k_density<KIND, 32, S>;
+    default: return k_density<KIND, 64, S>;
+    }
+}
+template <int KIND, bool S, bool E> static ForceFn force_fn(int lpa) {
+    switch (lpa) {
+    case 4: return k_force<KIND, 4, S, E>;
+    case 8: return k_force<KIND, 8, S, E>;
+    case 16: return k_force<KIND, 16, S, E>;
+    case 32: return k_force<KIND, 32, S, E>;
+    default: return k_force<KIND, 64, S, E>;
+    }
+}
+static DensFn pick_dens(int kind, int lpa, bool s) {
+    if (kind == MDL_QUINTIC)
+        return s ? dens_fn<MDL_QUINTIC, true>(lpa) : dens_fn<MDL_QUINTIC, false>(lpa);
+    return s ? dens_fn<MDL_BSPLINE, true>(lpa) : dens_fn<MDL_BSPLINE, false>(lpa);
+}
+static ForceFn pick_force(int kind, int lpa, bool s, bool e) {
+    if (kind == MDL_QUINTIC) {
+        if (e)
+            return s ? force_fn<MDL_QUINTIC, true, true>(lpa) : force_fn<MDL_QUINTIC, false, true>(lpa);
+        return s ? force_fn<MDL_QUINTIC, true, false>(lpa) : force_fn<MDL_QUINTIC, false, false>(lpa);
+    }
+    if (e)
+        return s ? force_fn<MDL_BSPLINE, true, true>(lpa) : force_fn<MDL_BSPLINE, false, true>(lpa);
+    return s ? force_fn<MDL_BSPLINE, true, false>(lpa) : force_fn<MDL_BSPLINE, false, false>(lpa);
+}
+
+__global__ void k_set_rebuild(DevState *gs) {
+    if (threadIdx.x == 0)
+        gs->force_rebuild = 1;
+}
+
+// Autotuner cache: (N, ntypes, kind, list stride bucket) -> lanes per atom
+static std::map<std::tuple<int64_t, int, int, int>, int> g_lpa_cache;
+
+struct Driver {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int N = 0, ntypes = 0, npair = 0, kind = MDL_QUINTIC, has_eam = 1, precision = MDL_FP64;
+    int lpa = 16;
+    bool use_cells = false;
+    int sample = 1;
+    int64_t cycle_host = 0;
+    bool failed = false;
+    AtomArrays a{};
+    ListArrays l{};
+    RTables rt{};
+    FTables ft{};
+    double rc2 = 0, rc_pair = 0, rc_rho = 0, rmin = 0;
+    double *d_rrec = nullptr, *d_frec = nullptr;
+    double *pA = nullptr, *pB = nullptr, *pK = nullptr;
+    int gridA = 8, gridB = 8, gridK = 1;
+    double *d_stage = nullptr;
+    DevState *gs = nullptr;
+    DevState hs{};                 // host mirror (refreshed by sync_state)
+    std::vector<double> init_pos;  // positions attribute before the first step
+    std::vector<void *> allocs;
+    // timing of the pair passes
+    std::vector<hipEvent_t> ev;
+    double t_dens = 0, t_force = 0, t_step = 0;
+
+    template <class T> mdl_status alloc(T **p, size_t n) {
+        void *q = nullptr;
+        MDL_HIP(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
+        allocs.push_back(q);
+        *p = static_cast<T *>(q);
+        return MDL_OK;
+    }
+    ~Driver() {
+        for (auto e : ev)
+            (void)hipEventDestroy(e);
+        for (void *p : allocs)
+            (void)hipFree(p);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+
+    int grid_pairs(int lanes) const {
+        int const apb = BLOCK / lanes;
+        int g = (N + apb - 1) / apb;
+        g = std::min(std::max(g, 1), GRID_MAX);
+        return (g + 7) / 8 * 8;
+    }
+
+    void launch_build() {
+        int const gN = (N + BLOCK - 1) / BLOCK;
+        k_decide<<<1, 1, 0, stream>>>(gs, use_cells ? 1 : 0, 1.0);
+        if (use_cells) {
+            k_bin<<<gN, BLOCK, 0, stream>>>(a, l, N, gs);
+            k_scan<<<1, 1024, 0, stream>>>(l, gs);
+            k_scatter<<<gN, BLOCK, 0, stream>>>(l, N, gs);
+            k_sort_cells<<<(l.ncells + BLOCK - 1) / BLOCK, BLOCK, 0, stream>>>(l, gs);
+            k_build_cells<<<gN, BLOCK, 0, stream>>>(a, l, N, gs);
+        } else {
+            k_build_brute<<<gN, BLOCK, 0, stream>>>(a, l, N, gs);
+        }
+    }
+
+    void launch_forces(bool s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
+                       hipEvent_t e2 = nullptr) {
+        if (e0)
+            (void)hipEventRecord(e0, stream);
+        if (has_eam) {
+            DensFn fa = pick_dens(kind, lpa, s);
+            fa<<<gridA, BLOCK, 0, stream>>>(a, l, rt, ft, N, npair, ntypes, rc2, rc_rho, gs, pA);
+        }
+        if (e1)
+            (void)hipEventRecord(e1, stream);
+        ForceFn fb = pick_force(kind, lpa, s, has_eam != 0);
+        fb<<<gridB, BLOCK, 0, stream>>>(a, l, rt, N, npair, rc2, rc_pair, rc_rho, rmin, gs, pB);
+        if (e2)
+            (void)hipEventRecord(e2, stream);
+    }
+
+    void launch_step(bool s, bool force_rebuild, double *rec_out, hipEvent_t *tev) {
+        k_pre<<<1, 64, 0, stream>>>(gs, s ? 1 : 0);
+        if (force_rebuild)
+            k_set_rebuild<<<1, 1, 0, stream>>>(gs);
+        k_kick_drift<<<gridK, BLOCK, 0, stream>>>(a, N, gs);
+        launch_build();
+        if (tev)
+            launch_forces(s, tev[0], tev[1], tev[2]);
+        else
+            launch_forces(s);
+        k_kick2<<<gridK, BLOCK, 0, stream>>>(a, N, gs, 0, pK);
+        k_post<<<1, BLOCK, 0, stream>>>(gs, pA, gridA, pB, gridB, pK, gridK, 1, rec_out);
+        ++cycle_host;
+    }
+
+    mdl_status sync_state() {
+        MDL_HIP(hipMemcpyAsync(&hs, gs, sizeof(DevState), hipMemcpyDeviceToHost, stream));
+        MDL_HIP(hipStreamSynchronize(stream));
+        return MDL_OK;
+    }
+
+    // Translate device error bits into the reference's FATAL messages.
+    mdl_status check_errors() {
+        int const e = hs.err;
+        if (!e)
+            return MDL_OK;
+        failed = true;
+        if (e & DE_RHOMAX) {
+            set_error("FATAL: EAM F rho >rhomax_fatal %g %g", hs.err_val[0], hs.err_val[1]);
+            return MDL_ERR_RHOMAX;
+        }
+        if (e & DE_VOLUME) {
+            set_error("FATAL: volume change too large... expe=%g Vold=%g Vnew=%g", hs.err_val[0],
+                      hs.err_val[1], hs.err_val[2]);
+            return MDL_ERR_VOLUME;
+        }
+        if (e & DE_NBR_OVERFLOW) {
+            set_error("neighbour list overflow: an atom has %lld neighbours within r_list, the "
+                      "list holds %d", hs.nbr_needed, l.stride);
+            return MDL_ERR_NEIGHBORS;
+        }
+        if (e & DE_CELL_GRID) {
+            set_error("the box shrank below the cell grid of the neighbour list (L = %g)",
+                      hs.err_val[0]);
+            return MDL_ERR_NEIGHBORS;
+        }
+        if (e & DE_MOMENTUM) {
+            set_error("FATAL: total momentum = (%g, %g, %g) scale=%g", hs.err_val[0],
+                      hs.err_val[1], hs.err_val[2], hs.err_val[3]);
+            return MDL_ERR_MOMENTUM;
+        }
+        if (e & DE_PRESSURE) {
+            set_error("FATAL PRESS err=%g %1.16E %1.16E", hs.err_val[0], hs.err_val[1],
+                      hs.err_val[2]);
+            return MDL_ERR_PRESSURE;
+        }
+        if (e & DE_CONSERVATION) {
+            set_error("FATAL: bad energy conservation... (err %g, current %g)", hs.err_val[0],
+                      hs.err_val[1]);
+            return MDL_ERR_CONSERVATION;
+        }
+        set_error("device error bits 0x%x", e);
+        return MDL_ERR_DEVICE;
+    }
+};
+
+static mdl_status bad(const char *msg) {
+    set_error("%s", msg);
+    return MDL_ERR_ARG;
+}
+
+// Time the two passes for each candidate lanes-per-atom and keep the fastest.
+static mdl_status autotune(Driver &d) {
+    auto key = std::make_tuple((int64_t)d.N, d.ntypes, d.kind, d.l.stride / 32);
+    auto it = g_lpa_cache.find(key);
+    if (it != g_lpa_cache.end()) {
+        d.lpa = it->second;
+        d.gridA = d.gridB = d.grid_pairs(d.lpa);
+        return MDL_OK;
+    }
+    hipEvent_t e0, e1;
+    MDL_HIP(hipEventCreate(&e0));
+    MDL_HIP(hipEventCreate(&e1));
+    double best = 1e300;
+    int best_lpa = 16;
+    int const cand[] = {4, 8, 16, 32, 64};
+    int const saved = d.lpa;
+    for (int lpa : cand) {
+        d.lpa = lpa;
+        d.gridA = d.gridB = d.grid_pairs(lpa);
+        d.launch_forces(true);   // warm-up
+        MDL_HIP(hipEventRecord(e0, d.stream));
+        for (int r = 0; r < 3; ++r)
+            d.launch_forces(true);
+        MDL_HIP(hipEventRecord(e1, d.stream));
+        MDL_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        MDL_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            best_lpa = lpa;
+        }
+    }
+    (void)saved;
+    d.lpa = best_lpa;
+    d.gridA = d.gridB = d.grid_pairs(best_lpa);
+    g_lpa_cache[key] = best_lpa;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return MDL_OK;
+}
+
+static mdl_status create(const mdl_config *c, Driver &d) {
+    if (!c || !c->positions || !c->velocities || !c->types || !c->mass || !c->pair)
+        return bad("mdl_driver_create: null argument");
+    if (c->N < 1 || c->N > (int64_t)1 << 30)
+        return bad("mdl_driver_create: N out of range");
+    if (c->ntypes < 1 || c->ntypes > MAX_TYPES) {
+        set_error("mdl_driver_create: %d species; the GPU path supports 1..%d", c->ntypes,
+                  MAX_TYPES);
+        return MDL_ERR_UNSUPPORTED;
+    }
+    if (c->th_M < 1 || c->th_M > MAX_CHAIN || c->bth_M < 1 || c->bth_M > MAX_CHAIN || !c->th_q ||
+        !c->th_p_eta || !c->bth_q || !c->bth_p_eta) {
+        set_error("mdl_driver_create: chain lengths must be 1..%d", MAX_CHAIN);
+        return MDL_ERR_UNSUPPORTED;
+    }
+    if (!(c->dt > 0.))
+        return bad("mdl_driver_create: dt must be > 0");
+    for (int a = 0; a < 3; ++a)
+        if (!(c->box[a] > 0.))
+            return bad("mdl_driver_create: box lengths must be > 0");
+    d.N = (int)c->N;
+    d.ntypes = c->ntypes;
+    d.npair = c->ntypes * (c->ntypes + 1) / 2;
+    d.has_eam = c->has_eam ? 1 : 0;
+    d.precision = c->precision;
+    if (d.precision != MDL_FP64) {
+        set_error("mdl_driver_create: only MDL_FP64 is implemented");
+        return MDL_ERR_UNSUPPORTED;
+    }
+    if (d.has_eam && (!c->rho_tab || !c->F_tab || !c->eam_rcut || !c->eam_rhomax))
+        return bad("mdl_driver_create: EAM tables missing");
+    // md_driver.cpp:91-106: positions inside the box, types in range
+    for (int64_t i = 0; i < c->N; ++i) {
+        const double *p = c->positions + 3 * i;
+        if (std::fabs(p[0]) >= 0.5 * c->box[0] || std::fabs(p[1]) >= 0.5 * c->box[1] ||
+            std::fabs(p[2]) >= 0.5 * c->box[2]) {
+            set_error("bad initial position");
+            return MDL_ERR_POSITION;
+        }
+        if (c->types[i] >= (uint32_t)c->ntypes) {
+            set_error("bad type");
+            return MDL_ERR_TYPE;
+        }
+    }
+    if (c->ensemble != MDL_NVE && c->ensemble != MDL_NVT && c->ensemble != MDL_NPT) {
+        set_error("bad ensamble %d", c->ensemble);
+        return MDL_ERR_ENSEMBLE;
+    }
+    // tables: one interpolation kind, a common r grid and a common rho grid
+    std::vector<const HostTable *> rtabs;
+    for (int p = 0; p < d.npair; ++p)
+        rtabs.push_back(reinterpret_cast<const HostTable *>(c->pair[p]));
+    if (d.has_eam)
+        for (int t = 0; t < d.ntypes; ++t)
+            rtabs.push_back(reinterpret_cast<const HostTable *>(c->rho_tab[t]));
+    for (auto *t : rtabs)
+        if (!t)
+            return bad("mdl_driver_create: null table");
+    d.kind = rtabs[0]->kind;
+    for (auto *t : rtabs)
+        if (t->kind != d.kind || t->h != rtabs[0]->h || t->n != rtabs[0]->n) {
+            set_error("the GPU path needs all r-tables on one grid with one interpolation");
+            return MDL_ERR_UNSUPPORTED;
+        }
+    d.rc_pair = c->pair_rcut[0];
+    d.rmin = c->pair_rmin ? c->pair_rmin[0] : 0.;
+    for (int p = 1; p < d.npair; ++p)
+        if (c->pair_rcut[p] != d.rc_pair || (c->pair_rmin && c->pair_rmin[p] != d.rmin)) {
+            set_error("the GPU path needs one cut-off and one r_min for all pair tables");
+            return MDL_ERR_UNSUPPORTED;
+        }
+    d.rc2 = c->pair_rcut[0] * c->pair_rcut[0];   // md_driver.cpp:42 (pot[0]->rcut)
+    d.rc_rho = 1e300;
+    double rhomax[MAX_TYPES] = {0};
+    if (d.has_eam) {
+        d.rc_rho = c->eam_rcut[0];
+        const HostTable *f0 = reinterpret_cast<const HostTable *>(c->F_tab[0]);
+        for (int t = 0; t < d.ntypes; ++t) {
+            const HostTable *ftab = reinterpret_cast<const HostTable *>(c->F_tab[t]);
+            if (!ftab)
+                return bad("mdl_driver_create: null F table");
+            if (c->eam_rcut[t] != d.rc_rho || ftab->kind != d.kind || ftab->h != f0->h ||
+                ftab->n != f0->n) {
+                set_error("the GPU path needs one rho cut-off and one F(rho) grid for all species");
+                return MDL_ERR_UNSUPPORTED;
+            }
+            rhomax[t] = c->eam_rhomax[t];
+        }
+    }
+    // interleaved r records: [interval][function][S]
+    int const S = rtabs[0]->stride;
+    int const nf = (int)rtabs.size();
+    int64_t const nr = rtabs[0]->n;
+    std::vector<double> rrec((size_t)nr * nf * S);
+    for (int64_t i = 0; i < nr; ++i)
+        for (int f = 0; f < nf; ++f)
+            memcpy(&rrec[((size_t)i * nf + f) * S], &rtabs[f]->rec[(size_t)i * S],
+                   sizeof(double) * S);
+    MDL_HIP(hipGetDevice(&d.device));
+    MDL_HIP(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
+    if (mdl_status s = d.alloc(&d.d_rrec, rrec.size()))
+        return s;
+    MDL_HIP(hipMemcpy(d.d_rrec, rrec.data(), sizeof(double) * rrec.size(), hipMemcpyHostToDevice));
+    d.rt.rec = d.d_rrec;
+    d.rt.nf = nf;
+    d.rt.n = (int)nr;
+    d.rt.inv_h = rtabs[0]->inv_h;
+    d.rt.inv_h2 = rtabs[0]->inv_h2;
+    if (d.has_eam) {
+        const HostTable *f0 = reinterpret_cast<const HostTable *>(c->F_tab[0]);
+        std::vector<double> frec((size_t)f0->n * d.ntypes * S);
+        for (int64_t i = 0; i < f0->n; ++i)
+            for (int t = 0; t < d.ntypes; ++t)
+                memcpy(&frec[((size_t)i * d.ntypes + t) * S],
+                       &reinterpret_cast<const HostTable *>(c->F_tab[t])->rec[(size_t)i * S],
+                       sizeof(double) * S);
+        if (mdl_status s = d.alloc(&d.d_frec, frec.size()))
+            return s;
+        MDL_HIP(hipMemcpy(d.d_frec, frec.data(), sizeof(double) * frec.size(),
+                          hipMemcpyHostToDevice));
+        d.ft.rec = d.d_frec;
+        d.ft.n = (int)f0->n;
+        d.ft.inv_h = f0->inv_h;
+        d.ft.inv_h2 = f0->inv_h2;
+        for (int t = 0; t < d.ntypes; ++t)
+            d.ft.rhomax[t] = rhomax[t];
+    }
+
+    // neighbour list geometry: the list is exact at every step (rebuilt on
+    // request and whenever a pair outside it could have entered r_c)
+    double const rc = c->pair_rcut[0];
+    double skin = c->neighbors_list ? c->r_skin : 1.2;
+    if (const char *env = getenv("MDL_SKIN"))
+        skin = atof(env);
+    skin = std::min(std::max(skin, 1.02), 2.0);
+    double const r_list = rc * skin;
+    int nc[3];
+    bool cells = true;
+    for (int a = 0; a < 3; ++a) {
+        nc[a] = (int)std::floor(c->box[a] / (r_list * 1.02));
+        if (nc[a] < 3)
+            cells = false;
+    }
+    if (getenv("MDL_BRUTE_LIST"))
+        cells = false;
+    d.use_cells = cells;
+    double const V = c->box[0] * (c->box[1] * c->box[2]);
+    double const expect = 4.18879020478639 * r_list * r_list * r_list * (double)d.N / V;
+    int stride = (int)std::ceil(1.5 * expect + 32.);
+    stride = std::min(stride, std::max(d.N - 1, 1));
+    stride = (stride + 7) / 8 * 8;
+    d.l.stride = stride;
+    d.l.ncells = cells ? nc[0] * nc[1] * nc[2] : 0;
+
+    int const N = d.N;
+    AtomArrays &a = d.a;
+#define ALLOC(p, n)                                                                            \
+    if (mdl_status s_ = d.alloc(&(p), (n)))                                                    \
+        return s_;
+    ALLOC(a.pos, N);
+    ALLOC(a.vx, N);
+    ALLOC(a.vy, N);
+    ALLOC(a.vz, N);
+    ALLOC(a.fx, N);
+    ALLOC(a.fy, N);
+    ALLOC(a.fz, N);
+    ALLOC(a.type, N);
+    ALLOC(a.cref, N);
+    ALLOC(a.fprime, N);
+    ALLOC(a.Femb, N);
+    ALLOC(a.rho, N);
+    ALLOC(a.epot, N);
+    ALLOC(d.l.nbr, (size_t)N * stride);
+    ALLOC(d.l.nnbr, N);
+    if (cells) {
+        ALLOC(d.l.cell_of, N);
+        ALLOC(d.l.cell_count, d.l.ncells);
+        ALLOC(d.l.cell_start, d.l.ncells + 1);
+        ALLOC(d.l.cell_fill, d.l.ncells);
+        ALLOC(d.l.cell_atoms, N);
+        MDL_HIP(hipMemset(d.l.cell_count, 0, sizeof(int) * d.l.ncells));
+        MDL_HIP(hipMemset(d.l.cell_fill, 0, sizeof(int) * d.l.ncells));
+    }
+    ALLOC(d.pA, (size_t)PART_STRIDE * PA_N);
+    ALLOC(d.pB, (size_t)PART_STRIDE * PB_N);
+    ALLOC(d.pK, (size_t)PART_STRIDE * PK_N);
+    ALLOC(d.d_stage, (size_t)3 * N);
+    ALLOC(d.gs, 1);
+#undef ALLOC
+    MDL_HIP(hipMemset(a.fx, 0, sizeof(double) * N));
+    MDL_HIP(hipMemset(a.fy, 0, sizeof(double) * N));
+    MDL_HIP(hipMemset(a.fz, 0, sizeof(double) * N));
+    MDL_HIP(hipMemset(a.Femb, 0, sizeof(double) * N));
+    MDL_HIP(hipMemset(a.fprime, 0, sizeof(double2) * N));
+    MDL_HIP(hipMemset(a.rho, 0, sizeof(double) * N));
+
+    // scaled coordinates (md_driver.cpp:162-173) and SoA velocities
+    std::vector<double4> pos(N);
+    std::vector<double> vx(N), vy(N), vz(N);
+    std::vector<int> typ(N);
+    auto nint = [](double x) { return (double)(x < 0. ? (int)(x - 0.5) : (int)(x + 0.5)); };
+    for (int i = 0; i < N; ++i) {
+        double s[3];
+        for (int k = 0; k < 3; ++k) {
+            double const u = c->positions[3 * i + k] / c->box[k];
+            s[k] = u - nint(u);
+        }
+        typ[i] = (int)c->types[i];
+        pos[i] = make_double4(s[0], s[1], s[2], (double)typ[i]);
+        vx[i] = c->velocities[3 * i];
+        vy[i] = c->velocities[3 * i + 1];
+        vz[i] = c->velocities[3 * i + 2];
+    }
+    d.init_pos.assign(c->positions, c->positions + 3 * (size_t)N);
+    MDL_HIP(hipMemcpy(a.pos, pos.data(), sizeof(double4) * N, hipMemcpyHostToDevice));
+    MDL_HIP(hipMemcpy(a.cref, pos.data(), sizeof(double4) * N, hipMemcpyHostToDevice));
+    MDL_HIP(hipMemcpy(a.vx, vx.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    MDL_HIP(hipMemcpy(a.vy, vy.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    MDL_HIP(hipMemcpy(a.vz, vz.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    MDL_HIP(hipMemcpy(a.type, typ.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+
+    DevState &s = d.hs;
+    memset(&s, 0, sizeof s);
+    s.N = N;
+    s.ntypes = d.ntypes;
+    s.ensemble = c->ensemble;
+    s.coupling = c->coupling;
+    s.has_eam = d.has_eam;
+    s.sample = 1;
+    s.Nf = c->Nf;
+    s.alpha = 1. + 3. / c->Nf;
+    s.dt = c->dt;
+    s.fatal = c->fatal_conservation;
+    for (int t = 0; t < d.ntypes; ++t)
+        s.mass[t] = c->mass[t];
+    for (int k = 0; k < 3; ++k)
+        s.L[k] = s.L0[k] = s.L_drift[k] = s.L_build[k] = s.drift_L[k] = c->box[k];
+    s.th.T = c->th_T;
+    s.th.Nf = c->th_Nf;
+    s.th.M = c->th_M;
+    for (int j = 0; j < c->th_M; ++j) {
+        s.th.q[j] = c->th_q[j];
+        s.th.p_eta[j] = c->th_p_eta[j];
+    }
+    s.bth.T = c->bth_T;
+    s.bth.Nf = c->bth_Nf;
+    s.bth.M = c->bth_M;
+    for (int j = 0; j < c->bth_M; ++j) {
+        s.bth.q[j] = c->bth_q[j];
+        s.bth.p_eta[j] = c->bth_p_eta[j];
+    }
+    s.bar_P = c->bar_P;
+    s.bar_W = c->bar_W;
+    s.peps = c->bar_p_eps;
+    s.vscale = 1.;
+    s.kick_s = 1.;
+    s.r_list = r_list;
+    s.rc = rc;
+    s.skin = r_list - rc;
+    s.force_rebuild = 1;
+    for (int k = 0; k < 3; ++k)
+        s.ncell[k] = cells ? nc[k] : 1;
+    MDL_HIP(hipMemcpy(d.gs, &s, sizeof(DevState), hipMemcpyHostToDevice));
+    // the uploads and memsets above ran on the null stream, which the driver's
+    // non-blocking stream does not wait for
+    MDL_HIP(hipDeviceSynchronize());
+
+    d.lpa = c->lanes_per_atom;
+    if (d.lpa != 0 && d.lpa != 4 && d.lpa != 8 && d.lpa != 16 && d.lpa != 32 && d.lpa != 64)
+        return bad("lanes_per_atom must be 0 (autotune), 4, 8, 16, 32 or 64");
+    bool const tune = d.lpa == 0;
+    if (tune)
+        d.lpa = 16;
+    d.gridA = d.gridB = d.grid_pairs(d.lpa);
+    d.gridK = std::min((N + BLOCK - 1) / BLOCK, GRID_ATOM_MAX);
+
+    // constructor: list, forces, thermodynamics (md_driver.cpp:112-118)
+    d.launch_build();
+    MDL_HIP(hipGetLastError());
+    if (mdl_status st = d.sync_state())
+        return st;
+    if (mdl_status st = d.check_errors())
+        return st;
+    if (tune && d.N >= 2048)
+        if (mdl_status st = autotune(d))
+            return st;
+    d.launch_forces(true);
+    k_kick2<<<d.gridK, BLOCK, 0, d.stream>>>(d.a, N, d.gs, 1, d.pK);
+    k_post<<<1, BLOCK, 0, d.stream>>>(d.gs, d.pA, d.gridA, d.pB, d.gridB, d.pK, d.gridK, 0,
+                                       nullptr);
+    MDL_HIP(hipGetLastError());
+    if (mdl_status st = d.sync_state())
+        return st;
+    return d.check_errors();
+}
+
+}  // namespace mdl
+
+using namespace mdl;
+
+static Driver *D(mdl_driver *p) { return reinterpret_cast<Driver *>(p); }
+
+extern "C" {
+
+mdl_status mdl_driver_create(const mdl_config *cfg, mdl_driver **out) {
+    if (!out)
+        return bad("mdl_driver_create: null output");
+    *out = nullptr;
+    auto d = std::make_unique<Driver>();
+    mdl_status s = create(cfg, *d);
+    if (s != MDL_OK)
+        return s;
+    *out = reinterpret_cast<mdl_driver *>(d.release());
+    return MDL_OK;
+}
+
+mdl_status mdl_driver_destroy(mdl_driver *p) {
+    Driver *d = D(p);
+    if (!d)
+        return MDL_OK;
+    (void)hipSetDevice(d->device);
+    (void)hipStreamSynchronize(d->stream);
+    delete d;
+    return MDL_OK;
+}
+
+static mdl_status guard(Driver *d) {
+    if (!d)
+        return bad("null driver");
+    if (d->failed) {
+        set_error("the driver failed earlier and accepts no more steps");
+        return MDL_ERR_STATE;
+    }
+    MDL_HIP(hipSetDevice(d->device));
+    return MDL_OK;
+}
+
+mdl_status mdl_driver_integrate(mdl_driver *p, int64_t nsteps) {
+    Driver *d = D(p);
+    if (mdl_status s = guard(d))
+        return s;
+    for (int64_t k = 0; k < nsteps; ++k)
+        d->launch_step(d->sample != 0, false, nullptr, nullptr);
+    MDL_HIP(hipGetLastError());
+    if (mdl_status s = d->sync_state())
+        return s;
+    return d->check_errors();
+}
+
+mdl_status mdl_driver_run(mdl_driver *p, int64_t nsteps, int64_t sample_stride,
+                          int64_t rebuild_every, double *records, int64_t *n_records) {
+    Driver *d = D(p);
+    if (mdl_status s = guard(d))
+        return s;
+    if (sample_stride < 1)
+        return bad("sample_stride must be >= 1");
+    int64_t nsamp = 0;
+    for (int64_t k = 0; k < nsteps; ++k)
+        if ((d->cycle_host + k + 1) % sample_stride == 0)
+            ++nsamp;
+    double *drec = nullptr;
+    if (records && nsamp > 0)
+        MDL_HIP(hipMalloc(&drec, sizeof(double) * MDL_NRECORD * (size_t)nsamp));
+    // events around the passes of up to 64 steps
+    int const ntime = (int)std::min<int64_t>(nsteps, 64);
+    while ((int)d->ev.size() < 4 * ntime) {
+        hipEvent_t e;
+        MDL_HIP(hipEventCreate(&e));
+        d->ev.push_back(e);
+    }
+    int64_t irec = 0;
+    int64_t const t0 = nsteps - ntime;   // time the last steps (after any warm-up)
+    for (int64_t k = 0; k < nsteps; ++k) {
+        int64_t const cyc = d->cycle_host;
+        bool const s = (cyc + 1) % sample_stride == 0;
+        bool const rb = rebuild_every > 0 && cyc % rebuild_every == 0;
+        double *ro = (drec && s) ? drec + MDL_NRECORD * irec : nullptr;
+        if (s)
+            ++irec;
+        hipEvent_t *tev = nullptr;
+        if (k >= t0) {
+            tev = &d->ev[4 * (k - t0)];
+            (void)hipEventRecord(tev[3], d->stream);
+        }
+        d->launch_step(s, rb, ro, tev);
+    }
+    MDL_HIP(hipGetLastError());
+    if (drec) {
+        hipError_t e = hipMemcpyAsync(records, drec, sizeof(double) * MDL_NRECORD * irec,
+                                      hipMemcpyDeviceToHost, d->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(d->stream);
+        (void)hipFree(drec);
+        if (e != hipSuccess)
+            return hip_fail(e, "mdl_driver_run records");
+    }
+    if (n_records)
+        *n_records = irec;
+    if (mdl_status s = d->sync_state())
+        return s;
+    // pass timings: [3] step start, [0] before A, [1] after A, [2] after B
+    double ta = 0, tb = 0, tsum = 0;
+    int nt = 0;
+    for (int k = 0; k < ntime; ++k) {
+        hipEvent_t *e = &d->ev[4 * k];
+        float x = 0, y = 0;
+        if (hipEventElapsedTime(&x, e[0], e[1]) == hipSuccess &&
+            hipEventElapsedTime(&y, e[1], e[2]) == hipSuccess) {
+            ta += x;
+            tb += y;
+            ++nt;
+        }
+        if (k + 1 < ntime) {
+            float z = 0;
+            if (hipEventElapsedTime(&z, e[3], d->ev[4 * (k + 1) + 3]) == hipSuccess)
+                tsum += z;
+        }
+    }
+    if (nt > 0) {
+        d->t_dens = ta / nt;
+        d->t_force = tb / nt;
+        d->t_step = ntime > 1 ? tsum / (ntime - 1) : 0.;
+    }
+    return d->check_errors();
+}
+
+mdl_status mdl_driver_calculate_neighbors(mdl_driver *p) {
+    Driver *d = D(p);
+    if (mdl_status s = guard(d))
+        return s;
+    k_set_rebuild<<<1, 1, 0, d->stream>>>(d->gs);
+    MDL_HIP(hipGetLastError());
+    return MDL_OK;
+}
+
+mdl_status mdl_driver_set_sample_observables(mdl_driver *p, int32_t flag) {
+    Driver *d = D(p);
+    if (!d)
+        return bad("null driver");
+    d->sample = flag ? 1 : 0;
+    return MDL_OK;
+}
+
+mdl_status mdl_driver_get_sample_observables(mdl_driver *p, int32_t *flag) {
+    Driver *d = D(p);
+    if (!d || !flag)
+        return bad("null argument");
+    *flag = d->sample;
+    return MDL_OK;
+}
+
+mdl_status mdl_driver_get_scalars(mdl_driver *p, double *out) {
+    Driver *d = D(p);
+    if (!d || !out)
+        return bad("null argument");
+    MDL_HIP(hipSetDevice(d->device));
+    if (mdl_status s = d->sync_state())
+        return s;
+    memcpy(out, d->hs.rec, sizeof(double) * MDL_NSCALARS);
+    out[MDL_S_NLIST_BUILDS] = (double)d->hs.nbuilds;
+    out[MDL_S_WARN_VIRIAL] = (double)d->hs.warn_virial;
+    out[MDL_S_WARN_RMIN] = (double)d->hs.warn_rmin;
+    for (int k = 0; k < 3; ++k)
+        out[MDL_S_BOX_INITIAL0 + k] = d->hs.L0[k];
+    out[MDL_S_NLIST_ENTRIES] = (double)d->hs.nlist_entries;
+    return MDL_OK;
+}
+
+mdl_status mdl_driver_get_atoms(mdl_driver *p, int32_t what, double *out) {
+    Driver *d = D(p);
+    if (!d || !out)
+        return bad("null argument");
+    if (what < MDL_A_POSITIONS || what > MDL_A_EKIN)
+        return bad("unknown per-atom array");
+    MDL_HIP(hipSetDevice(d->device));
+    if (what == MDL_A_POSITIONS) {
+        if (mdl_status s = d->sync_state())
+            return s;
+        if (d->hs.cycle == 0) {   // the ctor keeps the given positions (md_driver.cpp:31)
+            memcpy(out, d->init_pos.data(), sizeof(double) * d->init_pos.size());
+            return MDL_OK;
+        }
+    }
+    int const n = (what <= MDL_A_FORCES) ? 3 * d->N : d->N;
+    k_materialize<<<(d->N + 255) / 256, 256, 0, d->stream>>>(d->a, d->N, d->gs, what, d->d_stage);
+    MDL_HIP(hipGetLastError());
+    MDL_HIP(hipMemcpyAsync(out, d->d_stage, sizeof(double) * n, hipMemcpyDeviceToHost, d->stream));
+    MDL_HIP(hipStreamSynchronize(d->stream));
+    return MDL_OK;
+}
+
+mdl_status mdl_driver_get_chain(mdl_driver *p, int32_t which, double *p_eta, double *eta) {
+    Driver *d = D(p);
+    if (!d || !p_eta || !eta)
+        return bad("null argument");
+    MDL_HIP(hipSetDevice(d->device));
+    if (mdl_status s = d->sync_state())
+        return s;
+    const Chain &c = which ? d->hs.bth : d->hs.th;
+    memcpy(p_eta, c.p_eta, sizeof(double) * c.M);
+    memcpy(eta, c.eta, sizeof(double) * c.M);
+    return MDL_OK;
+}
+
+mdl_status mdl_driver_kernel_times(mdl_driver *p, double *out) {
+    Driver *d = D(p);
+    if (!d || !out)
+        return bad("null argument");
+    out[0] = d->t_dens;
+    out[1] = d->t_force;
+    out[2] = d->t_step;
+    return MDL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,197 @@
+// Internal definitions of libmdelast_hip: host table objects, the device state
+// record, and the device helpers shared by the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/mdelast.h"
+
+namespace mdl {
+
+// ----------------------------------------------------------------------------
+// Units (src/units.hpp:6-24) and fixed sizes
+// ----------------------------------------------------------------------------
+constexpr double K_BOLTZ = 1.38064852e-16;
+constexpr int MAX_TYPES = 8;                         // species per driver
+constexpr int MAX_PAIRS = MAX_TYPES * (MAX_TYPES + 1) / 2;
+constexpr int MAX_CHAIN = 64;                        // Nose-Hoover chain length
+constexpr int BLOCK = 256;                           // threads per block of the atom kernels
+
+// Per-block partial sums written by the atom kernels and reduced by the
+// scalar kernel in a fixed order (deterministic).
+enum PartialA { PA_EF = 0, PA_CB = 1 /*21*/, PA_LAP = 22, PA_N = 24 };
+enum PartialB {
+    PB_EPOT = 0, PB_VIR = 1, PB_SB = 2 /*6*/, PB_CB = 8 /*21*/, PB_LAP = 29, PB_WRMIN = 30,
+    PB_N = 32
+};
+enum PartialK { PK_K = 0, PK_MVV = 1 /*6*/, PK_MOM = 7 /*3*/, PK_F2 = 10, PK_N = 12 };
+
+// Error bits raised on the device (mirrors MDL_ERR_*).
+enum DevErr : int {
+    DE_RHOMAX = 1 << 0,
+    DE_VOLUME = 1 << 1,
+    DE_MOMENTUM = 1 << 2,
+    DE_PRESSURE = 1 << 3,
+    DE_CONSERVATION = 1 << 4,
+    DE_NBR_OVERFLOW = 1 << 5,
+    DE_CELL_GRID = 1 << 6,
+};
+
+// ----------------------------------------------------------------------------
+// Host table object (the C handle mdl_table)
+// ----------------------------------------------------------------------------
+struct HostTable {
+    int kind = MDL_QUINTIC;
+    double h = 0, inv_h = 0, inv_h2 = 0;
+    int64_t n = 0;                  // intervals
+    int stride = 6;                 // doubles per interval record (6 quintic, 12 B-spline)
+    std::vector<double> rec;        // [n][stride]
+    // device copy for mdl_table_eval (lazily created)
+    double *d_rec = nullptr;
+    int device = -1;
+};
+
+// ----------------------------------------------------------------------------
+// Device-resident scalar state of one driver (single copy, updated by the
+// one-thread scalar kernels; the host reads it back for attributes)
+// ----------------------------------------------------------------------------
+struct Chain {
+    double T, Nf;
+    int M;
+    double q[MAX_CHAIN], p_eta[MAX_CHAIN], eta[MAX_CHAIN];
+};
+
+struct DevState {
+    // configuration
+    int N, ntypes, ensemble, coupling, has_eam, sample;
+    double Nf, alpha, dt, fatal;
+    double mass[MAX_TYPES];
+    // box
+    double L[3], L0[3], L_drift[3], L_build[3];
+    // thermostat, barostat
+    Chain th, bth;
+    double bar_P, bar_W, eps, peps;
+    // velocity bookkeeping: stored v times vscale is the physical velocity
+    double vscale;
+    // coefficients of the current step (written by k_pre)
+    double kick_s, kick_e, kick_c, drift_cf, drift_L[3];
+    // thermodynamics (the MDL_S_* record)
+    double rec[MDL_NSCALARS];
+    double ekin_tot, virial, lap, f2, Tci;
+    double etot_cons_initial, etot_cons, etot_cons_err;
+    double time;
+    unsigned long long cycle;
+    // neighbour-list control
+    double r_list, skin, rc;
+    unsigned long long maxdisp2_bits;  // max squared displacement since the last build
+    int force_rebuild, do_rebuild;
+    long long nbuilds, nbr_needed, nlist_entries;
+    int ncell[3];
+    // errors and warnings
+    int err;
+    double err_val[4];
+    long long warn_virial, warn_rmin;
+    int init_done;
+};
+
+// ----------------------------------------------------------------------------
+// Device table lookups. Records of all r-functions (pair slots first, then
+// rho per type) are interleaved per interval: rec[(i*nf + f)*S + c], so the
+// functions a pair needs sit next to each other.
+// ----------------------------------------------------------------------------
+struct RTables {
+    const double *rec;
+    int nf, n;              // functions per interval, intervals
+    double inv_h, inv_h2;
+    double rc_f[MAX_PAIRS + MAX_TYPES];   // per-function cut-off (0 beyond)
+    double rmin_p[MAX_PAIRS];
+};
+struct FTables {
+    const double *rec;      // [n][ntypes][S]
+    int n;
+    double inv_h, inv_h2;
+    double rhomax[MAX_TYPES];
+};
+
+// Interval and local coordinate, range test before the conversion
+// (poly_table.hpp:105-114): NaN -> interval 0, outside -> end polynomial.
+__device__ __forceinline__ int tab_locate(double x, double inv_h, int n, double &t) {
+    double const u = x * inv_h;
+    int i = 0;
+    if (u >= (double)n)
+        i = n - 1;
+    else if (u > 0.)
+        i = (int)u;
+    t = u - (double)i;
+    return i;
+}
+
+template <int KIND> struct Poly;
+
+// Quintic: value/d1/d2 of one polynomial (poly_table.hpp:71-97)
+template <> struct Poly<MDL_QUINTIC> {
+    static constexpr int S = 6;
+    double c[6];
+    __device__ __forceinline__ void load(const double *p) {
+        const double2 *q = reinterpret_cast<const double2 *>(p);
+        double2 a = q[0], b = q[1], e = q[2];
+        c[0] = a.x; c[1] = a.y; c[2] = b.x; c[3] = b.y; c[4] = e.x; c[5] = e.y;
+    }
+    __device__ __forceinline__ double v(double t) const {
+        return c[0] + t * (c[1] + t * (c[2] + t * (c[3] + t * (c[4] + t * c[5]))));
+    }
+    __device__ __forceinline__ double d1(double t, double ih) const {
+        return (c[1] + t * (2. * c[2] + t * (3. * c[3] + t * (4. * c[4] + t * 5. * c[5])))) * ih;
+    }
+    __device__ __forceinline__ double d2(double t, double ih2) const {
+        return (2. * c[2] + t * (6. * c[3] + t * (12. * c[4] + t * 20. * c[5]))) * ih2;
+    }
+};
+
+// B-spline kind: three independent cubics (f, f', f'') per interval.
+template <> struct Poly<MDL_BSPLINE> {
+    static constexpr int S = 12;
+    double c[12];
+    __device__ __forceinline__ void load(const double *p) {
+        const double2 *q = reinterpret_cast<const double2 *>(p);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            double2 a = q[k];
+            c[2 * k] = a.x;
+            c[2 * k + 1] = a.y;
+        }
+    }
+    __device__ __forceinline__ double v(double t) const {
+        return c[0] + t * (c[1] + t * (c[2] + t * c[3]));
+    }
+    __device__ __forceinline__ double d1(double t, double) const {
+        return c[4] + t * (c[5] + t * (c[6] + t * c[7]));
+    }
+    __device__ __forceinline__ double d2(double t, double) const {
+        return c[8] + t * (c[9] + t * (c[10] + t * c[11]));
+    }
+};
+
+// Round half away from zero (md_driver.hpp:48-50): trunc(x -+ 0.5)
+__device__ __forceinline__ double nint_away(double x) {
+    return x < 0. ? trunc(x - 0.5) : trunc(x + 0.5);
+}
+
+}  // namespace mdl
+
+// Error reporting shared by the translation units
+namespace mdl {
+void set_error(const char *fmt, ...);
+mdl_status hip_fail(hipError_t e, const char *what);
+}  // namespace mdl
+
+#define MDL_HIP(call)                                                           \
+    do {                                                                        \
+        hipError_t e_ = (call);                                                 \
+        if (e_ != hipSuccess)                                                   \
+            return ::mdl::hip_fail(e_, #call);                                  \
+    } while (0)

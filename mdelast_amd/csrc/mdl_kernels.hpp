@@ -1,0 +1,1002 @@
+// Device kernels of one MD step. Included by mdl_driver.hip only.
+//
+// Step = k_pre (scalars: NHC, barostat, kick/drift coefficients)
+//      -> k_kick_drift (first half kick + drift, displacement since the list build)
+//      -> k_decide -> [list build kernels, each a no-op unless a rebuild is due]
+//      -> k_density (pass A) -> k_force (pass B) -> k_kick2 (second half kick)
+//      -> k_post (reductions, NHC, thermodynamics, checks)
+// which is MDDriver::integrate (md_driver.cpp:563-601) with the velocity
+// scalings of the Nose-Hoover chain folded into one factor per half step.
+#pragma once
+
+#include "mdl_internal.hpp"
+
+namespace mdl {
+
+struct AtomArrays {
+    double4 *pos;      // unwrapped scaled coordinates c (md_driver.hpp:97), type in .w
+    double *vx, *vy, *vz;
+    double *fx, *fy, *fz;
+    int *type;
+    double4 *cref;     // coordinates at the last list build
+    double2 *fprime;   // F'(rho_i), F''(rho_i)
+    double *Femb;      // F(rho_i)
+    double *rho;       // rho_i
+    double *epot;      // per-atom potential energy
+};
+
+struct ListArrays {
+    int *nbr;          // [N][stride] full list (both i->j and j->i)
+    int *nnbr;         // [N]
+    int stride;
+    int *cell_of, *cell_count, *cell_start, *cell_fill, *cell_atoms;
+    int ncells;
+};
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+// Partial records are stored component-major, part[k * PART_STRIDE + block],
+// so the final reduction reads them coalesced.
+constexpr int PART_STRIDE = 2048;
+
+template <int NV>
+__device__ __forceinline__ void block_sum_store(double (&acc)[NV], double *out) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double v = acc[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            v += __shfl_xor(v, off, 64);
+        acc[k] = v;
+    }
+    __shared__ double sm[BLOCK / 64][NV];
+    int const wave = threadIdx.x >> 6, l64 = threadIdx.x & 63;
+    if (l64 == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            sm[wave][k] = acc[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = 0.;
+#pragma unroll
+        for (int w = 0; w < BLOCK / 64; ++w)
+            s += sm[w][threadIdx.x];
+        out[(size_t)threadIdx.x * PART_STRIDE + blockIdx.x] = s;
+    }
+}
+
+// XCD-aware slice of [0, nitems): blocks are dispatched round-robin over the
+// 8 XCDs, so block b runs on XCD b % 8; give each XCD one contiguous chunk so
+// its L2 holds one spatial region of the neighbour gathers.
+struct XcdRange {
+    int first, end, step;
+    __device__ __forceinline__ XcdRange(int nitems) {
+        int const nper = gridDim.x >> 3;
+        int const xcd = blockIdx.x & 7, b = blockIdx.x >> 3;
+        int const chunk = (nitems + 7) >> 3;
+        first = xcd * chunk + b;
+        end = min(nitems, (xcd + 1) * chunk);
+        step = nper;
+    }
+};
+
+// Minimum-image pair vector d = (s - nint(s)) L of s = c_i - c_j
+// (md_driver.cpp:175-185, applied to the difference as at :222 and :333)
+__device__ __forceinline__ void min_image(const double4 &ci, const double4 &cj, const double *L,
+                                          double &dx, double &dy, double &dz) {
+    double const sx = ci.x - cj.x, sy = ci.y - cj.y, sz = ci.z - cj.z;
+    dx = (sx - nint_away(sx)) * L[0];
+    dy = (sy - nint_away(sy)) * L[1];
+    dz = (sz - nint_away(sz)) * L[2];
+}
+
+__device__ __forceinline__ void expx_operator(double x, double &expx, double &op) {
+    // numerics.hpp:11-18
+    const double c1 = -1.0 / 2.0, c2 = 1.0 / 6.0, c3 = -1.0 / 24.0, c4 = 1. / 120., c5 = -1. / 720.;
+    expx = exp(-x);
+    op = fabs(x) > 1e-3 ? (1.0 - expx) / x : 1.0 + x * (c1 + x * (c2 + x * (c3 + x * (c4 + x * c5))));
+}
+
+__constant__ const double WSY7[7] = {
+    0.784513610477560, 0.235573213359357, -1.17767998417887,
+    1. - 2. * (0.784513610477560 + 0.235573213359357 - 1.17767998417887),
+    -1.17767998417887, 0.235573213359357, 0.784513610477560};
+
+// thermostat.cpp:40-66
+__device__ void chain_U4(Chain &c, double dt, double e_couple, bool outward) {
+    double const kT = K_BOLTZ * c.T;
+    int const last = c.M - 1;
+    for (int s = 0; s < c.M; ++s) {
+        int const j = outward ? s : last - s;
+        double const G = j != 0 ? c.p_eta[j - 1] * c.p_eta[j - 1] / c.q[j - 1] - kT
+                                : 2. * e_couple - c.Nf * kT;
+        if (j != last) {
+            double e, op;
+            expx_operator(dt * c.p_eta[j + 1] / c.q[j + 1], e, op);
+            c.p_eta[j] = c.p_eta[j] * e + dt * G * op;
+        } else {
+            c.p_eta[j] += dt * G;
+        }
+    }
+}
+
+// thermostat.hpp:24-35: returns the momentum scale factor
+__device__ double chain_U3(Chain &c, double dt) {
+    double const s = exp(-dt * c.p_eta[0] / c.q[0]);
+    for (int j = 0; j < c.M; ++j)
+        c.eta[j] += dt * c.p_eta[j] / c.q[j];
+    return s;
+}
+
+// thermostat.cpp:24-37
+__device__ double chain_etot(const Chain &c) {
+    double e = c.Nf * c.eta[0];
+    for (int j = 1; j < c.M; ++j)
+        e += c.eta[j];
+    e *= K_BOLTZ * c.T;
+    for (int j = 0; j < c.M; ++j)
+        e += 0.5 * (c.p_eta[j] * c.p_eta[j]) / c.q[j];
+    return e;
+}
+
+__device__ __forceinline__ double box_volume(const double *L) { return L[0] * (L[1] * L[2]); }
+
+// md_driver.cpp:512-531
+__device__ double thermo_couple(const DevState &s) {
+    double e = s.ekin_tot;
+    if (s.coupling != MDL_KINETIC && s.Tci != 0.) {
+        double const econf = (K_BOLTZ / s.Tci) * s.Nf / 2.;
+        e = s.coupling == MDL_CONFIGURATIONAL ? econf : 0.5 * (s.ekin_tot + econf);
+    }
+    return e;
+}
+
+// U_NHC_thermo_propagator (md_driver.cpp:539-549) with U3 acting on a running
+// scale factor: K' = s^2 K replaces the O(N) recomputation (md_driver.cpp:504).
+__device__ double nhc_thermo(DevState &s, double dt) {
+    double scale = 1.;
+    for (int k = 0; k < 7; ++k) {
+        double const dtw = WSY7[k] * dt;
+        chain_U4(s.th, 0.5 * dtw, thermo_couple(s), false);
+        double const f = chain_U3(s.th, dtw);
+        scale *= f;
+        s.ekin_tot *= f * f;
+        chain_U4(s.th, 0.5 * dtw, thermo_couple(s), true);
+    }
+    return scale;
+}
+
+// U_NHC_baro_propagator (md_driver.cpp:551-561)
+__device__ void nhc_baro(DevState &s, double dt) {
+    for (int k = 0; k < 7; ++k) {
+        double const dtw = WSY7[k] * dt;
+        chain_U4(s.bth, 0.5 * dtw, 0.5 * (s.peps * s.peps) / s.bar_W, false);
+        s.peps *= chain_U3(s.bth, dtw);
+        chain_U4(s.bth, 0.5 * dtw, 0.5 * (s.peps * s.peps) / s.bar_W, true);
+    }
+}
+
+// U2p (md_driver.cpp:454-459)
+__device__ void U2p(DevState &s, double dt) {
+    double const pv = s.bar_P * box_volume(s.L);
+    s.peps += (2. * s.alpha * s.ekin_tot + 3. * s.virial - 3. * pv) * dt;
+}
+
+// The scalar kernels work on an LDS copy of the state: the chain updates are a
+// long serial sequence of dependent reads and writes, which must not each pay
+// a global-memory round trip.
+static_assert(sizeof(DevState) % 8 == 0, "DevState is copied in 8-byte words");
+__device__ __forceinline__ void state_load(DevState &sh, const DevState *gs) {
+    const unsigned long long *src = reinterpret_cast<const unsigned long long *>(gs);
+    unsigned long long *dst = reinterpret_cast<unsigned long long *>(&sh);
+    for (int k = threadIdx.x; k < (int)(sizeof(DevState) / 8); k += blockDim.x)
+        dst[k] = src[k];
+    __syncthreads();
+}
+__device__ __forceinline__ void state_store(DevState *gs, const DevState &sh) {
+    __syncthreads();
+    const unsigned long long *src = reinterpret_cast<const unsigned long long *>(&sh);
+    unsigned long long *dst = reinterpret_cast<unsigned long long *>(gs);
+    for (int k = threadIdx.x; k < (int)(sizeof(DevState) / 8); k += blockDim.x)
+        dst[k] = src[k];
+}
+
+// ---------------------------------------------------------------------------
+// k_pre: first half of integrate() up to the drift (md_driver.cpp:567-581)
+// ---------------------------------------------------------------------------
+__device__ void pre_step(DevState &s, int sample);
+
+__global__ __launch_bounds__(64) void k_pre(DevState *__restrict__ gs, int sample) {
+    if (gs->err)
+        return;
+    __shared__ DevState s;
+    state_load(s, gs);
+    if (threadIdx.x == 0)
+        pre_step(s, sample);
+    state_store(gs, s);
+}
+
+__device__ void pre_step(DevState &s, int sample) {
+    s.sample = sample;
+    s.maxdisp2_bits = 0ull;
+    bool const npt = s.ensemble == MDL_NPT;
+    bool const thermo = s.ensemble == MDL_NVT || npt;
+    double const h = 0.5 * s.dt;
+    double scale = 1.;
+    if (npt)
+        nhc_baro(s, h);
+    if (thermo)
+        scale = nhc_thermo(s, h);
+    if (npt)
+        U2p(s, h);
+    double e = 1., c = 1.;
+    if (npt)
+        expx_operator(s.alpha * h * s.peps / s.bar_W, e, c);
+    s.kick_s = s.vscale * scale;
+    s.vscale = 1.;
+    s.kick_e = e;
+    s.kick_c = c * h;
+    double cf = 1.;
+    if (npt) {
+        double const x = s.dt * s.peps / s.bar_W;
+        s.eps += x;
+        double ex;
+        expx_operator(x, ex, cf);
+    }
+    s.drift_cf = cf * s.dt;
+    for (int a = 0; a < 3; ++a) {
+        s.drift_L[a] = s.L[a];
+        s.L_drift[a] = s.L[a];
+    }
+    if (npt) {
+        double const expe = exp(s.eps);
+        double const vold = box_volume(s.L);
+        for (int a = 0; a < 3; ++a)
+            s.L[a] = s.L0[a] * expe;
+        if (fabs(box_volume(s.L) / vold - 1.) > 0.05) {
+            s.err |= DE_VOLUME;
+            s.err_val[0] = expe;
+            s.err_val[1] = vold;
+            s.err_val[2] = box_volume(s.L);
+        }
+    }
+}
+
+// First U2 (md_driver.cpp:433-452) and U1 drift (md_driver.cpp:462-499).
+__global__ __launch_bounds__(BLOCK) void k_kick_drift(AtomArrays a, int N,
+                                                     const DevState *__restrict__ gs) {
+    if (gs->err)
+        return;
+    double const s = gs->kick_s, e = gs->kick_e, ch = gs->kick_c, cf = gs->drift_cf;
+    double const Ld[3] = {gs->drift_L[0], gs->drift_L[1], gs->drift_L[2]};
+    double const Ln[3] = {gs->L[0], gs->L[1], gs->L[2]};
+    double dmax = 0.;
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < N; i += gridDim.x * BLOCK) {
+        double const m = gs->mass[a.type[i]];
+        double vx = a.vx[i] * s, vy = a.vy[i] * s, vz = a.vz[i] * s;
+        vx = vx * e + ch * a.fx[i] / m;
+        vy = vy * e + ch * a.fy[i] / m;
+        vz = vz * e + ch * a.fz[i] / m;
+        a.vx[i] = vx;
+        a.vy[i] = vy;
+        a.vz[i] = vz;
+        double4 c = a.pos[i];
+        c.x += cf * vx / Ld[0];
+        c.y += cf * vy / Ld[1];
+        c.z += cf * vz / Ld[2];
+        a.pos[i] = c;
+        double4 const r = a.cref[i];
+        double const ux = (c.x - r.x) * Ln[0], uy = (c.y - r.y) * Ln[1], uz = (c.z - r.z) * Ln[2];
+        dmax = fmax(dmax, ux * ux + uy * uy + uz * uz);
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(const_cast<unsigned long long *>(&gs->maxdisp2_bits),
+                  (unsigned long long)__double_as_longlong(dmax));
+}
+
+// Decide whether the list must be rebuilt: forced by calculate_neighbors(),
+// or any pair outside the list could now be inside r_c. For a list built with
+// radius r_list at box L_b, a pair not in it is at least lambda r_list - 2 u_max
+// apart now, lambda = min L/L_b, u_max the largest displacement since the build.
+__global__ void k_decide(DevState *__restrict__ gs, int use_cells, double cell_margin) {
+    if (threadIdx.x != 0 || gs->err)
+        return;
+    DevState &s = *gs;
+    double lam = 1e300;
+    for (int a = 0; a < 3; ++a)
+        lam = fmin(lam, s.L[a] / s.L_build[a]);
+    double const umax = sqrt(__longlong_as_double((long long)s.maxdisp2_bits));
+    bool const need = s.force_rebuild || !(lam * s.r_list - 2. * umax > s.rc);
+    s.do_rebuild = need ? 1 : 0;
+    if (!need)
+        return;
+    s.force_rebuild = 0;
+    s.nbuilds++;
+    s.nlist_entries = 0;
+    for (int a = 0; a < 3; ++a)
+        s.L_build[a] = s.L[a];
+    if (use_cells)
+        for (int a = 0; a < 3; ++a)
+            if (s.L[a] / s.ncell[a] < s.r_list * cell_margin) {
+                s.err |= DE_CELL_GRID;
+                s.err_val[0] = s.L[a];
+            }
+}
+
+__device__ __forceinline__ int cell_coord(double c, int n) {
+    double const w = c - nint_away(c);   // [-0.5, 0.5]
+    int k = (int)floor((w + 0.5) * n);
+    return k < 0 ? 0 : (k >= n ? n - 1 : k);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_bin(AtomArrays a, ListArrays l, int N,
+                                              const DevState *__restrict__ gs) {
+    if (gs->err || !gs->do_rebuild)
+        return;
+    int const i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= N)
+        return;
+    double4 const c = a.pos[i];
+    int const nx = gs->ncell[0], ny = gs->ncell[1], nz = gs->ncell[2];
+    int const cid = (cell_coord(c.z, nz) * ny + cell_coord(c.y, ny)) * nx + cell_coord(c.x, nx);
+    l.cell_of[i] = cid;
+    atomicAdd(&l.cell_count[cid], 1);
+}
+
+// Exclusive scan of the cell counts (one block), then zero counts and fills.
+__global__ __launch_bounds__(1024) void k_scan(ListArrays l, const DevState *__restrict__ gs) {
+    if (gs->err || !gs->do_rebuild)
+        return;
+    __shared__ int sm[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0)
+        carry = 0;
+    __syncthreads();
+    for (int base = 0; base < l.ncells; base += 1024) {
+        int const k = base + threadIdx.x;
+        int const v = k < l.ncells ? l.cell_count[k] : 0;
+        sm[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            int const t = threadIdx.x >= off ? sm[threadIdx.x - off] : 0;
+            __syncthreads();
+            sm[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (k < l.ncells) {
+            l.cell_start[k] = carry + sm[threadIdx.x] - v;
+            l.cell_count[k] = 0;
+            l.cell_fill[k] = 0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 1023)
+            carry += sm[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        l.cell_start[l.ncells] = carry;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scatter(ListArrays l, int N,
+                                                  const DevState *__restrict__ gs) {
+    if (gs->err || !gs->do_rebuild)
+        return;
+    int const i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= N)
+        return;
+    int const cid = l.cell_of[i];
+    int const slot = atomicAdd(&l.cell_fill[cid], 1);
+    l.cell_atoms[l.cell_start[cid] + slot] = i;
+}
+
+// Members of each cell in ascending index order, so the list (and the FP
+// summation order of the passes) does not depend on atomic arrival order.
+__global__ __launch_bounds__(BLOCK) void k_sort_cells(ListArrays l,
+                                                     const DevState *__restrict__ gs) {
+    if (gs->err || !gs->do_rebuild)
+        return;
+    int const c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= l.ncells)
+        return;
+    int const b = l.cell_start[c], e = l.cell_start[c + 1];
+    for (int k = b + 1; k < e; ++k) {
+        int const v = l.cell_atoms[k];
+        int m = k - 1;
+        while (m >= b && l.cell_atoms[m] > v) {
+            l.cell_atoms[m + 1] = l.cell_atoms[m];
+            --m;
+        }
+        l.cell_atoms[m + 1] = v;
+    }
+}
+
+__device__ __forceinline__ void list_push(ListArrays &l, int i, int &cnt, int j) {
+    if (cnt < l.stride)
+        l.nbr[(size_t)i * l.stride + cnt] = j;
+    ++cnt;
+}
+
+__device__ __forceinline__ void list_finish(ListArrays &l, AtomArrays &a, int i, int cnt,
+                                            DevState *gs) {
+    l.nnbr[i] = min(cnt, l.stride);
+    a.cref[i] = a.pos[i];
+    atomicAdd(reinterpret_cast<unsigned long long *>(&gs->nlist_entries),
+              (unsigned long long)min(cnt, l.stride));
+    if (cnt > l.stride) {
+        atomicOr(&gs->err, (int)DE_NBR_OVERFLOW);
+        atomicMax(reinterpret_cast<unsigned long long *>(&gs->nbr_needed),
+                  (unsigned long long)cnt);
+    }
+}
+
+// Full list from the 27 surrounding cells (>= 3 cells per direction).
+__global__ __launch_bounds__(BLOCK) void k_build_cells(AtomArrays a, ListArrays l, int N,
+                                                      DevState *__restrict__ gs) {
+    if (gs->err || !gs->do_rebuild)
+        return;
+    int const i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= N)
+        return;
+    double const L[3] = {gs->L[0], gs->L[1], gs->L[2]};
+    double const rl2 = gs->r_list * gs->r_list;
+    int const nx = gs->ncell[0], ny = gs->ncell[1], nz = gs->ncell[2];
+    double4 const ci = a.pos[i];
+    int const cx = cell_coord(ci.x, nx), cy = cell_coord(ci.y, ny), cz = cell_coord(ci.z, nz);
+    int cnt = 0;
+    for (int dz = -1; dz <= 1; ++dz) {
+        int const z = (cz + dz + nz) % nz;
+        for (int dy = -1; dy <= 1; ++dy) {
+            int const y = (cy + dy + ny) % ny;
+            for (int dx = -1; dx <= 1; ++dx) {
+                int const x = (cx + dx + nx) % nx;
+                int const cell = (z * ny + y) * nx + x;
+                int const b = l.cell_start[cell], e = l.cell_start[cell + 1];
+                for (int k = b; k < e; ++k) {
+                    int const j = l.cell_atoms[k];
+                    if (j == i)
+                        continue;
+                    double ddx, ddy, ddz;
+                    min_image(ci, a.pos[j], L, ddx, ddy, ddz);
+                    if (ddx * ddx + ddy * ddy + ddz * ddz <= rl2)
+                        list_push(l, i, cnt, j);
+                }
+            }
+        }
+    }
+    list_finish(l, a, i, cnt, gs);
+}
+
+// Full list by testing every other atom (small boxes, < 3 cells per direction).
+__global__ __launch_bounds__(BLOCK) void k_build_brute(AtomArrays a, ListArrays l, int N,
+                                                      DevState *__restrict__ gs) {
+    if (gs->err || !gs->do_rebuild)
+        return;
+    int const i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= N)
+        return;
+    double const L[3] = {gs->L[0], gs->L[1], gs->L[2]};
+    double const rl2 = gs->r_list * gs->r_list;
+    double4 const ci = a.pos[i];
+    int cnt = 0;
+    for (int j = 0; j < N; ++j) {
+        if (j == i)
+            continue;
+        double ddx, ddy, ddz;
+        min_image(ci, a.pos[j], L, ddx, ddy, ddz);
+        if (ddx * ddx + ddy * ddy + ddz * ddz <= rl2)
+            list_push(l, i, cnt, j);
+    }
+    list_finish(l, a, i, cnt, gs);
+}
+
+// ---------------------------------------------------------------------------
+// Pass A: densities (md_driver.cpp:195-277) over the full list; per atom
+// rho_i, and when sampled g_i (Voigt) and h_i; then F, F', F''. The per-atom
+// Born and Laplacian terms F''_i g_i g_i and F''_i |h_i|^2
+// (md_driver.cpp:410-419) are summed here, where g_i and h_i are complete.
+// ---------------------------------------------------------------------------
+template <int KIND, int LPA, bool SAMPLE>
+__global__ __launch_bounds__(BLOCK) void k_density(AtomArrays a, ListArrays l, RTables rt,
+                                                  FTables ft, int N, int npair, int ntypes,
+                                                  double rc2, double rc_rho,
+                                                  DevState *__restrict__ gs,
+                                                  double *__restrict__ part) {
+    if (gs->err)
+        return;
+    using P = Poly<KIND>;
+    constexpr int APB = BLOCK / LPA;
+    int const lane = threadIdx.x % LPA, slot = threadIdx.x / LPA;
+    double const L[3] = {gs->L[0], gs->L[1], gs->L[2]};
+    double acc[PA_N];
+#pragma unroll
+    for (int k = 0; k < PA_N; ++k)
+        acc[k] = 0.;
+    XcdRange const R((N + APB - 1) / APB);
+    for (int grp = R.first; grp < R.end; grp += R.step) {
+        int const i = grp * APB + slot;
+        bool const active = i < N;
+        double rho = 0., g0 = 0., g1 = 0., g2 = 0., g3 = 0., g4 = 0., g5 = 0.;
+        double hx = 0., hy = 0., hz = 0.;
+        double4 ci = make_double4(0., 0., 0., 0.);
+        if (active) {
+            ci = a.pos[i];
+            int const cnt = l.nnbr[i];
+            const int *nb = l.nbr + (size_t)i * l.stride;
+            for (int k = lane; k < cnt; k += LPA) {
+                int const j = nb[k];
+                double4 const cj = a.pos[j];
+                double dx, dy, dz;
+                min_image(ci, cj, L, dx, dy, dz);
+                double const r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 > rc2)
+                    continue;
+                double const r = sqrt(r2);
+                if (r > rc_rho)   // eam.hpp:30-44: rho and its derivatives are 0 beyond rcut
+                    continue;
+                int const tj = (int)cj.w;
+                double t;
+                int const it = tab_locate(r, rt.inv_h, rt.n, t);
+                P p;
+                p.load(rt.rec + ((size_t)it * rt.nf + npair + tj) * P::S);
+                rho += p.v(t);
+                if (SAMPLE) {
+                    double const w = p.d1(t, rt.inv_h) / r;
+                    g0 += w * dx * dx;
+                    g1 += w * dy * dy;
+                    g2 += w * dz * dz;
+                    g3 += w * dy * dz;
+                    g4 += w * dx * dz;
+                    g5 += w * dx * dy;
+                    hx += w * dx;
+                    hy += w * dy;
+                    hz += w * dz;
+                }
+            }
+        }
+#pragma unroll
+        for (int off = LPA / 2; off > 0; off >>= 1) {
+            rho += __shfl_xor(rho, off, LPA);
+            if (SAMPLE) {
+                g0 += __shfl_xor(g0, off, LPA);
+                g1 += __shfl_xor(g1, off, LPA);
+                g2 += __shfl_xor(g2, off, LPA);
+                g3 += __shfl_xor(g3, off, LPA);
+                g4 += __shfl_xor(g4, off, LPA);
+                g5 += __shfl_xor(g5, off, LPA);
+                hx += __shfl_xor(hx, off, LPA);
+                hy += __shfl_xor(hy, off, LPA);
+                hz += __shfl_xor(hz, off, LPA);
+            }
+        }
+        if (active && lane == 0) {
+            int const ti = (int)ci.w;
+            if (rho > ft.rhomax[ti]) {
+                atomicOr(&gs->err, (int)DE_RHOMAX);
+                gs->err_val[0] = rho;
+                gs->err_val[1] = ft.rhomax[ti];
+            }
+            double t;
+            int const it = tab_locate(rho, ft.inv_h, ft.n, t);
+            P p;
+            p.load(ft.rec + ((size_t)it * ntypes + ti) * P::S);
+            double const F = p.v(t), dF = p.d1(t, ft.inv_h);
+            double const d2F = SAMPLE ? p.d2(t, ft.inv_h2) : 0.;
+            a.Femb[i] = F;
+            a.rho[i] = rho;
+            a.fprime[i] = make_double2(dF, d2F);
+            acc[PA_EF] += F;
+            if (SAMPLE) {
+                double const g[6] = {g0, g1, g2, g3, g4, g5};
+                int q = 0;
+#pragma unroll
+                for (int v = 0; v < 6; ++v)
+#pragma unroll
+                    for (int w = v; w < 6; ++w)
+                        acc[PA_CB + q++] += d2F * g[v] * g[w];
+                acc[PA_LAP] += d2F * (hx * hx + hy * hy + hz * hz);
+            }
+        }
+    }
+    block_sum_store<PA_N>(acc, part);
+}
+
+// ---------------------------------------------------------------------------
+// Pass B: forces, energies, virial, Born stress; when sampled the Born
+// elasticity (upper 6x6) and the Laplacian pair terms (md_driver.cpp:300-430).
+// Each pair is visited from both atoms (full list): forces and energies are
+// gathered without atomics; the pair sums carry a factor 2, removed in k_post.
+// ---------------------------------------------------------------------------
+template <int KIND, int LPA, bool SAMPLE, bool EAM>
+__global__ __launch_bounds__(BLOCK) void k_force(AtomArrays a, ListArrays l, RTables rt, int N,
+                                                int npair, double rc2, double rc_pair,
+                                                double rc_rho, double rmin,
+                                                DevState *__restrict__ gs,
+                                                double *__restrict__ part) {
+    if (gs->err)
+        return;
+    using P = Poly<KIND>;
+    constexpr int APB = BLOCK / LPA;
+    int const lane = threadIdx.x % LPA, slot = threadIdx.x / LPA;
+    double const L[3] = {gs->L[0], gs->L[1], gs->L[2]};
+    double acc[PB_N];
+#pragma unroll
+    for (int k = 0; k < PB_N; ++k)
+        acc[k] = 0.;
+    XcdRange const R((N + APB - 1) / APB);
+    for (int grp = R.first; grp < R.end; grp += R.step) {
+        int const i = grp * APB + slot;
+        bool const active = i < N;
+        double fx = 0., fy = 0., fz = 0., e = 0.;
+        if (active) {
+            double4 const ci = a.pos[i];
+            int const ti = (int)ci.w;
+            double2 const Fi = EAM ? a.fprime[i] : make_double2(0., 0.);
+            int const cnt = l.nnbr[i];
+            const int *nb = l.nbr + (size_t)i * l.stride;
+            for (int k = lane; k < cnt; k += LPA) {
+                int const j = nb[k];
+                double4 const cj = a.pos[j];
+                double dx, dy, dz;
+                min_image(ci, cj, L, dx, dy, dz);
+                double const r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 > rc2)
+                    continue;
+                int const tj = (int)cj.w;
+                int const tmax = max(ti, tj), tmin = min(ti, tj);
+                int const pslot = tmax * (tmax + 1) / 2 + tmin;
+                double const r = sqrt(r2);
+                if (r < rmin)
+                    acc[PB_WRMIN] += 1.;
+                double t;
+                int const it = tab_locate(r, rt.inv_h, rt.n, t);
+                const double *base = rt.rec + (size_t)it * rt.nf * P::S;
+                double V = 0., dV = 0., d2V = 0.;
+                if (!(r > rc_pair)) {   // potential.hpp:97-103
+                    P pp;
+                    pp.load(base + pslot * P::S);
+                    V = pp.v(t);
+                    dV = pp.d1(t, rt.inv_h);
+                    if (SAMPLE)
+                        d2V = pp.d2(t, rt.inv_h2);
+                }
+                double dedr = dV, X = 0.;
+                if (SAMPLE) {
+                    X = d2V - dV / r;
+                    acc[PB_LAP] += 2. * (d2V + 2. * dV / r);
+                }
+                if (EAM && !(r > rc_rho)) {
+                    P pi, pj;
+                    pi.load(base + (npair + ti) * P::S);
+                    pj.load(base + (npair + tj) * P::S);
+                    double const dri = pi.d1(t, rt.inv_h), drj = pj.d1(t, rt.inv_h);
+                    double2 const Fj = a.fprime[j];
+                    dedr += Fi.x * drj + Fj.x * dri;
+                    if (SAMPLE) {
+                        double const d2ri = pi.d2(t, rt.inv_h2), d2rj = pj.d2(t, rt.inv_h2);
+                        acc[PB_LAP] += 2. * (Fj.x * (d2ri + 2. * dri / r) + Fi.x * (d2rj + 2. * drj / r));
+                        acc[PB_LAP] += Fi.y * (drj * drj) + Fj.y * (dri * dri);
+                        X += Fi.x * (d2rj - drj / r) + Fj.x * (d2ri - dri / r);
+                    }
+                }
+                X /= r * r;
+                double const s = -dedr / r;
+                fx += s * dx;
+                fy += s * dy;
+                fz += s * dz;
+                e += 0.5 * V;
+                acc[PB_VIR] += r * dedr;
+                double const d[3] = {dx, dy, dz};
+                double const dd[6] = {dx * dx, dy * dy, dz * dz, dy * dz, dx * dz, dx * dy};
+#pragma unroll
+                for (int v = 0; v < 6; ++v)
+                    acc[PB_SB + v] += dedr * dd[v] / r;
+                if (SAMPLE) {
+                    int q = 0;
+#pragma unroll
+                    for (int v = 0; v < 6; ++v) {
+                        double const xv = dd[v] * X;
+#pragma unroll
+                        for (int w = v; w < 6; ++w)
+                            acc[PB_CB + q++] += xv * dd[w];
+                    }
+                }
+                (void)d;
+            }
+        }
+#pragma unroll
+        for (int off = LPA / 2; off > 0; off >>= 1) {
+            fx += __shfl_xor(fx, off, LPA);
+            fy += __shfl_xor(fy, off, LPA);
+            fz += __shfl_xor(fz, off, LPA);
+            e += __shfl_xor(e, off, LPA);
+        }
+        if (active && lane == 0) {
+            a.fx[i] = fx;
+            a.fy[i] = fy;
+            a.fz[i] = fz;
+            double const ei = (EAM ? a.Femb[i] : 0.) + e;
+            a.epot[i] = ei;
+            acc[PB_EPOT] += ei;
+        }
+    }
+    block_sum_store<PB_N>(acc, part);
+}
+
+// Second U2 kick (md_driver.cpp:433-452) and the velocity sums the
+// thermodynamics need: K, sum m v_a v_b, momentum, sum |f|^2.
+__global__ __launch_bounds__(BLOCK) void k_kick2(AtomArrays a, int N,
+                                                const DevState *__restrict__ gs, int init,
+                                                double *__restrict__ part) {
+    if (gs->err)
+        return;
+    double const e = init ? 1. : gs->kick_e, ch = init ? 0. : gs->kick_c;
+    double acc[PK_N];
+#pragma unroll
+    for (int k = 0; k < PK_N; ++k)
+        acc[k] = 0.;
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < N; i += gridDim.x * BLOCK) {
+        double const m = gs->mass[a.type[i]];
+        double const fx = a.fx[i], fy = a.fy[i], fz = a.fz[i];
+        double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
+        if (!init) {
+            vx = vx * e + ch * fx / m;
+            vy = vy * e + ch * fy / m;
+            vz = vz * e + ch * fz / m;
+            a.vx[i] = vx;
+            a.vy[i] = vy;
+            a.vz[i] = vz;
+        }
+        acc[PK_K] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+        acc[PK_MVV + 0] += m * vx * vx;
+        acc[PK_MVV + 1] += m * vy * vy;
+        acc[PK_MVV + 2] += m * vz * vz;
+        acc[PK_MVV + 3] += m * vy * vz;
+        acc[PK_MVV + 4] += m * vx * vz;
+        acc[PK_MVV + 5] += m * vx * vy;
+        acc[PK_MOM + 0] += m * vx;
+        acc[PK_MOM + 1] += m * vy;
+        acc[PK_MOM + 2] += m * vz;
+        acc[PK_F2] += fx * fx + fy * fy + fz * fz;
+    }
+    block_sum_store<PK_N>(acc, part);
+}
+
+// Deterministic sum of nblocks partial records: thread t adds blocks t,
+// t+BLOCK, ... in order, then a fixed tree.
+template <int NV>
+__device__ void reduce_partials(const double *__restrict__ part, int nblocks, double *out) {
+    __shared__ double sm[BLOCK];
+    for (int k = 0; k < NV; ++k) {
+        double s = 0.;
+        for (int b = threadIdx.x; b < nblocks; b += BLOCK)
+            s += part[(size_t)k * PART_STRIDE + b];
+        sm[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = BLOCK / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w)
+                sm[threadIdx.x] += sm[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            out[k] = sm[0];
+        __syncthreads();
+    }
+}
+
+__device__ void post_step(DevState &s, const double *tA, const double *tB, const double *tK,
+                          int mode, double *__restrict__ rec_out);
+
+// ---------------------------------------------------------------------------
+// k_post: after the forces. mode 0 = constructor (md_driver.cpp:114-118),
+// mode 1 = end of integrate() (md_driver.cpp:583-600).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_post(DevState *__restrict__ gs,
+                                               const double *__restrict__ pA, int nA,
+                                               const double *__restrict__ pB, int nB,
+                                               const double *__restrict__ pK, int nK, int mode,
+                                               double *__restrict__ rec_out) {
+    if (gs->err)
+        return;
+    __shared__ double tA[PA_N], tB[PB_N], tK[PK_N];
+    if (gs->has_eam)
+        reduce_partials<PA_N>(pA, nA, tA);
+    else if (threadIdx.x < PA_N)
+        tA[threadIdx.x] = 0.;
+    reduce_partials<PB_N>(pB, nB, tB);
+    reduce_partials<PK_N>(pK, nK, tK);
+    __shared__ DevState s;
+    state_load(s, gs);
+    if (threadIdx.x == 0)
+        post_step(s, tA, tB, tK, mode, rec_out);
+    state_store(gs, s);
+}
+
+__device__ void post_step(DevState &s, const double *tA, const double *tB, const double *tK,
+                          int mode, double *__restrict__ rec_out) {
+    bool const sample = s.sample != 0;
+    bool const npt = s.ensemble == MDL_NPT;
+    bool const thermo = s.ensemble == MDL_NVT || npt;
+    double *rec = s.rec;
+    int const N = s.N;
+
+    // pair sums were taken over the full list: halve them (md_driver.cpp:389-407)
+    s.virial = -(0.5 * tB[PB_VIR]) / 3.;
+    double sB[6], CB[21];
+    for (int v = 0; v < 6; ++v)
+        sB[v] = 0.5 * tB[PB_SB + v];
+    if (sample) {
+        for (int q = 0; q < 21; ++q)
+            CB[q] = 0.5 * tB[PB_CB + q] + tA[PA_CB + q];
+        s.lap = 0.5 * tB[PB_LAP] + tA[PA_LAP];
+    }
+    s.warn_rmin += (long long)(0.5 * tB[PB_WRMIN] + 0.25);
+    {   // virial self-check (md_driver.cpp:422-429), a warning only
+        double const vtr = -(sB[0] + sB[1] + sB[2]);
+        if (fabs(s.virial / (vtr / 3.) - 1.) > 1e-7)
+            s.warn_virial++;
+    }
+    double const K = tK[PK_K];
+    s.ekin_tot = K;
+    // calculate_configurational_temperature (md_driver.cpp:630-646)
+    if (sample) {
+        s.f2 = tK[PK_F2];
+        s.Tci = (s.f2 != 0. && s.lap != 0.) ? K_BOLTZ * s.lap / s.f2 : 0.;
+    }
+    double scale = 1.;
+    if (mode == 1) {
+        double const h = 0.5 * s.dt;
+        if (npt)
+            U2p(s, h);
+        if (thermo)
+            scale = nhc_thermo(s, h);
+        if (npt)
+            nhc_baro(s, h);
+        s.vscale = scale;
+    }
+    // calculate_thermodynamics (md_driver.cpp:648-730) on the scaled velocities
+    double const ekin_atom = s.ekin_tot / N;
+    double mom[3];
+    for (int c = 0; c < 3; ++c)
+        mom[c] = scale * tK[PK_MOM + c];
+    double const p2 = mom[0] * mom[0] + mom[1] * mom[1] + mom[2] * mom[2];
+    double const sc2 = 2. * s.mass[0] * s.ekin_tot;
+    if (p2 / sc2 > 1e-14) {
+        s.err |= DE_MOMENTUM;
+        s.err_val[0] = mom[0];
+        s.err_val[1] = mom[1];
+        s.err_val[2] = mom[2];
+        s.err_val[3] = sqrt(sc2);
+    }
+    double const epot_atom = tB[PB_EPOT] / N;
+    double const epot_eam_atom = s.has_eam ? tA[PA_EF] / N : 0.;
+    double const etot_atom = ekin_atom + epot_atom;
+    double const old = s.etot_cons;
+    double ec = etot_atom * N;
+    double const V = box_volume(s.L);
+    if (thermo)
+        ec += chain_etot(s.th);
+    if (npt)
+        ec += V * s.bar_P + (0.5 * (s.peps * s.peps) / s.bar_W + chain_etot(s.bth));
+    s.etot_cons = ec;
+    s.etot_cons_err = fabs(ec / s.etot_cons_initial - 1.);
+    double const err_cur = fabs(ec / old - 1.);
+    double const Tk = 2. * s.ekin_tot / (K_BOLTZ * s.Nf);
+    double const P = (s.ekin_tot * 2. / 3. + s.virial) / V;
+    double st[6];
+    double const s2 = scale * scale;
+    for (int v = 0; v < 6; ++v)
+        st[v] = (-sB[v] + s2 * tK[PK_MVV + v]) / V;
+    double const ptr = st[0] + st[1] + st[2];
+    double const perr = fabs(P / (ptr / 3.) - 1.);
+    if (perr > 1e-5) {
+        s.err |= DE_PRESSURE;
+        s.err_val[0] = perr;
+        s.err_val[1] = P;
+        s.err_val[2] = ptr / 3.;
+    }
+    if ((s.etot_cons_err > s.fatal || err_cur > s.fatal) && s.cycle > 0) {
+        s.err |= DE_CONSERVATION;
+        s.err_val[0] = s.etot_cons_err;
+        s.err_val[1] = err_cur;
+    }
+    if (mode == 0)
+        s.etot_cons_initial = ec;
+    else {
+        s.time += s.dt;
+        s.cycle += 1;
+    }
+    rec[MDL_S_EPOT_EAM_ATOM] = epot_eam_atom;
+    rec[MDL_S_EPOT_ATOM] = epot_atom;
+    rec[MDL_S_EKIN_ATOM] = ekin_atom;
+    rec[MDL_S_ETOT_ATOM] = etot_atom;
+    rec[MDL_S_EKIN_TOT] = s.ekin_tot;
+    rec[MDL_S_ETOT_CONS_INITIAL] = s.etot_cons_initial;
+    rec[MDL_S_ETOT_CONS] = s.etot_cons;
+    rec[MDL_S_ETOT_CONS_ERR] = s.etot_cons_err;
+    rec[MDL_S_VIRIAL] = s.virial;
+    rec[MDL_S_LAPLACIAN] = s.lap;
+    rec[MDL_S_FORCE2] = s.f2;
+    rec[MDL_S_TKIN] = Tk;
+    rec[MDL_S_TCONF_INV] = s.Tci;
+    rec[MDL_S_PRESSURE] = P;
+    rec[MDL_S_TIME] = s.time;
+    rec[MDL_S_CYCLE] = (double)s.cycle;
+    for (int c = 0; c < 3; ++c) {
+        rec[MDL_S_BOX0 + c] = s.L[c];
+        rec[MDL_S_MOM0 + c] = mom[c];
+        rec[MDL_S_BOX_INITIAL0 + c] = s.L0[c];
+    }
+    rec[MDL_S_BAR_EPS] = s.eps;
+    rec[MDL_S_BAR_PEPS] = s.peps;
+    for (int v = 0; v < 6; ++v) {
+        rec[MDL_S_BORN_STRESS + v] = sB[v] / V;
+        rec[MDL_S_STRESS + v] = st[v];
+    }
+    if (sample) {
+        int q = 0;
+        for (int v = 0; v < 6; ++v)
+            for (int w = 0; w < 6; ++w)
+                rec[MDL_S_BORN_ELAST + 6 * v + w] = w >= v ? CB[q++] / V : 0.;
+    }
+    rec[MDL_S_NLIST_BUILDS] = (double)s.nbuilds;
+    rec[MDL_S_WARN_VIRIAL] = (double)s.warn_virial;
+    rec[MDL_S_WARN_RMIN] = (double)s.warn_rmin;
+    rec[MDL_S_NLIST_ENTRIES] = (double)s.nlist_entries;
+    if (rec_out && sample)
+        for (int k = 0; k < MDL_NRECORD; ++k)
+            rec_out[k] = rec[k];
+}
+
+// Host-visible per-atom arrays (md_driver_module.cpp:103-109)
+__global__ void k_materialize(AtomArrays a, int N, const DevState *__restrict__ gs, int what,
+                              double *__restrict__ out) {
+    int const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    double const vs = gs->vscale;
+    switch (what) {
+    case MDL_A_POSITIONS: {
+        double4 const c = a.pos[i];
+        out[3 * i + 0] = (c.x - nint_away(c.x)) * gs->L_drift[0];
+        out[3 * i + 1] = (c.y - nint_away(c.y)) * gs->L_drift[1];
+        out[3 * i + 2] = (c.z - nint_away(c.z)) * gs->L_drift[2];
+        break;
+    }
+    case MDL_A_COORDINATES: {
+        double4 const c = a.pos[i];
+        out[3 * i + 0] = c.x;
+        out[3 * i + 1] = c.y;
+        out[3 * i + 2] = c.z;
+        break;
+    }
+    case MDL_A_VELOCITIES:
+        out[3 * i + 0] = a.vx[i] * vs;
+        out[3 * i + 1] = a.vy[i] * vs;
+        out[3 * i + 2] = a.vz[i] * vs;
+        break;
+    case MDL_A_FORCES:
+        out[3 * i + 0] = a.fx[i];
+        out[3 * i + 1] = a.fy[i];
+        out[3 * i + 2] = a.fz[i];
+        break;
+    case MDL_A_RHO:
+        out[i] = a.rho[i];
+        break;
+    case MDL_A_EPOT:
+        out[i] = a.epot[i];
+        break;
+    case MDL_A_EKIN: {
+        double const vx = a.vx[i] * vs, vy = a.vy[i] * vs, vz = a.vz[i] * vs;
+        out[i] = 0.5 * gs->mass[a.type[i]] * (vx * vx + vy * vy + vz * vz);
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+}  // namespace mdl
