@@ -83,6 +83,7 @@ struct Driver {
     double *pA = nullptr, *pB = nullptr, *pK = nullptr;
     int gridA = 8, gridB = 8, gridK = 1;
     double *d_stage = nullptr;
+    double *d_totals = nullptr, *d_dmax = nullptr;
     DevState *gs = nullptr;
     DevState hs{};                 // host mirror (refreshed by sync_state)
     std::vector<double> init_pos;  // positions attribute before the first step
@@ -116,13 +117,15 @@ struct Driver {
 
     void launch_build() {
         int const gN = (N + BLOCK - 1) / BLOCK;
-        k_decide<<<1, 1, 0, stream>>>(gs, use_cells ? 1 : 0, 1.0);
+        k_decide<<<1, 64, 0, stream>>>(gs, use_cells ? 1 : 0, 1.0, d_dmax, gridK);
         if (use_cells) {
+            constexpr int BL = 8;   // lanes per atom of the list build
             k_bin<<<gN, BLOCK, 0, stream>>>(a, l, N, gs);
             k_scan<<<1, 1024, 0, stream>>>(l, gs);
             k_scatter<<<gN, BLOCK, 0, stream>>>(l, N, gs);
-            k_sort_cells<<<(l.ncells + BLOCK - 1) / BLOCK, BLOCK, 0, stream>>>(l, gs);
-            k_build_cells<<<gN, BLOCK, 0, stream>>>(a, l, N, gs);
+            k_sort_cells<<<l.ncells, 64, 0, stream>>>(l, gs);
+            k_build_cells<BL><<<(int)(((int64_t)N * BL + BLOCK - 1) / BLOCK), BLOCK, 0, stream>>>(
+                a, l, N, gs);
         } else {
             k_build_brute<<<gN, BLOCK, 0, stream>>>(a, l, N, gs);
         }
@@ -148,14 +151,18 @@ struct Driver {
         k_pre<<<1, 64, 0, stream>>>(gs, s ? 1 : 0);
         if (force_rebuild)
             k_set_rebuild<<<1, 1, 0, stream>>>(gs);
-        k_kick_drift<<<gridK, BLOCK, 0, stream>>>(a, N, gs);
+        k_kick_drift<<<gridK, BLOCK, 0, stream>>>(a, N, gs, d_dmax);
         launch_build();
         if (tev)
             launch_forces(s, tev[0], tev[1], tev[2]);
         else
             launch_forces(s);
-        k_kick2<<<gridK, BLOCK, 0, stream>>>(a, N, gs, 0, pK);
-        k_post<<<1, BLOCK, 0, stream>>>(gs, pA, gridA, pB, gridB, pK, gridK, 1, rec_out);
+        if (s)
+            k_kick2<true><<<gridK, BLOCK, 0, stream>>>(a, N, gs, 0, has_eam, pK);
+        else
+            k_kick2<false><<<gridK, BLOCK, 0, stream>>>(a, N, gs, 0, has_eam, pK);
+        k_reduce<<<NTOTAL, BLOCK, 0, stream>>>(gs, pA, gridA, pB, gridB, pK, gridK, d_totals);
+        k_post<<<1, 64, 0, stream>>>(gs, d_totals, 1, rec_out);
         ++cycle_host;
     }
 
@@ -425,11 +432,14 @@ static mdl_status create(const mdl_config *c, Driver &d) {
     ALLOC(a.type, N);
     ALLOC(a.cref, N);
     ALLOC(a.fprime, N);
+    ALLOC(a.geam, (size_t)4 * N);
     ALLOC(a.Femb, N);
     ALLOC(a.rho, N);
     ALLOC(a.epot, N);
     ALLOC(d.l.nbr, (size_t)N * stride);
     ALLOC(d.l.nnbr, N);
+    ALLOC(d.l.cnbr, (size_t)N * stride);
+    ALLOC(d.l.ccnt, N);
     if (cells) {
         ALLOC(d.l.cell_of, N);
         ALLOC(d.l.cell_count, d.l.ncells);
@@ -443,6 +453,9 @@ static mdl_status create(const mdl_config *c, Driver &d) {
     ALLOC(d.pB, (size_t)PART_STRIDE * PB_N);
     ALLOC(d.pK, (size_t)PART_STRIDE * PK_N);
     ALLOC(d.d_stage, (size_t)3 * N);
+    ALLOC(d.d_totals, NTOTAL);
+    ALLOC(d.d_dmax, GRID_ATOM_MAX);
+    MDL_HIP(hipMemset(d.d_dmax, 0, sizeof(double) * GRID_ATOM_MAX));
     ALLOC(d.gs, 1);
 #undef ALLOC
     MDL_HIP(hipMemset(a.fx, 0, sizeof(double) * N));
@@ -450,6 +463,7 @@ static mdl_status create(const mdl_config *c, Driver &d) {
     MDL_HIP(hipMemset(a.fz, 0, sizeof(double) * N));
     MDL_HIP(hipMemset(a.Femb, 0, sizeof(double) * N));
     MDL_HIP(hipMemset(a.fprime, 0, sizeof(double2) * N));
+    MDL_HIP(hipMemset(a.geam, 0, sizeof(double2) * 4 * N));
     MDL_HIP(hipMemset(a.rho, 0, sizeof(double) * N));
 
     // scaled coordinates (md_driver.cpp:162-173) and SoA velocities
@@ -543,9 +557,10 @@ static mdl_status create(const mdl_config *c, Driver &d) {
         if (mdl_status st = autotune(d))
             return st;
     d.launch_forces(true);
-    k_kick2<<<d.gridK, BLOCK, 0, d.stream>>>(d.a, N, d.gs, 1, d.pK);
-    k_post<<<1, BLOCK, 0, d.stream>>>(d.gs, d.pA, d.gridA, d.pB, d.gridB, d.pK, d.gridK, 0,
-                                       nullptr);
+    k_kick2<true><<<d.gridK, BLOCK, 0, d.stream>>>(d.a, N, d.gs, 1, d.has_eam, d.pK);
+    k_reduce<<<NTOTAL, BLOCK, 0, d.stream>>>(d.gs, d.pA, d.gridA, d.pB, d.gridB, d.pK, d.gridK,
+                                              d.d_totals);
+    k_post<<<1, 64, 0, d.stream>>>(d.gs, d.d_totals, 0, nullptr);
     MDL_HIP(hipGetLastError());
     if (mdl_status st = d.sync_state())
         return st;

@@ -23,12 +23,20 @@ constexpr int BLOCK = 256;                           // threads per block of the
 
 // Per-block partial sums written by the atom kernels and reduced by the
 // scalar kernel in a fixed order (deterministic).
-enum PartialA { PA_EF = 0, PA_CB = 1 /*21*/, PA_LAP = 22, PA_N = 24 };
+enum PartialA { PA_EF = 0, PA_N = 1 };
 enum PartialB {
-    PB_EPOT = 0, PB_VIR = 1, PB_SB = 2 /*6*/, PB_CB = 8 /*21*/, PB_LAP = 29, PB_WRMIN = 30,
-    PB_N = 32
+    PB_EPOT = 0, PB_VIR = 1, PB_SB = 2 /*6*/, PB_CB = 8 /*15 monomials*/, PB_LAP = 23,
+    PB_WRMIN = 24, PB_N = 25
 };
-enum PartialK { PK_K = 0, PK_MVV = 1 /*6*/, PK_MOM = 7 /*3*/, PK_F2 = 10, PK_N = 12 };
+enum PartialK {
+    PK_K = 0, PK_MVV = 1 /*6*/, PK_MOM = 7 /*3*/, PK_F2 = 10, PK_CBF = 11 /*21*/, PK_LAPF = 32,
+    PK_N = 33
+};
+// The 15 distinct degree-4 monomials of the pair vector d in the pair Born term
+enum Monomial {
+    M_X4, M_Y4, M_Z4, M_X3Y, M_X3Z, M_Y3X, M_Y3Z, M_Z3X, M_Z3Y, M_X2Y2, M_X2Z2, M_Y2Z2,
+    M_X2YZ, M_Y2XZ, M_Z2XY
+};
 
 // Error bits raised on the device (mirrors MDL_ERR_*).
 enum DevErr : int {

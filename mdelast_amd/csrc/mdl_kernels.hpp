@@ -20,6 +20,7 @@ struct AtomArrays {
     int *type;
     double4 *cref;     // coordinates at the last list build
     double2 *fprime;   // F'(rho_i), F''(rho_i)
+    double2 *geam;     // sampled steps: g_i (6, Voigt) and |h_i|^2, 4 double2 per atom
     double *Femb;      // F(rho_i)
     double *rho;       // rho_i
     double *epot;      // per-atom potential energy
@@ -28,6 +29,8 @@ struct AtomArrays {
 struct ListArrays {
     int *nbr;          // [N][stride] full list (both i->j and j->i)
     int *nnbr;         // [N]
+    int *cnbr;         // [N][stride] pairs inside r_c, compacted by pass A
+    int *ccnt;         // [N]
     int stride;
     int *cell_of, *cell_count, *cell_start, *cell_fill, *cell_atoms;
     int ncells;
@@ -81,6 +84,13 @@ struct XcdRange {
         step = nper;
     }
 };
+
+template <int LPA>
+__device__ __forceinline__ unsigned long long group_mask() {
+    int const l64 = threadIdx.x & 63;
+    unsigned long long const m = LPA == 64 ? ~0ull : ((1ull << LPA) - 1ull);
+    return m << (l64 & ~(LPA - 1));
+}
 
 // Minimum-image pair vector d = (s - nint(s)) L of s = c_i - c_j
 // (md_driver.cpp:175-185, applied to the difference as at :222 and :333)
@@ -153,9 +163,75 @@ __device__ double thermo_couple(const DevState &s) {
     return e;
 }
 
+// U4 and U3 of a chain of compile-time length M held in registers (the
+// chain sweep is a long chain of dependent FP64 operations; keeping p_eta,
+// eta and q out of memory shortens it). Same arithmetic as chain_U4/chain_U3.
+template <int M>
+__device__ __forceinline__ void reg_U4(double (&p)[M], const double (&q)[M], double Nf,
+                                       double kT, double dt, double e_couple, bool outward) {
+#pragma unroll
+    for (int s = 0; s < M; ++s) {
+        int const j = outward ? s : M - 1 - s;
+        double const G = j != 0 ? p[j - 1] * p[j - 1] / q[j - 1] - kT : 2. * e_couple - Nf * kT;
+        if (j != M - 1) {
+            double e, op;
+            expx_operator(dt * p[j + 1] / q[j + 1], e, op);
+            p[j] = p[j] * e + dt * G * op;
+        } else {
+            p[j] += dt * G;
+        }
+    }
+}
+
+template <int M>
+__device__ __forceinline__ double reg_U3(const double (&p)[M], const double (&q)[M],
+                                         double (&eta)[M], double dt) {
+    double const f = exp(-dt * p[0] / q[0]);
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+        eta[j] += dt * p[j] / q[j];
+    return f;
+}
+
+template <int M> struct RegChain {
+    double p[M], q[M], eta[M];
+    __device__ __forceinline__ void load(const Chain &c) {
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            p[j] = c.p_eta[j];
+            q[j] = c.q[j];
+            eta[j] = c.eta[j];
+        }
+    }
+    __device__ __forceinline__ void store(Chain &c) const {
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            c.p_eta[j] = p[j];
+            c.eta[j] = eta[j];
+        }
+    }
+};
+
 // U_NHC_thermo_propagator (md_driver.cpp:539-549) with U3 acting on a running
 // scale factor: K' = s^2 K replaces the O(N) recomputation (md_driver.cpp:504).
-__device__ double nhc_thermo(DevState &s, double dt) {
+template <int M> __device__ double nhc_thermo_m(DevState &s, double dt) {
+    RegChain<M> c;
+    c.load(s.th);
+    double const kT = K_BOLTZ * s.th.T;
+    double scale = 1.;
+    for (int k = 0; k < 7; ++k) {
+        double const dtw = WSY7[k] * dt;
+        reg_U4<M>(c.p, c.q, s.th.Nf, kT, 0.5 * dtw, thermo_couple(s), false);
+        double const f = reg_U3<M>(c.p, c.q, c.eta, dtw);
+        scale *= f;
+        s.ekin_tot *= f * f;
+        reg_U4<M>(c.p, c.q, s.th.Nf, kT, 0.5 * dtw, thermo_couple(s), true);
+    }
+    c.store(s.th);
+    return scale;
+}
+
+__device__ double nhc_thermo_mem(DevState &s, double dt) {
     double scale = 1.;
     for (int k = 0; k < 7; ++k) {
         double const dtw = WSY7[k] * dt;
@@ -169,12 +245,46 @@ __device__ double nhc_thermo(DevState &s, double dt) {
 }
 
 // U_NHC_baro_propagator (md_driver.cpp:551-561)
-__device__ void nhc_baro(DevState &s, double dt) {
+template <int M> __device__ void nhc_baro_m(DevState &s, double dt) {
+    RegChain<M> c;
+    c.load(s.bth);
+    double const kT = K_BOLTZ * s.bth.T;
+    for (int k = 0; k < 7; ++k) {
+        double const dtw = WSY7[k] * dt;
+        reg_U4<M>(c.p, c.q, s.bth.Nf, kT, 0.5 * dtw, 0.5 * (s.peps * s.peps) / s.bar_W, false);
+        s.peps *= reg_U3<M>(c.p, c.q, c.eta, dtw);
+        reg_U4<M>(c.p, c.q, s.bth.Nf, kT, 0.5 * dtw, 0.5 * (s.peps * s.peps) / s.bar_W, true);
+    }
+    c.store(s.bth);
+}
+
+__device__ void nhc_baro_mem(DevState &s, double dt) {
     for (int k = 0; k < 7; ++k) {
         double const dtw = WSY7[k] * dt;
         chain_U4(s.bth, 0.5 * dtw, 0.5 * (s.peps * s.peps) / s.bar_W, false);
         s.peps *= chain_U3(s.bth, dtw);
         chain_U4(s.bth, 0.5 * dtw, 0.5 * (s.peps * s.peps) / s.bar_W, true);
+    }
+}
+
+#define MDL_CHAIN_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) \
+    X(13) X(14) X(15) X(16)
+
+__device__ double nhc_thermo(DevState &s, double dt) {
+    switch (s.th.M) {
+#define MDL_CASE(m) case m: return nhc_thermo_m<m>(s, dt);
+        MDL_CHAIN_CASES(MDL_CASE)
+#undef MDL_CASE
+    default: return nhc_thermo_mem(s, dt);
+    }
+}
+
+__device__ void nhc_baro(DevState &s, double dt) {
+    switch (s.bth.M) {
+#define MDL_CASE(m) case m: nhc_baro_m<m>(s, dt); return;
+        MDL_CHAIN_CASES(MDL_CASE)
+#undef MDL_CASE
+    default: nhc_baro_mem(s, dt);
     }
 }
 
@@ -266,7 +376,8 @@ __device__ void pre_step(DevState &s, int sample) {
 
 // First U2 (md_driver.cpp:433-452) and U1 drift (md_driver.cpp:462-499).
 __global__ __launch_bounds__(BLOCK) void k_kick_drift(AtomArrays a, int N,
-                                                     const DevState *__restrict__ gs) {
+                                                     const DevState *__restrict__ gs,
+                                                     double *__restrict__ dmax_part) {
     if (gs->err)
         return;
     double const s = gs->kick_s, e = gs->kick_e, ch = gs->kick_c, cf = gs->drift_cf;
@@ -293,23 +404,39 @@ __global__ __launch_bounds__(BLOCK) void k_kick_drift(AtomArrays a, int N,
     }
     for (int off = 32; off > 0; off >>= 1)
         dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
+    __shared__ double sm[BLOCK / 64];
     if ((threadIdx.x & 63) == 0)
-        atomicMax(const_cast<unsigned long long *>(&gs->maxdisp2_bits),
-                  (unsigned long long)__double_as_longlong(dmax));
+        sm[threadIdx.x >> 6] = dmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = sm[0];
+        for (int w = 1; w < BLOCK / 64; ++w)
+            m = fmax(m, sm[w]);
+        dmax_part[blockIdx.x] = m;
+    }
 }
 
 // Decide whether the list must be rebuilt: forced by calculate_neighbors(),
 // or any pair outside the list could now be inside r_c. For a list built with
 // radius r_list at box L_b, a pair not in it is at least lambda r_list - 2 u_max
 // apart now, lambda = min L/L_b, u_max the largest displacement since the build.
-__global__ void k_decide(DevState *__restrict__ gs, int use_cells, double cell_margin) {
-    if (threadIdx.x != 0 || gs->err)
+__global__ __launch_bounds__(64) void k_decide(DevState *__restrict__ gs, int use_cells,
+                                              double cell_margin,
+                                              const double *__restrict__ dmax_part, int nparts) {
+    if (gs->err)
+        return;
+    double m = 0.;
+    for (int b = threadIdx.x; b < nparts; b += 64)
+        m = fmax(m, dmax_part[b]);
+    for (int off = 32; off > 0; off >>= 1)
+        m = fmax(m, __shfl_xor(m, off, 64));
+    if (threadIdx.x != 0)
         return;
     DevState &s = *gs;
     double lam = 1e300;
     for (int a = 0; a < 3; ++a)
         lam = fmin(lam, s.L[a] / s.L_build[a]);
-    double const umax = sqrt(__longlong_as_double((long long)s.maxdisp2_bits));
+    double const umax = sqrt(m);
     bool const need = s.force_rebuild || !(lam * s.r_list - 2. * umax > s.rc);
     s.do_rebuild = need ? 1 : 0;
     if (!need)
@@ -395,22 +522,30 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(ListArrays l, int N,
 
 // Members of each cell in ascending index order, so the list (and the FP
 // summation order of the passes) does not depend on atomic arrival order.
-__global__ __launch_bounds__(BLOCK) void k_sort_cells(ListArrays l,
-                                                     const DevState *__restrict__ gs) {
+// One wave per cell: members staged in LDS, each placed at its rank.
+constexpr int CELL_MAX = 1024;
+__global__ __launch_bounds__(64) void k_sort_cells(ListArrays l, DevState *__restrict__ gs) {
     if (gs->err || !gs->do_rebuild)
         return;
-    int const c = blockIdx.x * BLOCK + threadIdx.x;
-    if (c >= l.ncells)
-        return;
-    int const b = l.cell_start[c], e = l.cell_start[c + 1];
-    for (int k = b + 1; k < e; ++k) {
-        int const v = l.cell_atoms[k];
-        int m = k - 1;
-        while (m >= b && l.cell_atoms[m] > v) {
-            l.cell_atoms[m + 1] = l.cell_atoms[m];
-            --m;
+    int const c = blockIdx.x;
+    int const b = l.cell_start[c], n = l.cell_start[c + 1] - b;
+    if (n > CELL_MAX) {
+        if (threadIdx.x == 0) {
+            atomicOr(&gs->err, (int)DE_CELL_GRID);
+            gs->err_val[0] = (double)n;
         }
-        l.cell_atoms[m + 1] = v;
+        return;
+    }
+    __shared__ int v[CELL_MAX];
+    for (int k = threadIdx.x; k < n; k += 64)
+        v[k] = l.cell_atoms[b + k];
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += 64) {
+        int const x = v[k];
+        int rank = 0;
+        for (int m = 0; m < n; ++m)
+            rank += v[m] < x;
+        l.cell_atoms[b + rank] = x;
     }
 }
 
@@ -433,41 +568,56 @@ __device__ __forceinline__ void list_finish(ListArrays &l, AtomArrays &a, int i,
     }
 }
 
-// Full list from the 27 surrounding cells (>= 3 cells per direction).
+// Full list from the 27 surrounding cells (>= 3 cells per direction). LPA
+// lanes per atom walk each cell's members together; a ballot prefix keeps the
+// entries in (cell, member) order, the order one thread would produce.
+template <int LPA>
 __global__ __launch_bounds__(BLOCK) void k_build_cells(AtomArrays a, ListArrays l, int N,
                                                       DevState *__restrict__ gs) {
     if (gs->err || !gs->do_rebuild)
         return;
-    int const i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= N)
-        return;
+    int const lane = threadIdx.x % LPA;
+    int const i = (blockIdx.x * BLOCK + threadIdx.x) / LPA;
+    int const l64 = threadIdx.x & 63;
+    unsigned long long const gmask = group_mask<LPA>();
+    unsigned long long const below = (l64 == 0) ? 0ull : (~0ull >> (64 - l64));
+    bool const active = i < N;
     double const L[3] = {gs->L[0], gs->L[1], gs->L[2]};
     double const rl2 = gs->r_list * gs->r_list;
     int const nx = gs->ncell[0], ny = gs->ncell[1], nz = gs->ncell[2];
-    double4 const ci = a.pos[i];
+    double4 const ci = a.pos[active ? i : 0];
     int const cx = cell_coord(ci.x, nx), cy = cell_coord(ci.y, ny), cz = cell_coord(ci.z, nz);
+    int *row = l.nbr + (size_t)(active ? i : 0) * l.stride;
     int cnt = 0;
-    for (int dz = -1; dz <= 1; ++dz) {
-        int const z = (cz + dz + nz) % nz;
-        for (int dy = -1; dy <= 1; ++dy) {
-            int const y = (cy + dy + ny) % ny;
-            for (int dx = -1; dx <= 1; ++dx) {
-                int const x = (cx + dx + nx) % nx;
-                int const cell = (z * ny + y) * nx + x;
-                int const b = l.cell_start[cell], e = l.cell_start[cell + 1];
-                for (int k = b; k < e; ++k) {
-                    int const j = l.cell_atoms[k];
-                    if (j == i)
-                        continue;
+    for (int q = 0; q < 27; ++q) {
+        int const dz = q / 9 - 1, dy = (q / 3) % 3 - 1, dx = q % 3 - 1;
+        int const z = (cz + dz + nz) % nz, y = (cy + dy + ny) % ny, x = (cx + dx + nx) % nx;
+        int const cell = (z * ny + y) * nx + x;
+        int const b = active ? l.cell_start[cell] : 0;
+        int const e = active ? l.cell_start[cell + 1] : 0;
+        for (int k0 = b; k0 < e; k0 += LPA) {
+            int const k = k0 + lane;
+            bool in = false;
+            int j = 0;
+            if (k < e) {
+                j = l.cell_atoms[k];
+                if (j != i) {
                     double ddx, ddy, ddz;
                     min_image(ci, a.pos[j], L, ddx, ddy, ddz);
-                    if (ddx * ddx + ddy * ddy + ddz * ddz <= rl2)
-                        list_push(l, i, cnt, j);
+                    in = ddx * ddx + ddy * ddy + ddz * ddz <= rl2;
                 }
             }
+            unsigned long long const bal = __ballot(in) & gmask;
+            if (in) {
+                int const slot = cnt + __popcll(bal & below);
+                if (slot < l.stride)
+                    row[slot] = j;
+            }
+            cnt += __popcll(bal);
         }
     }
-    list_finish(l, a, i, cnt, gs);
+    if (active && lane == 0)
+        list_finish(l, a, i, cnt, gs);
 }
 
 // Full list by testing every other atom (small boxes, < 3 cells per direction).
@@ -494,11 +644,14 @@ __global__ __launch_bounds__(BLOCK) void k_build_brute(AtomArrays a, ListArrays 
 }
 
 // ---------------------------------------------------------------------------
-// Pass A: densities (md_driver.cpp:195-277) over the full list; per atom
-// rho_i, and when sampled g_i (Voigt) and h_i; then F, F', F''. The per-atom
-// Born and Laplacian terms F''_i g_i g_i and F''_i |h_i|^2
-// (md_driver.cpp:410-419) are summed here, where g_i and h_i are complete.
+// Pass A: densities (md_driver.cpp:195-277) over the full skin list. Per atom
+// rho_i, and when sampled g_i (Voigt) and h_i (md_driver.cpp:245-264); then
+// F, F', F'' (md_driver.cpp:267-276). The pass also compacts the pairs inside
+// r_c into the short list that pass B walks (positions do not change between
+// the passes, so pass B's r^2 test would select exactly these pairs). Within
+// a lane group the compacted order is the candidate order (ballot prefix).
 // ---------------------------------------------------------------------------
+
 template <int KIND, int LPA, bool SAMPLE>
 __global__ __launch_bounds__(BLOCK) void k_density(AtomArrays a, ListArrays l, RTables rt,
                                                   FTables ft, int N, int npair, int ntypes,
@@ -510,6 +663,9 @@ __global__ __launch_bounds__(BLOCK) void k_density(AtomArrays a, ListArrays l, R
     using P = Poly<KIND>;
     constexpr int APB = BLOCK / LPA;
     int const lane = threadIdx.x % LPA, slot = threadIdx.x / LPA;
+    int const l64 = threadIdx.x & 63;
+    unsigned long long const gmask = group_mask<LPA>();
+    unsigned long long const below = (l64 == 0) ? 0ull : (~0ull >> (64 - l64));
     double const L[3] = {gs->L[0], gs->L[1], gs->L[2]};
     double acc[PA_N];
 #pragma unroll
@@ -522,40 +678,53 @@ __global__ __launch_bounds__(BLOCK) void k_density(AtomArrays a, ListArrays l, R
         double rho = 0., g0 = 0., g1 = 0., g2 = 0., g3 = 0., g4 = 0., g5 = 0.;
         double hx = 0., hy = 0., hz = 0.;
         double4 ci = make_double4(0., 0., 0., 0.);
+        int cnt = 0;
         if (active) {
             ci = a.pos[i];
-            int const cnt = l.nnbr[i];
-            const int *nb = l.nbr + (size_t)i * l.stride;
-            for (int k = lane; k < cnt; k += LPA) {
-                int const j = nb[k];
-                double4 const cj = a.pos[j];
-                double dx, dy, dz;
+            cnt = l.nnbr[i];
+        }
+        const int *nb = l.nbr + (size_t)(active ? i : 0) * l.stride;
+        int *cb = l.cnbr + (size_t)(active ? i : 0) * l.stride;
+        int nin = 0;   // compacted entries so far (uniform in the group)
+        for (int k0 = 0; k0 < cnt; k0 += LPA) {
+            int const k = k0 + lane;
+            bool in = false;
+            int j = 0;
+            double dx = 0., dy = 0., dz = 0., r2 = 0.;
+            double4 cj;
+            if (k < cnt) {
+                j = nb[k];
+                cj = a.pos[j];
                 min_image(ci, cj, L, dx, dy, dz);
-                double const r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 > rc2)
-                    continue;
+                r2 = dx * dx + dy * dy + dz * dz;
+                in = !(r2 > rc2);
+            }
+            unsigned long long const bal = __ballot(in) & gmask;
+            if (in) {
+                cb[nin + __popcll(bal & below)] = j;
                 double const r = sqrt(r2);
-                if (r > rc_rho)   // eam.hpp:30-44: rho and its derivatives are 0 beyond rcut
-                    continue;
-                int const tj = (int)cj.w;
-                double t;
-                int const it = tab_locate(r, rt.inv_h, rt.n, t);
-                P p;
-                p.load(rt.rec + ((size_t)it * rt.nf + npair + tj) * P::S);
-                rho += p.v(t);
-                if (SAMPLE) {
-                    double const w = p.d1(t, rt.inv_h) / r;
-                    g0 += w * dx * dx;
-                    g1 += w * dy * dy;
-                    g2 += w * dz * dz;
-                    g3 += w * dy * dz;
-                    g4 += w * dx * dz;
-                    g5 += w * dx * dy;
-                    hx += w * dx;
-                    hy += w * dy;
-                    hz += w * dz;
+                if (!(r > rc_rho)) {   // eam.hpp:30-44: rho and derivatives are 0 beyond rcut
+                    int const tj = (int)cj.w;
+                    double t;
+                    int const it = tab_locate(r, rt.inv_h, rt.n, t);
+                    P p;
+                    p.load(rt.rec + ((size_t)it * rt.nf + npair + tj) * P::S);
+                    rho += p.v(t);
+                    if (SAMPLE) {
+                        double const w = p.d1(t, rt.inv_h) / r;
+                        g0 += w * dx * dx;
+                        g1 += w * dy * dy;
+                        g2 += w * dz * dz;
+                        g3 += w * dy * dz;
+                        g4 += w * dx * dz;
+                        g5 += w * dx * dy;
+                        hx += w * dx;
+                        hy += w * dy;
+                        hz += w * dz;
+                    }
                 }
             }
+            nin += __popcll(bal);
         }
 #pragma unroll
         for (int off = LPA / 2; off > 0; off >>= 1) {
@@ -573,6 +742,7 @@ __global__ __launch_bounds__(BLOCK) void k_density(AtomArrays a, ListArrays l, R
             }
         }
         if (active && lane == 0) {
+            l.ccnt[i] = nin;
             int const ti = (int)ci.w;
             if (rho > ft.rhomax[ti]) {
                 atomicOr(&gs->err, (int)DE_RHOMAX);
@@ -590,14 +760,11 @@ __global__ __launch_bounds__(BLOCK) void k_density(AtomArrays a, ListArrays l, R
             a.fprime[i] = make_double2(dF, d2F);
             acc[PA_EF] += F;
             if (SAMPLE) {
-                double const g[6] = {g0, g1, g2, g3, g4, g5};
-                int q = 0;
-#pragma unroll
-                for (int v = 0; v < 6; ++v)
-#pragma unroll
-                    for (int w = v; w < 6; ++w)
-                        acc[PA_CB + q++] += d2F * g[v] * g[w];
-                acc[PA_LAP] += d2F * (hx * hx + hy * hy + hz * hz);
+                double2 *gp = a.geam + 4 * (size_t)i;
+                gp[0] = make_double2(g0, g1);
+                gp[1] = make_double2(g2, g3);
+                gp[2] = make_double2(g4, g5);
+                gp[3] = make_double2(hx * hx + hy * hy + hz * hz, 0.);
             }
         }
     }
@@ -606,9 +773,12 @@ __global__ __launch_bounds__(BLOCK) void k_density(AtomArrays a, ListArrays l, R
 
 // ---------------------------------------------------------------------------
 // Pass B: forces, energies, virial, Born stress; when sampled the Born
-// elasticity (upper 6x6) and the Laplacian pair terms (md_driver.cpp:300-430).
-// Each pair is visited from both atoms (full list): forces and energies are
-// gathered without atomics; the pair sums carry a factor 2, removed in k_post.
+// elasticity and the Laplacian pair terms (md_driver.cpp:300-430), over the
+// compacted in-cut-off list. Each pair is visited from both atoms (full list):
+// forces and energies are gathered without atomics; the pair sums carry a
+// factor 2, removed in k_post. The pair Born term X d_a d_b d_c d_d is fully
+// symmetric in its four indices, so its 21 Voigt entries are 15 distinct
+// monomials of d, accumulated here and expanded in k_post.
 // ---------------------------------------------------------------------------
 template <int KIND, int LPA, bool SAMPLE, bool EAM>
 __global__ __launch_bounds__(BLOCK) void k_force(AtomArrays a, ListArrays l, RTables rt, int N,
@@ -626,6 +796,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(AtomArrays a, ListArrays l, RTa
 #pragma unroll
     for (int k = 0; k < PB_N; ++k)
         acc[k] = 0.;
+    int nrmin = 0;
     XcdRange const R((N + APB - 1) / APB);
     for (int grp = R.first; grp < R.end; grp += R.step) {
         int const i = grp * APB + slot;
@@ -635,22 +806,21 @@ __global__ __launch_bounds__(BLOCK) void k_force(AtomArrays a, ListArrays l, RTa
             double4 const ci = a.pos[i];
             int const ti = (int)ci.w;
             double2 const Fi = EAM ? a.fprime[i] : make_double2(0., 0.);
-            int const cnt = l.nnbr[i];
-            const int *nb = l.nbr + (size_t)i * l.stride;
+            int const cnt = EAM ? l.ccnt[i] : l.nnbr[i];
+            const int *nb = (EAM ? l.cnbr : l.nbr) + (size_t)i * l.stride;
             for (int k = lane; k < cnt; k += LPA) {
                 int const j = nb[k];
                 double4 const cj = a.pos[j];
                 double dx, dy, dz;
                 min_image(ci, cj, L, dx, dy, dz);
                 double const r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 > rc2)
+                if (!EAM && r2 > rc2)   // the compacted list is inside r_c already
                     continue;
                 int const tj = (int)cj.w;
                 int const tmax = max(ti, tj), tmin = min(ti, tj);
                 int const pslot = tmax * (tmax + 1) / 2 + tmin;
                 double const r = sqrt(r2);
-                if (r < rmin)
-                    acc[PB_WRMIN] += 1.;
+                nrmin += r < rmin;
                 double t;
                 int const it = tab_locate(r, rt.inv_h, rt.n, t);
                 const double *base = rt.rec + (size_t)it * rt.nf * P::S;
@@ -663,10 +833,11 @@ __global__ __launch_bounds__(BLOCK) void k_force(AtomArrays a, ListArrays l, RTa
                     if (SAMPLE)
                         d2V = pp.d2(t, rt.inv_h2);
                 }
+                double const ir = 1. / r;
                 double dedr = dV, X = 0.;
                 if (SAMPLE) {
-                    X = d2V - dV / r;
-                    acc[PB_LAP] += 2. * (d2V + 2. * dV / r);
+                    X = d2V - dV * ir;
+                    acc[PB_LAP] += 2. * (d2V + 2. * dV * ir);
                 }
                 if (EAM && !(r > rc_rho)) {
                     P pi, pj;
@@ -677,34 +848,46 @@ __global__ __launch_bounds__(BLOCK) void k_force(AtomArrays a, ListArrays l, RTa
                     dedr += Fi.x * drj + Fj.x * dri;
                     if (SAMPLE) {
                         double const d2ri = pi.d2(t, rt.inv_h2), d2rj = pj.d2(t, rt.inv_h2);
-                        acc[PB_LAP] += 2. * (Fj.x * (d2ri + 2. * dri / r) + Fi.x * (d2rj + 2. * drj / r));
+                        acc[PB_LAP] += 2. * (Fj.x * (d2ri + 2. * dri * ir) +
+                                             Fi.x * (d2rj + 2. * drj * ir));
                         acc[PB_LAP] += Fi.y * (drj * drj) + Fj.y * (dri * dri);
-                        X += Fi.x * (d2rj - drj / r) + Fj.x * (d2ri - dri / r);
+                        X += Fi.x * (d2rj - drj * ir) + Fj.x * (d2ri - dri * ir);
                     }
                 }
-                X /= r * r;
-                double const s = -dedr / r;
+                double const s = -dedr * ir;
                 fx += s * dx;
                 fy += s * dy;
                 fz += s * dz;
                 e += 0.5 * V;
                 acc[PB_VIR] += r * dedr;
-                double const d[3] = {dx, dy, dz};
-                double const dd[6] = {dx * dx, dy * dy, dz * dz, dy * dz, dx * dz, dx * dy};
-#pragma unroll
-                for (int v = 0; v < 6; ++v)
-                    acc[PB_SB + v] += dedr * dd[v] / r;
+                double const xx = dx * dx, yy = dy * dy, zz = dz * dz;
+                double const yz = dy * dz, xz = dx * dz, xy = dx * dy;
+                double const w = dedr * ir;
+                acc[PB_SB + 0] += w * xx;
+                acc[PB_SB + 1] += w * yy;
+                acc[PB_SB + 2] += w * zz;
+                acc[PB_SB + 3] += w * yz;
+                acc[PB_SB + 4] += w * xz;
+                acc[PB_SB + 5] += w * xy;
                 if (SAMPLE) {
-                    int q = 0;
-#pragma unroll
-                    for (int v = 0; v < 6; ++v) {
-                        double const xv = dd[v] * X;
-#pragma unroll
-                        for (int w = v; w < 6; ++w)
-                            acc[PB_CB + q++] += xv * dd[w];
-                    }
+                    X *= ir * ir;
+                    double const Xxx = X * xx, Xyy = X * yy, Xzz = X * zz;
+                    acc[PB_CB + M_X4] += Xxx * xx;
+                    acc[PB_CB + M_Y4] += Xyy * yy;
+                    acc[PB_CB + M_Z4] += Xzz * zz;
+                    acc[PB_CB + M_X3Y] += Xxx * xy;
+                    acc[PB_CB + M_X3Z] += Xxx * xz;
+                    acc[PB_CB + M_Y3X] += Xyy * xy;
+                    acc[PB_CB + M_Y3Z] += Xyy * yz;
+                    acc[PB_CB + M_Z3X] += Xzz * xz;
+                    acc[PB_CB + M_Z3Y] += Xzz * yz;
+                    acc[PB_CB + M_X2Y2] += Xxx * yy;
+                    acc[PB_CB + M_X2Z2] += Xxx * zz;
+                    acc[PB_CB + M_Y2Z2] += Xyy * zz;
+                    acc[PB_CB + M_X2YZ] += Xxx * yz;
+                    acc[PB_CB + M_Y2XZ] += Xyy * xz;
+                    acc[PB_CB + M_Z2XY] += Xzz * xy;
                 }
-                (void)d;
             }
         }
 #pragma unroll
@@ -723,14 +906,18 @@ __global__ __launch_bounds__(BLOCK) void k_force(AtomArrays a, ListArrays l, RTa
             acc[PB_EPOT] += ei;
         }
     }
+    acc[PB_WRMIN] = (double)nrmin;
     block_sum_store<PB_N>(acc, part);
 }
 
-// Second U2 kick (md_driver.cpp:433-452) and the velocity sums the
-// thermodynamics need: K, sum m v_a v_b, momentum, sum |f|^2.
+// Second U2 kick (md_driver.cpp:433-452) and the per-atom sums the
+// thermodynamics need: K, sum m v_a v_b, momentum, sum |f|^2; when sampled
+// also the embedding part of the Born tensor and Laplacian, F''_i g_i g_i
+// and F''_i |h_i|^2 (md_driver.cpp:410-419), from pass A's per-atom g, |h|^2.
+template <bool SAMPLE>
 __global__ __launch_bounds__(BLOCK) void k_kick2(AtomArrays a, int N,
                                                 const DevState *__restrict__ gs, int init,
-                                                double *__restrict__ part) {
+                                                int has_eam, double *__restrict__ part) {
     if (gs->err)
         return;
     double const e = init ? 1. : gs->kick_e, ch = init ? 0. : gs->kick_c;
@@ -761,29 +948,65 @@ __global__ __launch_bounds__(BLOCK) void k_kick2(AtomArrays a, int N,
         acc[PK_MOM + 1] += m * vy;
         acc[PK_MOM + 2] += m * vz;
         acc[PK_F2] += fx * fx + fy * fy + fz * fz;
+        if (SAMPLE && has_eam) {
+            double const d2F = a.fprime[i].y;
+            const double2 *gp = a.geam + 4 * (size_t)i;
+            double2 const ga = gp[0], gb = gp[1], gc = gp[2], gd = gp[3];
+            double const g[6] = {ga.x, ga.y, gb.x, gb.y, gc.x, gc.y};
+            int q = 0;
+#pragma unroll
+            for (int v = 0; v < 6; ++v) {
+                double const fg = d2F * g[v];
+#pragma unroll
+                for (int w = v; w < 6; ++w)
+                    acc[PK_CBF + q++] += fg * g[w];
+            }
+            acc[PK_LAPF] += d2F * gd.x;
+        }
     }
     block_sum_store<PK_N>(acc, part);
 }
 
-// Deterministic sum of nblocks partial records: thread t adds blocks t,
-// t+BLOCK, ... in order, then a fixed tree.
-template <int NV>
-__device__ void reduce_partials(const double *__restrict__ part, int nblocks, double *out) {
-    __shared__ double sm[BLOCK];
-    for (int k = 0; k < NV; ++k) {
-        double s = 0.;
-        for (int b = threadIdx.x; b < nblocks; b += BLOCK)
-            s += part[(size_t)k * PART_STRIDE + b];
-        sm[threadIdx.x] = s;
-        __syncthreads();
-        for (int w = BLOCK / 2; w > 0; w >>= 1) {
-            if (threadIdx.x < w)
-                sm[threadIdx.x] += sm[threadIdx.x + w];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0)
-            out[k] = sm[0];
-        __syncthreads();
+// Deterministic sums of the per-block partial records: one block per
+// component; thread t adds blocks t, t+BLOCK, ... in order, then a fixed tree.
+// Components are numbered A (PA_N), then B (PB_N), then K (PK_N).
+constexpr int NTOTAL = PA_N + PB_N + PK_N;
+__global__ __launch_bounds__(BLOCK) void k_reduce(const DevState *__restrict__ gs,
+                                                 const double *__restrict__ pA, int nA,
+                                                 const double *__restrict__ pB, int nB,
+                                                 const double *__restrict__ pK, int nK,
+                                                 double *__restrict__ totals) {
+    if (gs->err)
+        return;
+    int k = blockIdx.x;
+    const double *p;
+    int nb;
+    if (k < PA_N) {
+        p = pA;
+        nb = gs->has_eam ? nA : 0;
+    } else if (k < PA_N + PB_N) {
+        p = pB;
+        k -= PA_N;
+        nb = nB;
+    } else {
+        p = pK;
+        k -= PA_N + PB_N;
+        nb = nK;
+    }
+    double s = 0.;
+    for (int b = threadIdx.x; b < nb; b += BLOCK)
+        s += p[(size_t)k * PART_STRIDE + b];
+    for (int off = 32; off > 0; off >>= 1)
+        s += __shfl_xor(s, off, 64);
+    __shared__ double sm[BLOCK / 64];
+    if ((threadIdx.x & 63) == 0)
+        sm[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.;
+        for (int w = 0; w < BLOCK / 64; ++w)
+            t += sm[w];
+        totals[blockIdx.x] = t;
     }
 }
 
@@ -794,24 +1017,18 @@ __device__ void post_step(DevState &s, const double *tA, const double *tB, const
 // k_post: after the forces. mode 0 = constructor (md_driver.cpp:114-118),
 // mode 1 = end of integrate() (md_driver.cpp:583-600).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void k_post(DevState *__restrict__ gs,
-                                               const double *__restrict__ pA, int nA,
-                                               const double *__restrict__ pB, int nB,
-                                               const double *__restrict__ pK, int nK, int mode,
-                                               double *__restrict__ rec_out) {
+__global__ __launch_bounds__(64) void k_post(DevState *__restrict__ gs,
+                                            const double *__restrict__ totals, int mode,
+                                            double *__restrict__ rec_out) {
     if (gs->err)
         return;
-    __shared__ double tA[PA_N], tB[PB_N], tK[PK_N];
-    if (gs->has_eam)
-        reduce_partials<PA_N>(pA, nA, tA);
-    else if (threadIdx.x < PA_N)
-        tA[threadIdx.x] = 0.;
-    reduce_partials<PB_N>(pB, nB, tB);
-    reduce_partials<PK_N>(pK, nK, tK);
+    __shared__ double tot[NTOTAL];
+    for (int k = threadIdx.x; k < NTOTAL; k += 64)
+        tot[k] = totals[k];
     __shared__ DevState s;
     state_load(s, gs);
     if (threadIdx.x == 0)
-        post_step(s, tA, tB, tK, mode, rec_out);
+        post_step(s, tot, tot + PA_N, tot + PA_N + PB_N, mode, rec_out);
     state_store(gs, s);
 }
 
@@ -829,9 +1046,16 @@ __device__ void post_step(DevState &s, const double *tA, const double *tB, const
     for (int v = 0; v < 6; ++v)
         sB[v] = 0.5 * tB[PB_SB + v];
     if (sample) {
+        // Voigt (v <= w) -> monomial of the pair term, then the embedding term
+        const int mono[21] = {M_X4, M_X2Y2, M_X2Z2, M_X2YZ, M_X3Z, M_X3Y,
+                              M_Y4, M_Y2Z2, M_Y3Z, M_Y2XZ, M_Y3X,
+                              M_Z4, M_Z3Y, M_Z3X, M_Z2XY,
+                              M_Y2Z2, M_Z2XY, M_Y2XZ,
+                              M_X2Z2, M_X2YZ,
+                              M_X2Y2};
         for (int q = 0; q < 21; ++q)
-            CB[q] = 0.5 * tB[PB_CB + q] + tA[PA_CB + q];
-        s.lap = 0.5 * tB[PB_LAP] + tA[PA_LAP];
+            CB[q] = 0.5 * tB[PB_CB + mono[q]] + tK[PK_CBF + q];
+        s.lap = 0.5 * tB[PB_LAP] + tK[PK_LAPF];
     }
     s.warn_rmin += (long long)(0.5 * tB[PB_WRMIN] + 0.25);
     {   // virial self-check (md_driver.cpp:422-429), a warning only

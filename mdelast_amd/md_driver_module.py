@@ -468,7 +468,10 @@ class MDDriver:
     def integrate(self):
         """One MD step (md_driver.cpp:563-601)."""
         self._cache = None
-        self._call(nat.lib().mdl_driver_integrate(self.handle, 1))
+        try:
+            self._call(nat.lib().mdl_driver_integrate(self.handle, 1))
+        finally:
+            self._cache = None
         if VERBOSE and (self.cycle - 1) % 100 == 0:
             self._print_line(self.cycle - 1)
 
@@ -480,14 +483,19 @@ class MDDriver:
             records: bool = True) -> np.ndarray:
         """nsteps steps in one call (the loop of MolecularDynamics.run without Python
         per step); returns one MDL_S_* record per sampled step (extension)."""
-        self._cache = None
+        cycle0 = self.cycle
         nsamp = sum(1 for k in range(1, nsteps + 1)
-                    if (self.cycle + k) % sample_stride == 0) if records else 0
+                    if (cycle0 + k) % sample_stride == 0) if records else 0
         buf = np.zeros((max(nsamp, 1), nat.NRECORD))
         n = C.c_int64(0)
-        self._call(nat.lib().mdl_driver_run(self.handle, int(nsteps), int(sample_stride),
-                                            int(rebuild_every),
-                                            nat.dptr(buf) if records else None, C.byref(n)))
+        self._cache = None
+        try:
+            self._call(nat.lib().mdl_driver_run(self.handle, int(nsteps), int(sample_stride),
+                                                int(rebuild_every),
+                                                nat.dptr(buf) if records else None,
+                                                C.byref(n)))
+        finally:
+            self._cache = None
         return buf[:n.value]
 
     def kernel_times(self):
