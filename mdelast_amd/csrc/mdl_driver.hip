@@ -3,6 +3,7 @@
 // the attribute getters of md_driver_module.cpp:103-140, as the mdl_driver_*
 // C-ABI of include/mdelast.h.
 #include "mdl_kernels.hpp"
+#include "mdl_mixed.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -41,6 +42,37 @@ template <int KIND, bool S, bool E> static ForceFn force_fn(int lpa) {
     default: return k_force<KIND, 64, S, E>;
     }
 }
+using DensMxFn = void (*)(AtomArrays, ListArrays, RTablesF, FTablesF, int, int, int, double,
+                          DevState *, double *);
+using ForceMxFn = void (*)(AtomArrays, ListArrays, RTablesF, int, int, float, DevState *, double *);
+template <int KIND, bool S> static DensMxFn dens_mx_fn(int lpa) {
+    switch (lpa) {
+    case 4: return k_density_mx<KIND, 4, S>;
+    case 8: return k_density_mx<KIND, 8, S>;
+    case 16: return k_density_mx<KIND, 16, S>;
+    case 32: return k_density_mx<KIND, 32, S>;
+    default: return k_density_mx<KIND, 64, S>;
+    }
+}
+template <int KIND, bool S> static ForceMxFn force_mx_fn(int lpa) {
+    switch (lpa) {
+    case 4: return k_force_mx<KIND, 4, S>;
+    case 8: return k_force_mx<KIND, 8, S>;
+    case 16: return k_force_mx<KIND, 16, S>;
+    case 32: return k_force_mx<KIND, 32, S>;
+    default: return k_force_mx<KIND, 64, S>;
+    }
+}
+static DensMxFn pick_dens_mx(int kind, int lpa, bool s) {
+    if (kind == MDL_QUINTIC)
+        return s ? dens_mx_fn<MDL_QUINTIC, true>(lpa) : dens_mx_fn<MDL_QUINTIC, false>(lpa);
+    return s ? dens_mx_fn<MDL_BSPLINE, true>(lpa) : dens_mx_fn<MDL_BSPLINE, false>(lpa);
+}
+static ForceMxFn pick_force_mx(int kind, int lpa, bool s) {
+    if (kind == MDL_QUINTIC)
+        return s ? force_mx_fn<MDL_QUINTIC, true>(lpa) : force_mx_fn<MDL_QUINTIC, false>(lpa);
+    return s ? force_mx_fn<MDL_BSPLINE, true>(lpa) : force_mx_fn<MDL_BSPLINE, false>(lpa);
+}
 static DensFn pick_dens(int kind, int lpa, bool s) {
     if (kind == MDL_QUINTIC)
         return s ? dens_fn<MDL_QUINTIC, true>(lpa) : dens_fn<MDL_QUINTIC, false>(lpa);
@@ -78,6 +110,9 @@ struct Driver {
     ListArrays l{};
     RTables rt{};
     FTables ft{};
+    RTablesF rtf{};
+    FTablesF ftf{};
+    float *d_rrec_f = nullptr, *d_frec_f = nullptr;
     double rc2 = 0, rc_pair = 0, rc_rho = 0, rmin = 0;
     double *d_rrec = nullptr, *d_frec = nullptr;
     double *pA = nullptr, *pB = nullptr, *pK = nullptr;
@@ -115,26 +150,39 @@ struct Driver {
         return (g + 7) / 8 * 8;
     }
 
-    void launch_build() {
-        int const gN = (N + BLOCK - 1) / BLOCK;
-        k_decide<<<1, 64, 0, stream>>>(gs, use_cells ? 1 : 0, 1.0, d_dmax, gridK);
-        if (use_cells) {
-            constexpr int BL = 8;   // lanes per atom of the list build
-            k_bin<<<gN, BLOCK, 0, stream>>>(a, l, N, gs);
-            k_scan<<<1, 1024, 0, stream>>>(l, gs);
-            k_scatter<<<gN, BLOCK, 0, stream>>>(l, N, gs);
-            k_sort_cells<<<l.ncells, 64, 0, stream>>>(l, gs);
-            k_build_cells<BL><<<(int)(((int64_t)N * BL + BLOCK - 1) / BLOCK), BLOCK, 0, stream>>>(
-                a, l, N, gs);
-        } else {
-            k_build_brute<<<gN, BLOCK, 0, stream>>>(a, l, N, gs);
-        }
+    int gridL = 1;                  // resident blocks of the cooperative list kernel
+    unsigned *d_bar = nullptr, *d_ticket_kd = nullptr, *d_ticket_fin = nullptr;
+
+    hipError_t launch_list() {
+        int cells = use_cells ? 1 : 0;
+        int n = N;
+        void *args[] = {&a, &l, &n, &gs, &d_bar, &cells};
+        return hipLaunchCooperativeKernel((const void *)k_list, dim3(gridL), dim3(BLOCK), args, 0,
+                                          stream);
+    }
+
+    void launch_finish(int mode, double *rec_out, int next_sample, int next_force) {
+        k_finish<<<NTOTAL, BLOCK, 0, stream>>>(gs, pA, gridA, pB, gridB, pK, gridK, d_totals,
+                                               d_ticket_fin, mode, rec_out, next_sample,
+                                               next_force);
     }
 
     void launch_forces(bool s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
                        hipEvent_t e2 = nullptr) {
         if (e0)
             (void)hipEventRecord(e0, stream);
+        if (precision == MDL_MIXED) {
+            double const rc2A = rc2 * 1e16;
+            DensMxFn fa = pick_dens_mx(kind, lpa, s);
+            fa<<<gridA, BLOCK, 0, stream>>>(a, l, rtf, ftf, N, npair, ntypes, rc2A, gs, pA);
+            if (e1)
+                (void)hipEventRecord(e1, stream);
+            ForceMxFn fb = pick_force_mx(kind, lpa, s);
+            fb<<<gridB, BLOCK, 0, stream>>>(a, l, rtf, N, npair, (float)(rmin * 1e8), gs, pB);
+            if (e2)
+                (void)hipEventRecord(e2, stream);
+            return;
+        }
         if (has_eam) {
             DensFn fa = pick_dens(kind, lpa, s);
             fa<<<gridA, BLOCK, 0, stream>>>(a, l, rt, ft, N, npair, ntypes, rc2, rc_rho, gs, pA);
@@ -147,12 +195,15 @@ struct Driver {
             (void)hipEventRecord(e2, stream);
     }
 
-    void launch_step(bool s, bool force_rebuild, double *rec_out, hipEvent_t *tev) {
-        k_pre<<<1, 64, 0, stream>>>(gs, s ? 1 : 0);
-        if (force_rebuild)
-            k_set_rebuild<<<1, 1, 0, stream>>>(gs);
-        k_kick_drift<<<gridK, BLOCK, 0, stream>>>(a, N, gs, d_dmax);
-        launch_build();
+    // One step after its first half (k_pre or the previous k_finish) has run.
+    // next_sample < 0: end with the post step only; otherwise also run the next
+    // step's first half with that sampling flag and forced-rebuild flag.
+    hipError_t launch_step(bool s, double *rec_out, hipEvent_t *tev, int next_sample,
+                           int next_force) {
+        k_kick_drift<<<gridK, BLOCK, 0, stream>>>(a, N, gs, d_dmax, d_ticket_kd);
+        hipError_t e = launch_list();
+        if (e != hipSuccess)
+            return e;
         if (tev)
             launch_forces(s, tev[0], tev[1], tev[2]);
         else
@@ -161,9 +212,9 @@ struct Driver {
             k_kick2<true><<<gridK, BLOCK, 0, stream>>>(a, N, gs, 0, has_eam, pK);
         else
             k_kick2<false><<<gridK, BLOCK, 0, stream>>>(a, N, gs, 0, has_eam, pK);
-        k_reduce<<<NTOTAL, BLOCK, 0, stream>>>(gs, pA, gridA, pB, gridB, pK, gridK, d_totals);
-        k_post<<<1, 64, 0, stream>>>(gs, d_totals, 1, rec_out);
+        launch_finish(1, rec_out, next_sample, next_force);
         ++cycle_host;
+        return hipGetLastError();
     }
 
     mdl_status sync_state() {
@@ -288,8 +339,10 @@ static mdl_status create(const mdl_config *c, Driver &d) {
     d.npair = c->ntypes * (c->ntypes + 1) / 2;
     d.has_eam = c->has_eam ? 1 : 0;
     d.precision = c->precision;
-    if (d.precision != MDL_FP64) {
-        set_error("mdl_driver_create: only MDL_FP64 is implemented");
+    if (d.precision != MDL_FP64 && d.precision != MDL_MIXED)
+        return bad("mdl_driver_create: precision must be MDL_FP64 or MDL_MIXED");
+    if (d.precision == MDL_MIXED && !d.has_eam) {
+        set_error("mdl_driver_create: the mixed-precision path needs EAM tables");
         return MDL_ERR_UNSUPPORTED;
     }
     if (d.has_eam && (!c->rho_tab || !c->F_tab || !c->eam_rcut || !c->eam_rhomax))
@@ -391,6 +444,59 @@ static mdl_status create(const mdl_config *c, Driver &d) {
             d.ft.rhomax[t] = rhomax[t];
     }
 
+    if (d.precision == MDL_MIXED) {
+        // FP32 records in eV / Angstrom units: value scale per function, and for
+        // B-spline records the independent f' and f'' cubics also carry the
+        // argument's unit change (cm -> A for r, cm^-3 -> A^-3 for rho)
+        double const EV = 1.6021773e-12;
+        int const SF = d.kind == MDL_QUINTIC ? 8 : 12;
+        std::vector<float> rf((size_t)nr * nf * SF, 0.f);
+        for (int64_t i = 0; i < nr; ++i)
+            for (int f = 0; f < nf; ++f) {
+                double const sv = f < d.npair ? 1. / EV : 1e-24;
+                const double *src = &rtabs[f]->rec[(size_t)i * S];
+                float *dst = &rf[((size_t)i * nf + f) * SF];
+                if (d.kind == MDL_QUINTIC)
+                    for (int c = 0; c < 6; ++c)
+                        dst[c] = (float)(src[c] * sv);
+                else
+                    for (int c = 0; c < 12; ++c)
+                        dst[c] = (float)(src[c] * sv * (c < 4 ? 1. : (c < 8 ? 1e-8 : 1e-16)));
+            }
+        if (mdl_status st = d.alloc(&d.d_rrec_f, rf.size()))
+            return st;
+        MDL_HIP(hipMemcpy(d.d_rrec_f, rf.data(), sizeof(float) * rf.size(), hipMemcpyHostToDevice));
+        d.rtf.rec = d.d_rrec_f;
+        d.rtf.nf = nf;
+        d.rtf.n = (int)nr;
+        d.rtf.inv_h = (float)(rtabs[0]->inv_h * 1e-8);
+        d.rtf.inv_h2 = (float)(rtabs[0]->inv_h2 * 1e-16);
+        const HostTable *f0 = reinterpret_cast<const HostTable *>(c->F_tab[0]);
+        std::vector<float> ff((size_t)f0->n * d.ntypes * SF, 0.f);
+        for (int64_t i = 0; i < f0->n; ++i)
+            for (int t = 0; t < d.ntypes; ++t) {
+                const double *src =
+                    &reinterpret_cast<const HostTable *>(c->F_tab[t])->rec[(size_t)i * S];
+                float *dst = &ff[((size_t)i * d.ntypes + t) * SF];
+                if (d.kind == MDL_QUINTIC)
+                    for (int k = 0; k < 6; ++k)
+                        dst[k] = (float)(src[k] / EV);
+                else
+                    for (int k = 0; k < 12; ++k)
+                        dst[k] = (float)(src[k] / EV * (k < 4 ? 1. : (k < 8 ? 1e24 : 1e48)));
+            }
+        if (mdl_status st = d.alloc(&d.d_frec_f, ff.size()))
+            return st;
+        MDL_HIP(hipMemcpy(d.d_frec_f, ff.data(), sizeof(float) * ff.size(), hipMemcpyHostToDevice));
+        d.ftf.rec = d.d_frec_f;
+        d.ftf.n = (int)f0->n;
+        d.ftf.inv_h = f0->inv_h * 1e24;
+        d.ftf.inv_hf = (float)(f0->inv_h * 1e24);
+        d.ftf.inv_h2f = (float)(f0->inv_h2 * 1e48);
+        for (int t = 0; t < d.ntypes; ++t)
+            d.ftf.rhomax[t] = rhomax[t] * 1e-24;
+    }
+
     // neighbour list geometry: the list is exact at every step (rebuilt on
     // request and whenever a pair outside it could have entered r_c)
     double const rc = c->pair_rcut[0];
@@ -433,6 +539,7 @@ static mdl_status create(const mdl_config *c, Driver &d) {
     ALLOC(a.cref, N);
     ALLOC(a.fprime, N);
     ALLOC(a.geam, (size_t)4 * N);
+    ALLOC(a.fprime_f, N);
     ALLOC(a.Femb, N);
     ALLOC(a.rho, N);
     ALLOC(a.epot, N);
@@ -454,6 +561,12 @@ static mdl_status create(const mdl_config *c, Driver &d) {
     ALLOC(d.pK, (size_t)PART_STRIDE * PK_N);
     ALLOC(d.d_stage, (size_t)3 * N);
     ALLOC(d.d_totals, NTOTAL);
+    ALLOC(d.d_bar, 4);
+    ALLOC(d.d_ticket_kd, 4);
+    ALLOC(d.d_ticket_fin, 4);
+    MDL_HIP(hipMemset(d.d_bar, 0, 4 * sizeof(unsigned)));
+    MDL_HIP(hipMemset(d.d_ticket_kd, 0, 4 * sizeof(unsigned)));
+    MDL_HIP(hipMemset(d.d_ticket_fin, 0, 4 * sizeof(unsigned)));
     ALLOC(d.d_dmax, GRID_ATOM_MAX);
     MDL_HIP(hipMemset(d.d_dmax, 0, sizeof(double) * GRID_ATOM_MAX));
     ALLOC(d.gs, 1);
@@ -526,6 +639,9 @@ static mdl_status create(const mdl_config *c, Driver &d) {
     s.peps = c->bar_p_eps;
     s.vscale = 1.;
     s.kick_s = 1.;
+    s.mixed = d.precision == MDL_MIXED ? 1 : 0;
+    s.unit_E = s.mixed ? 1.6021773e-12 : 1.;
+    s.unit_L = s.mixed ? 1.6021773e-12 * 1e16 : 1.;
     s.r_list = r_list;
     s.rc = rc;
     s.skin = r_list - rc;
@@ -545,10 +661,22 @@ static mdl_status create(const mdl_config *c, Driver &d) {
         d.lpa = 16;
     d.gridA = d.gridB = d.grid_pairs(d.lpa);
     d.gridK = std::min((N + BLOCK - 1) / BLOCK, GRID_ATOM_MAX);
+    {   // the cooperative list kernel: as many blocks as can be resident at once
+        int per_cu = 0, cus = 0;
+        MDL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_list, BLOCK, 0));
+        MDL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d.device));
+        int coop = 0;
+        MDL_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, d.device));
+        if (!coop || per_cu < 1) {
+            set_error("the device does not support cooperative launches");
+            return MDL_ERR_DEVICE;
+        }
+        int64_t const want = ((int64_t)N * LIST_LANES + BLOCK - 1) / BLOCK;
+        d.gridL = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)per_cu * cus));
+    }
 
     // constructor: list, forces, thermodynamics (md_driver.cpp:112-118)
-    d.launch_build();
-    MDL_HIP(hipGetLastError());
+    MDL_HIP(d.launch_list());
     if (mdl_status st = d.sync_state())
         return st;
     if (mdl_status st = d.check_errors())
@@ -558,9 +686,7 @@ static mdl_status create(const mdl_config *c, Driver &d) {
             return st;
     d.launch_forces(true);
     k_kick2<true><<<d.gridK, BLOCK, 0, d.stream>>>(d.a, N, d.gs, 1, d.has_eam, d.pK);
-    k_reduce<<<NTOTAL, BLOCK, 0, d.stream>>>(d.gs, d.pA, d.gridA, d.pB, d.gridB, d.pK, d.gridK,
-                                              d.d_totals);
-    k_post<<<1, 64, 0, d.stream>>>(d.gs, d.d_totals, 0, nullptr);
+    d.launch_finish(0, nullptr, -1, 0);
     MDL_HIP(hipGetLastError());
     if (mdl_status st = d.sync_state())
         return st;
@@ -612,9 +738,11 @@ mdl_status mdl_driver_integrate(mdl_driver *p, int64_t nsteps) {
     Driver *d = D(p);
     if (mdl_status s = guard(d))
         return s;
+    int const smp = d->sample != 0 ? 1 : 0;
+    if (nsteps > 0)
+        k_pre<<<1, 64, 0, d->stream>>>(d->gs, smp, 0);
     for (int64_t k = 0; k < nsteps; ++k)
-        d->launch_step(d->sample != 0, false, nullptr, nullptr);
-    MDL_HIP(hipGetLastError());
+        MDL_HIP(d->launch_step(smp != 0, nullptr, nullptr, k + 1 < nsteps ? smp : -1, 0));
     if (mdl_status s = d->sync_state())
         return s;
     return d->check_errors();
@@ -643,10 +771,14 @@ mdl_status mdl_driver_run(mdl_driver *p, int64_t nsteps, int64_t sample_stride,
     }
     int64_t irec = 0;
     int64_t const t0 = nsteps - ntime;   // time the last steps (after any warm-up)
+    auto sampled = [&](int64_t cyc) { return (cyc + 1) % sample_stride == 0; };
+    auto forced = [&](int64_t cyc) { return rebuild_every > 0 && cyc % rebuild_every == 0; };
+    if (nsteps > 0)
+        k_pre<<<1, 64, 0, d->stream>>>(d->gs, sampled(d->cycle_host) ? 1 : 0,
+                                        forced(d->cycle_host) ? 1 : 0);
     for (int64_t k = 0; k < nsteps; ++k) {
         int64_t const cyc = d->cycle_host;
-        bool const s = (cyc + 1) % sample_stride == 0;
-        bool const rb = rebuild_every > 0 && cyc % rebuild_every == 0;
+        bool const s = sampled(cyc);
         double *ro = (drec && s) ? drec + MDL_NRECORD * irec : nullptr;
         if (s)
             ++irec;
@@ -655,9 +787,10 @@ mdl_status mdl_driver_run(mdl_driver *p, int64_t nsteps, int64_t sample_stride,
             tev = &d->ev[4 * (k - t0)];
             (void)hipEventRecord(tev[3], d->stream);
         }
-        d->launch_step(s, rb, ro, tev);
+        int const ns = k + 1 < nsteps ? (sampled(cyc + 1) ? 1 : 0) : -1;
+        int const nf = k + 1 < nsteps && forced(cyc + 1) ? 1 : 0;
+        MDL_HIP(d->launch_step(s, ro, tev, ns, nf));
     }
-    MDL_HIP(hipGetLastError());
     if (drec) {
         hipError_t e = hipMemcpyAsync(records, drec, sizeof(double) * MDL_NRECORD * irec,
                                       hipMemcpyDeviceToHost, d->stream);

@@ -47,6 +47,7 @@ enum DevErr : int {
     DE_CONSERVATION = 1 << 4,
     DE_NBR_OVERFLOW = 1 << 5,
     DE_CELL_GRID = 1 << 6,
+    DE_BARRIER = 1 << 7,
 };
 
 // ----------------------------------------------------------------------------
@@ -89,13 +90,15 @@ struct DevState {
     double kick_s, kick_e, kick_c, drift_cf, drift_L[3];
     // thermodynamics (the MDL_S_* record)
     double rec[MDL_NSCALARS];
+    double unit_E, unit_L;   // mixed mode: eV -> erg and eV/A^2 -> erg/cm^2 (1 in FP64)
+    int mixed, pad_mixed;
     double ekin_tot, virial, lap, f2, Tci;
     double etot_cons_initial, etot_cons, etot_cons_err;
     double time;
     unsigned long long cycle;
     // neighbour-list control
     double r_list, skin, rc;
-    unsigned long long maxdisp2_bits;  // max squared displacement since the last build
+    double umax2;            // max squared displacement since the last build (k_kick_drift)
     int force_rebuild, do_rebuild;
     long long nbuilds, nbr_needed, nlist_entries;
     int ncell[3];

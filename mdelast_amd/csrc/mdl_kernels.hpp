@@ -21,6 +21,7 @@ struct AtomArrays {
     double4 *cref;     // coordinates at the last list build
     double2 *fprime;   // F'(rho_i), F''(rho_i)
     double2 *geam;     // sampled steps: g_i (6, Voigt) and |h_i|^2, 4 double2 per atom
+    float2 *fprime_f;  // mixed mode: F', F'' in eV A^3, eV A^6 for the pair gathers
     double *Femb;      // F(rho_i)
     double *rho;       // rho_i
     double *epot;      // per-atom potential energy
@@ -318,19 +319,21 @@ __device__ __forceinline__ void state_store(DevState *gs, const DevState &sh) {
 // ---------------------------------------------------------------------------
 __device__ void pre_step(DevState &s, int sample);
 
-__global__ __launch_bounds__(64) void k_pre(DevState *__restrict__ gs, int sample) {
+__global__ __launch_bounds__(64) void k_pre(DevState *__restrict__ gs, int sample, int force) {
     if (gs->err)
         return;
     __shared__ DevState s;
     state_load(s, gs);
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         pre_step(s, sample);
+        if (force)
+            s.force_rebuild = 1;
+    }
     state_store(gs, s);
 }
 
 __device__ void pre_step(DevState &s, int sample) {
     s.sample = sample;
-    s.maxdisp2_bits = 0ull;
     bool const npt = s.ensemble == MDL_NPT;
     bool const thermo = s.ensemble == MDL_NVT || npt;
     double const h = 0.5 * s.dt;
@@ -376,8 +379,9 @@ __device__ void pre_step(DevState &s, int sample) {
 
 // First U2 (md_driver.cpp:433-452) and U1 drift (md_driver.cpp:462-499).
 __global__ __launch_bounds__(BLOCK) void k_kick_drift(AtomArrays a, int N,
-                                                     const DevState *__restrict__ gs,
-                                                     double *__restrict__ dmax_part) {
+                                                     DevState *__restrict__ gs,
+                                                     double *__restrict__ dmax_part,
+                                                     unsigned *__restrict__ ticket) {
     if (gs->err)
         return;
     double const s = gs->kick_s, e = gs->kick_e, ch = gs->kick_c, cf = gs->drift_cf;
@@ -408,151 +412,88 @@ __global__ __launch_bounds__(BLOCK) void k_kick_drift(AtomArrays a, int N,
     if ((threadIdx.x & 63) == 0)
         sm[threadIdx.x >> 6] = dmax;
     __syncthreads();
+    __shared__ int last;
     if (threadIdx.x == 0) {
         double m = sm[0];
         for (int w = 1; w < BLOCK / 64; ++w)
             m = fmax(m, sm[w]);
         dmax_part[blockIdx.x] = m;
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last)
+        return;
+    // the last block to finish reduces the per-block maxima
+    __threadfence();
+    if (threadIdx.x < 64) {
+        double m = 0.;
+        for (int b = threadIdx.x; b < (int)gridDim.x; b += 64)
+            m = fmax(m, __hip_atomic_load(&dmax_part[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        for (int off = 32; off > 0; off >>= 1)
+            m = fmax(m, __shfl_xor(m, off, 64));
+        if (threadIdx.x == 0) {
+            gs->umax2 = m;
+            *ticket = 0u;
+        }
     }
 }
 
-// Decide whether the list must be rebuilt: forced by calculate_neighbors(),
-// or any pair outside the list could now be inside r_c. For a list built with
-// radius r_list at box L_b, a pair not in it is at least lambda r_list - 2 u_max
-// apart now, lambda = min L/L_b, u_max the largest displacement since the build.
-__global__ __launch_bounds__(64) void k_decide(DevState *__restrict__ gs, int use_cells,
-                                              double cell_margin,
-                                              const double *__restrict__ dmax_part, int nparts) {
-    if (gs->err)
-        return;
-    double m = 0.;
-    for (int b = threadIdx.x; b < nparts; b += 64)
-        m = fmax(m, dmax_part[b]);
-    for (int off = 32; off > 0; off >>= 1)
-        m = fmax(m, __shfl_xor(m, off, 64));
-    if (threadIdx.x != 0)
-        return;
-    DevState &s = *gs;
-    double lam = 1e300;
-    for (int a = 0; a < 3; ++a)
-        lam = fmin(lam, s.L[a] / s.L_build[a]);
-    double const umax = sqrt(m);
-    bool const need = s.force_rebuild || !(lam * s.r_list - 2. * umax > s.rc);
-    s.do_rebuild = need ? 1 : 0;
-    if (!need)
-        return;
-    s.force_rebuild = 0;
-    s.nbuilds++;
-    s.nlist_entries = 0;
-    for (int a = 0; a < 3; ++a)
-        s.L_build[a] = s.L[a];
-    if (use_cells)
-        for (int a = 0; a < 3; ++a)
-            if (s.L[a] / s.ncell[a] < s.r_list * cell_margin) {
-                s.err |= DE_CELL_GRID;
-                s.err_val[0] = s.L[a];
-            }
-}
-
+// ---------------------------------------------------------------------------
+// Neighbour list: one cooperative kernel per step.
+//
+// Phase 0 (every block, no communication): decide whether the list must be
+// rebuilt - forced by calculate_neighbors(), or a pair outside the list could
+// now be inside r_c. For a list built with radius r_list at box L_b, a pair
+// not in it is at least lambda r_list - 2 u_max apart now, lambda = min L/L_b,
+// u_max the largest displacement since the build. Every block reads the same
+// inputs, so all take the same branch; on most steps they all return here and
+// the kernel costs one launch.
+// On a rebuild, phases separated by grid barriers: bin atoms into cells,
+// scan the counts (block 0), scatter, sort each cell's members by index (so
+// the list order, and the FP summation order of the passes, do not depend on
+// atomic arrival order), and build each atom's full list with lane groups.
+// The grid is sized so that all blocks are resident (cooperative launch); the
+// barrier spin is bounded and reports DE_BARRIER instead of waiting forever.
+// ---------------------------------------------------------------------------
 __device__ __forceinline__ int cell_coord(double c, int n) {
     double const w = c - nint_away(c);   // [-0.5, 0.5]
     int k = (int)floor((w + 0.5) * n);
     return k < 0 ? 0 : (k >= n ? n - 1 : k);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_bin(AtomArrays a, ListArrays l, int N,
-                                              const DevState *__restrict__ gs) {
-    if (gs->err || !gs->do_rebuild)
-        return;
-    int const i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= N)
-        return;
-    double4 const c = a.pos[i];
-    int const nx = gs->ncell[0], ny = gs->ncell[1], nz = gs->ncell[2];
-    int const cid = (cell_coord(c.z, nz) * ny + cell_coord(c.y, ny)) * nx + cell_coord(c.x, nx);
-    l.cell_of[i] = cid;
-    atomicAdd(&l.cell_count[cid], 1);
-}
+constexpr int CELL_MAX = 1024;          // members of one cell the sort stages in LDS
+constexpr int LIST_LANES = 8;           // lanes per atom of the list build
+constexpr long long BARRIER_SPINS = 1ll << 24;
 
-// Exclusive scan of the cell counts (one block), then zero counts and fills.
-__global__ __launch_bounds__(1024) void k_scan(ListArrays l, const DevState *__restrict__ gs) {
-    if (gs->err || !gs->do_rebuild)
-        return;
-    __shared__ int sm[1024];
-    __shared__ int carry;
-    if (threadIdx.x == 0)
-        carry = 0;
+// Sense-counting grid barrier over gridDim.x resident blocks.
+__device__ bool grid_barrier(unsigned *bar, DevState *gs) {
+    __threadfence();
     __syncthreads();
-    for (int base = 0; base < l.ncells; base += 1024) {
-        int const k = base + threadIdx.x;
-        int const v = k < l.ncells ? l.cell_count[k] : 0;
-        sm[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            int const t = threadIdx.x >= off ? sm[threadIdx.x - off] : 0;
-            __syncthreads();
-            sm[threadIdx.x] += t;
-            __syncthreads();
+    __shared__ int ok;
+    if (threadIdx.x == 0) {
+        unsigned const gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned const arrived =
+            __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 1;
+        if (arrived == gridDim.x - 1) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            long long spins = 0;
+            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > BARRIER_SPINS) {
+                    atomicOr(&gs->err, (int)DE_BARRIER);
+                    ok = 0;
+                    break;
+                }
+            }
         }
-        if (k < l.ncells) {
-            l.cell_start[k] = carry + sm[threadIdx.x] - v;
-            l.cell_count[k] = 0;
-            l.cell_fill[k] = 0;
-        }
-        __syncthreads();
-        if (threadIdx.x == 1023)
-            carry += sm[1023];
-        __syncthreads();
     }
-    if (threadIdx.x == 0)
-        l.cell_start[l.ncells] = carry;
-}
-
-__global__ __launch_bounds__(BLOCK) void k_scatter(ListArrays l, int N,
-                                                  const DevState *__restrict__ gs) {
-    if (gs->err || !gs->do_rebuild)
-        return;
-    int const i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= N)
-        return;
-    int const cid = l.cell_of[i];
-    int const slot = atomicAdd(&l.cell_fill[cid], 1);
-    l.cell_atoms[l.cell_start[cid] + slot] = i;
-}
-
-// Members of each cell in ascending index order, so the list (and the FP
-// summation order of the passes) does not depend on atomic arrival order.
-// One wave per cell: members staged in LDS, each placed at its rank.
-constexpr int CELL_MAX = 1024;
-__global__ __launch_bounds__(64) void k_sort_cells(ListArrays l, DevState *__restrict__ gs) {
-    if (gs->err || !gs->do_rebuild)
-        return;
-    int const c = blockIdx.x;
-    int const b = l.cell_start[c], n = l.cell_start[c + 1] - b;
-    if (n > CELL_MAX) {
-        if (threadIdx.x == 0) {
-            atomicOr(&gs->err, (int)DE_CELL_GRID);
-            gs->err_val[0] = (double)n;
-        }
-        return;
-    }
-    __shared__ int v[CELL_MAX];
-    for (int k = threadIdx.x; k < n; k += 64)
-        v[k] = l.cell_atoms[b + k];
     __syncthreads();
-    for (int k = threadIdx.x; k < n; k += 64) {
-        int const x = v[k];
-        int rank = 0;
-        for (int m = 0; m < n; ++m)
-            rank += v[m] < x;
-        l.cell_atoms[b + rank] = x;
-    }
-}
-
-__device__ __forceinline__ void list_push(ListArrays &l, int i, int &cnt, int j) {
-    if (cnt < l.stride)
-        l.nbr[(size_t)i * l.stride + cnt] = j;
-    ++cnt;
+    __threadfence();
+    return ok != 0;
 }
 
 __device__ __forceinline__ void list_finish(ListArrays &l, AtomArrays &a, int i, int cnt,
@@ -568,79 +509,180 @@ __device__ __forceinline__ void list_finish(ListArrays &l, AtomArrays &a, int i,
     }
 }
 
-// Full list from the 27 surrounding cells (>= 3 cells per direction). LPA
-// lanes per atom walk each cell's members together; a ballot prefix keeps the
-// entries in (cell, member) order, the order one thread would produce.
-template <int LPA>
-__global__ __launch_bounds__(BLOCK) void k_build_cells(AtomArrays a, ListArrays l, int N,
-                                                      DevState *__restrict__ gs) {
-    if (gs->err || !gs->do_rebuild)
+// Append the members of [b, e) of cell_atoms (or atoms b..e-1 when
+// cell_atoms is null) within r_list of atom i, LIST_LANES lanes per atom, in
+// member order.
+__device__ __forceinline__ void list_scan_range(const AtomArrays &a, const ListArrays &l,
+                                                const int *members, int b, int e, int i,
+                                                const double4 &ci, const double *L, double rl2,
+                                                int lane, unsigned long long gmask,
+                                                unsigned long long below, int *row, int &cnt) {
+    for (int k0 = b; k0 < e; k0 += LIST_LANES) {
+        int const k = k0 + lane;
+        bool in = false;
+        int j = 0;
+        if (k < e) {
+            j = members ? members[k] : k;
+            if (j != i) {
+                double dx, dy, dz;
+                min_image(ci, a.pos[j], L, dx, dy, dz);
+                in = dx * dx + dy * dy + dz * dz <= rl2;
+            }
+        }
+        unsigned long long const bal = __ballot(in) & gmask;
+        if (in) {
+            int const slot = cnt + __popcll(bal & below);
+            if (slot < l.stride)
+                row[slot] = j;
+        }
+        cnt += __popcll(bal);
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_list(AtomArrays a, ListArrays l, int N,
+                                               DevState *__restrict__ gs,
+                                               unsigned *__restrict__ bar, int use_cells) {
+    if (gs->err)
         return;
-    int const lane = threadIdx.x % LPA;
-    int const i = (blockIdx.x * BLOCK + threadIdx.x) / LPA;
-    int const l64 = threadIdx.x & 63;
-    unsigned long long const gmask = group_mask<LPA>();
-    unsigned long long const below = (l64 == 0) ? 0ull : (~0ull >> (64 - l64));
-    bool const active = i < N;
+    // phase 0: the decision, taken identically by every block from the same
+    // inputs (u_max^2 was reduced by k_kick_drift)
+    double lam = 1e300;
+    for (int c = 0; c < 3; ++c)
+        lam = fmin(lam, gs->L[c] / gs->L_build[c]);
+    bool const need = gs->force_rebuild || !(lam * gs->r_list - 2. * sqrt(gs->umax2) > gs->rc);
+    if (!need)
+        return;
+    bool ok = grid_barrier(bar, gs);   // everyone has read the decision inputs
+    int const nthreads = gridDim.x * BLOCK;
+    int const gtid = blockIdx.x * BLOCK + threadIdx.x;
     double const L[3] = {gs->L[0], gs->L[1], gs->L[2]};
     double const rl2 = gs->r_list * gs->r_list;
     int const nx = gs->ncell[0], ny = gs->ncell[1], nz = gs->ncell[2];
-    double4 const ci = a.pos[active ? i : 0];
-    int const cx = cell_coord(ci.x, nx), cy = cell_coord(ci.y, ny), cz = cell_coord(ci.z, nz);
-    int *row = l.nbr + (size_t)(active ? i : 0) * l.stride;
-    int cnt = 0;
-    for (int q = 0; q < 27; ++q) {
-        int const dz = q / 9 - 1, dy = (q / 3) % 3 - 1, dx = q % 3 - 1;
-        int const z = (cz + dz + nz) % nz, y = (cy + dy + ny) % ny, x = (cx + dx + nx) % nx;
-        int const cell = (z * ny + y) * nx + x;
-        int const b = active ? l.cell_start[cell] : 0;
-        int const e = active ? l.cell_start[cell + 1] : 0;
-        for (int k0 = b; k0 < e; k0 += LPA) {
-            int const k = k0 + lane;
-            bool in = false;
-            int j = 0;
-            if (k < e) {
-                j = l.cell_atoms[k];
-                if (j != i) {
-                    double ddx, ddy, ddz;
-                    min_image(ci, a.pos[j], L, ddx, ddy, ddz);
-                    in = ddx * ddx + ddy * ddy + ddz * ddz <= rl2;
+    if (gtid == 0) {
+        gs->force_rebuild = 0;
+        gs->do_rebuild = 1;
+        gs->nbuilds++;
+        gs->nlist_entries = 0;
+        for (int c = 0; c < 3; ++c)
+            gs->L_build[c] = gs->L[c];
+        if (use_cells)
+            for (int c = 0; c < 3; ++c)
+                if (gs->L[c] / gs->ncell[c] < gs->r_list) {
+                    atomicOr(&gs->err, (int)DE_CELL_GRID);
+                    gs->err_val[0] = gs->L[c];
                 }
+    }
+    int const lane = threadIdx.x % LIST_LANES;
+    int const l64 = threadIdx.x & 63;
+    unsigned long long const gmask = group_mask<LIST_LANES>();
+    unsigned long long const below = (l64 == 0) ? 0ull : (~0ull >> (64 - l64));
+    int const groups = nthreads / LIST_LANES;
+    if (use_cells) {
+        // phase 1: bin
+        for (int i = gtid; i < N; i += nthreads) {
+            double4 const c = a.pos[i];
+            int const cid = (cell_coord(c.z, nz) * ny + cell_coord(c.y, ny)) * nx +
+                            cell_coord(c.x, nx);
+            l.cell_of[i] = cid;
+            atomicAdd(&l.cell_count[cid], 1);
+        }
+        ok = grid_barrier(bar, gs) && ok;
+        // phase 2: exclusive scan of the counts (block 0); zero counts and fills
+        if (blockIdx.x == 0) {
+            __shared__ int sm[BLOCK];
+            __shared__ int carry;
+            if (threadIdx.x == 0)
+                carry = 0;
+            __syncthreads();
+            for (int base = 0; base < l.ncells; base += BLOCK) {
+                int const k = base + threadIdx.x;
+                int const v = k < l.ncells ? l.cell_count[k] : 0;
+                sm[threadIdx.x] = v;
+                __syncthreads();
+                for (int off = 1; off < BLOCK; off <<= 1) {
+                    int const t = threadIdx.x >= off ? sm[threadIdx.x - off] : 0;
+                    __syncthreads();
+                    sm[threadIdx.x] += t;
+                    __syncthreads();
+                }
+                if (k < l.ncells) {
+                    l.cell_start[k] = carry + sm[threadIdx.x] - v;
+                    l.cell_count[k] = 0;
+                    l.cell_fill[k] = 0;
+                }
+                __syncthreads();
+                if (threadIdx.x == BLOCK - 1)
+                    carry += sm[BLOCK - 1];
+                __syncthreads();
             }
-            unsigned long long const bal = __ballot(in) & gmask;
-            if (in) {
-                int const slot = cnt + __popcll(bal & below);
-                if (slot < l.stride)
-                    row[slot] = j;
+            if (threadIdx.x == 0)
+                l.cell_start[l.ncells] = carry;
+        }
+        ok = grid_barrier(bar, gs) && ok;
+        // phase 3: scatter
+        for (int i = gtid; i < N; i += nthreads) {
+            int const cid = l.cell_of[i];
+            int const slot = atomicAdd(&l.cell_fill[cid], 1);
+            l.cell_atoms[l.cell_start[cid] + slot] = i;
+        }
+        ok = grid_barrier(bar, gs) && ok;
+        // phase 4: one block per cell places each member at its rank
+        {
+            __shared__ int v[CELL_MAX];
+            for (int c = blockIdx.x; c < l.ncells; c += gridDim.x) {
+                int const b = l.cell_start[c], n = l.cell_start[c + 1] - b;
+                if (n > CELL_MAX) {
+                    if (threadIdx.x == 0) {
+                        atomicOr(&gs->err, (int)DE_CELL_GRID);
+                        gs->err_val[0] = (double)n;
+                    }
+                    continue;   // uniform in the block
+                }
+                for (int k = threadIdx.x; k < n; k += BLOCK)
+                    v[k] = l.cell_atoms[b + k];
+                __syncthreads();
+                for (int k = threadIdx.x; k < n; k += BLOCK) {
+                    int const x = v[k];
+                    int rank = 0;
+                    for (int m = 0; m < n; ++m)
+                        rank += v[m] < x;
+                    l.cell_atoms[b + rank] = x;
+                }
+                __syncthreads();
             }
-            cnt += __popcll(bal);
+        }
+        ok = grid_barrier(bar, gs) && ok;
+        if (!ok)
+            return;
+        // phase 5: lists from the 27 surrounding cells, in (cell, member) order
+        for (int i = gtid / LIST_LANES; i < N; i += groups) {
+            double4 const ci = a.pos[i];
+            int const cx = cell_coord(ci.x, nx), cy = cell_coord(ci.y, ny), cz = cell_coord(ci.z, nz);
+            int *row = l.nbr + (size_t)i * l.stride;
+            int cnt = 0;
+            for (int q = 0; q < 27; ++q) {
+                int const dz = q / 9 - 1, dy = (q / 3) % 3 - 1, dx = q % 3 - 1;
+                int const z = (cz + dz + nz) % nz, y = (cy + dy + ny) % ny, x = (cx + dx + nx) % nx;
+                int const cell = (z * ny + y) * nx + x;
+                list_scan_range(a, l, l.cell_atoms, l.cell_start[cell], l.cell_start[cell + 1], i,
+                                ci, L, rl2, lane, gmask, below, row, cnt);
+            }
+            if (lane == 0)
+                list_finish(l, a, i, cnt, gs);
+        }
+    } else {
+        if (!ok)
+            return;
+        // small boxes (< 3 cells per direction): test every other atom
+        for (int i = gtid / LIST_LANES; i < N; i += groups) {
+            double4 const ci = a.pos[i];
+            int *row = l.nbr + (size_t)i * l.stride;
+            int cnt = 0;
+            list_scan_range(a, l, nullptr, 0, N, i, ci, L, rl2, lane, gmask, below, row, cnt);
+            if (lane == 0)
+                list_finish(l, a, i, cnt, gs);
         }
     }
-    if (active && lane == 0)
-        list_finish(l, a, i, cnt, gs);
-}
-
-// Full list by testing every other atom (small boxes, < 3 cells per direction).
-__global__ __launch_bounds__(BLOCK) void k_build_brute(AtomArrays a, ListArrays l, int N,
-                                                      DevState *__restrict__ gs) {
-    if (gs->err || !gs->do_rebuild)
-        return;
-    int const i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= N)
-        return;
-    double const L[3] = {gs->L[0], gs->L[1], gs->L[2]};
-    double const rl2 = gs->r_list * gs->r_list;
-    double4 const ci = a.pos[i];
-    int cnt = 0;
-    for (int j = 0; j < N; ++j) {
-        if (j == i)
-            continue;
-        double ddx, ddy, ddz;
-        min_image(ci, a.pos[j], L, ddx, ddy, ddz);
-        if (ddx * ddx + ddy * ddy + ddz * ddz <= rl2)
-            list_push(l, i, cnt, j);
-    }
-    list_finish(l, a, i, cnt, gs);
 }
 
 // ---------------------------------------------------------------------------
@@ -967,15 +1009,26 @@ __global__ __launch_bounds__(BLOCK) void k_kick2(AtomArrays a, int N,
     block_sum_store<PK_N>(acc, part);
 }
 
-// Deterministic sums of the per-block partial records: one block per
-// component; thread t adds blocks t, t+BLOCK, ... in order, then a fixed tree.
+// Deterministic sums of the per-block partial records, one block per
+// component (thread t adds blocks t, t+BLOCK, ... in order, then a fixed
+// tree); the last block to finish then runs the scalar part of the step:
+// post_step (mode 0 = constructor, md_driver.cpp:114-118; mode 1 = end of
+// integrate(), md_driver.cpp:583-600) and, when next_sample >= 0, the first
+// half of the next step (pre_step), which needs nothing in between.
 // Components are numbered A (PA_N), then B (PB_N), then K (PK_N).
 constexpr int NTOTAL = PA_N + PB_N + PK_N;
-__global__ __launch_bounds__(BLOCK) void k_reduce(const DevState *__restrict__ gs,
+
+__device__ void post_step(DevState &s, const double *tA, const double *tB, const double *tK,
+                          int mode, double *__restrict__ rec_out);
+
+__global__ __launch_bounds__(BLOCK) void k_finish(DevState *__restrict__ gs,
                                                  const double *__restrict__ pA, int nA,
                                                  const double *__restrict__ pB, int nB,
                                                  const double *__restrict__ pK, int nK,
-                                                 double *__restrict__ totals) {
+                                                 double *__restrict__ totals,
+                                                 unsigned *__restrict__ ticket, int mode,
+                                                 double *__restrict__ rec_out, int next_sample,
+                                                 int next_force) {
     if (gs->err)
         return;
     int k = blockIdx.x;
@@ -993,42 +1046,42 @@ __global__ __launch_bounds__(BLOCK) void k_reduce(const DevState *__restrict__ g
         k -= PA_N + PB_N;
         nb = nK;
     }
-    double s = 0.;
+    double sum = 0.;
     for (int b = threadIdx.x; b < nb; b += BLOCK)
-        s += p[(size_t)k * PART_STRIDE + b];
+        sum += p[(size_t)k * PART_STRIDE + b];
     for (int off = 32; off > 0; off >>= 1)
-        s += __shfl_xor(s, off, 64);
+        sum += __shfl_xor(sum, off, 64);
     __shared__ double sm[BLOCK / 64];
+    __shared__ int last;
     if ((threadIdx.x & 63) == 0)
-        sm[threadIdx.x >> 6] = s;
+        sm[threadIdx.x >> 6] = sum;
     __syncthreads();
     if (threadIdx.x == 0) {
         double t = 0.;
         for (int w = 0; w < BLOCK / 64; ++w)
             t += sm[w];
         totals[blockIdx.x] = t;
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
     }
-}
-
-__device__ void post_step(DevState &s, const double *tA, const double *tB, const double *tK,
-                          int mode, double *__restrict__ rec_out);
-
-// ---------------------------------------------------------------------------
-// k_post: after the forces. mode 0 = constructor (md_driver.cpp:114-118),
-// mode 1 = end of integrate() (md_driver.cpp:583-600).
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_post(DevState *__restrict__ gs,
-                                            const double *__restrict__ totals, int mode,
-                                            double *__restrict__ rec_out) {
-    if (gs->err)
+    __syncthreads();
+    if (!last)
         return;
+    __threadfence();
     __shared__ double tot[NTOTAL];
-    for (int k = threadIdx.x; k < NTOTAL; k += 64)
-        tot[k] = totals[k];
+    for (int c = threadIdx.x; c < NTOTAL; c += BLOCK)
+        tot[c] = __hip_atomic_load(&totals[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __shared__ DevState s;
     state_load(s, gs);
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         post_step(s, tot, tot + PA_N, tot + PA_N + PB_N, mode, rec_out);
+        if (next_sample >= 0 && !s.err) {
+            pre_step(s, next_sample);
+            if (next_force)
+                s.force_rebuild = 1;
+        }
+        *ticket = 0u;
+    }
     state_store(gs, s);
 }
 
@@ -1040,11 +1093,15 @@ __device__ void post_step(DevState &s, const double *tA, const double *tB, const
     double *rec = s.rec;
     int const N = s.N;
 
-    // pair sums were taken over the full list: halve them (md_driver.cpp:389-407)
-    s.virial = -(0.5 * tB[PB_VIR]) / 3.;
+    // pair sums were taken over the full list: halve them (md_driver.cpp:389-407).
+    // unit_E / unit_L convert the mixed path's eV and eV/A^2 sums to CGS (1 in FP64).
+    double const uE = s.unit_E, uL = s.unit_L;
     double sB[6], CB[21];
     for (int v = 0; v < 6; ++v)
-        sB[v] = 0.5 * tB[PB_SB + v];
+        sB[v] = 0.5 * tB[PB_SB + v] * uE;
+    // mixed mode: the virial is the Born-stress trace, so P and tr(sigma)/3 agree
+    // to FP64 round-off as they do in the reference (md_driver.cpp:693-698)
+    s.virial = s.mixed ? -(sB[0] + sB[1] + sB[2]) / 3. : -(0.5 * tB[PB_VIR]) / 3.;
     if (sample) {
         // Voigt (v <= w) -> monomial of the pair term, then the embedding term
         const int mono[21] = {M_X4, M_X2Y2, M_X2Z2, M_X2YZ, M_X3Z, M_X3Y,
@@ -1054,8 +1111,8 @@ __device__ void post_step(DevState &s, const double *tA, const double *tB, const
                               M_X2Z2, M_X2YZ,
                               M_X2Y2};
         for (int q = 0; q < 21; ++q)
-            CB[q] = 0.5 * tB[PB_CB + mono[q]] + tK[PK_CBF + q];
-        s.lap = 0.5 * tB[PB_LAP] + tK[PK_LAPF];
+            CB[q] = (0.5 * tB[PB_CB + mono[q]] + tK[PK_CBF + q]) * uE;
+        s.lap = (0.5 * tB[PB_LAP] + tK[PK_LAPF]) * uL;
     }
     s.warn_rmin += (long long)(0.5 * tB[PB_WRMIN] + 0.25);
     {   // virial self-check (md_driver.cpp:422-429), a warning only
@@ -1095,8 +1152,8 @@ __device__ void post_step(DevState &s, const double *tA, const double *tB, const
         s.err_val[2] = mom[2];
         s.err_val[3] = sqrt(sc2);
     }
-    double const epot_atom = tB[PB_EPOT] / N;
-    double const epot_eam_atom = s.has_eam ? tA[PA_EF] / N : 0.;
+    double const epot_atom = tB[PB_EPOT] * uE / N;
+    double const epot_eam_atom = s.has_eam ? tA[PA_EF] * uE / N : 0.;
     double const etot_atom = ekin_atom + epot_atom;
     double const old = s.etot_cons;
     double ec = etot_atom * N;

@@ -1,0 +1,89 @@
+"""Mixed-precision path (FP32 tables and pair algebra, FP64 geometry and sums)
+against the FP64 oracle, with the per-quantity tolerances this path is held to.
+
+The reference changed its own tabulation by these amounts and treated the change
+as negligible (FIXES.md:19-22 of the reference): forces 5e-6, Born 4e-8 and
+T_conf 8e-8 relative. FP32 table arithmetic is held to:
+
+  energy per atom           2e-7 relative
+  forces                    5e-6 of max|F|
+  virial, Born stress       2e-6 of sum |pair term|
+  Born elasticity (21)      2e-6 of sum |pair term|
+  Laplacian, 1/T_conf       2e-6 relative
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import cells
+import harness as H
+
+pytestmark = pytest.mark.gpu
+
+TOL = dict(epot_atom=2e-7, epot_eam_atom=2e-7, forces=5e-6, virial=2e-6, born_stress=2e-6,
+           born_elasticity=2e-6, laplacian=2e-6, T_conf_inv=2e-6, pressure=2e-6,
+           energies_pot=2e-6, rho=2e-6, lower_triangle_zero=0.0)
+
+
+def _check(o, g):
+    errs = H.compare_static(o, g, 1.0)
+    print({k: f"{v:.2e}" for k, v in errs.items()})
+    bad = {k: v for k, v in errs.items() if v > TOL[k]}
+    assert not bad, bad
+    return errs
+
+
+@pytest.mark.parametrize("cellname", ["smoke", "nialco"])
+@pytest.mark.parametrize("interp", ["quintic", "bspline"])
+def test_mixed_static_vs_fp64_oracle(cellname, interp):
+    cell = cells.smoke_cell() if cellname == "smoke" else cells.nialco_cell()
+    setup = H.md_setup(cell, static=True)
+    o = H.oracle_md(cell, setup, interpolation=interp)
+    g = H.gpu_md(cell, setup, interpolation=interp, precision="mixed")
+    _check(o, g)
+
+
+def test_mixed_momentum_stays_at_fp64_roundoff():
+    # pair forces are bit-symmetric, so 1000 steps keep |P| at FP64 round-off
+    # (the reference aborts above |P|^2 / (2 m_0 K) = 1e-14, md_driver.cpp:620-627)
+    cell = cells.alloy_cell(10, 3.56, "Mishin_updated-Ni-Al-Co-2013.eam.alloy",
+                            [0.75, 0.20, 0.05], seed=11)
+    setup = H.md_setup(cell, T=600.0, M=10, seed=4)
+    g = H.gpu_md(cell, setup, ensemble="NVT", neighbors_list=True, r_skin=1.2,
+                 precision="mixed")
+    rec = g.run(1000, 50, 20)
+    m0 = cells.masses(cell["data"])[0]
+    p2 = (rec[:, 21:24] ** 2).sum(1)
+    ratio = p2 / (2 * m0 * rec[:, 4])
+    print("max |P|^2/(2 m0 K):", ratio.max())
+    assert ratio.max() < 1e-20
+
+
+def test_mixed_npt_pressure_trace_consistent():
+    # near P = 0 the reference's P-vs-trace check (1e-5) is tight; the mixed
+    # path takes the virial as the Born-stress trace, so the two agree
+    cell = cells.zn06_cell(8)
+    setup = H.md_setup(cell, T=300.0, M=10, Mb=10, seed=0)
+    g = H.gpu_md(cell, setup, ensemble="NPT", dt=2e-15, neighbors_list=True, r_skin=1.2,
+                 precision="mixed")
+    rec = g.run(400, 1, 20)
+    st = rec[:, 30:33].sum(1) / 3.0
+    err = np.abs(rec[:, 13] / st - 1.0)
+    assert err.max() < 1e-8, err.max()
+
+
+def test_mixed_trajectory_tracks_fp64():
+    # 200 NVT steps from one state: mixed vs FP64 GPU paths stay close in the
+    # thermodynamic observables (chaos amplifies the 1e-7 table differences slowly)
+    cell = cells.zn06_cell(5)
+    setup = H.md_setup(cell, T=300.0, M=3, seed=0)
+    kw = dict(ensemble="NVT", dt=2e-15, neighbors_list=True, r_skin=1.2)
+    a = H.gpu_md(cell, setup, precision="fp64", **kw)
+    b = H.gpu_md(cell, setup, precision="mixed", **kw)
+    ra, rb = a.run(200, 1, 20), b.run(200, 1, 20)
+    for k, tol in ((6, 1e-6), (11, 1e-3), (1, 1e-6)):
+        assert np.max(np.abs(rb[:, k] / ra[:, k] - 1)) < tol, k
+    # the E_cons error is set by dt; the mixed run's equals the FP64 run's
+    assert np.max(np.abs(rb[:, 7] - ra[:, 7])) < 2e-7
