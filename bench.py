@@ -198,6 +198,7 @@ def main():
             "precision": args.precision,
             "neighbor_entries_per_atom": entries / N,
             "list_builds": d.neighbor_list_builds,
+            "nhc_general_path": d.nhc_general_path_count,
             "kernel_ms": {k: round(v, 5) for k, v in kt.items()},
             "final_T_kin": float(rec[-1][11]),
             "final_etot_cons_err": float(rec[-1][7]),

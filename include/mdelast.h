@@ -161,6 +161,7 @@ enum {
     MDL_S_WARN_VIRIAL = 76,   /* steps whose virial self-check failed (md_driver.cpp:422-429) */
     MDL_S_WARN_RMIN = 77,     /* pair distances below r_min seen (potential.hpp:93-95) */
     MDL_S_NLIST_ENTRIES = 78, /* stored (full) list entries */
+    MDL_S_NHC_GENERAL = 79,   /* chain propagations that left the short-argument path */
     MDL_NSCALARS = 80
 };
 mdl_status mdl_driver_get_scalars(mdl_driver *d, double *out);

@@ -16,8 +16,8 @@
 
 namespace mdl {
 
-constexpr int GRID_MAX = 2048;      // blocks of the pair passes (multiple of 8)
-constexpr int GRID_ATOM_MAX = 1024; // blocks of the kick kernels
+constexpr int GRID_MAX = PART_STRIDE;   // blocks of the pair passes (multiple of 8)
+constexpr int GRID_ATOM_MAX = 1024;     // blocks of the kick kernels
 
 using DensFn = void (*)(AtomArrays, ListArrays, RTables, FTables, int, int, int, double, double,
                         DevState *, double *);
@@ -89,11 +89,6 @@ static ForceFn pick_force(int kind, int lpa, bool s, bool e) {
     return s ? force_fn<MDL_BSPLINE, true, false>(lpa) : force_fn<MDL_BSPLINE, false, false>(lpa);
 }
 
-__global__ void k_set_rebuild(DevState *gs) {
-    if (threadIdx.x == 0)
-        gs->force_rebuild = 1;
-}
-
 // Autotuner cache: (N, ntypes, kind, list stride bucket) -> lanes per atom
 static std::map<std::tuple<int64_t, int, int, int>, int> g_lpa_cache;
 
@@ -105,6 +100,7 @@ struct Driver {
     bool use_cells = false;
     int sample = 1;
     int64_t cycle_host = 0;
+    bool build_pending = true;     // calculate_neighbors() or the constructor asked for a build
     bool failed = false;
     AtomArrays a{};
     ListArrays l{};
@@ -150,21 +146,33 @@ struct Driver {
         return (g + 7) / 8 * 8;
     }
 
-    int gridL = 1;                  // resident blocks of the cooperative list kernel
-    unsigned *d_bar = nullptr, *d_ticket_kd = nullptr, *d_ticket_fin = nullptr;
+    int gridL = 1;                  // blocks of k_list: no more than the device holds at once
+    unsigned *d_bar = nullptr, *d_ticket_fin = nullptr;
 
-    hipError_t launch_list() {
-        int cells = use_cells ? 1 : 0;
-        int n = N;
-        void *args[] = {&a, &l, &n, &gs, &d_bar, &cells};
-        return hipLaunchCooperativeKernel((const void *)k_list, dim3(gridL), dim3(BLOCK), args, 0,
-                                          stream);
+    // The per-step list check (and the rare build it finds due).
+    void launch_list() {
+        k_list<<<gridL, BLOCK, 0, stream>>>(a, l, N, gs, d_bar, d_dmax, gridK, use_cells ? 1 : 0);
     }
 
-    void launch_finish(int mode, double *rec_out, int next_sample, int next_force) {
+    // A requested build, as a chain of kernels.
+    void launch_build() {
+        int const gN = (N + BLOCK - 1) / BLOCK;
+        if (use_cells) {
+            k_rb_bin<<<gN, BLOCK, 0, stream>>>(a, l, N, gs);
+            k_rb_scan<<<1, BLOCK, 0, stream>>>(l, gs);
+            k_rb_scatter<<<gN, BLOCK, 0, stream>>>(l, N, gs);
+            k_rb_sort<<<l.ncells, BLOCK, 0, stream>>>(l, gs);
+            k_rb_build<<<l.ncells, BLOCK, 0, stream>>>(a, l, N, gs, 1);
+        } else {
+            k_rb_begin<<<1, 64, 0, stream>>>(gs);
+            int const gB = (int)(((int64_t)N * LIST_LANES + BLOCK - 1) / BLOCK);
+            k_rb_build<<<gB, BLOCK, 0, stream>>>(a, l, N, gs, 0);
+        }
+    }
+
+    void launch_finish(int mode, double *rec_out, int next_sample) {
         k_finish<<<NTOTAL, BLOCK, 0, stream>>>(gs, pA, gridA, pB, gridB, pK, gridK, d_totals,
-                                               d_ticket_fin, mode, rec_out, next_sample,
-                                               next_force);
+                                               d_ticket_fin, mode, rec_out, next_sample);
     }
 
     void launch_forces(bool s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
@@ -196,14 +204,16 @@ struct Driver {
     }
 
     // One step after its first half (k_pre or the previous k_finish) has run.
+    // build: rebuild the list after the drift (else only check it).
     // next_sample < 0: end with the post step only; otherwise also run the next
-    // step's first half with that sampling flag and forced-rebuild flag.
-    hipError_t launch_step(bool s, double *rec_out, hipEvent_t *tev, int next_sample,
-                           int next_force) {
-        k_kick_drift<<<gridK, BLOCK, 0, stream>>>(a, N, gs, d_dmax, d_ticket_kd);
-        hipError_t e = launch_list();
-        if (e != hipSuccess)
-            return e;
+    // step's first half with that sampling flag.
+    hipError_t launch_step(bool s, double *rec_out, hipEvent_t *tev, bool build,
+                           int next_sample) {
+        k_kick_drift<<<gridK, BLOCK, 0, stream>>>(a, N, gs, d_dmax);
+        if (build)
+            launch_build();
+        else
+            launch_list();
         if (tev)
             launch_forces(s, tev[0], tev[1], tev[2]);
         else
@@ -212,7 +222,7 @@ struct Driver {
             k_kick2<true><<<gridK, BLOCK, 0, stream>>>(a, N, gs, 0, has_eam, pK);
         else
             k_kick2<false><<<gridK, BLOCK, 0, stream>>>(a, N, gs, 0, has_eam, pK);
-        launch_finish(1, rec_out, next_sample, next_force);
+        launch_finish(1, rec_out, next_sample);
         ++cycle_host;
         return hipGetLastError();
     }
@@ -562,10 +572,8 @@ static mdl_status create(const mdl_config *c, Driver &d) {
     ALLOC(d.d_stage, (size_t)3 * N);
     ALLOC(d.d_totals, NTOTAL);
     ALLOC(d.d_bar, 4);
-    ALLOC(d.d_ticket_kd, 4);
     ALLOC(d.d_ticket_fin, 4);
     MDL_HIP(hipMemset(d.d_bar, 0, 4 * sizeof(unsigned)));
-    MDL_HIP(hipMemset(d.d_ticket_kd, 0, 4 * sizeof(unsigned)));
     MDL_HIP(hipMemset(d.d_ticket_fin, 0, 4 * sizeof(unsigned)));
     ALLOC(d.d_dmax, GRID_ATOM_MAX);
     MDL_HIP(hipMemset(d.d_dmax, 0, sizeof(double) * GRID_ATOM_MAX));
@@ -625,6 +633,7 @@ static mdl_status create(const mdl_config *c, Driver &d) {
     s.th.M = c->th_M;
     for (int j = 0; j < c->th_M; ++j) {
         s.th.q[j] = c->th_q[j];
+        s.th.iq[j] = 1.0 / c->th_q[j];
         s.th.p_eta[j] = c->th_p_eta[j];
     }
     s.bth.T = c->bth_T;
@@ -632,6 +641,7 @@ static mdl_status create(const mdl_config *c, Driver &d) {
     s.bth.M = c->bth_M;
     for (int j = 0; j < c->bth_M; ++j) {
         s.bth.q[j] = c->bth_q[j];
+        s.bth.iq[j] = 1.0 / c->bth_q[j];
         s.bth.p_eta[j] = c->bth_p_eta[j];
     }
     s.bar_P = c->bar_P;
@@ -645,7 +655,7 @@ static mdl_status create(const mdl_config *c, Driver &d) {
     s.r_list = r_list;
     s.rc = rc;
     s.skin = r_list - rc;
-    s.force_rebuild = 1;
+    s.force_rebuild = 0;
     for (int k = 0; k < 3; ++k)
         s.ncell[k] = cells ? nc[k] : 1;
     MDL_HIP(hipMemcpy(d.gs, &s, sizeof(DevState), hipMemcpyHostToDevice));
@@ -661,22 +671,27 @@ static mdl_status create(const mdl_config *c, Driver &d) {
         d.lpa = 16;
     d.gridA = d.gridB = d.grid_pairs(d.lpa);
     d.gridK = std::min((N + BLOCK - 1) / BLOCK, GRID_ATOM_MAX);
-    {   // the cooperative list kernel: as many blocks as can be resident at once
+    {   // k_list's grid barriers need every block resident at once: at most
+        // what the device holds of it (occupancy x CUs), and at most 512
+        // blocks, since each barrier costs a serialised atomic per block
         int per_cu = 0, cus = 0;
         MDL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_list, BLOCK, 0));
         MDL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d.device));
-        int coop = 0;
-        MDL_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, d.device));
-        if (!coop || per_cu < 1) {
-            set_error("the device does not support cooperative launches");
+        if (per_cu < 1 || cus < 1) {
+            set_error("k_list does not fit on the device");
             return MDL_ERR_DEVICE;
         }
-        int64_t const want = ((int64_t)N * LIST_LANES + BLOCK - 1) / BLOCK;
-        d.gridL = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)per_cu * cus));
+        int64_t const want = d.use_cells
+                                 ? std::max<int64_t>(d.l.ncells, (N + BLOCK - 1) / BLOCK)
+                                 : ((int64_t)N * LIST_LANES + BLOCK - 1) / BLOCK;
+        int64_t const cap = std::min<int64_t>(512, (int64_t)per_cu * cus);
+        d.gridL = (int)std::max<int64_t>(1, std::min<int64_t>(want, cap));
     }
 
     // constructor: list, forces, thermodynamics (md_driver.cpp:112-118)
-    MDL_HIP(d.launch_list());
+    d.launch_build();
+    d.build_pending = false;
+    MDL_HIP(hipGetLastError());
     if (mdl_status st = d.sync_state())
         return st;
     if (mdl_status st = d.check_errors())
@@ -686,7 +701,7 @@ static mdl_status create(const mdl_config *c, Driver &d) {
             return st;
     d.launch_forces(true);
     k_kick2<true><<<d.gridK, BLOCK, 0, d.stream>>>(d.a, N, d.gs, 1, d.has_eam, d.pK);
-    d.launch_finish(0, nullptr, -1, 0);
+    d.launch_finish(0, nullptr, -1);
     MDL_HIP(hipGetLastError());
     if (mdl_status st = d.sync_state())
         return st;
@@ -740,9 +755,12 @@ mdl_status mdl_driver_integrate(mdl_driver *p, int64_t nsteps) {
         return s;
     int const smp = d->sample != 0 ? 1 : 0;
     if (nsteps > 0)
-        k_pre<<<1, 64, 0, d->stream>>>(d->gs, smp, 0);
-    for (int64_t k = 0; k < nsteps; ++k)
-        MDL_HIP(d->launch_step(smp != 0, nullptr, nullptr, k + 1 < nsteps ? smp : -1, 0));
+        k_pre<<<1, BLOCK, 0, d->stream>>>(d->gs, smp);
+    for (int64_t k = 0; k < nsteps; ++k) {
+        bool const build = d->build_pending;
+        d->build_pending = false;
+        MDL_HIP(d->launch_step(smp != 0, nullptr, nullptr, build, k + 1 < nsteps ? smp : -1));
+    }
     if (mdl_status s = d->sync_state())
         return s;
     return d->check_errors();
@@ -774,8 +792,7 @@ mdl_status mdl_driver_run(mdl_driver *p, int64_t nsteps, int64_t sample_stride,
     auto sampled = [&](int64_t cyc) { return (cyc + 1) % sample_stride == 0; };
     auto forced = [&](int64_t cyc) { return rebuild_every > 0 && cyc % rebuild_every == 0; };
     if (nsteps > 0)
-        k_pre<<<1, 64, 0, d->stream>>>(d->gs, sampled(d->cycle_host) ? 1 : 0,
-                                        forced(d->cycle_host) ? 1 : 0);
+        k_pre<<<1, BLOCK, 0, d->stream>>>(d->gs, sampled(d->cycle_host) ? 1 : 0);
     for (int64_t k = 0; k < nsteps; ++k) {
         int64_t const cyc = d->cycle_host;
         bool const s = sampled(cyc);
@@ -788,8 +805,9 @@ mdl_status mdl_driver_run(mdl_driver *p, int64_t nsteps, int64_t sample_stride,
             (void)hipEventRecord(tev[3], d->stream);
         }
         int const ns = k + 1 < nsteps ? (sampled(cyc + 1) ? 1 : 0) : -1;
-        int const nf = k + 1 < nsteps && forced(cyc + 1) ? 1 : 0;
-        MDL_HIP(d->launch_step(s, ro, tev, ns, nf));
+        bool const build = d->build_pending || forced(cyc);
+        d->build_pending = false;
+        MDL_HIP(d->launch_step(s, ro, tev, build, ns));
     }
     if (drec) {
         hipError_t e = hipMemcpyAsync(records, drec, sizeof(double) * MDL_NRECORD * irec,
@@ -834,8 +852,7 @@ mdl_status mdl_driver_calculate_neighbors(mdl_driver *p) {
     Driver *d = D(p);
     if (mdl_status s = guard(d))
         return s;
-    k_set_rebuild<<<1, 1, 0, d->stream>>>(d->gs);
-    MDL_HIP(hipGetLastError());
+    d->build_pending = true;   // the next step rebuilds the list after its drift
     return MDL_OK;
 }
 

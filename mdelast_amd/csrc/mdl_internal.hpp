@@ -71,7 +71,7 @@ struct HostTable {
 struct Chain {
     double T, Nf;
     int M;
-    double q[MAX_CHAIN], p_eta[MAX_CHAIN], eta[MAX_CHAIN];
+    double q[MAX_CHAIN], iq[MAX_CHAIN], p_eta[MAX_CHAIN], eta[MAX_CHAIN];   // iq = 1/q
 };
 
 struct DevState {
@@ -106,6 +106,7 @@ struct DevState {
     int err;
     double err_val[4];
     long long warn_virial, warn_rmin;
+    long long nhc_general;   // chain propagations that took the general (library exp) path
     int init_done;
 };
 

@@ -1,10 +1,12 @@
 // Device kernels of one MD step. Included by mdl_driver.hip only.
 //
-// Step = k_pre (scalars: NHC, barostat, kick/drift coefficients)
+// Step = k_pre, or the previous step's k_finish (scalars: NHC, barostat,
+//        kick/drift coefficients)
 //      -> k_kick_drift (first half kick + drift, displacement since the list build)
-//      -> k_decide -> [list build kernels, each a no-op unless a rebuild is due]
+//      -> k_list (rebuild decision; the rebuild itself when one is due)
 //      -> k_density (pass A) -> k_force (pass B) -> k_kick2 (second half kick)
-//      -> k_post (reductions, NHC, thermodynamics, checks)
+//      -> k_finish (reductions, NHC, thermodynamics, checks; then the next
+//         step's scalar first half)
 // which is MDDriver::integrate (md_driver.cpp:563-601) with the velocity
 // scalings of the Nose-Hoover chain folded into one factor per half step.
 #pragma once
@@ -103,11 +105,42 @@ __device__ __forceinline__ void min_image(const double4 &ci, const double4 &cj, 
     dz = (sz - nint_away(sz)) * L[2];
 }
 
+// exp(-x) and op(x) = (1 - exp(-x))/x of numerics.hpp:11-18. The reference
+// takes exp from the library and op from a 5th-order Taylor form for
+// |x| <= 1e-3, else from the quotient. The chain arguments are mostly small
+// (|x| ~ dt/tau): for |x| <= 1/8, op comes from its Taylor series to degree 10
+// (truncation below 3e-19 relative) and exp(-x) = 1 - x op(x) from one more
+// FMA (within rounding of the library exp; the quotient form of op loses up to
+// 1/|x| ulp to cancellation and agrees to that).
+constexpr double EXPX_SMALL = 0.125;
+
+__device__ __forceinline__ double op_series(double x) {
+    double const y = -x;
+    double o = 1. / 39916800.;
+    o = fma(o, y, 1. / 3628800.);
+    o = fma(o, y, 1. / 362880.);
+    o = fma(o, y, 1. / 40320.);
+    o = fma(o, y, 1. / 5040.);
+    o = fma(o, y, 1. / 720.);
+    o = fma(o, y, 1. / 120.);
+    o = fma(o, y, 1. / 24.);
+    o = fma(o, y, 1. / 6.);
+    o = fma(o, y, 0.5);
+    return fma(o, y, 1.);
+}
+
+__device__ __forceinline__ double exp_neg(double x) {
+    return fabs(x) <= EXPX_SMALL ? fma(-x, op_series(x), 1.) : exp(-x);
+}
+
 __device__ __forceinline__ void expx_operator(double x, double &expx, double &op) {
-    // numerics.hpp:11-18
-    const double c1 = -1.0 / 2.0, c2 = 1.0 / 6.0, c3 = -1.0 / 24.0, c4 = 1. / 120., c5 = -1. / 720.;
-    expx = exp(-x);
-    op = fabs(x) > 1e-3 ? (1.0 - expx) / x : 1.0 + x * (c1 + x * (c2 + x * (c3 + x * (c4 + x * c5))));
+    if (fabs(x) <= EXPX_SMALL) {
+        op = op_series(x);
+        expx = fma(-x, op, 1.);
+    } else {
+        expx = exp(-x);
+        op = (1.0 - expx) / x;
+    }
 }
 
 __constant__ const double WSY7[7] = {
@@ -115,17 +148,22 @@ __constant__ const double WSY7[7] = {
     1. - 2. * (0.784513610477560 + 0.235573213359357 - 1.17767998417887),
     -1.17767998417887, 0.235573213359357, 0.784513610477560};
 
+// The chain sweeps are one long sequence of dependent scalar FP64 operations
+// run by a single lane, so their cost is their instruction count: 1/q_j is
+// precomputed (Chain::iq) and p/q becomes p * (1/q) (one rounding more than
+// the reference's division, far below the parity tolerances).
+
 // thermostat.cpp:40-66
 __device__ void chain_U4(Chain &c, double dt, double e_couple, bool outward) {
     double const kT = K_BOLTZ * c.T;
     int const last = c.M - 1;
     for (int s = 0; s < c.M; ++s) {
         int const j = outward ? s : last - s;
-        double const G = j != 0 ? c.p_eta[j - 1] * c.p_eta[j - 1] / c.q[j - 1] - kT
+        double const G = j != 0 ? c.p_eta[j - 1] * c.p_eta[j - 1] * c.iq[j - 1] - kT
                                 : 2. * e_couple - c.Nf * kT;
         if (j != last) {
             double e, op;
-            expx_operator(dt * c.p_eta[j + 1] / c.q[j + 1], e, op);
+            expx_operator(dt * c.p_eta[j + 1] * c.iq[j + 1], e, op);
             c.p_eta[j] = c.p_eta[j] * e + dt * G * op;
         } else {
             c.p_eta[j] += dt * G;
@@ -135,9 +173,9 @@ __device__ void chain_U4(Chain &c, double dt, double e_couple, bool outward) {
 
 // thermostat.hpp:24-35: returns the momentum scale factor
 __device__ double chain_U3(Chain &c, double dt) {
-    double const s = exp(-dt * c.p_eta[0] / c.q[0]);
+    double const s = exp_neg(dt * c.p_eta[0] * c.iq[0]);
     for (int j = 0; j < c.M; ++j)
-        c.eta[j] += dt * c.p_eta[j] / c.q[j];
+        c.eta[j] += dt * c.p_eta[j] * c.iq[j];
     return s;
 }
 
@@ -148,7 +186,7 @@ __device__ double chain_etot(const Chain &c) {
         e += c.eta[j];
     e *= K_BOLTZ * c.T;
     for (int j = 0; j < c.M; ++j)
-        e += 0.5 * (c.p_eta[j] * c.p_eta[j]) / c.q[j];
+        e += 0.5 * (c.p_eta[j] * c.p_eta[j]) * c.iq[j];
     return e;
 }
 
@@ -164,43 +202,15 @@ __device__ double thermo_couple(const DevState &s) {
     return e;
 }
 
-// U4 and U3 of a chain of compile-time length M held in registers (the
-// chain sweep is a long chain of dependent FP64 operations; keeping p_eta,
-// eta and q out of memory shortens it). Same arithmetic as chain_U4/chain_U3.
-template <int M>
-__device__ __forceinline__ void reg_U4(double (&p)[M], const double (&q)[M], double Nf,
-                                       double kT, double dt, double e_couple, bool outward) {
-#pragma unroll
-    for (int s = 0; s < M; ++s) {
-        int const j = outward ? s : M - 1 - s;
-        double const G = j != 0 ? p[j - 1] * p[j - 1] / q[j - 1] - kT : 2. * e_couple - Nf * kT;
-        if (j != M - 1) {
-            double e, op;
-            expx_operator(dt * p[j + 1] / q[j + 1], e, op);
-            p[j] = p[j] * e + dt * G * op;
-        } else {
-            p[j] += dt * G;
-        }
-    }
-}
-
-template <int M>
-__device__ __forceinline__ double reg_U3(const double (&p)[M], const double (&q)[M],
-                                         double (&eta)[M], double dt) {
-    double const f = exp(-dt * p[0] / q[0]);
-#pragma unroll
-    for (int j = 0; j < M; ++j)
-        eta[j] += dt * p[j] / q[j];
-    return f;
-}
-
+// A chain of compile-time length M held in registers (no LDS round trip per
+// element).
 template <int M> struct RegChain {
-    double p[M], q[M], eta[M];
+    double p[M], iq[M], eta[M];
     __device__ __forceinline__ void load(const Chain &c) {
 #pragma unroll
         for (int j = 0; j < M; ++j) {
             p[j] = c.p_eta[j];
-            q[j] = c.q[j];
+            iq[j] = c.iq[j];
             eta[j] = c.eta[j];
         }
     }
@@ -213,27 +223,119 @@ template <int M> struct RegChain {
     }
 };
 
+// expx_operator for code that one lane runs, reporting an argument beyond
+// EXPX_SMALL in `general`. The branch is made uniform (readfirstlane), so the
+// path not taken is jumped over instead of issued with an empty exec mask.
+__device__ __forceinline__ void expx_flag(double x, double &expx, double &op, bool &general) {
+    if (__builtin_amdgcn_readfirstlane(fabs(x) <= EXPX_SMALL ? 1 : 0)) {
+        op = op_series(x);
+        expx = fma(-x, op, 1.);
+    } else {
+        expx = exp(-x);
+        op = (1.0 - expx) / x;
+        general = true;
+    }
+}
+
+// One U_NHC propagation (7 Suzuki-Yoshida weights; md_driver.cpp:539-561) of a
+// chain of compile-time length M in registers. A lone lane runs this, so its
+// cost is its instruction count: the inward sweep's G_j read p_{j-1} before its
+// update and are formed first, the outward sweep's exponentials read p_{j+1}
+// before its update and are formed first, and dt*(p^2/q - kT) is formed as
+// p^2*(dt/q) - dt*kT (thermostat.cpp:40-66 otherwise, element by element).
+// BARO: the coupling is p_eps^2/(2W) and U3 scales p_eps (held in kin); else
+// the coupling is cw*K + c0 (md_driver.cpp:512-531) and U3 scales K by f^2 and
+// the running velocity factor by f. Returns whether an exponential argument
+// exceeded EXPX_SMALL (took the library path).
+template <int M, bool BARO>
+__device__ __forceinline__ bool nhc_prop(double (&p)[M], double (&eta)[M], const double (&iq)[M],
+                                         double Nf, double kT, double dt, double &kin,
+                                         double &scale, double cw, double c0, double inv_W) {
+    bool general = false;
+#pragma unroll 1
+    for (int k = 0; k < 7; ++k) {
+        double const dtw = WSY7[k] * dt, dt4 = 0.5 * dtw;
+        double c[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            c[j] = dt4 * iq[j];
+        double const dtkT = dt4 * kT;
+        // U4, inward
+        double tG[M];
+        {
+            double const e = BARO ? 0.5 * (kin * kin) * inv_W : cw * kin + c0;
+            tG[0] = dt4 * (2. * e - Nf * kT);
+        }
+#pragma unroll
+        for (int j = 1; j < M; ++j)
+            tG[j] = fma(p[j - 1] * p[j - 1], c[j - 1], -dtkT);
+        p[M - 1] += tG[M - 1];
+#pragma unroll
+        for (int j = M - 2; j >= 0; --j) {
+            double e, o;
+            expx_flag(c[j + 1] * p[j + 1], e, o, general);
+            p[j] = fma(p[j], e, tG[j] * o);
+        }
+        // U3
+        {
+            double f, unused;
+            expx_flag((2. * c[0]) * p[0], f, unused, general);
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+                eta[j] = fma(2. * c[j], p[j], eta[j]);
+            if (BARO) {
+                kin *= f;
+            } else {
+                scale *= f;
+                kin *= f * f;
+            }
+        }
+        // U4, outward
+        double E[M], O[M];
+#pragma unroll
+        for (int j = 0; j + 1 < M; ++j)
+            expx_flag(c[j + 1] * p[j + 1], E[j], O[j], general);
+        double const e = BARO ? 0.5 * (kin * kin) * inv_W : cw * kin + c0;
+        double const tG0 = dt4 * (2. * e - Nf * kT);
+        if (M > 1)
+            p[0] = fma(p[0], E[0], tG0 * O[0]);
+        else
+            p[0] += tG0;
+#pragma unroll
+        for (int j = 1; j < M; ++j) {
+            double const t = fma(p[j - 1] * p[j - 1], c[j - 1], -dtkT);
+            if (j + 1 < M)
+                p[j] = fma(p[j], E[j], t * O[j]);
+            else
+                p[j] += t;
+        }
+    }
+    return general;
+}
+
 // U_NHC_thermo_propagator (md_driver.cpp:539-549) with U3 acting on a running
 // scale factor: K' = s^2 K replaces the O(N) recomputation (md_driver.cpp:504).
-template <int M> __device__ double nhc_thermo_m(DevState &s, double dt) {
+template <int M> __device__ __forceinline__ double nhc_thermo_m(DevState &s, double dt) {
     RegChain<M> c;
     c.load(s.th);
     double const kT = K_BOLTZ * s.th.T;
-    double scale = 1.;
-    for (int k = 0; k < 7; ++k) {
-        double const dtw = WSY7[k] * dt;
-        reg_U4<M>(c.p, c.q, s.th.Nf, kT, 0.5 * dtw, thermo_couple(s), false);
-        double const f = reg_U3<M>(c.p, c.q, c.eta, dtw);
-        scale *= f;
-        s.ekin_tot *= f * f;
-        reg_U4<M>(c.p, c.q, s.th.Nf, kT, 0.5 * dtw, thermo_couple(s), true);
+    double cw = 1., c0 = 0.;   // thermo_couple as cw K + c0
+    if (s.coupling != MDL_KINETIC && s.Tci != 0.) {
+        double const econf = (K_BOLTZ / s.Tci) * s.Nf / 2.;
+        cw = s.coupling == MDL_CONFIGURATIONAL ? 0. : 0.5;
+        c0 = s.coupling == MDL_CONFIGURATIONAL ? econf : 0.5 * econf;
     }
+    double kin = s.ekin_tot, scale = 1.;
+    if (nhc_prop<M, false>(c.p, c.eta, c.iq, s.th.Nf, kT, dt, kin, scale, cw, c0, 0.))
+        s.nhc_general++;
+    s.ekin_tot = kin;
     c.store(s.th);
     return scale;
 }
 
 __device__ double nhc_thermo_mem(DevState &s, double dt) {
     double scale = 1.;
+    s.nhc_general++;
     for (int k = 0; k < 7; ++k) {
         double const dtw = WSY7[k] * dt;
         chain_U4(s.th, 0.5 * dtw, thermo_couple(s), false);
@@ -246,20 +348,20 @@ __device__ double nhc_thermo_mem(DevState &s, double dt) {
 }
 
 // U_NHC_baro_propagator (md_driver.cpp:551-561)
-template <int M> __device__ void nhc_baro_m(DevState &s, double dt) {
+template <int M> __device__ __forceinline__ void nhc_baro_m(DevState &s, double dt) {
     RegChain<M> c;
     c.load(s.bth);
     double const kT = K_BOLTZ * s.bth.T;
-    for (int k = 0; k < 7; ++k) {
-        double const dtw = WSY7[k] * dt;
-        reg_U4<M>(c.p, c.q, s.bth.Nf, kT, 0.5 * dtw, 0.5 * (s.peps * s.peps) / s.bar_W, false);
-        s.peps *= reg_U3<M>(c.p, c.q, c.eta, dtw);
-        reg_U4<M>(c.p, c.q, s.bth.Nf, kT, 0.5 * dtw, 0.5 * (s.peps * s.peps) / s.bar_W, true);
-    }
+    double peps = s.peps, unused = 1.;
+    if (nhc_prop<M, true>(c.p, c.eta, c.iq, s.bth.Nf, kT, dt, peps, unused, 0., 0.,
+                          1. / s.bar_W))
+        s.nhc_general++;
+    s.peps = peps;
     c.store(s.bth);
 }
 
 __device__ void nhc_baro_mem(DevState &s, double dt) {
+    s.nhc_general++;
     for (int k = 0; k < 7; ++k) {
         double const dtw = WSY7[k] * dt;
         chain_U4(s.bth, 0.5 * dtw, 0.5 * (s.peps * s.peps) / s.bar_W, false);
@@ -268,10 +370,9 @@ __device__ void nhc_baro_mem(DevState &s, double dt) {
     }
 }
 
-#define MDL_CHAIN_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) \
-    X(13) X(14) X(15) X(16)
+#define MDL_CHAIN_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 
-__device__ double nhc_thermo(DevState &s, double dt) {
+__device__ __forceinline__ double nhc_thermo(DevState &s, double dt) {
     switch (s.th.M) {
 #define MDL_CASE(m) case m: return nhc_thermo_m<m>(s, dt);
         MDL_CHAIN_CASES(MDL_CASE)
@@ -280,7 +381,7 @@ __device__ double nhc_thermo(DevState &s, double dt) {
     }
 }
 
-__device__ void nhc_baro(DevState &s, double dt) {
+__device__ __forceinline__ void nhc_baro(DevState &s, double dt) {
     switch (s.bth.M) {
 #define MDL_CASE(m) case m: nhc_baro_m<m>(s, dt); return;
         MDL_CHAIN_CASES(MDL_CASE)
@@ -299,40 +400,56 @@ __device__ void U2p(DevState &s, double dt) {
 // long serial sequence of dependent reads and writes, which must not each pay
 // a global-memory round trip.
 static_assert(sizeof(DevState) % 8 == 0, "DevState is copied in 8-byte words");
+// Every word of the copy is loaded before any is stored, so the copy costs one
+// memory latency rather than one per loop trip. Blocks of BLOCK threads.
+constexpr int STATE_WORDS = (int)(sizeof(DevState) / 8);
+constexpr int STATE_PER_THREAD = (STATE_WORDS + BLOCK - 1) / BLOCK;
 __device__ __forceinline__ void state_load(DevState &sh, const DevState *gs) {
     const unsigned long long *src = reinterpret_cast<const unsigned long long *>(gs);
     unsigned long long *dst = reinterpret_cast<unsigned long long *>(&sh);
-    for (int k = threadIdx.x; k < (int)(sizeof(DevState) / 8); k += blockDim.x)
-        dst[k] = src[k];
+    unsigned long long r[STATE_PER_THREAD];
+#pragma unroll
+    for (int u = 0; u < STATE_PER_THREAD; ++u) {
+        int const k = threadIdx.x + u * BLOCK;
+        r[u] = k < STATE_WORDS ? src[k] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < STATE_PER_THREAD; ++u) {
+        int const k = threadIdx.x + u * BLOCK;
+        if (k < STATE_WORDS)
+            dst[k] = r[u];
+    }
     __syncthreads();
 }
 __device__ __forceinline__ void state_store(DevState *gs, const DevState &sh) {
     __syncthreads();
     const unsigned long long *src = reinterpret_cast<const unsigned long long *>(&sh);
     unsigned long long *dst = reinterpret_cast<unsigned long long *>(gs);
-    for (int k = threadIdx.x; k < (int)(sizeof(DevState) / 8); k += blockDim.x)
-        dst[k] = src[k];
+#pragma unroll
+    for (int u = 0; u < STATE_PER_THREAD; ++u) {
+        int const k = threadIdx.x + u * BLOCK;
+        if (k < STATE_WORDS)
+            dst[k] = src[k];
+    }
 }
 
 // ---------------------------------------------------------------------------
 // k_pre: first half of integrate() up to the drift (md_driver.cpp:567-581)
 // ---------------------------------------------------------------------------
-__device__ void pre_step(DevState &s, int sample);
+__device__ __forceinline__ void pre_step(DevState &s, int sample);
 
-__global__ __launch_bounds__(64) void k_pre(DevState *__restrict__ gs, int sample, int force) {
+__global__ __launch_bounds__(BLOCK) void k_pre(DevState *__restrict__ gs, int sample) {
     if (gs->err)
         return;
     __shared__ DevState s;
     state_load(s, gs);
     if (threadIdx.x == 0) {
         pre_step(s, sample);
-        if (force)
-            s.force_rebuild = 1;
     }
     state_store(gs, s);
 }
 
-__device__ void pre_step(DevState &s, int sample) {
+__device__ __forceinline__ void pre_step(DevState &s, int sample) {
     s.sample = sample;
     bool const npt = s.ensemble == MDL_NPT;
     bool const thermo = s.ensemble == MDL_NVT || npt;
@@ -378,10 +495,11 @@ __device__ void pre_step(DevState &s, int sample) {
 }
 
 // First U2 (md_driver.cpp:433-452) and U1 drift (md_driver.cpp:462-499).
+// Each block leaves the largest squared displacement of its atoms since the
+// last list build in dmax_part[blockIdx.x].
 __global__ __launch_bounds__(BLOCK) void k_kick_drift(AtomArrays a, int N,
                                                      DevState *__restrict__ gs,
-                                                     double *__restrict__ dmax_part,
-                                                     unsigned *__restrict__ ticket) {
+                                                     double *__restrict__ dmax_part) {
     if (gs->err)
         return;
     double const s = gs->kick_s, e = gs->kick_e, ch = gs->kick_c, cf = gs->drift_cf;
@@ -412,96 +530,180 @@ __global__ __launch_bounds__(BLOCK) void k_kick_drift(AtomArrays a, int N,
     if ((threadIdx.x & 63) == 0)
         sm[threadIdx.x >> 6] = dmax;
     __syncthreads();
-    __shared__ int last;
     if (threadIdx.x == 0) {
         double m = sm[0];
         for (int w = 1; w < BLOCK / 64; ++w)
             m = fmax(m, sm[w]);
         dmax_part[blockIdx.x] = m;
-        __threadfence();
-        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last)
-        return;
-    // the last block to finish reduces the per-block maxima
-    __threadfence();
-    if (threadIdx.x < 64) {
-        double m = 0.;
-        for (int b = threadIdx.x; b < (int)gridDim.x; b += 64)
-            m = fmax(m, __hip_atomic_load(&dmax_part[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        for (int off = 32; off > 0; off >>= 1)
-            m = fmax(m, __shfl_xor(m, off, 64));
-        if (threadIdx.x == 0) {
-            gs->umax2 = m;
-            *ticket = 0u;
-        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Neighbour list: one cooperative kernel per step.
+// Neighbour list (full list, both i->j and j->i, radius r_list = r_c r_skin).
 //
-// Phase 0 (every block, no communication): decide whether the list must be
-// rebuilt - forced by calculate_neighbors(), or a pair outside the list could
-// now be inside r_c. For a list built with radius r_list at box L_b, a pair
-// not in it is at least lambda r_list - 2 u_max apart now, lambda = min L/L_b,
-// u_max the largest displacement since the build. Every block reads the same
-// inputs, so all take the same branch; on most steps they all return here and
-// the kernel costs one launch.
-// On a rebuild, phases separated by grid barriers: bin atoms into cells,
-// scan the counts (block 0), scatter, sort each cell's members by index (so
-// the list order, and the FP summation order of the passes, do not depend on
-// atomic arrival order), and build each atom's full list with lane groups.
-// The grid is sized so that all blocks are resident (cooperative launch); the
-// barrier spin is bounded and reports DE_BARRIER instead of waiting forever.
+// A build bins the atoms into cells, scans the counts, scatters, sorts each
+// cell's members by atom index (so neither the list order nor the summation
+// order of the passes depends on atomic arrival order), then builds the lists
+// one cell per block: the members of the 27 surrounding cells are staged in
+// LDS as Cartesian FP32 coordinates relative to the cell centre, with the
+// periodic image fixed by the cell offset, and lane groups test them against
+// each member of the cell. The FP32 test accepts up to r_list (1 + 1e-5),
+// more than its rounding error, so the list holds every pair within r_list (a
+// few beyond it change nothing: the passes select pairs by r <= r_c in FP64).
+// Boxes with fewer than 3 cells per direction test every atom pair in FP64.
+//
+// Builds the host asks for (calculate_neighbors(), the run schedule, the
+// constructor) run as a chain of kernels, k_rb_*. Every other step runs
+// k_list: it decides whether a pair outside the list could now be inside r_c
+// - for a list built with radius r_list at box L_b, such a pair is at least
+// lambda r_list - 2 u_max apart now, lambda = min L/L_b, u_max the largest
+// displacement since the build - and on most steps every block returns there.
+// When a build is due it runs the same phases in that one kernel, separated
+// by grid barriers (its grid is no larger than what the device holds at once).
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ double wrap_scaled(double c) { return c - nint_away(c); }
+
 __device__ __forceinline__ int cell_coord(double c, int n) {
-    double const w = c - nint_away(c);   // [-0.5, 0.5]
-    int k = (int)floor((w + 0.5) * n);
+    int k = (int)floor((wrap_scaled(c) + 0.5) * n);
     return k < 0 ? 0 : (k >= n ? n - 1 : k);
 }
 
 constexpr int CELL_MAX = 1024;          // members of one cell the sort stages in LDS
+constexpr int STAGE_CAP = 2048;         // candidates staged in LDS at once
 constexpr int LIST_LANES = 8;           // lanes per atom of the list build
-constexpr long long BARRIER_SPINS = 1ll << 24;
+constexpr long long BARRIER_TICKS = 100000000ll;   // 1 s of the 100 MHz real-time counter
 
-// Sense-counting grid barrier over gridDim.x resident blocks.
-__device__ bool grid_barrier(unsigned *bar, DevState *gs) {
-    __threadfence();
-    __syncthreads();
-    __shared__ int ok;
-    if (threadIdx.x == 0) {
-        unsigned const gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned const arrived =
-            __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        ok = 1;
-        if (arrived == gridDim.x - 1) {
-            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            long long spins = 0;
-            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-                __builtin_amdgcn_s_sleep(2);
-                if (++spins > BARRIER_SPINS) {
-                    atomicOr(&gs->err, (int)DE_BARRIER);
-                    ok = 0;
-                    break;
-                }
+struct ListSmem {
+    float4 stage[STAGE_CAP];            // also the sort buffer (CELL_MAX ints)
+    int nb_start[27], nb_off[28];
+    double nb_shift[27][3];
+    int sc[BLOCK];
+    unsigned long long entries;
+};
+static_assert(CELL_MAX * sizeof(int) <= sizeof(float4) * STAGE_CAP, "sort buffer in stage");
+
+// Start of a build (one thread): list bookkeeping and the cell-size check.
+__device__ __forceinline__ void rb_begin(DevState *gs, int use_cells) {
+    gs->force_rebuild = 0;
+    gs->do_rebuild = 1;
+    gs->nbuilds++;
+    for (int c = 0; c < 3; ++c)
+        gs->L_build[c] = gs->L[c];
+    if (use_cells)
+        for (int c = 0; c < 3; ++c)
+            if (gs->L[c] / gs->ncell[c] < gs->r_list) {
+                atomicOr(&gs->err, (int)DE_CELL_GRID);
+                gs->err_val[0] = gs->L[c];
             }
-        }
-    }
-    __syncthreads();
-    __threadfence();
-    return ok != 0;
 }
 
+// Lanes of a wave holding the same key add their count with one atomic; each
+// active lane gets base[key] before the add plus its rank among those lanes.
+// Called by every lane of the wave.
+__device__ __forceinline__ int wave_add(int *base, int key, bool active) {
+    int const l64 = threadIdx.x & 63;
+    unsigned long long const below = (l64 == 0) ? 0ull : (~0ull >> (64 - l64));
+    unsigned long long pending = __ballot(active);
+    int res = 0;
+    while (pending) {
+        int const leader = __ffsll((long long)pending) - 1;
+        int const lkey = __shfl(key, leader, 64);
+        unsigned long long const same = __ballot(active && key == lkey) & pending;
+        int old = 0;
+        if (l64 == leader)
+            old = atomicAdd(&base[lkey], __popcll(same));
+        old = __shfl(old, leader, 64);
+        if ((same >> l64) & 1ull)
+            res = old + __popcll(same & below);
+        pending &= ~same;
+    }
+    return res;
+}
+
+// Atoms of block slots [first, N) step `step` (multiples of BLOCK).
+__device__ __forceinline__ void rb_bin(const AtomArrays &a, ListArrays &l, int N,
+                                       const DevState *gs, int first, int step) {
+    int const nx = gs->ncell[0], ny = gs->ncell[1], nz = gs->ncell[2];
+    for (int b = first; b < N; b += step) {
+        int const i = b + threadIdx.x;
+        bool const active = i < N;
+        int cid = 0;
+        if (active) {
+            double4 const c = a.pos[i];
+            cid = (cell_coord(c.z, nz) * ny + cell_coord(c.y, ny)) * nx + cell_coord(c.x, nx);
+            l.cell_of[i] = cid;
+        }
+        wave_add(l.cell_count, cid, active);
+    }
+}
+
+// One block: exclusive scan of the counts into cell_start; zero counts, fills.
+__device__ __forceinline__ void rb_scan(ListArrays &l, int *sc) {
+    int const per = (l.ncells + BLOCK - 1) / BLOCK;
+    int const b0 = min(l.ncells, (int)threadIdx.x * per), e0 = min(l.ncells, b0 + per);
+    int s = 0;
+    for (int k = b0; k < e0; ++k)
+        s += l.cell_count[k];
+    sc[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < BLOCK; off <<= 1) {
+        int const t = threadIdx.x >= off ? sc[threadIdx.x - off] : 0;
+        __syncthreads();
+        sc[threadIdx.x] += t;
+        __syncthreads();
+    }
+    int run = sc[threadIdx.x] - s;
+    for (int k = b0; k < e0; ++k) {
+        int const v = l.cell_count[k];
+        l.cell_start[k] = run;
+        run += v;
+        l.cell_count[k] = 0;
+        l.cell_fill[k] = 0;
+    }
+    if (threadIdx.x == BLOCK - 1)
+        l.cell_start[l.ncells] = sc[BLOCK - 1];
+}
+
+__device__ __forceinline__ void rb_scatter(ListArrays &l, int N, int first, int step) {
+    for (int b = first; b < N; b += step) {
+        int const i = b + threadIdx.x;
+        bool const active = i < N;
+        int const cid = active ? l.cell_of[i] : 0;
+        int const slot = wave_add(l.cell_fill, cid, active);
+        if (active)
+            l.cell_atoms[l.cell_start[cid] + slot] = i;
+    }
+}
+
+// One block: members of cell c in ascending atom order (each placed at its rank).
+__device__ __forceinline__ void rb_sort_cell(ListArrays &l, int c, int *v, DevState *gs) {
+    int const b = l.cell_start[c], n = l.cell_start[c + 1] - b;
+    if (n > CELL_MAX) {
+        if (threadIdx.x == 0) {
+            atomicOr(&gs->err, (int)DE_CELL_GRID);
+            gs->err_val[0] = (double)n;
+        }
+        return;   // uniform in the block
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += BLOCK)
+        v[k] = l.cell_atoms[b + k];
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += BLOCK) {
+        int const x = v[k];
+        int rank = 0;
+        for (int q = 0; q < n; ++q)
+            rank += v[q] < x;
+        l.cell_atoms[b + rank] = x;
+    }
+}
+
+// Per-atom end of a list build; entries are summed per block in *blk.
 __device__ __forceinline__ void list_finish(ListArrays &l, AtomArrays &a, int i, int cnt,
-                                            DevState *gs) {
+                                            DevState *gs, unsigned long long *blk) {
     l.nnbr[i] = min(cnt, l.stride);
     a.cref[i] = a.pos[i];
-    atomicAdd(reinterpret_cast<unsigned long long *>(&gs->nlist_entries),
-              (unsigned long long)min(cnt, l.stride));
+    atomicAdd(blk, (unsigned long long)min(cnt, l.stride));
     if (cnt > l.stride) {
         atomicOr(&gs->err, (int)DE_NBR_OVERFLOW);
         atomicMax(reinterpret_cast<unsigned long long *>(&gs->nbr_needed),
@@ -509,180 +711,304 @@ __device__ __forceinline__ void list_finish(ListArrays &l, AtomArrays &a, int i,
     }
 }
 
-// Append the members of [b, e) of cell_atoms (or atoms b..e-1 when
-// cell_atoms is null) within r_list of atom i, LIST_LANES lanes per atom, in
-// member order.
-__device__ __forceinline__ void list_scan_range(const AtomArrays &a, const ListArrays &l,
-                                                const int *members, int b, int e, int i,
-                                                const double4 &ci, const double *L, double rl2,
-                                                int lane, unsigned long long gmask,
-                                                unsigned long long below, int *row, int &cnt) {
-    for (int k0 = b; k0 < e; k0 += LIST_LANES) {
-        int const k = k0 + lane;
-        bool in = false;
-        int j = 0;
-        if (k < e) {
-            j = members ? members[k] : k;
-            if (j != i) {
-                double dx, dy, dz;
-                min_image(ci, a.pos[j], L, dx, dy, dz);
-                in = dx * dx + dy * dy + dz * dz <= rl2;
-            }
-        }
-        unsigned long long const bal = __ballot(in) & gmask;
-        if (in) {
-            int const slot = cnt + __popcll(bal & below);
-            if (slot < l.stride)
-                row[slot] = j;
-        }
-        cnt += __popcll(bal);
+// Ordered append of the lanes' hits: within a lane group the entries keep the
+// candidate order (ballot prefix).
+__device__ __forceinline__ void list_append(bool in, int j, unsigned long long gmask,
+                                            unsigned long long below, int stride, int *row,
+                                            int &cnt) {
+    unsigned long long const bal = __ballot(in) & gmask;
+    if (in) {
+        int const slot = cnt + __popcll(bal & below);
+        if (slot < stride)
+            row[slot] = j;
     }
+    cnt += __popcll(bal);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_list(AtomArrays a, ListArrays l, int N,
-                                               DevState *__restrict__ gs,
-                                               unsigned *__restrict__ bar, int use_cells) {
-    if (gs->err)
-        return;
-    // phase 0: the decision, taken identically by every block from the same
-    // inputs (u_max^2 was reduced by k_kick_drift)
-    double lam = 1e300;
-    for (int c = 0; c < 3; ++c)
-        lam = fmin(lam, gs->L[c] / gs->L_build[c]);
-    bool const need = gs->force_rebuild || !(lam * gs->r_list - 2. * sqrt(gs->umax2) > gs->rc);
-    if (!need)
-        return;
-    bool ok = grid_barrier(bar, gs);   // everyone has read the decision inputs
-    int const nthreads = gridDim.x * BLOCK;
-    int const gtid = blockIdx.x * BLOCK + threadIdx.x;
-    double const L[3] = {gs->L[0], gs->L[1], gs->L[2]};
-    double const rl2 = gs->r_list * gs->r_list;
+// One block: the lists of the members of cell c.
+__device__ __forceinline__ void rb_build_cell(AtomArrays &a, ListArrays &l, int c,
+                                              DevState *gs, const double *L, double r_list,
+                                              ListSmem &sm) {
     int const nx = gs->ncell[0], ny = gs->ncell[1], nz = gs->ncell[2];
-    if (gtid == 0) {
-        gs->force_rebuild = 0;
-        gs->do_rebuild = 1;
-        gs->nbuilds++;
-        gs->nlist_entries = 0;
-        for (int c = 0; c < 3; ++c)
-            gs->L_build[c] = gs->L[c];
-        if (use_cells)
-            for (int c = 0; c < 3; ++c)
-                if (gs->L[c] / gs->ncell[c] < gs->r_list) {
-                    atomicOr(&gs->err, (int)DE_CELL_GRID);
-                    gs->err_val[0] = gs->L[c];
-                }
-    }
     int const lane = threadIdx.x % LIST_LANES;
     int const l64 = threadIdx.x & 63;
     unsigned long long const gmask = group_mask<LIST_LANES>();
     unsigned long long const below = (l64 == 0) ? 0ull : (~0ull >> (64 - l64));
-    int const groups = nthreads / LIST_LANES;
-    if (use_cells) {
-        // phase 1: bin
-        for (int i = gtid; i < N; i += nthreads) {
-            double4 const c = a.pos[i];
-            int const cid = (cell_coord(c.z, nz) * ny + cell_coord(c.y, ny)) * nx +
-                            cell_coord(c.x, nx);
-            l.cell_of[i] = cid;
-            atomicAdd(&l.cell_count[cid], 1);
-        }
-        ok = grid_barrier(bar, gs) && ok;
-        // phase 2: exclusive scan of the counts (block 0); zero counts and fills
-        if (blockIdx.x == 0) {
-            __shared__ int sm[BLOCK];
-            __shared__ int carry;
-            if (threadIdx.x == 0)
-                carry = 0;
-            __syncthreads();
-            for (int base = 0; base < l.ncells; base += BLOCK) {
-                int const k = base + threadIdx.x;
-                int const v = k < l.ncells ? l.cell_count[k] : 0;
-                sm[threadIdx.x] = v;
-                __syncthreads();
-                for (int off = 1; off < BLOCK; off <<= 1) {
-                    int const t = threadIdx.x >= off ? sm[threadIdx.x - off] : 0;
-                    __syncthreads();
-                    sm[threadIdx.x] += t;
-                    __syncthreads();
-                }
-                if (k < l.ncells) {
-                    l.cell_start[k] = carry + sm[threadIdx.x] - v;
-                    l.cell_count[k] = 0;
-                    l.cell_fill[k] = 0;
-                }
-                __syncthreads();
-                if (threadIdx.x == BLOCK - 1)
-                    carry += sm[BLOCK - 1];
-                __syncthreads();
-            }
-            if (threadIdx.x == 0)
-                l.cell_start[l.ncells] = carry;
-        }
-        ok = grid_barrier(bar, gs) && ok;
-        // phase 3: scatter
-        for (int i = gtid; i < N; i += nthreads) {
-            int const cid = l.cell_of[i];
-            int const slot = atomicAdd(&l.cell_fill[cid], 1);
-            l.cell_atoms[l.cell_start[cid] + slot] = i;
-        }
-        ok = grid_barrier(bar, gs) && ok;
-        // phase 4: one block per cell places each member at its rank
-        {
-            __shared__ int v[CELL_MAX];
-            for (int c = blockIdx.x; c < l.ncells; c += gridDim.x) {
-                int const b = l.cell_start[c], n = l.cell_start[c + 1] - b;
-                if (n > CELL_MAX) {
-                    if (threadIdx.x == 0) {
-                        atomicOr(&gs->err, (int)DE_CELL_GRID);
-                        gs->err_val[0] = (double)n;
-                    }
-                    continue;   // uniform in the block
-                }
-                for (int k = threadIdx.x; k < n; k += BLOCK)
-                    v[k] = l.cell_atoms[b + k];
-                __syncthreads();
-                for (int k = threadIdx.x; k < n; k += BLOCK) {
-                    int const x = v[k];
-                    int rank = 0;
-                    for (int m = 0; m < n; ++m)
-                        rank += v[m] < x;
-                    l.cell_atoms[b + rank] = x;
-                }
-                __syncthreads();
-            }
-        }
-        ok = grid_barrier(bar, gs) && ok;
-        if (!ok)
-            return;
-        // phase 5: lists from the 27 surrounding cells, in (cell, member) order
-        for (int i = gtid / LIST_LANES; i < N; i += groups) {
-            double4 const ci = a.pos[i];
-            int const cx = cell_coord(ci.x, nx), cy = cell_coord(ci.y, ny), cz = cell_coord(ci.z, nz);
-            int *row = l.nbr + (size_t)i * l.stride;
-            int cnt = 0;
-            for (int q = 0; q < 27; ++q) {
-                int const dz = q / 9 - 1, dy = (q / 3) % 3 - 1, dx = q % 3 - 1;
-                int const z = (cz + dz + nz) % nz, y = (cy + dy + ny) % ny, x = (cx + dx + nx) % nx;
-                int const cell = (z * ny + y) * nx + x;
-                list_scan_range(a, l, l.cell_atoms, l.cell_start[cell], l.cell_start[cell + 1], i,
-                                ci, L, rl2, lane, gmask, below, row, cnt);
-            }
-            if (lane == 0)
-                list_finish(l, a, i, cnt, gs);
-        }
-    } else {
-        if (!ok)
-            return;
-        // small boxes (< 3 cells per direction): test every other atom
-        for (int i = gtid / LIST_LANES; i < N; i += groups) {
-            double4 const ci = a.pos[i];
-            int *row = l.nbr + (size_t)i * l.stride;
-            int cnt = 0;
-            list_scan_range(a, l, nullptr, 0, N, i, ci, L, rl2, lane, gmask, below, row, cnt);
-            if (lane == 0)
-                list_finish(l, a, i, cnt, gs);
-        }
+    float const rl2f = (float)(r_list * r_list * (1. + 1e-5));
+    int const cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+    double const scx = (cx + 0.5) / nx - 0.5, scy = (cy + 0.5) / ny - 0.5,
+                 scz = (cz + 0.5) / nz - 0.5;
+    __syncthreads();   // the previous cell's staging is consumed
+    if (threadIdx.x < 27) {
+        int const q = threadIdx.x;
+        int x = cx + q % 3 - 1, y = cy + (q / 3) % 3 - 1, z = cz + q / 9 - 1;
+        double shx = 0., shy = 0., shz = 0.;
+        if (x < 0) { x += nx; shx = -1.; } else if (x >= nx) { x -= nx; shx = 1.; }
+        if (y < 0) { y += ny; shy = -1.; } else if (y >= ny) { y -= ny; shy = 1.; }
+        if (z < 0) { z += nz; shz = -1.; } else if (z >= nz) { z -= nz; shz = 1.; }
+        int const cell = (z * ny + y) * nx + x;
+        sm.nb_start[q] = l.cell_start[cell];
+        sm.nb_off[q + 1] = l.cell_start[cell + 1] - l.cell_start[cell];
+        sm.nb_shift[q][0] = shx;
+        sm.nb_shift[q][1] = shy;
+        sm.nb_shift[q][2] = shz;
     }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        sm.nb_off[0] = 0;
+        for (int q = 0; q < 27; ++q)
+            sm.nb_off[q + 1] += sm.nb_off[q];
+    }
+    __syncthreads();
+    int const T = sm.nb_off[27];
+    int const cb = l.cell_start[c], mc = l.cell_start[c + 1] - cb;
+    int const nchunks = (T + STAGE_CAP - 1) / STAGE_CAP;
+    for (int r0 = 0; r0 < mc; r0 += BLOCK / LIST_LANES) {
+        int const k = r0 + threadIdx.x / LIST_LANES;
+        bool const active = k < mc;
+        int const i = active ? l.cell_atoms[cb + k] : -1;
+        float xi = 0.f, yi = 0.f, zi = 0.f;
+        if (active) {
+            double4 const p = a.pos[i];
+            xi = (float)((wrap_scaled(p.x) - scx) * L[0]);
+            yi = (float)((wrap_scaled(p.y) - scy) * L[1]);
+            zi = (float)((wrap_scaled(p.z) - scz) * L[2]);
+        }
+        int *row = l.nbr + (size_t)(active ? i : 0) * l.stride;
+        int cnt = 0;
+        for (int ch = 0; ch < nchunks; ++ch) {
+            int const t0 = ch * STAGE_CAP, tn = min(T - t0, STAGE_CAP);
+            if (nchunks > 1 || r0 == 0) {
+                __syncthreads();
+                for (int t = threadIdx.x; t < tn; t += BLOCK) {
+                    int const tt = t0 + t;
+                    int q = 0;
+                    while (sm.nb_off[q + 1] <= tt)
+                        ++q;
+                    int const j = l.cell_atoms[sm.nb_start[q] + (tt - sm.nb_off[q])];
+                    double4 const p = a.pos[j];
+                    sm.stage[t] = make_float4(
+                        (float)((wrap_scaled(p.x) + sm.nb_shift[q][0] - scx) * L[0]),
+                        (float)((wrap_scaled(p.y) + sm.nb_shift[q][1] - scy) * L[1]),
+                        (float)((wrap_scaled(p.z) + sm.nb_shift[q][2] - scz) * L[2]),
+                        __int_as_float(j));
+                }
+                __syncthreads();
+            }
+            for (int u0 = 0; u0 < tn; u0 += LIST_LANES) {
+                int const u = u0 + lane;
+                bool in = false;
+                int j = 0;
+                if (active && u < tn) {
+                    float4 const p = sm.stage[u];
+                    j = __float_as_int(p.w);
+                    float const dx = p.x - xi, dy = p.y - yi, dz = p.z - zi;
+                    in = j != i && dx * dx + dy * dy + dz * dz <= rl2f;
+                }
+                list_append(in, j, gmask, below, l.stride, row, cnt);
+            }
+        }
+        if (active && lane == 0)
+            list_finish(l, a, i, cnt, gs, &sm.entries);
+    }
+}
+
+// Lane groups over atoms [first group, N) step `groups`: every other atom,
+// FP64 minimum image (boxes with fewer than 3 cells per direction).
+__device__ __forceinline__ void rb_build_brute(AtomArrays &a, ListArrays &l, int N,
+                                               DevState *gs, const double *L, double r_list,
+                                               int group0, int groups,
+                                               unsigned long long *blk) {
+    int const lane = threadIdx.x % LIST_LANES;
+    int const l64 = threadIdx.x & 63;
+    unsigned long long const gmask = group_mask<LIST_LANES>();
+    unsigned long long const below = (l64 == 0) ? 0ull : (~0ull >> (64 - l64));
+    double const rl2 = r_list * r_list;
+    for (int i = group0; i < N; i += groups) {
+        double4 const ci = a.pos[i];
+        int *row = l.nbr + (size_t)i * l.stride;
+        int cnt = 0;
+        for (int k0 = 0; k0 < N; k0 += LIST_LANES) {
+            int const j = k0 + lane;
+            bool in = false;
+            if (j < N && j != i) {
+                double dx, dy, dz;
+                min_image(ci, a.pos[j], L, dx, dy, dz);
+                in = dx * dx + dy * dy + dz * dz <= rl2;
+            }
+            list_append(in, j, gmask, below, l.stride, row, cnt);
+        }
+        if (lane == 0)
+            list_finish(l, a, i, cnt, gs, blk);
+    }
+}
+
+__device__ __forceinline__ void rb_flush_entries(DevState *gs, unsigned long long &blk) {
+    __syncthreads();
+    if (threadIdx.x == 0 && blk)
+        atomicAdd(reinterpret_cast<unsigned long long *>(&gs->nlist_entries), blk);
+}
+
+// --- the chain of a requested build ----------------------------------------
+// k_rb_begin (brute builds) or k_rb_bin (cell builds, block 0 begins the
+// build) -> k_rb_scan -> k_rb_scatter -> k_rb_sort -> k_rb_build.
+__global__ __launch_bounds__(64) void k_rb_begin(DevState *__restrict__ gs) {
+    if (gs->err || threadIdx.x != 0)
+        return;
+    rb_begin(gs, 0);
+    gs->nlist_entries = 0;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rb_bin(AtomArrays a, ListArrays l, int N,
+                                                 DevState *__restrict__ gs) {
+    if (gs->err)
+        return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        rb_begin(gs, 1);
+        gs->nlist_entries = 0;
+    }
+    rb_bin(a, l, N, gs, blockIdx.x * BLOCK, gridDim.x * BLOCK);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rb_scan(ListArrays l, const DevState *__restrict__ gs) {
+    if (gs->err)
+        return;
+    __shared__ int sc[BLOCK];
+    rb_scan(l, sc);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rb_scatter(ListArrays l, int N,
+                                                     const DevState *__restrict__ gs) {
+    if (gs->err)
+        return;
+    rb_scatter(l, N, blockIdx.x * BLOCK, gridDim.x * BLOCK);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rb_sort(ListArrays l, DevState *__restrict__ gs) {
+    if (gs->err)
+        return;
+    __shared__ int v[CELL_MAX];
+    rb_sort_cell(l, blockIdx.x, v, gs);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rb_build(AtomArrays a, ListArrays l, int N,
+                                                   DevState *__restrict__ gs, int use_cells) {
+    if (gs->err)
+        return;
+    __shared__ ListSmem sm;
+    if (threadIdx.x == 0)
+        sm.entries = 0ull;
+    double const L[3] = {gs->L[0], gs->L[1], gs->L[2]};
+    double const r_list = gs->r_list;
+    if (use_cells)
+        rb_build_cell(a, l, blockIdx.x, gs, L, r_list, sm);
+    else
+        rb_build_brute(a, l, N, gs, L, r_list, (blockIdx.x * BLOCK + threadIdx.x) / LIST_LANES,
+                       gridDim.x * BLOCK / LIST_LANES, &sm.entries);
+    rb_flush_entries(gs, sm.entries);
+}
+
+// --- the per-step check, with the build in one kernel when one is due ------
+// Generation-counting grid barrier over gridDim.x resident blocks. Waiting
+// blocks poll with relaxed loads (an acquiring poll would invalidate the
+// caches on every iteration) and fence once when released. The wait is
+// bounded: past BARRIER_TICKS the block reports DE_BARRIER and goes on.
+__device__ bool grid_barrier(unsigned *bar, DevState *gs) {
+    __syncthreads();
+    __shared__ int ok;
+    if (threadIdx.x == 0) {
+        ok = 1;
+        __threadfence();
+        unsigned const gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned const arrived =
+            __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == gridDim.x - 1) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            long long const t0 = wall_clock64();
+            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                __builtin_amdgcn_s_sleep(8);
+                if (wall_clock64() - t0 > BARRIER_TICKS) {
+                    atomicOr(&gs->err, (int)DE_BARRIER);
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+    return ok != 0;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_list(AtomArrays a, ListArrays l, int N,
+                                               DevState *__restrict__ gs,
+                                               unsigned *__restrict__ bar,
+                                               const double *__restrict__ dmax_part, int nparts,
+                                               int use_cells) {
+    if (gs->err)
+        return;
+    // the decision, from the same inputs in every block (u_max^2 from
+    // k_kick_drift's per-block maxima, nparts <= 4 BLOCK)
+    __shared__ double red[BLOCK / 64];
+    double m = 0.;
+    {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            int const b = threadIdx.x + u * BLOCK;
+            v[u] = b < nparts ? dmax_part[b] : 0.;
+        }
+        m = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        m = fmax(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    m = red[0];
+    for (int w = 1; w < BLOCK / 64; ++w)
+        m = fmax(m, red[w]);
+    double lam = 1e300;
+    for (int c = 0; c < 3; ++c)
+        lam = fmin(lam, gs->L[c] / gs->L_build[c]);
+    bool const need = gs->force_rebuild || !(lam * gs->r_list - 2. * sqrt(m) > gs->rc);
+    if (!need)
+        return;
+
+    __shared__ ListSmem sm;
+    if (threadIdx.x == 0) {
+        sm.entries = 0ull;
+        if (blockIdx.x == 0)
+            gs->nlist_entries = 0;   // not a decision input; summed only after barrier 1
+    }
+    bool ok = grid_barrier(bar, gs);   // every block has read the decision inputs
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        rb_begin(gs, use_cells);
+    double const L[3] = {gs->L[0], gs->L[1], gs->L[2]};
+    double const r_list = gs->r_list;
+    int const nthreads = gridDim.x * BLOCK;
+    if (!use_cells) {
+        if (ok)
+            rb_build_brute(a, l, N, gs, L, r_list, (blockIdx.x * BLOCK + threadIdx.x) / LIST_LANES,
+                           nthreads / LIST_LANES, &sm.entries);
+    } else {
+        rb_bin(a, l, N, gs, blockIdx.x * BLOCK, nthreads);
+        ok = grid_barrier(bar, gs) && ok;
+        if (blockIdx.x == 0)
+            rb_scan(l, sm.sc);
+        ok = grid_barrier(bar, gs) && ok;
+        rb_scatter(l, N, blockIdx.x * BLOCK, nthreads);
+        ok = grid_barrier(bar, gs) && ok;
+        for (int c = blockIdx.x; c < l.ncells; c += gridDim.x)
+            rb_sort_cell(l, c, reinterpret_cast<int *>(sm.stage), gs);
+        ok = grid_barrier(bar, gs) && ok;
+        if (ok)
+            for (int c = blockIdx.x; c < l.ncells; c += gridDim.x)
+                rb_build_cell(a, l, c, gs, L, r_list, sm);
+    }
+    rb_flush_entries(gs, sm.entries);
 }
 
 // ---------------------------------------------------------------------------
@@ -818,9 +1144,9 @@ __global__ __launch_bounds__(BLOCK) void k_density(AtomArrays a, ListArrays l, R
 // elasticity and the Laplacian pair terms (md_driver.cpp:300-430), over the
 // compacted in-cut-off list. Each pair is visited from both atoms (full list):
 // forces and energies are gathered without atomics; the pair sums carry a
-// factor 2, removed in k_post. The pair Born term X d_a d_b d_c d_d is fully
+// factor 2, removed in post_step. The pair Born term X d_a d_b d_c d_d is fully
 // symmetric in its four indices, so its 21 Voigt entries are 15 distinct
-// monomials of d, accumulated here and expanded in k_post.
+// monomials of d, accumulated here and expanded in post_step.
 // ---------------------------------------------------------------------------
 template <int KIND, int LPA, bool SAMPLE, bool EAM>
 __global__ __launch_bounds__(BLOCK) void k_force(AtomArrays a, ListArrays l, RTables rt, int N,
@@ -1010,15 +1336,15 @@ __global__ __launch_bounds__(BLOCK) void k_kick2(AtomArrays a, int N,
 }
 
 // Deterministic sums of the per-block partial records, one block per
-// component (thread t adds blocks t, t+BLOCK, ... in order, then a fixed
-// tree); the last block to finish then runs the scalar part of the step:
+// component (thread t loads blocks t, t+BLOCK, ..., all summed in fixed
+// trees); the last block to finish then runs the scalar part of the step:
 // post_step (mode 0 = constructor, md_driver.cpp:114-118; mode 1 = end of
 // integrate(), md_driver.cpp:583-600) and, when next_sample >= 0, the first
 // half of the next step (pre_step), which needs nothing in between.
 // Components are numbered A (PA_N), then B (PB_N), then K (PK_N).
 constexpr int NTOTAL = PA_N + PB_N + PK_N;
 
-__device__ void post_step(DevState &s, const double *tA, const double *tB, const double *tK,
+__device__ __forceinline__ void post_step(DevState &s, const double *tA, const double *tB, const double *tK,
                           int mode, double *__restrict__ rec_out);
 
 __global__ __launch_bounds__(BLOCK) void k_finish(DevState *__restrict__ gs,
@@ -1027,8 +1353,7 @@ __global__ __launch_bounds__(BLOCK) void k_finish(DevState *__restrict__ gs,
                                                  const double *__restrict__ pK, int nK,
                                                  double *__restrict__ totals,
                                                  unsigned *__restrict__ ticket, int mode,
-                                                 double *__restrict__ rec_out, int next_sample,
-                                                 int next_force) {
+                                                 double *__restrict__ rec_out, int next_sample) {
     if (gs->err)
         return;
     int k = blockIdx.x;
@@ -1046,9 +1371,20 @@ __global__ __launch_bounds__(BLOCK) void k_finish(DevState *__restrict__ gs,
         k -= PA_N + PB_N;
         nb = nK;
     }
-    double sum = 0.;
-    for (int b = threadIdx.x; b < nb; b += BLOCK)
-        sum += p[(size_t)k * PART_STRIDE + b];
+    // nb <= PART_STRIDE: every partial of a thread is loaded before the adds
+    constexpr int PER = PART_STRIDE / BLOCK;
+    double v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        int const b = threadIdx.x + u * BLOCK;
+        v[u] = b < nb ? p[(size_t)k * PART_STRIDE + b] : 0.;
+    }
+#pragma unroll
+    for (int w = 1; w < PER; w <<= 1)
+#pragma unroll
+        for (int u = 0; u + w < PER; u += 2 * w)
+            v[u] += v[u + w];
+    double sum = v[0];
     for (int off = 32; off > 0; off >>= 1)
         sum += __shfl_xor(sum, off, 64);
     __shared__ double sm[BLOCK / 64];
@@ -1075,17 +1411,14 @@ __global__ __launch_bounds__(BLOCK) void k_finish(DevState *__restrict__ gs,
     state_load(s, gs);
     if (threadIdx.x == 0) {
         post_step(s, tot, tot + PA_N, tot + PA_N + PB_N, mode, rec_out);
-        if (next_sample >= 0 && !s.err) {
+        if (next_sample >= 0 && !s.err)
             pre_step(s, next_sample);
-            if (next_force)
-                s.force_rebuild = 1;
-        }
         *ticket = 0u;
     }
     state_store(gs, s);
 }
 
-__device__ void post_step(DevState &s, const double *tA, const double *tB, const double *tK,
+__device__ __forceinline__ void post_step(DevState &s, const double *tA, const double *tB, const double *tK,
                           int mode, double *__restrict__ rec_out) {
     bool const sample = s.sample != 0;
     bool const npt = s.ensemble == MDL_NPT;
@@ -1227,6 +1560,7 @@ __device__ void post_step(DevState &s, const double *tA, const double *tB, const
     rec[MDL_S_WARN_VIRIAL] = (double)s.warn_virial;
     rec[MDL_S_WARN_RMIN] = (double)s.warn_rmin;
     rec[MDL_S_NLIST_ENTRIES] = (double)s.nlist_entries;
+    rec[MDL_S_NHC_GENERAL] = (double)s.nhc_general;
     if (rec_out && sample)
         for (int k = 0; k < MDL_NRECORD; ++k)
             rec_out[k] = rec[k];

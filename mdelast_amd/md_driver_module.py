@@ -687,6 +687,12 @@ class MDDriver:
         return int(self._scalars()[nat.S_NLIST_ENTRIES])
 
     @property
+    def nhc_general_path_count(self):
+        """Chain propagations that took the general (library exp) path: a chain
+        argument dt p/q beyond 1/8 (diagnostic; no counterpart in the reference)."""
+        return int(self._scalars()[nat.S_NHC_GENERAL])
+
+    @property
     def warnings(self):
         s = self._scalars()
         return {"virial": int(s[nat.S_WARN_VIRIAL]), "rmin": int(s[nat.S_WARN_RMIN])}

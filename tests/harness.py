@@ -16,8 +16,9 @@ ENS = {"NVE": 0, "NVT": 1, "NPT": 2}
 
 
 def md_setup(cell, *, T=300.0, tau_T=5e-14, tau_P=5e-13, P=1e-10 * U.GPa, M=3, Mb=3, seed=0,
-             velocities="maxwell", static=False):
-    """Velocities and chain parameters as the reference's MolecularDynamics builds them."""
+             velocities="maxwell", static=False, tau_B=None):
+    """Velocities and chain parameters as the reference's MolecularDynamics builds them.
+    tau_B: time constant of the barostat's chain (default tau_P, as the reference)."""
     N = len(cell["pos"])
     Nf = 3.0 * (N - 1.0)
     if static:
@@ -35,7 +36,7 @@ def md_setup(cell, *, T=300.0, tau_T=5e-14, tau_P=5e-13, P=1e-10 * U.GPa, M=3, M
     q = np.ones(M) * tau_T ** 2 * kT
     q[0] *= Nf
     p_eta = rs.normal(0.0, np.sqrt(q * kT))
-    qb = np.ones(Mb) * tau_P ** 2 * kT
+    qb = np.ones(Mb) * (tau_P if tau_B is None else tau_B) ** 2 * kT
     p_eta_b = rs.normal(0.0, np.sqrt(qb * kT))
     W = Nf * tau_P ** 2 * kT
     p_eps = rs.normal(0.0, np.sqrt(W * kT))

@@ -111,9 +111,9 @@ def test_table_eval_vs_oracle(interp):
             assert np.max(np.abs(a - b)) / np.max(np.abs(b)) < tol
 
 
-def _traj_compare(ensemble, nsteps, interp="quintic", neighbors=True):
+def _traj_compare(ensemble, nsteps, interp="quintic", neighbors=True, chain_kw=None):
     cell = cells.zn06_cell(5)
-    setup = H.md_setup(cell, T=300.0, M=3, Mb=3, seed=0)
+    setup = H.md_setup(cell, T=300.0, M=3, Mb=3, seed=0, **(chain_kw or {}))
     kw = dict(interpolation=interp, ensemble=ensemble, dt=2e-15, neighbors_list=neighbors,
               r_skin=1.2)
     o = H.oracle_md(cell, setup, **kw)
@@ -142,15 +142,37 @@ def _traj_compare(ensemble, nsteps, interp="quintic", neighbors=True):
     po, eo = o.chain(0)
     th = g.thermostat
     worst["p_eta"] = float(np.max(np.abs(np.array(th.p_eta) - po) / np.abs(po).max()))
-    return worst
+    if ensemble == "NPT":
+        pb, _ = o.chain(1)
+        worst["p_eta_baro"] = float(np.max(np.abs(np.array(g.barostat.thermostat.p_eta) - pb)
+                                           / np.abs(pb).max()))
+    return worst, g
 
 
 @pytest.mark.parametrize("ensemble", ["NVE", "NVT", "NPT"])
 def test_trajectory_vs_oracle(ensemble):
     # 40 steps of 2 fs on the 500-atom Zn06 cell; FP round-off differences grow
-    # slowly over so short a window, so all state agrees to ~1e-11.
-    worst = _traj_compare(ensemble, 40)
-    print(ensemble, worst)
+    # slowly over so short a window, so all state agrees to ~1e-11. (The cell
+    # starts on its lattice, so the chains are driven hard and some chain
+    # arguments dt p/q exceed 1/8.)
+    worst, g = _traj_compare(ensemble, 40)
+    print(ensemble, worst, g.nhc_general_path_count)
+    for k, v in worst.items():
+        assert v < 1e-10, (k, v)
+
+
+@pytest.mark.parametrize("ensemble", ["NVT", "NPT"])
+@pytest.mark.parametrize("path", ["series", "general"])
+def test_trajectory_chain_paths(ensemble, path):
+    # tau = 500 fs keeps every chain argument dt p/q far below 1/8 (series path
+    # only); tau = 4 fs puts many beyond (library exp and quotient)
+    tau = 5e-13 if path == "series" else 4e-15
+    worst, g = _traj_compare(ensemble, 20, chain_kw=dict(tau_T=tau, tau_B=tau))
+    print(ensemble, path, worst, g.nhc_general_path_count)
+    if path == "series":
+        assert g.nhc_general_path_count == 0
+    else:
+        assert g.nhc_general_path_count > 0
     for k, v in worst.items():
         assert v < 1e-10, (k, v)
 
