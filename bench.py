@@ -11,11 +11,14 @@ accumulated every step, neighbour list r_skin 1.2 rebuilt every 20 steps
 A step is one full MD step (md_driver.cpp:563-601 of the reference): NHC,
 kick, drift, list check/build, density pass, force pass, kick, NHC and
 thermodynamics. value = atoms * steps / wall time of the timed steps, inputs
-resident in HBM. The line also carries the force kernel's roofline position
-(algorithmic bytes / measured kernel time, HIP events on the driver's stream)
-and the CPU oracle timed on a bounded sample of the same workload on one core.
+resident in HBM. Default arithmetic: mixed (FP32 tables and pair algebra,
+FP64 positions, chains and accumulators); --precision fp64 runs the all-FP64
+path. The line also carries the force kernel's roofline position (algorithmic
+bytes / measured kernel time, HIP events on the driver's stream; HBM traffic
+from the committed rocprofv3 --pmc summary) and the CPU oracle timed on a
+bounded sample of the same workload on one core.
 
-  python bench.py [--steps K] [--warmup W] [--gpus 1] [--precision fp64]
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--precision mixed|fp64]
 """
 
 from __future__ import annotations
@@ -83,16 +86,13 @@ def make_driver(w, precision="fp64", lanes_per_atom=0):
                         precision=precision, lanes_per_atom=lanes_per_atom), (pots, eams)
 
 
-def force_pass_bytes(N, entries, precision):
-    """Compulsory HBM bytes of one force-pass launch (k_force).
-
-    Per atom: own position record 32 B (double4), F'/F'' 16 B, F(rho) 8 B, list
-    count 4 B and list row 4 B per entry (read); force 24 B and energy 8 B
-    (written). Neighbour positions, F' of neighbours and the tables are gathers
-    of data other atoms also read: re-use, not compulsory traffic.
-    """
-    per_atom = 32 + 16 + 8 + 4 + 24 + 8
-    return N * per_atom + 4.0 * entries
+def force_pass_bytes(N, entries):
+    """Algorithmic HBM bytes of one force-pass launch (k_force): the pass-B share of
+    the per-atom-step byte budget of the path, 41 B per atom (position 24 B, type
+    1 B, F'(rho) 4 B read; force 12 B written) plus 4 B per stored list entry.
+    Neighbour positions, F' of neighbours and the tables are gathers of data
+    other atoms also read: re-use, not compulsory traffic."""
+    return 41.0 * N + 4.0 * entries
 
 
 def read_pmc_profile():
@@ -109,6 +109,28 @@ def read_pmc_profile():
         return float(vals["hbm_bytes_per_dispatch"])
     except (KeyError, ValueError):
         return None
+
+
+def copy_calibration(mib):
+    """hipMemcpy device-to-device of a buffer of mib MiB, three times: the HIP
+    runtime's copy kernel reads and writes exactly that many bytes per launch,
+    so a profiler's byte counters for it give their scale factor."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = mib * (1 << 20)
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, ctypes.c_size_t(nbytes))
+    hip.hipMemset(dst, 0, ctypes.c_size_t(nbytes))
+    for _ in range(3):
+        hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3)   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    return {"kernel": "HIP runtime device-to-device copy", "bytes_read": nbytes,
+            "bytes_written": nbytes, "launches": 3}
 
 
 def cpu_baseline(w, nsteps=2, rebuild_every=20):
@@ -151,10 +173,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--nside", type=int, default=30)
     ap.add_argument("--lanes-per-atom", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the run, copy a buffer of this many MiB three times "
+                         "(a kernel of known HBM traffic, to scale rocprofv3 --pmc "
+                         "FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only; the workload is a single "
@@ -172,9 +198,10 @@ def main():
     N = w["N"]
     value = N * args.steps / wall
     entries = d.neighbor_list_entries
-    fbytes = force_pass_bytes(N, entries, args.precision)
+    fbytes = force_pass_bytes(N, entries)
     achieved = fbytes / (kt["force_ms"] * 1e-3) / 1e9
-    traffic = read_pmc_profile()
+    # the committed --pmc summary measures the mixed-precision force kernel
+    traffic = read_pmc_profile() if args.precision == "mixed" else None
     out = {
         "metric": "atom-steps/s",
         "value": value,
@@ -217,6 +244,8 @@ def main():
     }
     if not args.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
         out["cpu_baseline"] = cpu_baseline(w)
+    if args.copy_calibration_mib > 0:
+        out["copy_calibration"] = copy_calibration(args.copy_calibration_mib)
     print(json.dumps(out), flush=True)
 
 

@@ -18,8 +18,9 @@ HEADERS = ["mdl_internal.hpp", "mdl_kernels.hpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MDL_OFFLOAD_ARCH", "gfx950")
 
-# -packed-fp32-ops: no v_pk_*_f32 are generated (they have given wrong results on
-# this hardware pool); the host pass warns that it does not know the feature.
+# -packed-fp32-ops: no packed FP32 instructions are generated (some of their
+# forms have given wrong results on MI355X); the host pass warns that it does
+# not know the feature.
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wall", "-Wno-unused-result"]
