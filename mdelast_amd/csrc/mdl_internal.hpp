@@ -106,7 +106,7 @@ struct DevState {
     int err;
     double err_val[4];
     long long warn_virial, warn_rmin;
-    long long nhc_general;   // chain propagations that took the general (library exp) path
+    long long nhc_general;   // chain propagations with an argument beyond the series range
     int init_done;
 };
 

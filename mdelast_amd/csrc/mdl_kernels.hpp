@@ -129,8 +129,34 @@ __device__ __forceinline__ double op_series(double x) {
     return fma(o, y, 1.);
 }
 
+// exp(-x) for any x: y = -x = k ln2 + r with |r| <= ln2/2 (Cody-Waite: ln2
+// split into a 32-bit head, so k*head is exact, and a tail), e^r = 1 + r p(r)
+// with p the Taylor series of (e^r - 1)/r to degree 12 (truncation below 5e-18
+// relative), then scaled by 2^k. Within 1 ulp of the correctly rounded value
+// over the whole range; overflow gives inf and underflow 0 through the scaling.
+__device__ __forceinline__ double exp_neg_general(double x) {
+    double const y = -x;
+    double const k = rint(y * 1.4426950408889634);   // y / ln 2
+    double r = fma(k, -0x1.62e42ff000000p-1, y);     // ln2 head (32 significant bits)
+    r = fma(k, 0x1.718432a1b0e26p-35, r);            // -(ln2 tail)
+    double p = 1. / 6227020800.;
+    p = fma(p, r, 1. / 479001600.);
+    p = fma(p, r, 1. / 39916800.);
+    p = fma(p, r, 1. / 3628800.);
+    p = fma(p, r, 1. / 362880.);
+    p = fma(p, r, 1. / 40320.);
+    p = fma(p, r, 1. / 5040.);
+    p = fma(p, r, 1. / 720.);
+    p = fma(p, r, 1. / 120.);
+    p = fma(p, r, 1. / 24.);
+    p = fma(p, r, 1. / 6.);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.);
+    return ldexp(fma(r, p, 1.), (int)k);
+}
+
 __device__ __forceinline__ double exp_neg(double x) {
-    return fabs(x) <= EXPX_SMALL ? fma(-x, op_series(x), 1.) : exp(-x);
+    return fabs(x) <= EXPX_SMALL ? fma(-x, op_series(x), 1.) : exp_neg_general(x);
 }
 
 __device__ __forceinline__ void expx_operator(double x, double &expx, double &op) {
@@ -138,7 +164,7 @@ __device__ __forceinline__ void expx_operator(double x, double &expx, double &op
         op = op_series(x);
         expx = fma(-x, op, 1.);
     } else {
-        expx = exp(-x);
+        expx = exp_neg_general(x);
         op = (1.0 - expx) / x;
     }
 }
@@ -231,7 +257,7 @@ __device__ __forceinline__ void expx_flag(double x, double &expx, double &op, bo
         op = op_series(x);
         expx = fma(-x, op, 1.);
     } else {
-        expx = exp(-x);
+        expx = exp_neg_general(x);
         op = (1.0 - expx) / x;
         general = true;
     }

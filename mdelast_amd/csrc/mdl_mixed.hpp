@@ -65,13 +65,12 @@ template <> struct PolyF<MDL_BSPLINE> {
     __device__ __forceinline__ float d2(float t, float) const { return c[8] + t * (c[9] + t * (c[10] + t * c[11])); }
 };
 
-__device__ __forceinline__ int tab_locate_f(float x, float inv_h, int n, float &t) {
+// Interval and local coordinate as tab_locate (range test before the
+// conversion, NaN -> interval 0, outside -> end polynomial), without branches:
+// clamping u to [0, n-1] first gives the same interval. nm1 = n - 1.
+__device__ __forceinline__ int tab_locate_f(float x, float inv_h, float nm1, float &t) {
     float const u = x * inv_h;
-    int i = 0;
-    if (u >= (float)n)
-        i = n - 1;
-    else if (u > 0.f)
-        i = (int)u;
+    int const i = (int)fminf(fmaxf(u, 0.f), nm1);
     t = u - (float)i;
     return i;
 }
@@ -103,6 +102,8 @@ __global__ __launch_bounds__(BLOCK) void k_density_mx(AtomArrays a, ListArrays l
     unsigned long long const gmask = group_mask<LPA>();
     unsigned long long const below = (l64 == 0) ? 0ull : (~0ull >> (64 - l64));
     double const L[3] = {gs->L[0] * 1e8, gs->L[1] * 1e8, gs->L[2] * 1e8};   // A
+    int const rowS = rt.nf * P::S;           // floats per interval (fits int)
+    float const nm1 = (float)(rt.n - 1);
     double acc[PA_N];
 #pragma unroll
     for (int k = 0; k < PA_N; ++k)
@@ -120,46 +121,53 @@ __global__ __launch_bounds__(BLOCK) void k_density_mx(AtomArrays a, ListArrays l
             ci = a.pos[i];
             cnt = l.nnbr[i];
         }
-        const int *nb = l.nbr + (size_t)(active ? i : 0) * l.stride;
-        int *cb = l.cnbr + (size_t)(active ? i : 0) * l.stride;
+        int const ia = active ? i : 0;
+        const int *nb = l.nbr + (size_t)ia * l.stride;
+        int *cb = l.cnbr + (size_t)ia * l.stride;
         int nin = 0;
+        // Software pipeline: the position of the next candidate and the index
+        // of the one after are loaded while this candidate is processed (the
+        // table record of this candidate is requested before them, so waiting
+        // for it does not wait for them). Lanes past the end use atom ia itself
+        // as a harmless dummy; every contribution is selected by `in`.
+        int j0 = lane < cnt ? nb[lane] : ia;
+        int j1 = LPA + lane < cnt ? nb[LPA + lane] : ia;
+        double4 c0 = a.pos[j0];
         for (int k0 = 0; k0 < cnt; k0 += LPA) {
             int const k = k0 + lane;
-            bool in = false;
-            int j = 0;
-            double dx = 0., dy = 0., dz = 0.;
-            double4 cj;
-            if (k < cnt) {
-                j = nb[k];
-                cj = a.pos[j];
-                min_image(ci, cj, L, dx, dy, dz);
-                in = !(dx * dx + dy * dy + dz * dz > rc2);
-            }
+            int const j = j0;
+            double4 const cj = c0;
+            double dx, dy, dz;
+            min_image(ci, cj, L, dx, dy, dz);
+            bool const in = k < cnt && !(dx * dx + dy * dy + dz * dz > rc2);
+            float const fx = (float)dx, fy = (float)dy, fz = (float)dz;
+            float const r = __builtin_amdgcn_sqrtf(fx * fx + fy * fy + fz * fz);
+            int const tj = (int)cj.w;
+            float t;
+            int const it = tab_locate_f(r, rt.inv_h, nm1, t);
+            P p;
+            p.load(rt.rec + (it * rowS + (npair + tj) * P::S));
+            c0 = a.pos[j1];
+            j0 = j1;
+            j1 = k + 2 * LPA < cnt ? nb[k + 2 * LPA] : ia;
             unsigned long long const bal = __ballot(in) & gmask;
-            if (in) {
+            if (in)
                 cb[nin + __popcll(bal & below)] = j;
-                float const fx = (float)dx, fy = (float)dy, fz = (float)dz;
-                float const r = sqrtf(fx * fx + fy * fy + fz * fz);
-                int const tj = (int)cj.w;
-                float t;
-                int const it = tab_locate_f(r, rt.inv_h, rt.n, t);
-                P p;
-                p.load(rt.rec + ((size_t)it * rt.nf + npair + tj) * P::S);
-                rho += (double)p.v(t);
-                if (SAMPLE) {
-                    float const w = p.d1(t, rt.inv_h) / r;
-                    g0 += w * fx * fx;
-                    g1 += w * fy * fy;
-                    g2 += w * fz * fz;
-                    g3 += w * fy * fz;
-                    g4 += w * fx * fz;
-                    g5 += w * fx * fy;
-                    hx += w * fx;
-                    hy += w * fy;
-                    hz += w * fz;
-                }
-            }
             nin += __popcll(bal);
+            rho += in ? (double)p.v(t) : 0.;
+            if (SAMPLE) {
+                float const w = in ? p.d1(t, rt.inv_h) * __builtin_amdgcn_rcpf(r) : 0.f;
+                float const wx = w * fx, wy = w * fy, wz = w * fz;
+                g0 += wx * fx;
+                g1 += wy * fy;
+                g2 += wz * fz;
+                g3 += wy * fz;
+                g4 += wx * fz;
+                g5 += wx * fy;
+                hx += wx;
+                hy += wy;
+                hz += wz;
+            }
         }
 #pragma unroll
         for (int off = LPA / 2; off > 0; off >>= 1) {
@@ -231,6 +239,8 @@ __global__ __launch_bounds__(BLOCK) void k_force_mx(AtomArrays a, ListArrays l, 
     constexpr double E_CGS = 1.6021773e-12;          // eV -> erg
     int const lane = threadIdx.x % LPA, slot = threadIdx.x / LPA;
     double const L[3] = {gs->L[0] * 1e8, gs->L[1] * 1e8, gs->L[2] * 1e8};
+    int const rowS = rt.nf * P::S;           // floats per interval (fits int)
+    float const nm1 = (float)(rt.n - 1);
     float sb0 = 0.f, sb1 = 0.f, sb2 = 0.f, sb3 = 0.f, sb4 = 0.f, sb5 = 0.f;
     float cb[15];
 #pragma unroll
@@ -250,29 +260,37 @@ __global__ __launch_bounds__(BLOCK) void k_force_mx(AtomArrays a, ListArrays l, 
             float2 const Fi = a.fprime_f[i];
             int const cnt = l.ccnt[i];
             const int *nb = l.cnbr + (size_t)i * l.stride;
+            // Software pipeline as in pass A: the next pair's position and the
+            // index after it are requested after this pair's table records.
+            int j0 = lane < cnt ? nb[lane] : i;
+            int j1 = LPA + lane < cnt ? nb[LPA + lane] : i;
+            double4 c0 = a.pos[j0];
             for (int k = lane; k < cnt; k += LPA) {
-                int const j = nb[k];
-                double4 const cj = a.pos[j];
+                int const j = j0;
+                double4 const cj = c0;
                 double dx, dy, dz;
                 min_image(ci, cj, L, dx, dy, dz);
                 float const x = (float)dx, y = (float)dy, z = (float)dz;
                 float const xx = x * x, yy = y * y, zz = z * z;
-                float const r = sqrtf(xx + yy + zz);
+                float const r = __builtin_amdgcn_sqrtf(xx + yy + zz);
                 int const tj = (int)cj.w;
                 int const tmax = max(ti, tj), tmin = min(ti, tj);
-                int const pslot = tmax * (tmax + 1) / 2 + tmin;
+                int const pslot = ((tmax * (tmax + 1)) >> 1) + tmin;
                 nrmin += r < rmin;
                 float t;
-                int const it = tab_locate_f(r, rt.inv_h, rt.n, t);
-                const float *base = rt.rec + (size_t)it * rt.nf * P::S;
+                int const it = tab_locate_f(r, rt.inv_h, nm1, t);
+                const float *base = rt.rec + it * rowS;
                 P pp, pi, pj;
                 pp.load(base + pslot * P::S);
                 pi.load(base + (npair + ti) * P::S);
                 pj.load(base + (npair + tj) * P::S);
                 float2 const Fj = a.fprime_f[j];
+                c0 = a.pos[j1];
+                j0 = j1;
+                j1 = k + 2 * LPA < cnt ? nb[k + 2 * LPA] : i;
                 float const V = pp.v(t), dV = pp.d1(t, rt.inv_h);
                 float const dri = pi.d1(t, rt.inv_h), drj = pj.d1(t, rt.inv_h);
-                float const ir = 1.f / r;
+                float const ir = __builtin_amdgcn_rcpf(r);
                 float const dedr = sym_dedr(dV, Fi.x, drj, Fj.x, dri);
                 float const s = -dedr * ir;
                 fx += (double)s * dx;

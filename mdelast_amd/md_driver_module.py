@@ -688,7 +688,7 @@ class MDDriver:
 
     @property
     def nhc_general_path_count(self):
-        """Chain propagations that took the general (library exp) path: a chain
+        """Chain propagations that took the general (range-reduced exp) path: a chain
         argument dt p/q beyond 1/8 (diagnostic; no counterpart in the reference)."""
         return int(self._scalars()[nat.S_NHC_GENERAL])
 

@@ -165,7 +165,7 @@ def test_trajectory_vs_oracle(ensemble):
 @pytest.mark.parametrize("path", ["series", "general"])
 def test_trajectory_chain_paths(ensemble, path):
     # tau = 500 fs keeps every chain argument dt p/q far below 1/8 (series path
-    # only); tau = 4 fs puts many beyond (library exp and quotient)
+    # only); tau = 4 fs puts many beyond (range-reduced exp and quotient)
     tau = 5e-13 if path == "series" else 4e-15
     worst, g = _traj_compare(ensemble, 20, chain_kw=dict(tau_T=tau, tau_B=tau))
     print(ensemble, path, worst, g.nhc_general_path_count)
