@@ -1,0 +1,128 @@
+"""End-to-end statistics of the reference's sample case on the GPU path.
+
+6 at.% Zn alpha-brass, 500 atoms, 300 K (sample_case/run.py:210-297 of the
+reference): NPT 50 k + 100 k steps of 2 fs from the FCC lattice (a = 3.6345 A,
+30 Zn by RandomState(42)), then NVT 50 k + 100 k steps from the last NPT dump
+scaled to the production-mean box. The results are compared with the
+reference's own run of this protocol (benchmarks/expected/zn06_300K.json) within
+its tolerances, tol = 4 sqrt(2) max(SE of a run, spread over velocity seeds):
+independent trajectories agree only statistically.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from mdelast_amd import get_potential
+from mdelast_amd.md_driver_module import MDDriver, Units, Vector3d
+from mdelast_amd.molecular_dynamics import Material, MolecularDynamics
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = json.load(open(os.path.join(HERE, "golden", "zn06_300K.json")))["stages"]
+GPa = Units.GPa
+
+
+def _lattice(a, side):
+    L = side * a
+    basis = np.array([[0, 0, 0], [.5, .5, 0], [.5, 0, .5], [0, .5, .5]])
+    pos = [Vector3d(*(-L / 2.0 + (np.array([i, j, k]) + 0.25 + b) * a))
+           for i in range(side) for j in range(side) for k in range(side) for b in basis]
+    return pos, L
+
+
+def _stage(material, ensemble, seed, work_dir):
+    return MolecularDynamics(
+        work_dir=str(work_dir), ensamble=ensemble, material=material, dt_global=2e-15,
+        tau_thermostat=5e-14, tau_barostat=5e-13, temperature_initial=300.0,
+        temperature_thermostat=300.0, pressure_barostat=1e-10 * GPa, dump_cycles=50_000,
+        plot_graphs=False, fatal_conservation=1e3, cycles_thermalization=50_000,
+        plot_freq=100_000, cycles_end=150_000, neighbors=True, r_skin=1.2, cycles_neigbors=20,
+        gr_stride=100, random_seed=seed, sample_stride=1, precision="mixed").run()
+
+
+def _quantities(md, elastic):
+    s = md.series
+    cut = md.samples_thermalization
+    out = {
+        "L_L0": s.L_L0.mean,
+        "T_kin": s.temperature_kin.mean,
+        "T_conf_recip": 1.0 / s.temperature_conf_inverse.mean,
+        "pressure": s.pressure.mean / GPa,
+        "CB11": s.born_elasticity_tensor_11.mean / GPa,
+        "CB12": s.born_elasticity_tensor_12.mean / GPa,
+        "CB44": s.born_elasticity_tensor_44.mean / GPa,
+        "econs_err_max": float(np.max(s.etot_cons_err.data)),
+    }
+    if elastic:
+        tr = s.thermal_results
+        sf = s.stress_fluc.data
+        out.update({
+            "C11": tr.elasticity_tensor_11 / GPa,
+            "C12": tr.elasticity_tensor_12 / GPa,
+            "C44": tr.elasticity_tensor_44 / GPa,
+            "fluc11": (sf[0][0].mean + sf[1][1].mean + sf[2][2].mean) / 3 / GPa,
+            "fluc12": (sf[0][1].mean + sf[0][2].mean + sf[1][2].mean) / 3 / GPa,
+            "fluc44": (sf[3][3].mean + sf[4][4].mean + sf[5][5].mean) / 3 / GPa,
+            "kin11": s.elasticity_ideal_gas.data[0][0].mean / GPa,
+        })
+    assert cut < len(s.time)
+    return out
+
+
+def _check(got, stage):
+    bad = {}
+    for k, v in got.items():
+        q = GOLDEN[stage]["quantities"].get(k)
+        if q is None:
+            continue
+        if k == "econs_err_max":
+            ok = v <= q["max"]
+            ref = q["max"]
+        elif "tol" in q:
+            ok = abs(v - q["expected"]) <= q["tol"]
+            ref = q["expected"]
+        else:
+            continue
+        print(f"{stage} {k:14s} {v:.6g}  expected {ref:.6g}" + ("" if ok else "  <-- out"))
+        if not ok:
+            bad[k] = (v, ref)
+    return bad
+
+
+def test_zn06_npt_to_nvt_statistics(tmp_path):
+    elements, pots, eams = get_potential.get_EAM_alloy_file("ZnCu.eam.alloy")
+    pos, L = _lattice(3.6345 * Units.angstrom, 5)
+    types = [1] * len(pos)
+    for idx in np.random.RandomState(42).choice(len(pos), size=30, replace=False):
+        types[int(idx)] = 0
+    mk = lambda p, t, Lb: Material(potential=pots, atomic_weight=[e[2] for e in elements],
+                                   atom_name="-".join(e[0] for e in elements), eam=eams,
+                                   lattice=p, atom_type=t, box_length=(Lb, Lb, Lb))
+    npt = _stage(mk(pos, types, L), MDDriver.NPT, 0, tmp_path / "npt")
+    got_npt = _quantities(npt, elastic=False)
+
+    # handoff (sample_case/run.py:254-283): the last NPT dump, scaled to the
+    # production-mean box
+    dumps = sorted((tmp_path / "npt" / "dump").glob("*.xyz"), key=lambda p: int(p.stem))
+    species = {e[0]: k for k, e in enumerate(elements)}
+    npos, ntypes = [], []
+    for line in dumps[-1].read_text().splitlines()[2:]:
+        f = line.split()
+        if len(f) < 7:
+            continue
+        npos.append([float(v) * Units.angstrom for v in f[1:4]])
+        ntypes.append(species[f[0]])
+    scale = npt.series.V_V0.mean ** (1.0 / 3.0)
+    npos = [Vector3d(*(np.array(p) * scale)) for p in npos]
+    nvt = _stage(mk(npos, ntypes, L * scale), MDDriver.NVT, 0, tmp_path / "nvt")
+    got_nvt = _quantities(nvt, elastic=True)
+
+    bad = _check(got_npt, "npt")
+    bad.update({f"nvt {k}": v for k, v in _check(got_nvt, "nvt").items()})
+    assert not bad, bad

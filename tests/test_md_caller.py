@@ -1,0 +1,214 @@
+"""The MolecularDynamics caller and its elasticity estimator on the CPU.
+
+The estimator runs on synthetic sample records (no driver): its terms are
+checked against closed forms and against independent numpy statistics, the
+kinetic term against the reference's golden value for the Zn06 case, and the
+device-loop schedule (stretches between events) against the reference's
+one-step loop (molecular_dynamics.py:1240-1285) on a recording stand-in for the
+driver.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import types
+
+import numpy as np
+import pytest
+
+from mdelast_amd import _native as nat
+from mdelast_amd import molecular_dynamics as MDmod
+from mdelast_amd.md_driver_module import MDDriver, Units
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden", "zn06_300K.json")
+
+
+class FakeDriver:
+    """Advances cycles and returns MDL_S_* records; logs the schedule."""
+
+    def __init__(self, L0, ensamble, T, P=1e-10 * Units.GPa, record_fn=None):
+        self.cycle = 0
+        self.time = 0.0
+        self.ensamble = ensamble
+        self._L0 = L0
+        self.box_initial = types.SimpleNamespace(volume=lambda: L0 ** 3)
+        self.thermostat = types.SimpleNamespace(temperature=T)
+        self.barostat = types.SimpleNamespace(pressure=P)
+        self.record_fn = record_fn
+        self.rebuilds, self.samples = [], []
+
+    def run(self, nsteps, sample_stride, rebuild_every, records=True):
+        out = []
+        for _ in range(nsteps):
+            if rebuild_every > 0 and self.cycle % rebuild_every == 0:
+                self.rebuilds.append(self.cycle)
+            self.cycle += 1
+            self.time += 1.0
+            if self.cycle % sample_stride == 0:
+                self.samples.append(self.cycle)
+                out.append(self.record_fn(self.cycle) if self.record_fn else np.zeros(72))
+        return np.array(out).reshape(-1, 72)
+
+
+def _md(driver, *, ensamble, N, T, therm, plot_freq, end, stride=1, dump=10 ** 9, nbr=20,
+        P=1e-10 * Units.GPa):
+    md = MDmod.MolecularDynamics.__new__(MDmod.MolecularDynamics)
+    md.driver, md.ensamble, md.Nparticles = driver, ensamble, N
+    md.temperature_thermostat, md.pressure_barostat = T, P
+    md.cycles_thermalization, md.plot_freq, md.cycles_end = therm, plot_freq, end
+    md.sample_stride, md.dump_cycles = stride, dump
+    md.samples_thermalization = therm // stride
+    md.neighbors, md.cycles_neigbors = True, nbr
+    md.ndensity = N / driver.box_initial.volume()
+    md.series = MDmod.Series()
+    md.gr_sum, md.gr_samples, md.rg = np.zeros(4), 0, np.zeros(4)
+    md.thermalized = False
+    md.runtime_begin = 0.0
+    md.dumps, md.averages = [], []
+    md.dump = lambda: md.dumps.append(driver.cycle)
+    orig = md.perform_thermal_average
+    md.perform_thermal_average = lambda: (md.averages.append(driver.cycle), orig())
+    return md
+
+
+def _record(L, sb, cb, T=300.0, etot=0.0):
+    r = np.zeros(72)
+    r[nat.S_INDEX["box0"]] = r[nat.S_INDEX["box1"]] = r[nat.S_INDEX["box2"]] = L
+    r[nat.S_INDEX["temperature_kin"]] = T
+    r[nat.S_INDEX["temperature_conf_inverse"]] = 1.0 / T
+    r[nat.S_INDEX["pressure"]] = 1.0
+    r[nat.S_INDEX["etot_atom"]] = etot
+    r[nat.S_INDEX["ekin_atom"]] = 1.0
+    r[nat.S_INDEX["epot_atom"]] = 1.0
+    r[nat.S_BORN_STRESS:nat.S_BORN_STRESS + 6] = sb
+    r[nat.S_BORN_ELAST:nat.S_BORN_ELAST + 36] = cb.ravel()
+    return r
+
+
+def _reference_schedule(therm, plot_freq, end, stride, dump, nbr):
+    """molecular_dynamics.py:1240-1285, one step at a time."""
+    cycle, dumps, rebuilds, samples, averages = 0, [], [], [], []
+    while cycle <= end:
+        if cycle % dump == 0:
+            dumps.append(cycle)
+        if cycle % nbr == 0:
+            rebuilds.append(cycle)
+        cycle += 1
+        if cycle % stride == 0:
+            samples.append(cycle)
+        if cycle > therm and (cycle - therm) % plot_freq == 0:
+            averages.append(cycle)
+    return dumps, rebuilds, samples, averages
+
+
+@pytest.mark.parametrize("therm,plot_freq,end,stride,dump,nbr",
+                         [(50, 30, 140, 1, 25, 20), (40, 40, 120, 2, 50, 7), (0, 10, 35, 5, 100, 15)])
+def test_device_loop_schedule_matches_step_loop(therm, plot_freq, end, stride, dump, nbr):
+    L0 = 1e-7
+    cb = np.eye(6)
+    drv = FakeDriver(L0, MDDriver.NVT, 300.0, record_fn=lambda c: _record(L0, np.ones(6), cb))
+    md = _md(drv, ensamble=MDDriver.NVT, N=100, T=300.0, therm=therm, plot_freq=plot_freq,
+             end=end, stride=stride, dump=dump, nbr=nbr)
+    md.run()
+    ref = _reference_schedule(therm, plot_freq, end, stride, dump, nbr)
+    assert md.dumps == ref[0]
+    assert drv.rebuilds == ref[1]
+    assert drv.samples == ref[2]
+    assert md.averages == ref[3]
+    assert drv.cycle == end + 1
+
+
+def _synthetic_run(ensamble, n=4000, seed=3):
+    """Records whose Born stress has a known covariance and, in NPT, a volume
+    that fluctuates."""
+    rs = np.random.RandomState(seed)
+    L0 = 1.8e-7
+    T = 300.0
+    A = rs.normal(size=(6, 6)) * 1e8
+    sb = rs.normal(size=(n, 6)) @ A.T + 1e9
+    cb = np.triu(rs.normal(size=(6, 6)) * 1e11 + 1e12)
+    L = L0 * (1.0 + (rs.normal(size=n) * 1e-3 if ensamble == MDDriver.NPT else np.zeros(n)))
+    etot = rs.normal(size=n) * 1e-13
+    recs = [_record(L[k], sb[k], cb, T=T, etot=etot[k]) for k in range(n)]
+    drv = FakeDriver(L0, ensamble, T, record_fn=lambda c: recs[c - 1])
+    therm = n // 4
+    md = _md(drv, ensamble=ensamble, N=500, T=T, therm=therm, plot_freq=n - therm, end=n - 1)
+    md.run()
+    return md, dict(sb=sb, cb=cb, L=L, L0=L0, T=T, therm=therm, etot=etot)
+
+
+@pytest.mark.parametrize("ens", ["NVT", "NPT"])
+def test_estimator_terms_against_closed_forms(ens):
+    ensamble = MDDriver.NVT if ens == "NVT" else MDDriver.NPT
+    md, x = _synthetic_run(ensamble)
+    tr, s = md.series.thermal_results, md.series
+    kT = Units.k_boltz * x["T"]
+    cut = x["therm"]
+    V = x["L"][cut:] ** 3
+    rho = 500.0 / V
+    fl = x["sb"][cut:].copy()
+    if ensamble == MDDriver.NPT:
+        fl[:, :3] -= (rho * kT)[:, None]
+    cov = np.cov(fl.T, bias=True)                        # population covariance
+    fluc = V.mean() / kT * cov
+    kin = np.diag([2, 2, 2, 1, 1, 1]) * (rho * kT).mean()
+    K = V.mean() * kT / V.var() if ensamble == MDDriver.NPT else 0.0
+    Kmat = np.zeros((6, 6))
+    Kmat[:3, :3] = K
+    expect = x["cb"] - fluc + kin + Kmat
+    assert np.allclose(tr.elasticity_tensor, expect, rtol=1e-9, atol=1e-9 * np.abs(expect).max())
+    for i in range(6):
+        for j in range(6):
+            assert np.isclose(s.stress_fluc.data[i][j].mean, fluc[i, j], rtol=1e-9,
+                              atol=1e-12 * np.abs(fluc).max())
+    assert np.isclose(tr.elasticity_tensor_11, np.trace(expect[:3, :3]) / 3, rtol=1e-12)
+    assert np.isclose(tr.elasticity_tensor_12, (expect[0, 1] + expect[0, 2] + expect[1, 2]) / 3,
+                      rtol=1e-12)
+    assert np.isclose(tr.elasticity_tensor_44, np.trace(expect[3:, 3:]) / 3, rtol=1e-12)
+    # the Born tensor's lower triangle is zero in the driver's record, and so
+    # enters the lower triangle of C only through the other terms
+    assert np.isclose(tr.elasticity_tensor[1, 0], -fluc[1, 0] + Kmat[1, 0], rtol=1e-9)
+    if ensamble == MDDriver.NPT:
+        assert np.isclose(tr.bulk_modulus, K, rtol=1e-9)
+        assert np.isclose(tr.compressibility, 1.0 / K, rtol=1e-9)
+
+
+def test_cumulative_statistics():
+    rs = np.random.RandomState(0)
+    v = rs.normal(size=500)
+    ts = MDmod.TimeSeries()
+    ts.data = list(v)
+    ts.set_averages(100, 0)
+    assert np.isclose(ts.mean, v[100:].mean()) and np.isclose(ts.std, v[100:].std())
+    for k in (1, 17, 499):
+        assert np.isclose(ts.mean_cum[k - 1], v[:k].mean())
+        assert np.isclose(ts.std_cum[k - 1], v[:k].std(), atol=1e-12)
+
+
+def test_kinetic_term_is_the_reference_golden_value():
+    # NVT stage of the Zn06 case: 500 atoms in the production-mean NPT box,
+    # kin11 = 2 rho k T (benchmarks/expected/zn06_300K.json, tol 3.3e-5 GPa)
+    g = json.load(open(GOLDEN))["stages"]
+    L = 5 * 3.6345 * Units.angstrom * g["npt"]["quantities"]["L_L0"]["expected"]
+    drv = FakeDriver(L, MDDriver.NVT, 300.0,
+                     record_fn=lambda c: _record(L, np.zeros(6), np.zeros((6, 6))))
+    md = _md(drv, ensamble=MDDriver.NVT, N=500, T=300.0, therm=10, plot_freq=10, end=19)
+    md.run()
+    kin11 = md.series.elasticity_ideal_gas.data[0][0].mean / Units.GPa
+    q = g["nvt"]["quantities"]["kin11"]
+    assert abs(kin11 - q["expected"]) < q["tol"], (kin11, q["expected"])
+    # and the shear component is half of it
+    assert np.isclose(md.series.elasticity_ideal_gas.data[3][3].mean * 2,
+                      md.series.elasticity_ideal_gas.data[0][0].mean, rtol=1e-14)
+
+
+def test_keyword_validation():
+    with pytest.raises(ValueError, match="missing"):
+        MDmod.MolecularDynamics(material=None)
+    base = dict(material=None, temperature_initial=1, temperature_thermostat=1, tau_thermostat=1,
+                pressure_barostat=0, tau_barostat=1, ensamble="NVT", dump_cycles=1,
+                cycles_thermalization=1, plot_freq=1, cycles_end=1, dt_global=1)
+    with pytest.raises(ValueError, match="extra keys"):
+        MDmod.MolecularDynamics(bogus=1, **base)
