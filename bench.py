@@ -111,6 +111,50 @@ def read_pmc_profile():
         return None
 
 
+def bench_replicas(args):
+    """Aggregate atom-steps/s of independent NVT replicas of uneven size,
+    composition and temperature, advanced together (one host call per stretch)."""
+    from mdelast_amd import get_potential, replicas as R
+    elements, pots, eams = get_potential.get_EAM_alloy_file("ZnCu.eam.alloy")
+    masses = [e[2] / U.Navogadro for e in elements]
+    specs = R.batched_replica_specs(args.replicas)
+    drivers = [R.make_replica_driver(s, pots, eams, masses, precision=args.precision)
+               for s in specs]
+    natoms = sum(s["N"] for s in specs)
+    R.run_replicas(drivers, args.warmup, 1, 20, records=False)
+    t0 = time.perf_counter()
+    recs = R.run_replicas(drivers, args.steps, 1, 20, records=True)
+    wall = time.perf_counter() - t0
+    Tk = [float(r[-1][11]) for r in recs]
+    out = {
+        "metric": "atom-steps/s",
+        "value": natoms * args.steps / wall,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{len(specs)} independent NVT replicas, ZnCu EAM, FCC edge 5-20 cells "
+                        "(500-32,000 atoms), pure Cu or Cu-Zn up to 10 at.% Zn, T 100-900 K, "
+                        "dt 1 fs, NHC M=10, Born every step, list r_skin 1.2 rebuilt every 20",
+            "replicas": len(specs),
+            "atoms_total": natoms,
+            "atoms_min": min(s["N"] for s in specs),
+            "atoms_max": max(s["N"] for s in specs),
+            "precision": args.precision,
+            "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "default"),
+            "T_kin_final_minmax": [min(Tk), max(Tk)],
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def copy_calibration(mib):
     """hipMemcpy device-to-device of a buffer of mib MiB, three times: the HIP
     runtime's copy kernel reads and writes exactly that many bytes per launch,
@@ -177,6 +221,13 @@ def main():
     ap.add_argument("--nside", type=int, default=30)
     ap.add_argument("--lanes-per-atom", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="nialco108k", choices=["nialco108k", "replicas"],
+                    help="nialco108k: the single 108,000-atom instance (default); replicas: "
+                         "independent NVT Cu / Cu-Zn replicas of 500-32,000 atoms advanced side "
+                         "by side")
+    ap.add_argument("--replicas", type=int, default=64)
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="hardware queues of the process (GPU_MAX_HW_QUEUES, at most 32)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB three times "
                          "(a kernel of known HBM traffic, to scale rocprofv3 --pmc "
@@ -187,6 +238,11 @@ def main():
               "instance on one MI355X (multi-GPU sharding is design-only).", file=sys.stderr)
         sys.exit(2)
     mdm.VERBOSE = False
+    if args.hw_queues:
+        # read by the HIP runtime when it initialises (before the library loads)
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(1, args.hw_queues)))
+    if args.workload == "replicas":
+        return bench_replicas(args)
     w = build_workload(nside=args.nside)
     d, keep = make_driver(w, args.precision, args.lanes_per_atom)
     d.run(args.warmup, 1, 20, records=False)

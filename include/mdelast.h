@@ -139,6 +139,15 @@ mdl_status mdl_driver_integrate(mdl_driver *d, int64_t nsteps);
  * *n_records receives the count. */
 mdl_status mdl_driver_run(mdl_driver *d, int64_t nsteps, int64_t sample_stride,
                           int64_t rebuild_every, double *records, int64_t *n_records);
+/* mdl_driver_run on n drivers at once: the launches of step k of every driver,
+ * each on its own stream, then those of step k + 1, so the systems advance
+ * side by side on the device (independent replicas of uneven size,
+ * composition and temperature). records[i] and n_records[i] as for
+ * mdl_driver_run (records or records[i] may be NULL). No counterpart in the
+ * reference, whose replicas run as separate processes. */
+mdl_status mdl_drivers_run(mdl_driver *const *drivers, int32_t n, int64_t nsteps,
+                           int64_t sample_stride, int64_t rebuild_every, double *const *records,
+                           int64_t *n_records);
 /* Force a rebuild before the next force evaluation (md_driver.cpp:279-298). */
 mdl_status mdl_driver_calculate_neighbors(mdl_driver *d);
 /* MDDriver::sample_observables (md_driver.hpp:163-167) */

@@ -766,34 +766,48 @@ mdl_status mdl_driver_integrate(mdl_driver *p, int64_t nsteps) {
     return d->check_errors();
 }
 
-mdl_status mdl_driver_run(mdl_driver *p, int64_t nsteps, int64_t sample_stride,
-                          int64_t rebuild_every, double *records, int64_t *n_records) {
-    Driver *d = D(p);
-    if (mdl_status s = guard(d))
-        return s;
-    if (sample_stride < 1)
-        return bad("sample_stride must be >= 1");
-    int64_t nsamp = 0;
-    for (int64_t k = 0; k < nsteps; ++k)
-        if ((d->cycle_host + k + 1) % sample_stride == 0)
-            ++nsamp;
-    double *drec = nullptr;
-    if (records && nsamp > 0)
-        MDL_HIP(hipMalloc(&drec, sizeof(double) * MDL_NRECORD * (size_t)nsamp));
-    // events around the passes of up to 64 steps
-    int const ntime = (int)std::min<int64_t>(nsteps, 64);
-    while ((int)d->ev.size() < 4 * ntime) {
-        hipEvent_t e;
-        MDL_HIP(hipEventCreate(&e));
-        d->ev.push_back(e);
+}  // extern "C"
+
+// One run() of a driver, split into begin / one launch per step / end so that
+// the launches of several drivers (each on its own stream) can be interleaved
+// from one host thread.
+struct RunPlan {
+    Driver *d = nullptr;
+    int64_t nsteps = 0, stride = 1, rebuild = 0, k = 0, irec = 0, t0 = 0;
+    int ntime = 0;
+    double *records = nullptr, *drec = nullptr;
+    int64_t *n_records = nullptr;
+
+    bool sampled(int64_t cyc) const { return (cyc + 1) % stride == 0; }
+    bool forced(int64_t cyc) const { return rebuild > 0 && cyc % rebuild == 0; }
+
+    mdl_status begin() {
+        if (mdl_status s = guard(d))
+            return s;
+        if (stride < 1)
+            return bad("sample_stride must be >= 1");
+        int64_t nsamp = 0;
+        for (int64_t j = 0; j < nsteps; ++j)
+            if ((d->cycle_host + j + 1) % stride == 0)
+                ++nsamp;
+        if (records && nsamp > 0)
+            MDL_HIP(hipMalloc(&drec, sizeof(double) * MDL_NRECORD * (size_t)nsamp));
+        // events around the passes of up to 64 steps
+        ntime = (int)std::min<int64_t>(nsteps, 64);
+        while ((int)d->ev.size() < 4 * ntime) {
+            hipEvent_t e;
+            MDL_HIP(hipEventCreate(&e));
+            d->ev.push_back(e);
+        }
+        t0 = nsteps - ntime;   // time the last steps (after any warm-up)
+        if (nsteps > 0)
+            k_pre<<<1, BLOCK, 0, d->stream>>>(d->gs, sampled(d->cycle_host) ? 1 : 0);
+        MDL_HIP(hipGetLastError());
+        return MDL_OK;
     }
-    int64_t irec = 0;
-    int64_t const t0 = nsteps - ntime;   // time the last steps (after any warm-up)
-    auto sampled = [&](int64_t cyc) { return (cyc + 1) % sample_stride == 0; };
-    auto forced = [&](int64_t cyc) { return rebuild_every > 0 && cyc % rebuild_every == 0; };
-    if (nsteps > 0)
-        k_pre<<<1, BLOCK, 0, d->stream>>>(d->gs, sampled(d->cycle_host) ? 1 : 0);
-    for (int64_t k = 0; k < nsteps; ++k) {
+
+    // Launch step k (asynchronous).
+    mdl_status step() {
         int64_t const cyc = d->cycle_host;
         bool const s = sampled(cyc);
         double *ro = (drec && s) ? drec + MDL_NRECORD * irec : nullptr;
@@ -807,45 +821,106 @@ mdl_status mdl_driver_run(mdl_driver *p, int64_t nsteps, int64_t sample_stride,
         int const ns = k + 1 < nsteps ? (sampled(cyc + 1) ? 1 : 0) : -1;
         bool const build = d->build_pending || forced(cyc);
         d->build_pending = false;
+        ++k;
         MDL_HIP(d->launch_step(s, ro, tev, build, ns));
+        return MDL_OK;
     }
-    if (drec) {
-        hipError_t e = hipMemcpyAsync(records, drec, sizeof(double) * MDL_NRECORD * irec,
-                                      hipMemcpyDeviceToHost, d->stream);
-        if (e == hipSuccess)
-            e = hipStreamSynchronize(d->stream);
-        (void)hipFree(drec);
-        if (e != hipSuccess)
-            return hip_fail(e, "mdl_driver_run records");
+
+    mdl_status end() {
+        if (drec) {
+            hipError_t e = hipMemcpyAsync(records, drec, sizeof(double) * MDL_NRECORD * irec,
+                                          hipMemcpyDeviceToHost, d->stream);
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(d->stream);
+            (void)hipFree(drec);
+            drec = nullptr;
+            if (e != hipSuccess)
+                return hip_fail(e, "mdl_driver_run records");
+        }
+        if (n_records)
+            *n_records = irec;
+        if (mdl_status s = d->sync_state())
+            return s;
+        // pass timings: [3] step start, [0] before A, [1] after A, [2] after B
+        double ta = 0, tb = 0, tsum = 0;
+        int nt = 0;
+        for (int j = 0; j < ntime; ++j) {
+            hipEvent_t *e = &d->ev[4 * j];
+            float x = 0, y = 0;
+            if (hipEventElapsedTime(&x, e[0], e[1]) == hipSuccess &&
+                hipEventElapsedTime(&y, e[1], e[2]) == hipSuccess) {
+                ta += x;
+                tb += y;
+                ++nt;
+            }
+            if (j + 1 < ntime) {
+                float z = 0;
+                if (hipEventElapsedTime(&z, e[3], d->ev[4 * (j + 1) + 3]) == hipSuccess)
+                    tsum += z;
+            }
+        }
+        if (nt > 0) {
+            d->t_dens = ta / nt;
+            d->t_force = tb / nt;
+            d->t_step = ntime > 1 ? tsum / (ntime - 1) : 0.;
+        }
+        return d->check_errors();
     }
-    if (n_records)
-        *n_records = irec;
-    if (mdl_status s = d->sync_state())
+};
+
+extern "C" {
+
+mdl_status mdl_driver_run(mdl_driver *p, int64_t nsteps, int64_t sample_stride,
+                          int64_t rebuild_every, double *records, int64_t *n_records) {
+    RunPlan r;
+    r.d = D(p);
+    r.nsteps = nsteps;
+    r.stride = sample_stride;
+    r.rebuild = rebuild_every;
+    r.records = records;
+    r.n_records = n_records;
+    if (mdl_status s = r.begin())
         return s;
-    // pass timings: [3] step start, [0] before A, [1] after A, [2] after B
-    double ta = 0, tb = 0, tsum = 0;
-    int nt = 0;
-    for (int k = 0; k < ntime; ++k) {
-        hipEvent_t *e = &d->ev[4 * k];
-        float x = 0, y = 0;
-        if (hipEventElapsedTime(&x, e[0], e[1]) == hipSuccess &&
-            hipEventElapsedTime(&y, e[1], e[2]) == hipSuccess) {
-            ta += x;
-            tb += y;
-            ++nt;
-        }
-        if (k + 1 < ntime) {
-            float z = 0;
-            if (hipEventElapsedTime(&z, e[3], d->ev[4 * (k + 1) + 3]) == hipSuccess)
-                tsum += z;
-        }
+    for (int64_t k = 0; k < nsteps; ++k)
+        if (mdl_status s = r.step())
+            return s;
+    return r.end();
+}
+
+mdl_status mdl_drivers_run(mdl_driver *const *drivers, int32_t n, int64_t nsteps,
+                           int64_t sample_stride, int64_t rebuild_every, double *const *records,
+                           int64_t *n_records) {
+    if (!drivers || n < 1)
+        return bad("no drivers");
+    std::vector<RunPlan> plans((size_t)n);
+    for (int32_t i = 0; i < n; ++i) {
+        for (int32_t j = 0; j < i; ++j)
+            if (drivers[j] == drivers[i])
+                return bad("a driver appears twice in mdl_drivers_run");
+        RunPlan &r = plans[(size_t)i];
+        r.d = D(drivers[i]);
+        r.nsteps = nsteps;
+        r.stride = sample_stride;
+        r.rebuild = rebuild_every;
+        r.records = records ? records[i] : nullptr;
+        r.n_records = n_records ? n_records + i : nullptr;
+        if (mdl_status s = r.begin())
+            return s;
     }
-    if (nt > 0) {
-        d->t_dens = ta / nt;
-        d->t_force = tb / nt;
-        d->t_step = ntime > 1 ? tsum / (ntime - 1) : 0.;
+    // step k of every driver, then step k + 1: the streams run side by side
+    for (int64_t k = 0; k < nsteps; ++k)
+        for (RunPlan &r : plans) {
+            MDL_HIP(hipSetDevice(r.d->device));
+            if (mdl_status s = r.step())
+                return s;
+        }
+    mdl_status first = MDL_OK;
+    for (RunPlan &r : plans) {
+        mdl_status s = r.end();
+        if (s != MDL_OK && first == MDL_OK)
+            first = s;
     }
-    return d->check_errors();
+    return first;
 }
 
 mdl_status mdl_driver_calculate_neighbors(mdl_driver *p) {

@@ -1,0 +1,34 @@
+"""Replicas advanced side by side (mdl_drivers_run) against the same replicas
+advanced one at a time: the interleaving of launches across streams changes
+nothing in any replica's arithmetic, so the records agree bit for bit."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from mdelast_amd import get_potential, replicas as R
+from mdelast_amd import units as U
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_side_by_side_equals_one_at_a_time(precision):
+    elements, pots, eams = get_potential.get_EAM_alloy_file("ZnCu.eam.alloy")
+    masses = [e[2] / U.Navogadro for e in elements]
+    specs = [dict(edge=5, N=500, x_zn=0.06, T=300.0, place_seed=1, vel_seed=2),
+             dict(edge=6, N=864, x_zn=0.0, T=800.0, place_seed=3, vel_seed=4),
+             dict(edge=7, N=1372, x_zn=0.10, T=150.0, place_seed=5, vel_seed=6)]
+    a = [R.make_replica_driver(s, pots, eams, masses, precision=precision) for s in specs]
+    b = [R.make_replica_driver(s, pots, eams, masses, precision=precision) for s in specs]
+    ra = R.run_replicas(a, 60, 1, 20)
+    rb = [d.run(60, 1, 20) for d in b]
+    for x, y in zip(ra, rb):
+        assert x.shape == (60, 72)
+        assert np.array_equal(x, y)
+    for da, db in zip(a, b):
+        assert np.array_equal(da.forces_array(), db.forces_array())
+    # temperatures are those of three different systems
+    T = [r[-1][11] for r in ra]
+    assert len(set(np.round(T, 6))) == 3

@@ -212,3 +212,18 @@ def test_keyword_validation():
                 cycles_thermalization=1, plot_freq=1, cycles_end=1, dt_global=1)
     with pytest.raises(ValueError, match="extra keys"):
         MDmod.MolecularDynamics(bogus=1, **base)
+
+
+def test_batched_replica_table():
+    from mdelast_amd import replicas as R
+    specs = R.batched_replica_specs(64)
+    N = np.array([s["N"] for s in specs])
+    x = np.array([s["x_zn"] for s in specs])
+    T = np.array([s["T"] for s in specs])
+    assert len(specs) == 64 and N.min() >= 500 and N.max() <= 32000
+    assert np.all(N == 4 * np.array([s["edge"] for s in specs]) ** 3)
+    assert np.all((x >= 0) & (x <= 0.10)) and 10 < np.sum(x == 0) < 54
+    assert np.all((T >= 100) & (T <= 900))
+    assert R.batched_replica_specs(64) == specs        # seeded
+    pos = R.fcc_positions(5, 3.6345e-8)
+    assert pos.shape == (500, 3) and np.all(np.abs(pos) < 5 * 3.6345e-8 / 2)
