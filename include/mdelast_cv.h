@@ -1,0 +1,58 @@
+/*
+ * mdelast-mi355x: device kernels of the learned Stein control variate
+ * (libmdelast_hip.so). No counterpart in the reference: the control variate
+ * is new work on top of its stress-fluctuation estimator
+ * (molecular_dynamics.py:825-1087), whose plain form it must not change in
+ * expectation: E_NVT[div g - beta g . grad U] = 0 for any smooth periodic g.
+ *
+ * All pointers are device pointers (float32 unless stated); `stream` is a
+ * hipStream_t (NULL = the null stream). Calls are asynchronous; errors of the
+ * launch are reported, and mdl_last_error() (mdelast.h) holds the message.
+ */
+#ifndef MDELAST_CV_H_
+#define MDELAST_CV_H_
+
+#include "mdelast.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Radial-basis descriptors of N atoms of two species (types 0/1) and their
+ * derivatives along each atom's own coordinates:
+ *   D[i][t*32 + k]    = sum_{j of type t, r_ij < rc} phi_k(r_ij)
+ *   J[a][i][t*32 + k] = d D[i][t*32 + k] / d x_{i,a}
+ * phi_k(r) = exp(-(r - mu0 - k dmu)^2 / (2 width^2)) (cos(pi r / rc) + 1) / 2,
+ * k = 0..31. pos: N*3 FP64 in A (minimum image in the box), box: 3 FP64 on the
+ * HOST, D: [N][64], J: [3][N][64]. MDL_ERR_ARG unless 2 rc <= each box edge. */
+mdl_status mdl_cv_descriptors(const double *pos, const int32_t *types, int32_t N,
+                              const double *box, float rc, float mu0, float dmu, float width,
+                              float *D, float *J, void *stream);
+
+/* C[R][N] = A[R][K] B + bias on MFMA (bf16 operands rounded to nearest even,
+ * fp32 accumulation); B = W[K][N], or W^T with W[N][K] when transB != 0.
+ * K and N positive multiples of 16; bias may be NULL. */
+mdl_status mdl_cv_gemm(const float *A, const float *W, int32_t transB, const float *bias,
+                       float *C, int64_t R, int32_t K, int32_t N, void *stream);
+
+/* C[K][N] = sum_r A[r][K]^T B[r][N] (bf16 MFMA, fp32), summed per chunk of
+ * rows_per_chunk rows (a multiple of 16) into work[ceil(R/rows)][K][N] and then
+ * over chunks in a fixed order. */
+mdl_status mdl_cv_gemm_tn(const float *A, const float *B, float *C, float *work, int64_t R,
+                          int32_t K, int32_t N, int32_t rows_per_chunk, void *stream);
+
+/* out[n] = sum_r X[r][n], fixed order. */
+mdl_status mdl_cv_colsum(const float *X, float *out, int64_t R, int32_t N, void *stream);
+
+/* Activation with the three tangents: Y = act(Z), Yt[a] = act'(Z) Zt[a]
+ * (Z, Y: MN floats; Zt, Yt: [3][MN]). act 0 = tanh, 1 = identity. */
+mdl_status mdl_cv_act_forward(const float *Z, const float *Zt, float *Y, float *Yt, int64_t MN,
+                              int32_t act, void *stream);
+/* Its backward: gZ = gY act'(Z) + sum_a gYt[a] Zt[a] act''(Z), gZt[a] = gYt[a] act'(Z). */
+mdl_status mdl_cv_act_backward(const float *Z, const float *Zt, const float *gY, const float *gYt,
+                               float *gZ, float *gZt, int64_t MN, int32_t act, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MDELAST_CV_H_ */

@@ -1,0 +1,342 @@
+// Learned Stein control variate: device kernels (C-ABI in include/mdelast_cv.h).
+//
+// A per-atom network g(x) maps local radial-basis descriptors D_i to a 3-vector
+// per estimator head; the control variate of a head is
+//   s(x) = sum_i [ div_i g_i + beta g_i . F_i ],   E_NVT[s] = 0,
+// where div_i g_i = sum_a d g_{i,a} / d x_{i,a} needs, for each atom, the
+// derivatives of the outputs along the three coordinates of that atom. They
+// are carried through the network in forward mode: every layer maps inputs X
+// and three tangents T_a (X' along x_{i,a}) to Y = act(X W + b) and
+// Y'_a = act'(X W + b) (T_a W). The descriptor kernel supplies the first
+// tangents, J_a = d D_i / d x_{i,a}. The dense products run on MFMA
+// (bf16 operands, fp32 accumulation, 16x16x16 tiles); the backward pass of a
+// layer is three more products of the same kinds.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+
+#include "mdl_internal.hpp"
+#include "../../include/mdelast_cv.h"
+
+namespace {
+
+using mdl::set_error;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short bf16_rne(float x) {
+    // round to nearest even (inputs are finite)
+    unsigned const u = __float_as_uint(x);
+    unsigned const r = u + 0x7fffu + ((u >> 16) & 1u);
+    return (short)(r >> 16);
+}
+
+// ---------------------------------------------------------------------------
+// Descriptors. One wave per two atoms: lanes 0-31 take atom 2w, lanes 32-63
+// atom 2w+1, lane k%32 owns radial function k for both neighbour species.
+//   D[i][t*nb + k]      = sum_{j of species t, r_ij < rc} phi_k(r_ij)
+//   J[a][i][t*nb + k]   = sum_j phi_k'(r_ij) (x_i - x_j)_a / r_ij
+// phi_k(r) = exp(-(r - mu_k)^2 / (2 w^2)) fc(r), fc = (cos(pi r / rc) + 1) / 2.
+// Positions in A (FP64), minimum image in FP64, basis in FP32.
+// ---------------------------------------------------------------------------
+constexpr int CV_NB = 32;
+
+__global__ __launch_bounds__(256) void k_cv_descriptors(const double *__restrict__ pos,
+                                                        const int *__restrict__ types, int N,
+                                                        double Lx, double Ly, double Lz, float rc,
+                                                        float mu0, float dmu, float width,
+                                                        float *__restrict__ D,
+                                                        float *__restrict__ J) {
+    int const lane = threadIdx.x & 63, k = lane & 31;
+    int const i = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 2 + (lane >> 5);
+    bool const active = i < N;
+    int const ia = active ? i : 0;
+    double const xi = pos[3 * ia], yi = pos[3 * ia + 1], zi = pos[3 * ia + 2];
+    float const mu = mu0 + dmu * (float)k;
+    float const iw2 = 1.f / (width * width);
+    float const rc2 = rc * rc;
+    float const pirc = 3.14159265358979f / rc;
+    float d0 = 0.f, d1 = 0.f, jx0 = 0.f, jy0 = 0.f, jz0 = 0.f, jx1 = 0.f, jy1 = 0.f, jz1 = 0.f;
+    for (int j = 0; j < N; ++j) {
+        double sx = xi - pos[3 * j], sy = yi - pos[3 * j + 1], sz = zi - pos[3 * j + 2];
+        sx -= Lx * rint(sx / Lx);
+        sy -= Ly * rint(sy / Ly);
+        sz -= Lz * rint(sz / Lz);
+        float const x = (float)sx, y = (float)sy, z = (float)sz;
+        float const r2 = x * x + y * y + z * z;
+        if (j == ia || !(r2 < rc2))
+            continue;   // uniform within each half-wave's atom; lanes of one atom agree
+        float const r = sqrtf(r2);
+        float const ir = 1.f / r;
+        float const u = r - mu;
+        float const g = __expf(-0.5f * u * u * iw2);
+        float const c = cosf(pirc * r), sn = sinf(pirc * r);
+        float const fc = 0.5f * (c + 1.f), dfc = -0.5f * pirc * sn;
+        float const phi = g * fc;
+        float const dphi = g * (dfc - u * iw2 * fc);
+        float const w = dphi * ir;
+        if (types[j] == 0) {
+            d0 += phi;
+            jx0 += w * x;
+            jy0 += w * y;
+            jz0 += w * z;
+        } else {
+            d1 += phi;
+            jx1 += w * x;
+            jy1 += w * y;
+            jz1 += w * z;
+        }
+    }
+    if (!active)
+        return;
+    int const F = 2 * CV_NB;
+    D[(size_t)i * F + k] = d0;
+    D[(size_t)i * F + CV_NB + k] = d1;
+    size_t const plane = (size_t)N * F;
+    J[(size_t)i * F + k] = jx0;
+    J[(size_t)i * F + CV_NB + k] = jx1;
+    J[plane + (size_t)i * F + k] = jy0;
+    J[plane + (size_t)i * F + CV_NB + k] = jy1;
+    J[2 * plane + (size_t)i * F + k] = jz0;
+    J[2 * plane + (size_t)i * F + CV_NB + k] = jz1;
+}
+
+// ---------------------------------------------------------------------------
+// C[R][N] = A[R][K] B + bias, B = W[K][N] (or W^T when transB, W[N][K]).
+// One wave per 16x16 tile of C; K, N multiples of 16; rows past R are zero.
+// MFMA 16x16x16 bf16 lane layout: A row l%16, k 4(l/16)+i; B col l%16,
+// k 4(l/16)+i; D col l%16, row 4(l/16)+r.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gemm_nn(const float *__restrict__ A,
+                                                 const float *__restrict__ W, int transB,
+                                                 const float *__restrict__ bias,
+                                                 float *__restrict__ C, int R, int K, int N) {
+    int const lane = threadIdx.x & 63;
+    int const tile = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    int const ntn = N / 16;
+    int const tm = tile / ntn, tn = tile % ntn;
+    int const r0 = tm * 16, n0 = tn * 16;
+    if (r0 >= R)
+        return;   // whole wave
+    int const row = r0 + (lane & 15), col = n0 + (lane & 15), kq = 4 * (lane >> 4);
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        bf16x4 a, b;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            int const kk = k0 + kq + q;
+            a[q] = bf16_rne(row < R ? A[(size_t)row * K + kk] : 0.f);
+            b[q] = bf16_rne(transB ? W[(size_t)col * K + kk] : W[(size_t)kk * N + col]);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        int const rr = r0 + kq + q;
+        if (rr < R)
+            C[(size_t)rr * N + col] = acc[q] + (bias ? bias[col] : 0.f);
+    }
+}
+
+// part[c][K][N] = sum over rows r of chunk c of A[r][K]^T B[r][N]; chunks of
+// `rows` rows (a multiple of 16). One wave per (chunk, 16x16 tile of C).
+__global__ __launch_bounds__(256) void k_gemm_tn(const float *__restrict__ A,
+                                                 const float *__restrict__ B,
+                                                 float *__restrict__ part, int R, int K, int N,
+                                                 int rows, int nchunk) {
+    int const lane = threadIdx.x & 63;
+    int const w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    int const ntk = K / 16, ntn = N / 16;
+    int const per = ntk * ntn;
+    int const c = w / per, t = w % per;
+    if (c >= nchunk)
+        return;
+    int const k0 = (t / ntn) * 16, n0 = (t % ntn) * 16;
+    int const kq = 4 * (lane >> 4);
+    int const rb = c * rows, re = min(R, rb + rows);
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int s0 = rb; s0 < re; s0 += 16) {
+        bf16x4 a, b;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            int const rr = s0 + kq + q;
+            bool const ok = rr < re;
+            a[q] = bf16_rne(ok ? A[(size_t)rr * K + k0 + (lane & 15)] : 0.f);
+            b[q] = bf16_rne(ok ? B[(size_t)rr * N + n0 + (lane & 15)] : 0.f);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    float *out = part + (size_t)c * K * N;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        out[(size_t)(k0 + kq + q) * N + n0 + (lane & 15)] = acc[q];
+}
+
+// out[e] = sum_c part[c][e], fixed order (deterministic)
+__global__ void k_sum_chunks(const float *__restrict__ part, float *__restrict__ out, int nchunk,
+                             int n) {
+    int const e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n)
+        return;
+    float s = 0.f;
+    for (int c = 0; c < nchunk; ++c)
+        s += part[(size_t)c * n + e];
+    out[e] = s;
+}
+
+// out[n] = sum_r X[r][n] (column sums in fixed order)
+__global__ void k_colsum(const float *__restrict__ X, float *__restrict__ out, int R, int N) {
+    int const n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N)
+        return;
+    float s = 0.f;
+    for (int r = 0; r < R; ++r)
+        s += X[(size_t)r * N + n];
+    out[n] = s;
+}
+
+// Activation with tangents. act 0: tanh; 1: identity.
+// Y = act(Z); Yt[a] = act'(Z) Zt[a], a = 0..2 (Zt, Yt are [3][M][N])
+__global__ void k_act_fwd(const float *__restrict__ Z, const float *__restrict__ Zt,
+                          float *__restrict__ Y, float *__restrict__ Yt, int64_t MN, int act) {
+    int64_t const e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= MN)
+        return;
+    float const z = Z[e];
+    float y = z, s = 1.f;
+    if (act == 0) {
+        y = tanhf(z);
+        s = 1.f - y * y;
+    }
+    Y[e] = y;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        Yt[a * MN + e] = s * Zt[a * MN + e];
+}
+
+// Backward of k_act_fwd: gZ = gY act'(Z) + sum_a gYt[a] Zt[a] act''(Z);
+// gZt[a] = gYt[a] act'(Z).
+__global__ void k_act_bwd(const float *__restrict__ Z, const float *__restrict__ Zt,
+                          const float *__restrict__ gY, const float *__restrict__ gYt,
+                          float *__restrict__ gZ, float *__restrict__ gZt, int64_t MN, int act) {
+    int64_t const e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= MN)
+        return;
+    float s = 1.f, s2 = 0.f;
+    if (act == 0) {
+        float const y = tanhf(Z[e]);
+        s = 1.f - y * y;
+        s2 = -2.f * y * s;
+    }
+    float g = gY[e] * s;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float const gt = gYt[a * MN + e];
+        g += gt * Zt[a * MN + e] * s2;
+        gZt[a * MN + e] = gt * s;
+    }
+    gZ[e] = g;
+}
+
+mdl_status launch_ok(const char *what) {
+    hipError_t const e = hipGetLastError();
+    if (e != hipSuccess)
+        return mdl::hip_fail(e, what);
+    return MDL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+mdl_status mdl_cv_descriptors(const double *pos, const int32_t *types, int32_t N,
+                              const double *box, float rc, float mu0, float dmu, float width,
+                              float *D, float *J, void *stream) {
+    if (!pos || !types || !box || !D || !J || N < 1 || !(rc > 0.f) || !(width > 0.f)) {
+        set_error("mdl_cv_descriptors: bad arguments");
+        return MDL_ERR_ARG;
+    }
+    // the minimum image is exact (and the descriptors smooth and periodic) only
+    // while the cut-off sphere fits in the box
+    if (!(2.0 * rc <= box[0] && 2.0 * rc <= box[1] && 2.0 * rc <= box[2])) {
+        set_error("mdl_cv_descriptors: the cut-off exceeds half the box");
+        return MDL_ERR_ARG;
+    }
+    int const waves = (N + 1) / 2;
+    int const blocks = (waves * 64 + 255) / 256;
+    k_cv_descriptors<<<blocks, 256, 0, (hipStream_t)stream>>>(pos, types, N, box[0], box[1],
+                                                              box[2], rc, mu0, dmu, width, D, J);
+    return launch_ok("mdl_cv_descriptors");
+}
+
+mdl_status mdl_cv_gemm(const float *A, const float *W, int32_t transB, const float *bias,
+                       float *C, int64_t R, int32_t K, int32_t N, void *stream) {
+    if (!A || !W || !C || R < 0 || K < 16 || N < 16 || K % 16 || N % 16 || R > (1ll << 31) - 16) {
+        set_error("mdl_cv_gemm: K and N must be positive multiples of 16");
+        return MDL_ERR_ARG;
+    }
+    if (R == 0)
+        return MDL_OK;
+    int64_t const tiles = ((R + 15) / 16) * (N / 16);
+    int64_t const blocks = (tiles * 64 + 255) / 256;
+    k_gemm_nn<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(A, W, transB, bias, C, (int)R,
+                                                                 K, N);
+    return launch_ok("mdl_cv_gemm");
+}
+
+mdl_status mdl_cv_gemm_tn(const float *A, const float *B, float *C, float *work, int64_t R,
+                          int32_t K, int32_t N, int32_t rows_per_chunk, void *stream) {
+    if (!A || !B || !C || !work || R < 1 || K % 16 || N % 16 || K < 16 || N < 16 ||
+        rows_per_chunk < 16 || rows_per_chunk % 16) {
+        set_error("mdl_cv_gemm_tn: bad arguments");
+        return MDL_ERR_ARG;
+    }
+    int const nchunk = (int)((R + rows_per_chunk - 1) / rows_per_chunk);
+    int64_t const waves = (int64_t)nchunk * (K / 16) * (N / 16);
+    k_gemm_tn<<<(unsigned)((waves * 64 + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+        A, B, work, (int)R, K, N, rows_per_chunk, nchunk);
+    if (mdl_status s = launch_ok("mdl_cv_gemm_tn"))
+        return s;
+    int const n = K * N;
+    k_sum_chunks<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(work, C, nchunk, n);
+    return launch_ok("mdl_cv_gemm_tn reduce");
+}
+
+mdl_status mdl_cv_colsum(const float *X, float *out, int64_t R, int32_t N, void *stream) {
+    if (!X || !out || R < 0 || N < 1) {
+        set_error("mdl_cv_colsum: bad arguments");
+        return MDL_ERR_ARG;
+    }
+    k_colsum<<<(N + 63) / 64, 64, 0, (hipStream_t)stream>>>(X, out, (int)R, N);
+    return launch_ok("mdl_cv_colsum");
+}
+
+mdl_status mdl_cv_act_forward(const float *Z, const float *Zt, float *Y, float *Yt, int64_t MN,
+                              int32_t act, void *stream) {
+    if (!Z || !Zt || !Y || !Yt || MN < 0 || (act != 0 && act != 1)) {
+        set_error("mdl_cv_act_forward: bad arguments");
+        return MDL_ERR_ARG;
+    }
+    if (MN == 0)
+        return MDL_OK;
+    k_act_fwd<<<(unsigned)((MN + 255) / 256), 256, 0, (hipStream_t)stream>>>(Z, Zt, Y, Yt, MN,
+                                                                             act);
+    return launch_ok("mdl_cv_act_forward");
+}
+
+mdl_status mdl_cv_act_backward(const float *Z, const float *Zt, const float *gY, const float *gYt,
+                               float *gZ, float *gZt, int64_t MN, int32_t act, void *stream) {
+    if (!Z || !Zt || !gY || !gYt || !gZ || !gZt || MN < 0 || (act != 0 && act != 1)) {
+        set_error("mdl_cv_act_backward: bad arguments");
+        return MDL_ERR_ARG;
+    }
+    if (MN == 0)
+        return MDL_OK;
+    k_act_bwd<<<(unsigned)((MN + 255) / 256), 256, 0, (hipStream_t)stream>>>(Z, Zt, gY, gYt, gZ,
+                                                                             gZt, MN, act);
+    return launch_ok("mdl_cv_act_backward");
+}
+
+}  // extern "C"

@@ -1,0 +1,139 @@
+"""CPU restatement of the learned Stein control variate (test infrastructure).
+
+Only the tests and the smoke check import this module: it is the checker of
+the device kernels behind mdelast_amd/cv.py (include/mdelast_cv.h), never part
+of the product path. The control variate has no counterpart in the reference;
+what pins this restatement is mathematics, checked in tests/test_cv_oracle.py:
+
+* descriptor derivatives against central finite differences of the
+  descriptors (FP64);
+* the forward-mode tangents of the network against finite differences of the
+  network output along each atom's own coordinates;
+* the Stein identity E[s] = 0 by exact Monte Carlo on a harmonic system whose
+  canonical distribution is Gaussian and sampled directly, for the learned
+  local field and for every global field (a sign error in the beta term, a
+  missing div term or a missing grad-U term makes the mean drift by many
+  standard errors).
+
+Dynamics that hold the centre of mass sample the canonical distribution on
+the plane sum_i m_i x_i = const. The local field is a function of
+interatomic vectors only, so its Stein sum is translation invariant and has the
+same mean on that plane as on the equal-weight plane sum_i x_i = const, where
+its projection g_i - mean_j g_j is tangent, has the same divergence (the
+derivatives of g along a common translation vanish) and the same g . grad U
+(the forces sum to zero): the sum needs no correction. The displacement fields
+are not translation invariant; they are tangent to the mass-weighted plane and
+lose one unit of divergence per axis there (``global_stein``'s ndiv).
+
+Units: positions and displacements in A, forces in eV/A, energies in eV,
+beta in 1/eV.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+NB = 32
+
+
+def bf16_round(x):
+    """Round float32 values to bfloat16 (nearest even), returned as float64."""
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    u = a.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def min_image(d, box):
+    box = np.asarray(box, dtype=float)
+    return d - box * np.rint(d / box)
+
+
+def descriptors(pos, types, box, rc=6.8, mu0=2.0, dmu=0.15, width=0.15):
+    """D [N, 64] and J [3, N, 64] = d D_i / d x_{i,a}, brute force over pairs.
+    phi_k(r) = exp(-(r - mu_k)^2 / (2 w^2)) (cos(pi r / rc) + 1) / 2."""
+    pos = np.asarray(pos, dtype=float)
+    N = len(pos)
+    mu = mu0 + dmu * np.arange(NB)
+    D = np.zeros((N, 2 * NB))
+    J = np.zeros((3, N, 2 * NB))
+    for i in range(N):
+        d = min_image(pos[i] - pos, box)
+        r = np.sqrt((d ** 2).sum(1))
+        m = (r < rc) & (np.arange(N) != i)
+        d, r, t = d[m], r[m], np.asarray(types)[m]
+        u = r[:, None] - mu[None, :]
+        g = np.exp(-0.5 * u ** 2 / width ** 2)
+        fc = 0.5 * (np.cos(np.pi * r / rc) + 1.0)
+        dfc = -0.5 * np.pi / rc * np.sin(np.pi * r / rc)
+        phi = g * fc[:, None]
+        dphi = g * (dfc[:, None] - u / width ** 2 * fc[:, None])
+        w = dphi / r[:, None]
+        for s in (0, 1):
+            sel = t == s
+            D[i, s * NB:(s + 1) * NB] = phi[sel].sum(0)
+            for a in range(3):
+                J[a, i, s * NB:(s + 1) * NB] = (w[sel] * d[sel, a:a + 1]).sum(0)
+    return D, J
+
+
+def dense_forward(X, T, W, b, act, emulate_bf16=False):
+    """Y = act(X W + b), Yt[a] = act'(X W + b) (T[a] W). With emulate_bf16 the
+    operands of both products are rounded to bfloat16 as the MFMA path does."""
+    if emulate_bf16:
+        X, T, W = bf16_round(X), bf16_round(T), bf16_round(W)
+    Z = X @ W + b
+    Zt = np.einsum("amk,kn->amn", T, W)
+    if act == 0:
+        Y = np.tanh(Z)
+        s = 1.0 - Y ** 2
+    else:
+        Y, s = Z, np.ones_like(Z)
+    return Y, s[None] * Zt
+
+
+def net_forward(params, D, J, scale=0.1, emulate_bf16=False):
+    """The control variate's network: 64 -> 64 -> 64 -> 16 (tanh, tanh,
+    identity); returns g [M, 9] and its tangents [3, M, 9]."""
+    W1, b1, W2, b2, W3, b3 = params
+    h, ht = dense_forward(D * scale, J * scale, W1, b1, 0, emulate_bf16)
+    if emulate_bf16:
+        h, ht = h.astype(np.float32), ht.astype(np.float32)
+    h, ht = dense_forward(h, ht, W2, b2, 0, emulate_bf16)
+    if emulate_bf16:
+        h, ht = h.astype(np.float32), ht.astype(np.float32)
+    o, ot = dense_forward(h, ht, W3, b3, 1, emulate_bf16)
+    return o[:, :9], ot[:, :, :9]
+
+
+def local_stein(g, gt, F, beta, out_scale=1.0):
+    """Per head c: sum_i div_i g^c_i + beta g^c_i . F_i (g, gt of one frame)."""
+    s = np.zeros(3)
+    for c in range(3):
+        div = gt[0, :, 3 * c] + gt[1, :, 3 * c + 1] + gt[2, :, 3 * c + 2]
+        s[c] = div.sum() + beta * (g[:, 3 * c:3 * c + 3] * F).sum()
+    return out_scale * s
+
+
+def global_stein(u, F, U, U0, beta, ndiv=None):
+    """Zero-mean Stein sums of the displacement fields, per frame.
+
+    u: displacements from the lattice sites [N, 3]; F: forces [N, 3]; U: the
+    configurational energy; U0: any constant; ndiv: the divergence of the
+    field e_a u_ia on the configuration space, N when every atom moves freely
+    and N - 1 when the centre of mass is held (the fields are tangent to that
+    constraint because sum_i m_i u_i = 0, and the divergence along the
+    constraint drops the one normal direction per axis). Fields and sums:
+      e_a u_ib          S_ab = ndiv d_ab + beta sum_i u_ib F_ia
+      (U - U0) e_a u_ib T_ab = (U - U0) S_ab - sum_i F_ia u_ib
+      (U - U0)^2 e_a u_ib Q_ab = (U - U0)^2 S_ab - 2 (U - U0) sum_i F_ia u_ib
+    (div of h(U) v = h'(U) grad U . v + h(U) div v, grad_i U = -F_i).
+    Returns [27]: S, T, Q flattened row-major over (a, b)."""
+    N = len(u)
+    nd = N if ndiv is None else ndiv
+    M = np.einsum("ia,ib->ab", F, u)          # M[a, b] = sum_i F_ia u_ib
+    S = nd * np.eye(3) + beta * M
+    dU = U - U0
+    T = dU * S - M
+    Q = dU ** 2 * S - 2.0 * dU * M
+    return np.concatenate([S.ravel(), T.ravel(), Q.ravel()])

@@ -1,0 +1,196 @@
+"""Control-variate kernels (include/mdelast_cv.h) against the CPU restatement
+(oracle/cv_oracle.py) and an FP64 autograd reference, and the trained variate
+on molecular-dynamics frames of the device loop.
+
+Tolerances: the products round their operands to bfloat16 and accumulate in
+FP32; the oracle rounds the same operands the same way, so forward values and
+tangents agree to FP32 accumulation (norm-wise 1e-4; an operand that lands on
+the other side of a bfloat16 rounding boundary moves one element by 2^-9
+relative). Gradients are compared with an FP64 reference that does not round,
+so they agree to the bfloat16 operand precision (norm-wise 2e-2).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import cv_oracle as O
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from mdelast_amd import cv  # noqa: E402
+
+DEV = "cuda:0"
+
+
+def _cell(seed=0, edge=4, amp=0.12, a=3.667):
+    basis = np.array([[0, 0, 0], [.5, .5, 0], [.5, 0, .5], [0, .5, .5]])
+    idx = np.indices((edge,) * 3).reshape(3, -1).T
+    sites = ((idx[:, None, :] + basis[None]) * a).reshape(-1, 3)
+    rs = np.random.RandomState(seed)
+    pos = sites + rs.normal(scale=amp, size=sites.shape)
+    types = (rs.rand(len(sites)) < 0.25).astype(np.int32)
+    return sites, pos, types, np.full(3, edge * a)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def test_descriptors_match_the_oracle():
+    _, pos, types, box = _cell()
+    print("descriptors: 256 atoms", flush=True)
+    D, J = cv.descriptors(torch.tensor(pos, device=DEV), torch.tensor(types, device=DEV), box)
+    Do, Jo = O.descriptors(pos, types, box)
+    assert np.allclose(D.cpu().numpy(), Do, rtol=2e-5, atol=2e-6 * np.abs(Do).max())
+    assert np.allclose(J.cpu().numpy(), Jo, rtol=2e-5, atol=2e-6 * np.abs(Jo).max())
+
+
+def test_descriptors_refuse_a_cutoff_beyond_half_the_box():
+    _, pos, types, box = _cell(edge=3)
+    with pytest.raises(Exception, match="half the box"):
+        cv.descriptors(torch.tensor(pos, device=DEV), torch.tensor(types, device=DEV), box)
+
+
+@pytest.mark.parametrize("R,K,N,transB", [(100, 64, 48, 0), (37, 64, 64, 1), (16, 16, 16, 0),
+                                          (2049, 64, 16, 1)])
+def test_gemm_matches_bf16_operands_in_fp64(R, K, N, transB):
+    rs = np.random.RandomState(R)
+    A = rs.normal(size=(R, K)).astype(np.float32)
+    W = rs.normal(size=(N, K) if transB else (K, N)).astype(np.float32)
+    b = rs.normal(size=N).astype(np.float32)
+    print(f"gemm {R}x{K}x{N} transB={transB}", flush=True)
+    C = cv._gemm(torch.tensor(A, device=DEV), torch.tensor(W, device=DEV),
+                 torch.tensor(b, device=DEV), transB=bool(transB)).cpu().numpy()
+    Wr = O.bf16_round(W)
+    ref = O.bf16_round(A) @ (Wr.T if transB else Wr) + b
+    scale = np.abs(O.bf16_round(A)) @ np.abs(Wr.T if transB else Wr)
+    assert np.all(np.abs(C - ref) <= 1e-6 * scale + 1e-6)
+
+
+@pytest.mark.parametrize("R,rows", [(1000, 512), (48, 16), (33, 32)])
+def test_gemm_tn_and_colsum(R, rows):
+    rs = np.random.RandomState(R)
+    A = rs.normal(size=(R, 64)).astype(np.float32)
+    B = rs.normal(size=(R, 32)).astype(np.float32)
+    print(f"gemm_tn R={R} rows={rows}", flush=True)
+    C = cv._gemm_tn(torch.tensor(A, device=DEV), torch.tensor(B, device=DEV), rows=rows)
+    ref = O.bf16_round(A).T @ O.bf16_round(B)
+    scale = np.abs(O.bf16_round(A)).T @ np.abs(O.bf16_round(B))
+    assert np.all(np.abs(C.cpu().numpy() - ref) <= 1e-6 * scale + 1e-6)
+    out = torch.empty(32, device=DEV)
+    cv._check(cv.nat.lib().mdl_cv_colsum(cv._p(torch.tensor(B, device=DEV)), cv._p(out), R, 32,
+                                         cv._stream()))
+    assert np.allclose(out.cpu().numpy(), B.astype(np.float64).sum(0), rtol=1e-5, atol=1e-4)
+
+
+def _net(seed=3):
+    net = cv.SteinNet(seed=seed, device=DEV)
+    with torch.no_grad():
+        net.W3.mul_(30.0)            # outputs of order one
+        net.b1.normal_(0, 0.1)
+        net.b3[:9].normal_(0, 0.1)
+    return net
+
+
+def test_network_forward_and_tangents_match_the_oracle():
+    _, pos, types, box = _cell(seed=1)
+    D, J = O.descriptors(pos, types, box)
+    net = _net()
+    print("network forward: 256 atoms", flush=True)
+    g, gt = net.forward(torch.tensor(D, dtype=torch.float32, device=DEV),
+                        torch.tensor(J, dtype=torch.float32, device=DEV))
+    go, gto = O.net_forward(net.numpy_params(), D.astype(np.float32), J.astype(np.float32),
+                            emulate_bf16=True)
+    assert _rel(g.detach().cpu().numpy(), go) < 1e-4
+    assert _rel(gt.detach().cpu().numpy(), gto) < 1e-4
+    # and the stein sums
+    F = np.random.RandomState(2).normal(size=(len(pos), 3))
+    s = net.stein(torch.tensor(D, dtype=torch.float32, device=DEV),
+                  torch.tensor(J, dtype=torch.float32, device=DEV),
+                  torch.tensor(F, dtype=torch.float32, device=DEV), 38.7, 1)
+    so = O.local_stein(go, gto, F, 38.7, net.out_scale)
+    assert _rel(s.detach().cpu().numpy()[0], so) < 1e-3
+
+
+def _torch_reference(params, D, J, F, beta, out_scale, scale=0.1):
+    """FP64 autograd reference of the Stein sums (CPU, no rounding)."""
+    W1, b1, W2, b2, W3, b3 = params
+    X, T = D * scale, J * scale
+
+    def layer(X, T, W, b, act):
+        Z = X @ W + b
+        Zt = torch.einsum("amk,kn->amn", T, W)
+        if act == 0:
+            Y = torch.tanh(Z)
+            return Y, (1 - Y ** 2)[None] * Zt
+        return Z, Zt
+
+    h, ht = layer(X, T, W1, b1, 0)
+    h, ht = layer(h, ht, W2, b2, 0)
+    o, ot = layer(h, ht, W3, b3, 1)
+    g, gt = o[:, :9], ot[:, :, :9]
+    s = []
+    for c in range(3):
+        div = gt[0, :, 3 * c] + gt[1, :, 3 * c + 1] + gt[2, :, 3 * c + 2]
+        s.append(out_scale * (div.sum() + beta * (g[:, 3 * c:3 * c + 3] * F).sum()))
+    return torch.stack(s)
+
+
+def test_backward_matches_fp64_autograd():
+    _, pos, types, box = _cell(seed=4)
+    D, J = O.descriptors(pos, types, box)
+    F = np.random.RandomState(5).normal(size=(len(pos), 3))
+    net = _net(seed=6)
+    wts = torch.tensor([0.7, -1.3, 0.4], dtype=torch.float64)
+    print("network backward: 256 atoms", flush=True)
+    s = net.stein(torch.tensor(D, dtype=torch.float32, device=DEV),
+                  torch.tensor(J, dtype=torch.float32, device=DEV),
+                  torch.tensor(F, dtype=torch.float32, device=DEV), 38.7, 1)[0]
+    loss = (s.double().cpu() * wts).sum() + (s.double() ** 2).sum().cpu()
+    loss.backward()
+    grads = [p.grad.detach().double().cpu().numpy() for p in net.parameters()]
+    ref_p = [torch.tensor(O.bf16_round(p) if p.ndim == 2 else p, requires_grad=True)
+             for p in net.numpy_params()]
+    sr = _torch_reference(ref_p, torch.tensor(D), torch.tensor(J), torch.tensor(F), 38.7,
+                          net.out_scale)
+    lr = (sr * wts).sum() + (sr ** 2).sum()
+    lr.backward()
+    for k, (g, p) in enumerate(zip(grads, ref_p)):
+        ref = p.grad.numpy()
+        if k == 5:                      # padded outputs have no gradient
+            assert np.all(g[9:] == 0)
+            g, ref = g[:9], ref[:9]
+        if k == 4:
+            assert np.all(g[:, 9:] == 0)
+            g, ref = g[:, :9], ref[:, :9]
+        assert _rel(g, ref) < 2e-2, k
+
+
+def test_stein_identity_and_fallback_on_md_frames():
+    """500-atom alpha-brass NVT: the trained control variate keeps a zero mean
+    on a fresh trajectory (Stein), switched-off heads reproduce the plain
+    estimate exactly, and the variance of no enabled head grows."""
+    print("brass 500 atoms: frames", flush=True)
+    drv, sites, types = cv.brass_driver(edge=5)
+    drv.run(4000, sample_stride=4000)
+    fr = cv.collect_frames(drv, 500, 10, 300.0)
+    train, valid = cv.subset(fr, slice(0, 400)), cv.subset(fr, slice(400, None))
+    cvm = cv.ControlVariate(sites, fr.box, 300.0, device=DEV)
+    cvm.set_types(types)
+    cvm.cache(train)
+    print("brass 500 atoms: training", flush=True)
+    cvm.fit(train, valid, epochs=3)
+    d, _, _ = cv.brass_driver(edge=5, vel_seed=999)
+    d.run(4000, sample_stride=4000)
+    ev = cv.collect_frames(d, 1280, 10, 300.0)
+    s_all = cvm.controls(ev, enabled_only=False)
+    se = np.array([cv.block_bootstrap_se(s_all[:, h]) for h in range(3)])
+    print("mean s", s_all.mean(0), "se", se, flush=True)
+    assert np.all(np.abs(s_all.mean(0)) < 4.5 * se)
+    cvm.enabled[:] = False
+    Cp, sep = cv.estimate(ev)
+    Cc, sec = cv.estimate(ev, cvm.controls(ev))
+    assert np.array_equal(Cp, Cc) and np.array_equal(sep, sec)
