@@ -18,16 +18,30 @@
 extern "C" {
 #endif
 
-/* Radial-basis descriptors of N atoms of two species (types 0/1) and their
- * derivatives along each atom's own coordinates:
+/* Radial-basis descriptors of N atoms of two species (types 0/1), their
+ * derivatives along each atom's own coordinates and their Laplacians there:
  *   D[i][t*32 + k]    = sum_{j of type t, r_ij < rc} phi_k(r_ij)
  *   J[a][i][t*32 + k] = d D[i][t*32 + k] / d x_{i,a}
+ *   L[i][t*32 + k]    = sum_a d^2 D[i][t*32 + k] / d x_{i,a}^2
  * phi_k(r) = exp(-(r - mu0 - k dmu)^2 / (2 width^2)) (cos(pi r / rc) + 1) / 2,
  * k = 0..31. pos: N*3 FP64 in A (minimum image in the box), box: 3 FP64 on the
- * HOST, D: [N][64], J: [3][N][64]. MDL_ERR_ARG unless 2 rc <= each box edge. */
+ * HOST, D: [N][64], J: [3][N][64], L: [N][64] or NULL. MDL_ERR_ARG unless
+ * 2 rc <= each box edge. Brute force over all pairs. */
 mdl_status mdl_cv_descriptors(const double *pos, const int32_t *types, int32_t N,
                               const double *box, float rc, float mu0, float dmu, float width,
-                              float *D, float *J, void *stream);
+                              float *D, float *J, float *L, void *stream);
+
+/* The same descriptors of the driver's current configuration, from its
+ * neighbour list (lengths in A; the driver's species index is the type).
+ * rc must not exceed the list's guaranteed cut-off and the driver must have
+ * two species. Runs on the driver's stream and returns when done. */
+mdl_status mdl_driver_cv_descriptors(mdl_driver *d, float rc, float mu0, float dmu, float width,
+                                     float *D, float *J, float *L);
+
+/* Device-to-device copy of a per-atom array of the driver (MDL_A_*, the
+ * layout and units of mdl_driver_get_atoms) into device memory `out`;
+ * returns when done. */
+mdl_status mdl_driver_copy_atoms(mdl_driver *d, int32_t what, double *out);
 
 /* C[R][N] = A[R][K] B + bias on MFMA (bf16 operands rounded to nearest even,
  * fp32 accumulation); B = W[K][N], or W^T with W[N][K] when transB != 0.
@@ -41,8 +55,10 @@ mdl_status mdl_cv_gemm(const float *A, const float *W, int32_t transB, const flo
 mdl_status mdl_cv_gemm_tn(const float *A, const float *B, float *C, float *work, int64_t R,
                           int32_t K, int32_t N, int32_t rows_per_chunk, void *stream);
 
-/* out[n] = sum_r X[r][n], fixed order. */
-mdl_status mdl_cv_colsum(const float *X, float *out, int64_t R, int32_t N, void *stream);
+/* out[n] = sum_r X[r][n] in a fixed order (per chunk of 1024 rows into
+ * work[ceil(R/1024)][N], then over chunks). */
+mdl_status mdl_cv_colsum(const float *X, float *out, float *work, int64_t R, int32_t N,
+                         void *stream);
 
 /* Activation with the three tangents: Y = act(Z), Yt[a] = act'(Z) Zt[a]
  * (Z, Y: MN floats; Zt, Yt: [3][MN]). act 0 = tanh, 1 = identity. */

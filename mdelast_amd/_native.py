@@ -96,12 +96,15 @@ SIGNATURES = {
     "mdl_driver_kernel_times": (C.c_int32, [_vp, _dp]),
     # learned control variate (include/mdelast_cv.h); device pointers as c_void_p
     "mdl_cv_descriptors": (C.c_int32, [_vp, _vp, C.c_int32, _dp, C.c_float, C.c_float,
-                                       C.c_float, C.c_float, _vp, _vp, _vp]),
+                                       C.c_float, C.c_float, _vp, _vp, _vp, _vp]),
+    "mdl_driver_cv_descriptors": (C.c_int32, [_vp, C.c_float, C.c_float, C.c_float, C.c_float,
+                                              _vp, _vp, _vp]),
+    "mdl_driver_copy_atoms": (C.c_int32, [_vp, C.c_int32, _vp]),
     "mdl_cv_gemm": (C.c_int32, [_vp, _vp, C.c_int32, _vp, _vp, C.c_int64, C.c_int32, C.c_int32,
                                 _vp]),
     "mdl_cv_gemm_tn": (C.c_int32, [_vp, _vp, _vp, _vp, C.c_int64, C.c_int32, C.c_int32,
                                    C.c_int32, _vp]),
-    "mdl_cv_colsum": (C.c_int32, [_vp, _vp, C.c_int64, C.c_int32, _vp]),
+    "mdl_cv_colsum": (C.c_int32, [_vp, _vp, _vp, C.c_int64, C.c_int32, _vp]),
     "mdl_cv_act_forward": (C.c_int32, [_vp, _vp, _vp, _vp, C.c_int64, C.c_int32, _vp]),
     "mdl_cv_act_backward": (C.c_int32, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int64, C.c_int32,
                                         _vp]),

@@ -17,6 +17,7 @@
 #include <cstring>
 
 #include "mdl_internal.hpp"
+#include "mdl_cv.hpp"
 #include "../../include/mdelast_cv.h"
 
 namespace {
@@ -34,31 +35,24 @@ __device__ __forceinline__ short bf16_rne(float x) {
 }
 
 // ---------------------------------------------------------------------------
-// Descriptors. One wave per two atoms: lanes 0-31 take atom 2w, lanes 32-63
-// atom 2w+1, lane k%32 owns radial function k for both neighbour species.
-//   D[i][t*nb + k]      = sum_{j of species t, r_ij < rc} phi_k(r_ij)
-//   J[a][i][t*nb + k]   = sum_j phi_k'(r_ij) (x_i - x_j)_a / r_ij
-// phi_k(r) = exp(-(r - mu_k)^2 / (2 w^2)) fc(r), fc = (cos(pi r / rc) + 1) / 2.
+// Descriptors by brute force over all pairs (the reference form for positions
+// handed over by the caller; the driver's own frames use its neighbour list).
+// One wave per two atoms: lanes 0-31 take atom 2w, lanes 32-63 atom 2w+1,
+// lane k%32 owns radial function k for both neighbour species.
 // Positions in A (FP64), minimum image in FP64, basis in FP32.
 // ---------------------------------------------------------------------------
-constexpr int CV_NB = 32;
-
 __global__ __launch_bounds__(256) void k_cv_descriptors(const double *__restrict__ pos,
                                                         const int *__restrict__ types, int N,
-                                                        double Lx, double Ly, double Lz, float rc,
-                                                        float mu0, float dmu, float width,
-                                                        float *__restrict__ D,
-                                                        float *__restrict__ J) {
+                                                        double Lx, double Ly, double Lz,
+                                                        mdl::CvBasis basis, float *__restrict__ D,
+                                                        float *__restrict__ J,
+                                                        float *__restrict__ L) {
     int const lane = threadIdx.x & 63, k = lane & 31;
     int const i = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 2 + (lane >> 5);
     bool const active = i < N;
     int const ia = active ? i : 0;
     double const xi = pos[3 * ia], yi = pos[3 * ia + 1], zi = pos[3 * ia + 2];
-    float const mu = mu0 + dmu * (float)k;
-    float const iw2 = 1.f / (width * width);
-    float const rc2 = rc * rc;
-    float const pirc = 3.14159265358979f / rc;
-    float d0 = 0.f, d1 = 0.f, jx0 = 0.f, jy0 = 0.f, jz0 = 0.f, jx1 = 0.f, jy1 = 0.f, jz1 = 0.f;
+    mdl::CvAcc acc;
     for (int j = 0; j < N; ++j) {
         double sx = xi - pos[3 * j], sy = yi - pos[3 * j + 1], sz = zi - pos[3 * j + 2];
         sx -= Lx * rint(sx / Lx);
@@ -66,41 +60,12 @@ __global__ __launch_bounds__(256) void k_cv_descriptors(const double *__restrict
         sz -= Lz * rint(sz / Lz);
         float const x = (float)sx, y = (float)sy, z = (float)sz;
         float const r2 = x * x + y * y + z * z;
-        if (j == ia || !(r2 < rc2))
-            continue;   // uniform within each half-wave's atom; lanes of one atom agree
-        float const r = sqrtf(r2);
-        float const ir = 1.f / r;
-        float const u = r - mu;
-        float const g = __expf(-0.5f * u * u * iw2);
-        float const c = cosf(pirc * r), sn = sinf(pirc * r);
-        float const fc = 0.5f * (c + 1.f), dfc = -0.5f * pirc * sn;
-        float const phi = g * fc;
-        float const dphi = g * (dfc - u * iw2 * fc);
-        float const w = dphi * ir;
-        if (types[j] == 0) {
-            d0 += phi;
-            jx0 += w * x;
-            jy0 += w * y;
-            jz0 += w * z;
-        } else {
-            d1 += phi;
-            jx1 += w * x;
-            jy1 += w * y;
-            jz1 += w * z;
-        }
+        if (j == ia || !(r2 < basis.rc2))
+            continue;   // the same for the 32 lanes of an atom
+        acc.add(basis, k, types[j], x, y, z, sqrtf(r2));
     }
-    if (!active)
-        return;
-    int const F = 2 * CV_NB;
-    D[(size_t)i * F + k] = d0;
-    D[(size_t)i * F + CV_NB + k] = d1;
-    size_t const plane = (size_t)N * F;
-    J[(size_t)i * F + k] = jx0;
-    J[(size_t)i * F + CV_NB + k] = jx1;
-    J[plane + (size_t)i * F + k] = jy0;
-    J[plane + (size_t)i * F + CV_NB + k] = jy1;
-    J[2 * plane + (size_t)i * F + k] = jz0;
-    J[2 * plane + (size_t)i * F + CV_NB + k] = jz1;
+    if (active)
+        acc.store(i, k, N, D, J, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -186,15 +151,25 @@ __global__ void k_sum_chunks(const float *__restrict__ part, float *__restrict__
     out[e] = s;
 }
 
-// out[n] = sum_r X[r][n] (column sums in fixed order)
-__global__ void k_colsum(const float *__restrict__ X, float *__restrict__ out, int R, int N) {
-    int const n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N)
-        return;
+// part[c][n] = sum of X[r][n] over the rows r of chunk c (CS_ROWS rows per
+// block); a block takes 64 columns (blockIdx.y) with 4 row lanes, then adds
+// the lanes in a fixed order.
+constexpr int CS_ROWS = 1024;
+
+__global__ __launch_bounds__(256) void k_colsum_part(const float *__restrict__ X,
+                                                     float *__restrict__ part, int R, int N) {
+    __shared__ float sm[256];
+    int const n = blockIdx.y * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
+    int const r0 = blockIdx.x * CS_ROWS, r1 = min(R, r0 + CS_ROWS);
     float s = 0.f;
-    for (int r = 0; r < R; ++r)
-        s += X[(size_t)r * N + n];
-    out[n] = s;
+    if (n < N)
+        for (int r = r0 + g; r < r1; r += 4)
+            s += X[(size_t)r * N + n];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < 64 && n < N)
+        part[(size_t)blockIdx.x * N + n] = (sm[threadIdx.x] + sm[64 + threadIdx.x]) +
+                                           (sm[128 + threadIdx.x] + sm[192 + threadIdx.x]);
 }
 
 // Activation with tangents. act 0: tanh; 1: identity.
@@ -253,7 +228,7 @@ extern "C" {
 
 mdl_status mdl_cv_descriptors(const double *pos, const int32_t *types, int32_t N,
                               const double *box, float rc, float mu0, float dmu, float width,
-                              float *D, float *J, void *stream) {
+                              float *D, float *J, float *L, void *stream) {
     if (!pos || !types || !box || !D || !J || N < 1 || !(rc > 0.f) || !(width > 0.f)) {
         set_error("mdl_cv_descriptors: bad arguments");
         return MDL_ERR_ARG;
@@ -266,8 +241,8 @@ mdl_status mdl_cv_descriptors(const double *pos, const int32_t *types, int32_t N
     }
     int const waves = (N + 1) / 2;
     int const blocks = (waves * 64 + 255) / 256;
-    k_cv_descriptors<<<blocks, 256, 0, (hipStream_t)stream>>>(pos, types, N, box[0], box[1],
-                                                              box[2], rc, mu0, dmu, width, D, J);
+    k_cv_descriptors<<<blocks, 256, 0, (hipStream_t)stream>>>(
+        pos, types, N, box[0], box[1], box[2], mdl::cv_basis(rc, mu0, dmu, width), D, J, L);
     return launch_ok("mdl_cv_descriptors");
 }
 
@@ -304,13 +279,19 @@ mdl_status mdl_cv_gemm_tn(const float *A, const float *B, float *C, float *work,
     return launch_ok("mdl_cv_gemm_tn reduce");
 }
 
-mdl_status mdl_cv_colsum(const float *X, float *out, int64_t R, int32_t N, void *stream) {
-    if (!X || !out || R < 0 || N < 1) {
+mdl_status mdl_cv_colsum(const float *X, float *out, float *work, int64_t R, int32_t N,
+                         void *stream) {
+    if (!X || !out || !work || R < 1 || R > (1ll << 31) - 1 || N < 1) {
         set_error("mdl_cv_colsum: bad arguments");
         return MDL_ERR_ARG;
     }
-    k_colsum<<<(N + 63) / 64, 64, 0, (hipStream_t)stream>>>(X, out, (int)R, N);
-    return launch_ok("mdl_cv_colsum");
+    int const nchunk = (int)((R + CS_ROWS - 1) / CS_ROWS);
+    k_colsum_part<<<dim3(nchunk, (N + 63) / 64), 256, 0, (hipStream_t)stream>>>(X, work, (int)R,
+                                                                               N);
+    if (mdl_status s = launch_ok("mdl_cv_colsum"))
+        return s;
+    k_sum_chunks<<<(N + 255) / 256, 256, 0, (hipStream_t)stream>>>(work, out, nchunk, N);
+    return launch_ok("mdl_cv_colsum reduce");
 }
 
 mdl_status mdl_cv_act_forward(const float *Z, const float *Zt, float *Y, float *Yt, int64_t MN,

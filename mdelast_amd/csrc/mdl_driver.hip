@@ -4,6 +4,8 @@
 // C-ABI of include/mdelast.h.
 #include "mdl_kernels.hpp"
 #include "mdl_mixed.hpp"
+#include "mdl_cv.hpp"
+#include "../../include/mdelast_cv.h"
 
 #include <algorithm>
 #include <cmath>
@@ -712,6 +714,38 @@ static mdl_status create(const mdl_config *c, Driver &d) {
 
 using namespace mdl;
 
+// Control-variate descriptors of the current configuration from the full
+// neighbour list (the list holds every pair within its guaranteed cut-off).
+// One wave per two atoms, lane k%32 = radial function k; lengths in A.
+__global__ __launch_bounds__(256) void k_cv_desc_list(AtomArrays a, ListArrays l, int N,
+                                                      const DevState *__restrict__ gs,
+                                                      double to_A, CvBasis basis,
+                                                      float *__restrict__ D,
+                                                      float *__restrict__ J,
+                                                      float *__restrict__ Lp) {
+    int const lane = threadIdx.x & 63, k = lane & 31;
+    int const i = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 2 + (lane >> 5);
+    bool const active = i < N;
+    int const ia = active ? i : 0;
+    double const L[3] = {gs->L[0], gs->L[1], gs->L[2]};
+    double4 const ci = a.pos[ia];
+    int const cnt = active ? l.nnbr[ia] : 0;
+    const int *nb = l.nbr + (size_t)ia * l.stride;
+    CvAcc acc;
+    for (int q = 0; q < cnt; ++q) {
+        double4 const cj = a.pos[nb[q]];
+        double dx, dy, dz;
+        min_image(ci, cj, L, dx, dy, dz);
+        float const x = (float)(dx * to_A), y = (float)(dy * to_A), z = (float)(dz * to_A);
+        float const r2 = x * x + y * y + z * z;
+        if (!(r2 < basis.rc2))
+            continue;
+        acc.add(basis, k, (int)cj.w, x, y, z, sqrtf(r2));
+    }
+    if (active)
+        acc.store(i, k, N, D, J, Lp);
+}
+
 static Driver *D(mdl_driver *p) { return reinterpret_cast<Driver *>(p); }
 
 extern "C" {
@@ -983,6 +1017,47 @@ mdl_status mdl_driver_get_atoms(mdl_driver *p, int32_t what, double *out) {
     k_materialize<<<(d->N + 255) / 256, 256, 0, d->stream>>>(d->a, d->N, d->gs, what, d->d_stage);
     MDL_HIP(hipGetLastError());
     MDL_HIP(hipMemcpyAsync(out, d->d_stage, sizeof(double) * n, hipMemcpyDeviceToHost, d->stream));
+    MDL_HIP(hipStreamSynchronize(d->stream));
+    return MDL_OK;
+}
+
+mdl_status mdl_driver_cv_descriptors(mdl_driver *p, float rc, float mu0, float dmu, float width,
+                                     float *Dp, float *Jp, float *Lp) {
+    Driver *d = D(p);
+    if (!d || !Dp || !Jp || !(rc > 0.f) || !(width > 0.f))
+        return bad("mdl_driver_cv_descriptors: bad arguments");
+    if (d->ntypes > 2)
+        return bad("mdl_driver_cv_descriptors: the descriptors have two species channels");
+    // lengths of the driver are in cm; the list holds every pair within hs.rc
+    if (!((double)rc * 1e-8 <= d->hs.rc))
+        return bad("mdl_driver_cv_descriptors: rc exceeds the neighbour list's cut-off");
+    MDL_HIP(hipSetDevice(d->device));
+    if (d->failed)
+        return bad("the driver stopped on an earlier error");
+    int const waves = (d->N + 1) / 2;
+    k_cv_desc_list<<<(waves * 64 + 255) / 256, 256, 0, d->stream>>>(
+        d->a, d->l, d->N, d->gs, 1e8, cv_basis(rc, mu0, dmu, width), Dp, Jp, Lp);
+    MDL_HIP(hipGetLastError());
+    MDL_HIP(hipStreamSynchronize(d->stream));
+    return MDL_OK;
+}
+
+mdl_status mdl_driver_copy_atoms(mdl_driver *p, int32_t what, double *out) {
+    Driver *d = D(p);
+    if (!d || !out)
+        return bad("null argument");
+    if (what < MDL_A_POSITIONS || what > MDL_A_EKIN)
+        return bad("unknown per-atom array");
+    MDL_HIP(hipSetDevice(d->device));
+    int const n = (what <= MDL_A_FORCES) ? 3 * d->N : d->N;
+    if (what == MDL_A_POSITIONS && d->cycle_host == 0) {
+        // the ctor keeps the given positions (md_driver.cpp:31)
+        MDL_HIP(hipMemcpyAsync(out, d->init_pos.data(), sizeof(double) * n,
+                               hipMemcpyHostToDevice, d->stream));
+    } else {
+        k_materialize<<<(d->N + 255) / 256, 256, 0, d->stream>>>(d->a, d->N, d->gs, what, out);
+        MDL_HIP(hipGetLastError());
+    }
     MDL_HIP(hipStreamSynchronize(d->stream));
     return MDL_OK;
 }

@@ -3,30 +3,48 @@
 New work (no counterpart in the reference). For the canonical distribution
 p(x) ~ exp(-beta U(x)) and any smooth vector field g that is periodic in the
 box, E_p[div g - beta g . grad U] = 0 (integrate div(g p) over the torus). So
-with F = -grad U,
+with F = -grad U the Stein sum
 
-    s_c(x) = sum_i [ div_i g^c_i(x) + beta g^c_i(x) . F_i ],     E[s_c] = 0,
+    s[g](x) = sum_i [ div_i g_i(x) + beta g_i(x) . F_i ]
 
-and the controlled estimate mean(y_c - s_c) of a head c has the same
-expectation as the plain mean(y_c), for any g. Here g^c_i is a per-atom
-3-vector from a small network on radial-basis descriptors of atom i's
-neighbourhood (32 Gaussians per neighbour species, 2 hidden tanh layers of
-width 64, heads for C11, C12, C44). The network is trained to minimise
-Var(y_c - s_c) on frames of the plain run; the plain estimator stays the
-judge, and a head whose variance the variate does not cut is switched off.
+has zero mean, and mean(y - sum_q theta_q s[g_q]) estimates the same C_ij as
+the plain mean(y) for any fields g_q and fixed coefficients theta.
 
-The device work is hand-written HIP (include/mdelast_cv.h): descriptors and
-their derivatives along each atom's own coordinates, and a dense layer that
-carries those three tangents forward (Y = act(XW + b), Y'_a = act'(XW + b) T_a W),
-with its backward, on bf16 MFMA with fp32 accumulation. PyTorch holds the
-device memory and runs the optimiser; ``TangentDense`` is the
-torch.autograd.Function over the kernels.
+The per-frame integrand of C11 or C12 is dominated by (V/kT) ds_a ds_b, a
+product of two global stress fluctuations; a sum of local terms cannot
+follow it. The fields therefore include products with a global scalar whose
+gradient is known, h(x) = U(x) - U0 (grad_i U = -F_i):
+s[h g] = h s[g] - sum_i F_i . g_i. The fields are
+
+* the displacements from the lattice sites, e_a u_ib and (U - U0) e_a u_ib
+  (their sums need only u, F and U: ``global_features``);
+* learned fields c_i = sum_k w_k(D_i) grad_i D_ik: a per-atom network on the
+  radial-basis descriptors D_i (32 Gaussians per neighbour species, two hidden
+  tanh layers of width 64) weights the descriptor gradients, which gives
+  vectors along the bonds of atom i (equivariant). Its divergence,
+  sum_k [w_k L_k + grad_i w_k . grad_i D_k] with L the descriptor Laplacian,
+  needs the derivatives of the weights along each atom's own coordinates,
+  which the network carries forward. Six fields: per head (C11, C12, C44) one
+  used as is and one weighted by U - U0.
+
+The coefficients of each head are fitted by least squares on frames of a
+plain run after the network has been trained on them (Adam on the residual
+variance); a head whose held-out variance the variate does not cut is switched
+off, and the plain estimator stays the judge.
+
+Device work is hand-written HIP (include/mdelast_cv.h): descriptors with
+their derivatives and Laplacians from the driver's neighbour list, and the
+dense layers with their tangents (Y = act(XW + b), Y'_a = act'(XW + b) T_a W)
+and backward on bf16 MFMA with fp32 accumulation. PyTorch holds the device
+memory, runs the optimiser and the per-atom contractions; ``TangentDense`` is
+the torch.autograd.Function over the kernels.
 """
 
 from __future__ import annotations
 
 import ctypes as C
 import math
+import time
 from dataclasses import dataclass
 
 import numpy as np
@@ -37,8 +55,11 @@ from . import units as U
 NB = 32            # radial functions per neighbour species
 NF = 2 * NB        # descriptor length (two species)
 HIDDEN = 64
-NOUT = 16          # 3 heads x 3 components, padded to the MFMA tile
+NFIELD = 6         # per head: one field as is, one weighted by U - U0
+NOUT = NFIELD * NF
 HEADS = ("C11", "C12", "C44")
+NGLOBAL = 27       # global_features: S, T, Q
+NGLOBAL_USED = 18  # S and T enter the controls
 
 
 def _torch():
@@ -68,17 +89,41 @@ class Basis:
 
 
 def descriptors(pos_A, types, box_A, basis: Basis = Basis()):
-    """D [N, 64] and J [3, N, 64] on the device for FP64 positions in A."""
+    """D [N, 64], J [3, N, 64] and L [N, 64] on the device for FP64 positions
+    in A (brute force over pairs)."""
     torch = _torch()
     pos = pos_A.contiguous()
     N = pos.shape[0]
     D = torch.empty((N, NF), device=pos.device, dtype=torch.float32)
     J = torch.empty((3, N, NF), device=pos.device, dtype=torch.float32)
+    L = torch.empty((N, NF), device=pos.device, dtype=torch.float32)
     box = (C.c_double * 3)(*[float(b) for b in box_A])
     _check(nat.lib().mdl_cv_descriptors(_p(pos), _p(types.contiguous()), N, box,
                                         basis.rc, basis.mu0, basis.dmu, basis.width,
-                                        _p(D), _p(J), _stream()))
-    return D, J
+                                        _p(D), _p(J), _p(L), _stream()))
+    return D, J, L
+
+
+def driver_descriptors(drv, basis: Basis = Basis(), device="cuda:0"):
+    """D, J, L of the driver's current configuration from its neighbour list."""
+    torch = _torch()
+    N = drv.N
+    D = torch.empty((N, NF), device=device, dtype=torch.float32)
+    J = torch.empty((3, N, NF), device=device, dtype=torch.float32)
+    L = torch.empty((N, NF), device=device, dtype=torch.float32)
+    torch.cuda.current_stream().synchronize()
+    _check(nat.lib().mdl_driver_cv_descriptors(drv.handle, basis.rc, basis.mu0, basis.dmu,
+                                               basis.width, _p(D), _p(J), _p(L)))
+    return D, J, L
+
+
+def driver_atoms(drv, what, device="cuda:0"):
+    """A per-atom [N, 3] array of the driver (MDL_A_* layout, CGS) copied on the device."""
+    torch = _torch()
+    out = torch.empty((drv.N, 3), device=device, dtype=torch.float64)
+    torch.cuda.current_stream().synchronize()
+    _check(nat.lib().mdl_driver_copy_atoms(drv.handle, nat.A_INDEX[what], _p(out)))
+    return out
 
 
 def _gemm(A, W, bias=None, transB=False):
@@ -102,6 +147,16 @@ def _gemm_tn(A, B, rows=512):
     work = torch.empty((nchunk, K, N), device=A.device, dtype=torch.float32)
     out = torch.empty((K, N), device=A.device, dtype=torch.float32)
     _check(nat.lib().mdl_cv_gemm_tn(_p(A), _p(B), _p(out), _p(work), R, K, N, rows, _stream()))
+    return out
+
+
+def _colsum(X):
+    torch = _torch()
+    X = X.contiguous()
+    R, N = X.shape
+    work = torch.empty(((R + 1023) // 1024, N), device=X.device, dtype=torch.float32)
+    out = torch.empty((N,), device=X.device, dtype=torch.float32)
+    _check(nat.lib().mdl_cv_colsum(_p(X), _p(out), _p(work), R, N, _stream()))
     return out
 
 
@@ -142,9 +197,7 @@ class TangentDense:
                 gT = _gemm(gZt.reshape(3 * M, N), W, transB=True).reshape(3, M, K)
                 gW = _gemm_tn(torch.cat([X, T.reshape(3 * M, K)]),
                               torch.cat([gZ, gZt.reshape(3 * M, N)]))
-                gb = torch.empty((N,), device=X.device, dtype=torch.float32)
-                _check(nat.lib().mdl_cv_colsum(_p(gZ), _p(gb), M, N, _stream()))
-                return gX, gT, gW, gb, None
+                return gX, gT, gW, _colsum(gZ), None
 
         return _F
 
@@ -158,9 +211,10 @@ class TangentDense:
 
 
 class SteinNet:
-    """g: descriptors -> 3 heads x 3-vectors; parameters in FP32 (rounded to
-    bf16 at the products). The Stein sums are scaled by ``out_scale`` so that
-    outputs of order one give sums of order GPa."""
+    """Weights of the learned fields: descriptors -> 6 fields x 64 weights per
+    atom, with their tangents along each atom's own coordinates. Parameters in
+    FP32, rounded to bf16 at the products. Field sums are scaled by
+    ``out_scale``."""
 
     def __init__(self, seed=0, device="cuda:0", scale=0.1, out_scale=1e-2):
         torch = _torch()
@@ -173,9 +227,7 @@ class SteinNet:
         self.b1 = torch.zeros(HIDDEN, device=device, requires_grad=True)
         self.W2 = lin(HIDDEN, HIDDEN, 1.0 / math.sqrt(HIDDEN))
         self.b2 = torch.zeros(HIDDEN, device=device, requires_grad=True)
-        W3 = torch.randn((HIDDEN, NOUT), generator=gen) * (1e-2 / math.sqrt(HIDDEN))
-        W3[:, 9:] = 0.0
-        self.W3 = W3.to(device).requires_grad_()
+        self.W3 = lin(HIDDEN, NOUT, 1e-2 / math.sqrt(HIDDEN))
         self.b3 = torch.zeros(NOUT, device=device, requires_grad=True)
         self.scale = scale
         self.out_scale = out_scale
@@ -187,23 +239,34 @@ class SteinNet:
         return [p.detach().double().cpu().numpy() for p in self.parameters()]
 
     def forward(self, D, J):
-        """g [M, 9] and its tangents [3, M, 9] (d g / d x_{i,a} of each atom's own a)."""
+        """w [M, 6, 64] and its tangents wt [3, M, 6, 64]."""
         X, T = D * self.scale, J * self.scale
         h, ht = TangentDense.apply(X, T, self.W1, self.b1, 0)
         h, ht = TangentDense.apply(h, ht, self.W2, self.b2, 0)
         o, ot = TangentDense.apply(h, ht, self.W3, self.b3, 1)
-        return o[:, :9], ot[:, :, :9]
+        M = D.shape[0]
+        return o.view(M, NFIELD, NF), ot.view(3, M, NFIELD, NF)
 
-    def stein(self, D, J, F, beta, nframes):
-        """s [nframes, 3]: per head out_scale * sum_i (div_i g_i + beta g_i . F_i).
-        D [B*N, 64], J [3, B*N, 64], F [B*N, 3] (eV/A), beta in 1/eV."""
+    def field_sums(self, D, J, L, F, beta, nframes):
+        """Per frame and field: A = sum_i div_i c_i + beta c_i . F_i and
+        C = sum_i c_i . F_i, both times out_scale ([nframes, 6] each), for
+        D [B*N, 64], J [3, B*N, 64], L [B*N, 64], F [B*N, 3] (eV/A), beta in 1/eV."""
         torch = _torch()
-        g, gt = self.forward(D, J)
-        div = torch.stack([gt[0, :, 3 * c] + gt[1, :, 3 * c + 1] + gt[2, :, 3 * c + 2]
-                           for c in range(3)], dim=1)                   # [M, 3]
-        gf = torch.stack([(g[:, 3 * c:3 * c + 3] * F).sum(1) for c in range(3)], dim=1)
-        per_atom = div + beta * gf
-        return self.out_scale * per_atom.reshape(nframes, -1, 3).sum(1)
+        w, wt = self.forward(D, J)
+        c = torch.einsum("mfk,amk->mfa", w, J)
+        div = torch.einsum("mfk,mk->mf", w, L) + torch.einsum("amfk,amk->mf", wt, J)
+        cF = torch.einsum("mfa,ma->mf", c, F)
+        A = (div + beta * cF).view(nframes, -1, NFIELD).sum(1)
+        Cs = cF.view(nframes, -1, NFIELD).sum(1)
+        return self.out_scale * A, self.out_scale * Cs
+
+
+def net_features(A, Cs, dU):
+    """[B, 3, 2] per head: the Stein sum of its plain field and that of its
+    U-weighted field, dU A - C."""
+    torch = _torch()
+    A, Cs = A.double(), Cs.double()
+    return torch.stack([A[:, 0::2], dU[:, None] * A[:, 1::2] - Cs[:, 1::2]], dim=2)
 
 
 # ---------------------------------------------------------------------------
@@ -289,15 +352,15 @@ def block_bootstrap_se(x, nblocks=64, nresample=2000, seed=0):
 # ---------------------------------------------------------------------------
 @dataclass
 class Frames:
-    """Sampled configurations of one NVT run (host arrays, A / eV / GPa)."""
-    pos: np.ndarray          # [n, N, 3]
-    forces: np.ndarray       # [n, N, 3]
+    """Scalars of the sampled steps of one NVT run (A / eV / GPa); with
+    ``store`` the per-atom inputs of the controls stay on the device."""
     epot: np.ndarray         # [n] configurational energy
     born_stress: np.ndarray  # [n, 6]
     born_elast: np.ndarray   # [n, 6, 6]
     box: np.ndarray          # [3]
     T: float
     rho_n: float             # atoms per A^3
+    store: dict = None       # D [n,N,64], J [n,3,N,64], L [n,N,64], u [n,N,3], F [n,N,3]
 
     @property
     def n(self):
@@ -309,7 +372,7 @@ class Frames:
 
     def y(self, sigma_mean=None):
         V = float(np.prod(self.box))
-        # stresses in GPa, V/kT in A^3/eV: 1 GPa A^3 / eV = 1/160.2176...
+        # stresses in GPa, V/kT in A^3/eV: 1 GPa A^3 = 1e-21 J
         f = 1e9 * 1e-30 / (U.ev * 1e-7)
         return head_integrands(self.born_stress, self.born_elast, V * f, self.kT, sigma_mean)
 
@@ -318,38 +381,69 @@ class Frames:
         return kinetic_terms(self.rho_n, self.kT) / f
 
 
-def collect_frames(driver, nframes, stride, T):
-    """Advance ``driver`` nframes x stride steps and keep positions, forces,
-    energies and the Born stress and elasticity of every stride-th step."""
-    N = driver.N
-    pos = np.zeros((nframes, N, 3))
-    frc = np.zeros((nframes, N, 3))
+def subset(fr: Frames, sl) -> Frames:
+    st = None if fr.store is None else {k: v[sl] for k, v in fr.store.items()}
+    return Frames(fr.epot[sl], fr.born_stress[sl], fr.born_elast[sl], fr.box, fr.T, fr.rho_n, st)
+
+
+def _record_scalars(r, N):
+    return (r[nat.S_INDEX["epot_atom"]] * N / U.ev,
+            r[nat.S_BORN_STRESS:nat.S_BORN_STRESS + 6] / U.GPa,
+            r[nat.S_BORN_ELAST:nat.S_BORN_ELAST + 36].reshape(6, 6) / U.GPa)
+
+
+def frame_inputs(drv, sites, box, basis: Basis = Basis(), device="cuda:0"):
+    """The per-atom inputs of the controls for the driver's current step, all on
+    the device: descriptors, displacements from the sites (A) and forces (eV/A)."""
+    torch = _torch()
+    D, J, L = driver_descriptors(drv, basis, device)
+    pos = driver_atoms(drv, "positions", device) / U.angstrom
+    F = driver_atoms(drv, "forces", device) * (U.angstrom / U.ev)
+    u = displacements(pos, sites, box)
+    return D, J, L, u, F
+
+
+def collect_frames(drv, nframes, stride, T, sites=None, basis: Basis = Basis(),
+                   device="cuda:0"):
+    """Advance the driver nframes x stride steps and keep the scalars of every
+    stride-th step, and with ``sites`` its per-atom inputs on the device."""
+    torch = _torch()
+    N = drv.N
     ep = np.zeros(nframes)
     sb = np.zeros((nframes, 6))
     cb = np.zeros((nframes, 6, 6))
+    box = np.diag(drv.box.h) / U.angstrom
+    store = None
+    if sites is not None:
+        st = torch.as_tensor(np.asarray(sites), dtype=torch.float64, device=device)
+        store = dict(D=torch.empty((nframes, N, NF), device=device),
+                     J=torch.empty((nframes, 3, N, NF), device=device),
+                     L=torch.empty((nframes, N, NF), device=device),
+                     u=torch.empty((nframes, N, 3), device=device, dtype=torch.float64),
+                     F=torch.empty((nframes, N, 3), device=device, dtype=torch.float64))
     for k in range(nframes):
-        rec = driver.run(stride, sample_stride=stride)
-        r = rec[-1]
-        pos[k] = driver.positions_array() / U.angstrom
-        frc[k] = driver.forces_array() * U.angstrom / U.ev
-        ep[k] = r[nat.S_INDEX["epot_atom"]] * N / U.ev
-        sb[k] = r[nat.S_BORN_STRESS:nat.S_BORN_STRESS + 6] / U.GPa
-        cb[k] = r[nat.S_BORN_ELAST:nat.S_BORN_ELAST + 36].reshape(6, 6) / U.GPa
-    box = np.diag(driver.box.h) / U.angstrom
-    return Frames(pos, frc, ep, sb, cb, box, T, N / float(np.prod(box)))
+        r = drv.run(stride, sample_stride=stride)[-1]
+        ep[k], sb[k], cb[k] = _record_scalars(r, N)
+        if store is not None:
+            D, J, L, u, F = frame_inputs(drv, st, box, basis, device)
+            store["D"][k], store["J"][k], store["L"][k] = D, J, L
+            store["u"][k], store["F"][k] = u, F
+    return Frames(ep, sb, cb, box, T, N / float(np.prod(box)), store)
 
 
 # ---------------------------------------------------------------------------
 # the control variate
 # ---------------------------------------------------------------------------
 class ControlVariate:
-    """s_h(x) = theta_h . t(x) + phi_h s_loc,h(x) per head h: t are the 27 global
-    Stein sums (scaled by their training spread), s_loc the learned local sum.
+    """s_h(x) = theta_h . X_h(x) per head h, X_h: the 18 global Stein sums (scaled
+    by their training spread) and the two learned-field sums of the head.
     Heads whose held-out variance the variate does not cut are switched off
     (s_h = 0), so the controlled estimate falls back to the plain one."""
 
+    NX = NGLOBAL_USED + 2
+
     def __init__(self, sites, box, T, device="cuda:0", seed=0, basis: Basis = Basis(),
-                 use_global=True, use_net=True):
+                 use_net=True):
         torch = _torch()
         self.device = device
         self.sites = torch.as_tensor(np.asarray(sites), dtype=torch.float64, device=device)
@@ -357,138 +451,115 @@ class ControlVariate:
         self.beta = 1.0 / (U.k_boltz * T / U.ev)
         self.basis = basis
         self.net = SteinNet(seed=seed, device=device)
-        self.use_global, self.use_net = use_global, use_net
-        self.t_scale = np.ones(NGLOBAL)
+        self.use_net = use_net
+        self.t_scale = np.ones(NGLOBAL_USED)
         self.U0 = 0.0
-        self.coef = np.zeros((3, NGLOBAL + 1))
+        self.coef = np.zeros((3, self.NX))
         self.enabled = np.ones(3, dtype=bool)
-        self.y_mean = np.zeros(3)
 
-    # -- per-frame inputs on the device --------------------------------------
-    def cache(self, fr: Frames):
-        """Keep the descriptors of every frame of ``fr`` on the device (4 MB per
-        frame of 4000 atoms) so that training epochs do not recompute them."""
+    def _x(self, D, J, L, u, F, Ut, grad=False):
+        """X [B, 3, NX] (torch, FP64) for B frames of per-atom inputs."""
         torch = _torch()
-        pos = torch.as_tensor(fr.pos, dtype=torch.float64, device=self.device)
-        Ds, Js = [], []
-        for b in range(fr.n):
-            D, J = descriptors(pos[b], self._types, self.box, self.basis)
-            Ds.append(D)
-            Js.append(J)
-        fr._dev = (torch.stack(Ds), torch.stack(Js))       # [n, N, 64], [n, 3, N, 64]
-
-    def _inputs(self, fr: Frames, sl):
-        torch = _torch()
-        pos = torch.as_tensor(fr.pos[sl], dtype=torch.float64, device=self.device)
-        F64 = torch.as_tensor(fr.forces[sl], dtype=torch.float64, device=self.device)
-        B, N, _ = pos.shape
-        if getattr(fr, "_dev", None) is not None:
-            D = fr._dev[0][sl].reshape(B * N, NF)
-            J = fr._dev[1][sl].permute(1, 0, 2, 3).reshape(3, B * N, NF)
+        B, N, _ = u.shape
+        t = global_features(u, F, Ut, self.U0, self.beta, N - 1)[:, :NGLOBAL_USED]
+        t = t / torch.as_tensor(self.t_scale, dtype=t.dtype, device=t.device)
+        if self.use_net:
+            with torch.set_grad_enabled(grad):
+                A, Cs = self.net.field_sums(D.reshape(B * N, NF),
+                                            J.permute(1, 0, 2, 3).reshape(3, B * N, NF),
+                                            L.reshape(B * N, NF), F.reshape(B * N, 3).float(),
+                                            self.beta, B)
+                nx = net_features(A, Cs, Ut - self.U0)
         else:
-            Ds, Js = [], []
-            for b in range(B):
-                D, J = descriptors(pos[b], self._types, self.box, self.basis)
-                Ds.append(D)
-                Js.append(J)
-            D = torch.cat(Ds, 0)
-            J = torch.cat(Js, 1)
-        u = displacements(pos, self.sites, self.box)
+            nx = torch.zeros((B, 3, 2), dtype=t.dtype, device=t.device)
+        return torch.cat([t[:, None, :].expand(B, 3, NGLOBAL_USED), nx], dim=2)
+
+    def _batch(self, fr: Frames, sl, grad=False):
+        torch = _torch()
+        st = fr.store
         Ut = torch.as_tensor(fr.epot[sl], dtype=torch.float64, device=self.device)
-        t = global_features(u, F64, Ut, self.U0, self.beta, N - 1)
-        return D, J, F64.reshape(B * N, 3).float(), t
-
-    def set_types(self, types):
-        torch = _torch()
-        self._types = torch.as_tensor(np.asarray(types, dtype=np.int32), device=self.device)
-
-    def local_sums(self, fr: Frames, sl, grad=False):
-        torch = _torch()
-        D, J, F, t = self._inputs(fr, sl)
-        B = t.shape[0]
-        if not self.use_net:
-            return torch.zeros((B, 3), device=self.device), t
-        with torch.set_grad_enabled(grad):
-            s = self.net.stein(D, J, F, self.beta, B)
-        return s, t
+        return self._x(st["D"][sl], st["J"][sl], st["L"][sl], st["u"][sl], st["F"][sl], Ut, grad)
 
     def features(self, fr: Frames, batch=8):
-        """[n, 3, 28] per head: the scaled global sums and the local sum."""
-        out = np.zeros((fr.n, 3, NGLOBAL + 1))
+        out = np.zeros((fr.n, 3, self.NX))
         for k0 in range(0, fr.n, batch):
             sl = slice(k0, min(fr.n, k0 + batch))
-            s, t = self.local_sums(fr, sl)
-            tt = t.cpu().numpy() / self.t_scale
-            if not self.use_global:
-                tt[:] = 0.0
-            out[sl, :, :NGLOBAL] = tt[:, None, :]
-            out[sl, :, NGLOBAL] = s.detach().double().cpu().numpy()
+            out[sl] = self._batch(fr, sl).detach().cpu().numpy()
         return out
 
-    def _fit_linear(self, X, yc, ridge=1e-6):
-        coef = np.zeros((3, NGLOBAL + 1))
+    @staticmethod
+    def _fit_linear(X, yc, ridge=1e-6):
+        coef = np.zeros((3, X.shape[2]))
         for h in range(3):
             A = X[:, h, :]
             G = A.T @ A
-            G += ridge * np.trace(G) / len(G) * np.eye(len(G))
+            G += ridge * max(np.trace(G), 1e-300) / len(G) * np.eye(len(G))
             coef[h] = np.linalg.solve(G, A.T @ yc[:, h])
         return coef
 
-    def fit(self, train: Frames, valid: Frames, epochs=30, batch=8, lr=2e-3, log=None):
-        """Train on ``train``: the global coefficients by least squares, then the
-        network on the residual with Adam, then all coefficients again; switch off
-        the heads whose variance on ``valid`` does not drop."""
+    def fit(self, train: Frames, valid: Frames, epochs=20, batch=8, lr=2e-3, log=None):
+        """Least squares for the global coefficients, then Adam on the network
+        for the residual variance, then least squares for all coefficients;
+        switch off the heads whose variance on ``valid`` does not drop."""
         torch = _torch()
         self.U0 = float(train.epot.mean())
-        # spread of the global sums on the training frames
-        self.t_scale = np.ones(NGLOBAL)
-        t_all = self.features(train)[:, 0, :NGLOBAL]
-        self.t_scale = np.maximum(t_all.std(0), 1e-30)
-        y = train.y()
-        self.y_mean = y.mean(0)
-        yc = y - self.y_mean
+        self.t_scale = np.ones(NGLOBAL_USED)
+        use_net, self.use_net = self.use_net, False
         X = self.features(train)
-        X[:, :, NGLOBAL] = 0.0
-        self.coef = self._fit_linear(X, yc)
-        w = 1.0 / np.maximum(yc.var(0), 1e-30)
+        self.t_scale = np.maximum(X[:, 0, :NGLOBAL_USED].std(0), 1e-30)
+        X[:, :, :NGLOBAL_USED] /= self.t_scale
+        self.use_net = use_net
+        y = train.y()
+        yc = y - y.mean(0)
+        self.coef = self._fit_linear(X[:, :, :NGLOBAL_USED], yc)
+        self.coef = np.concatenate([self.coef, np.ones((3, 2))], axis=1)
+        history = []
         if self.use_net and epochs > 0:
             opt = torch.optim.Adam(self.net.parameters(), lr=lr)
-            lin = torch.as_tensor(self.coef[:, :NGLOBAL], dtype=torch.float64, device=self.device)
-            ts = torch.as_tensor(self.t_scale, dtype=torch.float64, device=self.device)
-            wt = torch.as_tensor(w, dtype=torch.float64, device=self.device)
+            co = torch.as_tensor(self.coef, dtype=torch.float64, device=self.device)
+            wt = torch.as_tensor(1.0 / np.maximum(yc.var(0), 1e-30), dtype=torch.float64,
+                                 device=self.device)
             ytc = torch.as_tensor(yc, dtype=torch.float64, device=self.device)
             rs = np.random.RandomState(1234)
             for ep in range(epochs):
-                order = rs.permutation(train.n // batch)
                 tot = 0.0
+                order = rs.permutation(train.n // batch)
                 for bi in order:
                     sl = slice(bi * batch, (bi + 1) * batch)
-                    s, t = self.local_sums(train, sl, grad=True)
-                    tl = (t / ts) if self.use_global else torch.zeros_like(t)
-                    r = ytc[sl] - torch.einsum("bq,hq->bh", tl, lin) - s.double()
+                    Xb = self._batch(train, sl, grad=True)
+                    r = ytc[sl] - (Xb * co[None]).sum(2)
                     loss = ((r ** 2) * wt).mean()
                     opt.zero_grad()
                     loss.backward()
                     opt.step()
-                    tot += float(loss)
+                    tot += float(loss.detach())
+                history.append(tot / len(order))
                 if log:
-                    log(f"epoch {ep} loss {tot / len(order):.4f}")
+                    log(f"epoch {ep}: residual variance / plain {history[-1]:.4f}")
         X = self.features(train)
-        self.coef = self._fit_linear(X, yc)
+        self.coef = self._fit_linear(X if self.use_net else X[:, :, :NGLOBAL_USED], yc)
+        if not self.use_net:
+            self.coef = np.concatenate([self.coef, np.zeros((3, 2))], axis=1)
         yv = valid.y()
         sv = self.controls(valid, enabled_only=False)
-        var_plain = yv.var(0)
-        var_cv = (yv - sv).var(0)
+        var_plain, var_cv = yv.var(0), (yv - sv).var(0)
         self.enabled = var_cv < var_plain
-        return dict(var_plain_valid=var_plain, var_cv_valid=var_cv, enabled=self.enabled.copy())
+        return dict(var_plain_valid=var_plain, var_cv_valid=var_cv, enabled=self.enabled.copy(),
+                    history=history)
 
     def controls(self, fr: Frames, enabled_only=True):
-        """s [n, 3] of each frame (zero for switched-off heads)."""
-        X = self.features(fr)
-        s = np.einsum("nhq,hq->nh", X, self.coef)
-        if enabled_only:
-            s = s * self.enabled[None, :]
-        return s
+        """s [n, 3] of stored frames (zero for switched-off heads)."""
+        s = np.einsum("nhq,hq->nh", self.features(fr), self.coef)
+        return s * self.enabled[None, :] if enabled_only else s
+
+    def control_of_driver(self, drv, epot, enabled_only=True):
+        """s [3] of the driver's current step (epot: its configurational energy in eV)."""
+        torch = _torch()
+        D, J, L, u, F = frame_inputs(drv, self.sites, self.box, self.basis, self.device)
+        Ut = torch.tensor([epot], dtype=torch.float64, device=self.device)
+        X = self._x(D[None], J[None], L[None], u[None], F[None], Ut)[0].detach().cpu().numpy()
+        s = (X * self.coef).sum(1)
+        return s * self.enabled if enabled_only else s
 
 
 def estimate(fr: Frames, s=None, nblocks=64):
@@ -502,24 +573,19 @@ def estimate(fr: Frames, s=None, nblocks=64):
     return C, se
 
 
-def subset(fr: Frames, sl) -> Frames:
-    return Frames(fr.pos[sl], fr.forces[sl], fr.epot[sl], fr.born_stress[sl], fr.born_elast[sl],
-                  fr.box, fr.T, fr.rho_n)
-
-
 def frames_needed(se, n, target=1.0):
     """Frames for 2 SE <= target (GPa) at the spread seen in n frames."""
     return n * (2.0 * np.asarray(se) / target) ** 2
 
 
 # ---------------------------------------------------------------------------
-# the 4000-atom alpha-brass experiment
+# the 4000-atom alpha-brass case
 # ---------------------------------------------------------------------------
 A_300K = 3.6345 * 1.0089030761549767   # A: the zero-pressure lattice constant at 300 K
 
 
 def brass_driver(edge=10, x_zn=0.06, T=300.0, place_seed=100, vel_seed=200, precision="mixed"):
-    """NVT alpha-brass cell on the device loop, and its lattice sites (A) and types."""
+    """NVT alpha-brass cell on the device loop, its lattice sites (A) and types."""
     from . import get_potential
     from . import replicas as R
     elements, pots, eams = get_potential.get_EAM_alloy_file("ZnCu.eam.alloy")
@@ -529,90 +595,82 @@ def brass_driver(edge=10, x_zn=0.06, T=300.0, place_seed=100, vel_seed=200, prec
     drv = R.make_replica_driver(spec, pots, eams, masses, a=A_300K * U.angstrom,
                                 precision=precision)
     sites = R.fcc_positions(edge, A_300K)
-    types = np.asarray(drv.types, dtype=np.int32)
-    return drv, sites, types
+    return drv, sites, np.asarray(drv.types, dtype=np.int32)
 
 
 def time_steps(drv, nsteps):
-    """Seconds per MD step of the device loop (records off, list rebuilds on schedule)."""
-    import time
+    """Seconds per MD step of the device loop (no records)."""
     drv.run(200, sample_stride=200)
     t0 = time.perf_counter()
     drv.run(nsteps, sample_stride=nsteps)
     return (time.perf_counter() - t0) / nsteps
 
 
-def time_controls(cvm: ControlVariate, fr: Frames, nframes=32):
-    """Device seconds per frame of the controls: descriptors, network and sums."""
-    torch = _torch()
-    fr = subset(fr, slice(0, nframes))
-    fr._dev = None
-    cvm.features(fr, batch=8)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    cvm.features(fr, batch=8)
-    e1.record()
-    torch.cuda.synchronize()
-    return e0.elapsed_time(e1) * 1e-3 / nframes
-
-
-def brass_experiment(n_train=2000, n_valid=400, n_eval=4000, eval_runs=2, stride=10,
-                     equil=20_000, edge=10, epochs=30, T=300.0, use_net=True, log=None,
+def brass_experiment(n_train=2000, n_valid=400, n_eval=4000, eval_runs=2, stride=20,
+                     equil=20_000, edge=10, epochs=20, T=300.0, use_net=True, log=None,
                      precision="mixed"):
     """Train the control variate on a plain run and compare controlled and plain
-    C11, C12, C44 on fresh trajectories (new velocity seeds, same alloy sample).
-    Returns a dict of estimates, SEs, variance ratios and GPU-seconds to
-    2 SE <= 1 GPa."""
+    C11, C12, C44 on fresh trajectories (new velocity seeds, same alloy
+    sample). The controls of the evaluation runs are computed in the loop from
+    the driver's device state, as a production run would. Returns estimates,
+    SEs, variance ratios and GPU-seconds to 2 SE <= 1 GPa."""
     torch = _torch()
     say = log or (lambda *_: None)
     drv, sites, types = brass_driver(edge=edge, T=T, vel_seed=200, precision=precision)
     drv.run(equil, sample_stride=equil)
     say(f"training run: {n_train + n_valid} frames every {stride} steps")
-    fr = collect_frames(drv, n_train + n_valid, stride, T)
+    fr = collect_frames(drv, n_train + n_valid, stride, T, sites=sites)
     train, valid = subset(fr, slice(0, n_train)), subset(fr, slice(n_train, None))
     cvm = ControlVariate(sites, fr.box, T, use_net=use_net)
-    cvm.set_types(types)
-    cvm.cache(train)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    fit = cvm.fit(train, valid, epochs=epochs, log=say)
-    e1.record()
     torch.cuda.synchronize()
-    train_s = e0.elapsed_time(e1) * 1e-3
-    train._dev = None
-    t_step = time_steps(drv, 2000)
-    t_cv = time_controls(cvm, valid)
-    say(f"step {t_step * 1e6:.1f} us, controls {t_cv * 1e6:.1f} us/frame, training {train_s:.1f} s")
+    t0 = time.perf_counter()
+    fit = cvm.fit(train, valid, epochs=epochs, log=say)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    del fr, train, valid
+    t_step = time_steps(drv, 4000)
+    say(f"step {t_step * 1e6:.1f} us; training {train_s:.1f} s; enabled {cvm.enabled}")
     runs = []
+    t_ctrl = []
     for r in range(eval_runs):
         d, _, _ = brass_driver(edge=edge, T=T, vel_seed=300 + r, precision=precision)
         d.run(equil, sample_stride=equil)
-        ev = collect_frames(d, n_eval, stride, T)
-        s = cvm.controls(ev)
+        ep = np.zeros(n_eval)
+        sb = np.zeros((n_eval, 6))
+        cb = np.zeros((n_eval, 6, 6))
+        s = np.zeros((n_eval, 3))
+        for k in range(n_eval):
+            rec = d.run(stride, sample_stride=stride)[-1]
+            ep[k], sb[k], cb[k] = _record_scalars(rec, d.N)
+            t0 = time.perf_counter()
+            s[k] = cvm.control_of_driver(d, ep[k], enabled_only=False)
+            t_ctrl.append(time.perf_counter() - t0)
+        ev = Frames(ep, sb, cb, np.diag(d.box.h) / U.angstrom, T, d.N / np.prod(
+            np.diag(d.box.h) / U.angstrom))
+        sc = s * cvm.enabled[None, :]
         Cp, sep = estimate(ev)
-        Cc, sec = estimate(ev, s)
+        Cc, sec = estimate(ev, sc)
         y = ev.y()
         runs.append(dict(C_plain=Cp, se_plain=sep, C_cv=Cc, se_cv=sec,
-                         var_ratio=(y - s).var(0) / y.var(0),
-                         mean_s=s.mean(0), se_mean_s=np.array(
-                             [block_bootstrap_se(s[:, h]) for h in range(3)])))
+                         var_ratio_all_heads=(y - s).var(0) / y.var(0),
+                         mean_s=s.mean(0),
+                         se_mean_s=np.array([block_bootstrap_se(s[:, h]) for h in range(3)])))
         say(f"eval run {r}: plain {np.round(Cp, 2)} +- {np.round(sep, 3)}; "
-            f"cv {np.round(Cc, 2)} +- {np.round(sec, 3)}")
+            f"controlled {np.round(Cc, 2)} +- {np.round(sec, 3)}")
         del d
     se_p = np.sqrt(np.mean([x["se_plain"] ** 2 for x in runs], 0))
     se_c = np.sqrt(np.mean([x["se_cv"] ** 2 for x in runs], 0))
+    t_cv = float(np.median(t_ctrl))
     cost_p = stride * t_step
     cost_c = stride * t_step + t_cv
     gpu_s_plain = frames_needed(se_p, n_eval) * cost_p
     gpu_s_cv = frames_needed(se_c, n_eval) * cost_c
-    out = dict(N=4 * edge ** 3, stride=stride, n_train=n_train, n_eval=n_eval, eval_runs=eval_runs,
-               t_step_s=t_step, t_controls_s=t_cv, train_s=train_s, enabled=cvm.enabled.tolist(),
-               fit=fit, runs=runs, se_plain=se_p, se_cv=se_c,
+    out = dict(N=4 * edge ** 3, stride=stride, n_train=n_train, n_valid=n_valid, n_eval=n_eval,
+               eval_runs=eval_runs, t_step_s=t_step, t_controls_s=t_cv, train_s=train_s,
+               enabled=cvm.enabled.tolist(), fit=fit, runs=runs, se_plain=se_p, se_cv=se_c,
                gpu_s_plain=gpu_s_plain, gpu_s_cv=gpu_s_cv,
                gpu_s_plain_worst=float(gpu_s_plain.max()),
                gpu_s_cv_worst=float(gpu_s_cv.max()),
-               gpu_s_cv_worst_with_training=float(gpu_s_cv.max() + train_s +
-                                                  (n_train + n_valid) * stride * t_step))
+               training_gpu_s=float(train_s + (n_train + n_valid) * stride * t_step))
     out["speedup"] = out["gpu_s_plain_worst"] / out["gpu_s_cv_worst"]
     return out

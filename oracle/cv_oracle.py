@@ -7,8 +7,9 @@ what pins this restatement is mathematics, checked in tests/test_cv_oracle.py:
 
 * descriptor derivatives against central finite differences of the
   descriptors (FP64);
-* the forward-mode tangents of the network against finite differences of the
-  network output along each atom's own coordinates;
+* the descriptor Laplacians against central differences of the derivatives;
+* the forward-mode tangents of the network, and the divergence of the fields
+  it weights, against finite differences along each atom's own coordinates;
 * the Stein identity E[s] = 0 by exact Monte Carlo on a harmonic system whose
   canonical distribution is Gaussian and sampled directly, for the learned
   local field and for every global field (a sign error in the beta term, a
@@ -50,13 +51,16 @@ def min_image(d, box):
 
 
 def descriptors(pos, types, box, rc=6.8, mu0=2.0, dmu=0.15, width=0.15):
-    """D [N, 64] and J [3, N, 64] = d D_i / d x_{i,a}, brute force over pairs.
-    phi_k(r) = exp(-(r - mu_k)^2 / (2 w^2)) (cos(pi r / rc) + 1) / 2."""
+    """D [N, 64], J [3, N, 64] = d D_i / d x_{i,a} and L [N, 64] =
+    sum_a d^2 D_i / d x_{i,a}^2, brute force over pairs.
+    phi_k(r) = exp(-(r - mu_k)^2 / (2 w^2)) (cos(pi r / rc) + 1) / 2; the
+    Laplacian of a radial function is phi'' + 2 phi' / r."""
     pos = np.asarray(pos, dtype=float)
     N = len(pos)
     mu = mu0 + dmu * np.arange(NB)
     D = np.zeros((N, 2 * NB))
     J = np.zeros((3, N, 2 * NB))
+    L = np.zeros((N, 2 * NB))
     for i in range(N):
         d = min_image(pos[i] - pos, box)
         r = np.sqrt((d ** 2).sum(1))
@@ -64,17 +68,24 @@ def descriptors(pos, types, box, rc=6.8, mu0=2.0, dmu=0.15, width=0.15):
         d, r, t = d[m], r[m], np.asarray(types)[m]
         u = r[:, None] - mu[None, :]
         g = np.exp(-0.5 * u ** 2 / width ** 2)
-        fc = 0.5 * (np.cos(np.pi * r / rc) + 1.0)
-        dfc = -0.5 * np.pi / rc * np.sin(np.pi * r / rc)
-        phi = g * fc[:, None]
-        dphi = g * (dfc[:, None] - u / width ** 2 * fc[:, None])
-        w = dphi / r[:, None]
+        c = np.cos(np.pi * r / rc)[:, None]
+        sn = np.sin(np.pi * r / rc)[:, None]
+        fc = 0.5 * (c + 1.0)
+        dfc = -0.5 * np.pi / rc * sn
+        d2fc = -0.5 * (np.pi / rc) ** 2 * c
+        ui = u / width ** 2
+        phi = g * fc
+        d1 = g * (dfc - ui * fc)
+        d2 = g * ((ui ** 2 - 1.0 / width ** 2) * fc - 2.0 * ui * dfc + d2fc)
+        w = d1 / r[:, None]
+        lap = d2 + 2.0 * w
         for s in (0, 1):
             sel = t == s
             D[i, s * NB:(s + 1) * NB] = phi[sel].sum(0)
+            L[i, s * NB:(s + 1) * NB] = lap[sel].sum(0)
             for a in range(3):
                 J[a, i, s * NB:(s + 1) * NB] = (w[sel] * d[sel, a:a + 1]).sum(0)
-    return D, J
+    return D, J, L
 
 
 def dense_forward(X, T, W, b, act, emulate_bf16=False):
@@ -93,8 +104,9 @@ def dense_forward(X, T, W, b, act, emulate_bf16=False):
 
 
 def net_forward(params, D, J, scale=0.1, emulate_bf16=False):
-    """The control variate's network: 64 -> 64 -> 64 -> 16 (tanh, tanh,
-    identity); returns g [M, 9] and its tangents [3, M, 9]."""
+    """The control variate's network: 64 -> 64 -> 64 -> 6 x 64 (tanh, tanh,
+    identity); returns the weights w [M, 6, 64] of the six fields and their
+    tangents wt [3, M, 6, 64] along each atom's own coordinates."""
     W1, b1, W2, b2, W3, b3 = params
     h, ht = dense_forward(D * scale, J * scale, W1, b1, 0, emulate_bf16)
     if emulate_bf16:
@@ -103,16 +115,33 @@ def net_forward(params, D, J, scale=0.1, emulate_bf16=False):
     if emulate_bf16:
         h, ht = h.astype(np.float32), ht.astype(np.float32)
     o, ot = dense_forward(h, ht, W3, b3, 1, emulate_bf16)
-    return o[:, :9], ot[:, :, :9]
+    M = len(D)
+    return o.reshape(M, -1, 2 * NB), ot.reshape(3, M, -1, 2 * NB)
 
 
-def local_stein(g, gt, F, beta, out_scale=1.0):
-    """Per head c: sum_i div_i g^c_i + beta g^c_i . F_i (g, gt of one frame)."""
-    s = np.zeros(3)
-    for c in range(3):
-        div = gt[0, :, 3 * c] + gt[1, :, 3 * c + 1] + gt[2, :, 3 * c + 2]
-        s[c] = div.sum() + beta * (g[:, 3 * c:3 * c + 3] * F).sum()
-    return out_scale * s
+def fields(w, wt, J, L):
+    """The fields c_{f,i} = sum_k w_{f,ik} grad_i D_ik (sums of radial vectors
+    along the bonds of atom i, equivariant) and their divergences along each
+    atom's own coordinates:
+      div_i c_{f,i} = sum_k [ w_{f,ik} L_ik + sum_a (d w_{f,ik} / d x_{i,a}) J_{a,ik} ].
+    Returns c [M, 6, 3] and div [M, 6]."""
+    c = np.einsum("mfk,amk->mfa", w, J)
+    div = np.einsum("mfk,mk->mf", w, L) + np.einsum("amfk,amk->mf", wt, J)
+    return c, div
+
+
+def field_sums(c, div, F, beta):
+    """Per field f of one frame: A_f = sum_i div_i c_f,i + beta c_f,i . F_i (a
+    Stein sum) and C_f = sum_i c_f,i . F_i."""
+    cF = np.einsum("mfa,ma->mf", c, F)
+    return (div + beta * cF).sum(0), cF.sum(0)
+
+
+def net_features(A, C, dU, out_scale=1.0):
+    """Per head h: the Stein sum of field 2h, and that of the field (U - U0) c
+    of field 2h + 1, dU A - C (grad_i U = -F_i). Returns [3, 2]."""
+    A, C = out_scale * np.asarray(A), out_scale * np.asarray(C)
+    return np.stack([A[0::2], dU * A[1::2] - C[1::2]], axis=1)
 
 
 def global_stein(u, F, U, U0, beta, ndiv=None):

@@ -41,40 +41,54 @@ def _params(seed=1, w3=0.3):
     rs = np.random.RandomState(seed)
     return [rs.normal(size=(64, 64)) / 8, rs.normal(size=64) * 0.1,
             rs.normal(size=(64, 64)) / 8, rs.normal(size=64) * 0.1,
-            np.concatenate([rs.normal(size=(64, 9)) * w3, np.zeros((64, 7))], 1),
-            np.concatenate([rs.normal(size=9) * 0.1, np.zeros(7)])]
+            rs.normal(size=(64, 384)) * w3, rs.normal(size=384) * 0.1]
+
+
+def _fd(f, pos, i, a, h=1e-5):
+    p1, p0 = pos.copy(), pos.copy()
+    p1[i, a] += h
+    p0[i, a] -= h
+    return (f(p1) - f(p0)) / (2 * h)
 
 
 def test_descriptor_derivatives_are_finite_differences():
     _, pos, types, box = _cell()
-    D, J = O.descriptors(pos, types, box)
-    assert D.shape == (len(pos), 64) and J.shape == (3, len(pos), 64)
+    D, J, L = O.descriptors(pos, types, box)
+    assert D.shape == (len(pos), 64) and J.shape == (3, len(pos), 64) and L.shape == D.shape
     assert D.max() > 0.5          # shells populate the basis
-    h = 1e-5
     for i in (0, 17, 100):
+        lap = np.zeros(64)
         for a in range(3):
-            p1, p0 = pos.copy(), pos.copy()
-            p1[i, a] += h
-            p0[i, a] -= h
-            fd = (O.descriptors(p1, types, box)[0][i] - O.descriptors(p0, types, box)[0][i]) / (2 * h)
+            fd = _fd(lambda p: O.descriptors(p, types, box)[0][i], pos, i, a)
             assert np.allclose(J[a, i], fd, rtol=1e-6, atol=1e-8 * np.abs(J).max())
+            lap += _fd(lambda p: O.descriptors(p, types, box)[1][a, i], pos, i, a)
+        assert np.allclose(L[i], lap, rtol=1e-5, atol=1e-7 * np.abs(L).max())
 
 
-def test_network_tangents_are_finite_differences():
+def test_network_tangents_and_field_divergence_are_finite_differences():
     _, pos, types, box = _cell(seed=2)
     P = _params()
-    D, J = O.descriptors(pos, types, box)
-    g, gt = O.net_forward(P, D, J)
-    h = 1e-5
+    D, J, L = O.descriptors(pos, types, box)
+    w, wt = O.net_forward(P, D, J)
+    c, div = O.fields(w, wt, J, L)
+
+    def field(p, i):
+        Dp, Jp, Lp = O.descriptors(p, types, box)
+        wp, wtp = O.net_forward(P, Dp, Jp)
+        return O.fields(wp, wtp, Jp, Lp)[0][i]          # [6, 3]
+
     for i in (3, 50):
+        fd_div = np.zeros(6)
         for a in range(3):
-            p1, p0 = pos.copy(), pos.copy()
-            p1[i, a] += h
-            p0[i, a] -= h
-            g1 = O.net_forward(P, *O.descriptors(p1, types, box))[0][i]
-            g0 = O.net_forward(P, *O.descriptors(p0, types, box))[0][i]
-            assert np.allclose(gt[a, i], (g1 - g0) / (2 * h), rtol=1e-6,
-                               atol=1e-8 * np.abs(gt).max())
+            fw = _fd(lambda p: O.net_forward(P, *O.descriptors(p, types, box)[:2])[0][i], pos, i, a)
+            assert np.allclose(wt[a, i], fw, rtol=1e-6, atol=1e-8 * np.abs(wt).max())
+            fd_div += _fd(lambda p: field(p, i)[:, a], pos, i, a)
+        assert np.allclose(div[i], fd_div, rtol=1e-5, atol=1e-7 * np.abs(div).max())
+    # the field of a single weight is the descriptor gradient itself
+    wk = np.zeros_like(w)
+    wk[:, 0, 5] = 1.0
+    ck, _ = O.fields(wk, np.zeros_like(wt), J, L)
+    assert np.allclose(ck[:, 0, :], J[:, :, 5].T)
 
 
 def _harmonic_samples(n, N, k, beta, seed):
@@ -85,7 +99,7 @@ def _harmonic_samples(n, N, k, beta, seed):
 
 def _random_sites(n, L, dmin, seed):
     """Sites at random in a cubic box, no two closer than dmin (a disordered
-    reference, so that a local field's divergence has a non-zero mean)."""
+    reference, so that the fields do not vanish on average by symmetry)."""
     rs = np.random.RandomState(seed)
     out = []
     while len(out) < n:
@@ -95,45 +109,44 @@ def _random_sites(n, L, dmin, seed):
     return np.array(out)
 
 
-def test_stein_identity_of_the_local_field():
+def test_stein_identity_of_the_learned_fields():
     L = 14.0
     sites = _random_sites(110, L, 2.1, seed=3)
     box = np.full(3, L)
     types = (np.arange(len(sites)) % 5 == 0).astype(np.int32)
-    P = _params(seed=4)
+    P = _params(seed=4, w3=0.05)
     k, beta = 2.0, 1.0 / 0.0258
-    # output layer aligned with the hidden tangents at the sites: a field whose
-    # divergence has a large mean, so that the test has power (a random output
-    # layer gives a mean divergence near zero)
-    D, J = O.descriptors(sites, types, box)
-    h, ht = O.dense_forward(D * 0.1, J * 0.1, P[0], P[1], 0)
-    h, ht = O.dense_forward(h, ht, P[2], P[3], 0)
-    W3 = np.zeros((64, 16))
-    for c in range(3):
-        for a in range(3):
-            W3[:, 3 * c + a] = ht[a].sum(0)
-    P[4], P[5] = W3 / np.abs(W3).max(), np.zeros(16)
     us = _harmonic_samples(200, len(sites), k, beta, seed=5)
-    # the field at the sites is a constant field (zero divergence) whose Stein
-    # sum beta g0 . F has zero mean on its own; taking it out leaves a field of
-    # order u and a far quieter test statistic
-    g0 = O.net_forward(P, *O.descriptors(sites, types, box))[0]
-    s, s_wrong, divs = [], [], []
+    D0, J0, L0 = O.descriptors(sites, types, box)
+    # output biases aligned with the descriptor Laplacians at the sites give
+    # divergences with a large mean, so that the checks below have power
+    P[5] = 0.05 * np.tile(np.sign(L0.sum(0)), 6) * np.repeat([1, -1, 2, -2, 1, 3], 64)
+    # the fields at the sites form a constant field (zero divergence) whose
+    # Stein sum beta c0 . F has zero mean on its own; taking it out of the
+    # additive sums leaves a quieter statistic for the power checks
+    c0 = O.fields(*O.net_forward(P, D0, J0), J0, L0)[0]
+    feats, quiet, divs, wrong = [], [], [], []
     for u in us:
-        D, J = O.descriptors(sites + u, types, box)
-        g, gt = O.net_forward(P, D, J)
+        D, J, Lp = O.descriptors(sites + u, types, box)
+        w, wt = O.net_forward(P, D, J)
+        c, div = O.fields(w, wt, J, Lp)
         F = -k * u
-        div = sum(gt[a, :, 3 * c + a].sum() for c in range(3) for a in range(3))
-        gf = sum(((g - g0)[:, 3 * c:3 * c + 3] * F).sum() for c in range(3))
-        s.append(O.local_stein(g - g0, gt, F, beta).sum())
-        s_wrong.append(div - beta * gf)
-        divs.append(div)
-    s, s_wrong, divs = map(np.asarray, (s, s_wrong, divs))
-    se = s.std() / np.sqrt(len(s))
-    assert abs(s.mean()) < 4 * se
-    # the divergence alone is far from zero, and a sign error is caught
-    assert abs(divs.mean()) > 8 * se
-    assert abs(s_wrong.mean()) > 10 * s_wrong.std() / np.sqrt(len(s))
+        U = 0.5 * k * (u ** 2).sum()
+        A, C = O.field_sums(c, div, F, beta)
+        feats.append(O.net_features(A, C, U - 1.0).ravel())
+        quiet.append(A - beta * np.einsum("mfa,ma->f", c0, F))
+        divs.append(div.sum(0))
+        wrong.append(div.sum(0) - beta * np.einsum("mfa,ma->f", c - c0, F))
+    feats, quiet, divs, wrong = map(np.asarray, (feats, quiet, divs, wrong))
+    se = feats.std(0) / np.sqrt(len(feats))
+    assert np.all(np.abs(feats.mean(0)) < 4.5 * se)
+    sq = quiet.std(0) / np.sqrt(len(quiet))
+    assert np.all(np.abs(quiet.mean(0)) < 4.5 * sq)
+    # where a field's divergence alone is far from zero, a sign error is caught
+    strong = np.abs(divs.mean(0)) > 6 * sq
+    assert strong.sum() >= 4
+    sw = wrong.std(0) / np.sqrt(len(wrong))
+    assert np.all(np.abs(wrong.mean(0))[strong] > 8 * sw[strong])
 
 
 @pytest.mark.parametrize("com_fixed", [False, True])
@@ -206,8 +219,8 @@ def test_integrands_average_to_the_callers_estimate():
     sb = s.born_stress_tensor.get_time_array()[:, cut:].T / Units.GPa
     cb = s.born_elasticity_tensor.get_time_array()[:, :, cut:].transpose(2, 0, 1) / Units.GPa
     box = np.full(3, x["L0"] / Units.angstrom)
-    fr = cv.Frames(pos=None, forces=None, epot=np.zeros(len(sb)), born_stress=sb,
-                   born_elast=cb, box=box, T=x["T"], rho_n=500 / np.prod(box))
+    fr = cv.Frames(epot=np.zeros(len(sb)), born_stress=sb, born_elast=cb, box=box, T=x["T"],
+                   rho_n=500 / np.prod(box))
     C, se = cv.estimate(fr)
     expect = np.array([tr.elasticity_tensor_11, tr.elasticity_tensor_12,
                        tr.elasticity_tensor_44]) / Units.GPa

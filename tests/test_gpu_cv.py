@@ -42,10 +42,30 @@ def _rel(a, b):
 def test_descriptors_match_the_oracle():
     _, pos, types, box = _cell()
     print("descriptors: 256 atoms", flush=True)
-    D, J = cv.descriptors(torch.tensor(pos, device=DEV), torch.tensor(types, device=DEV), box)
-    Do, Jo = O.descriptors(pos, types, box)
+    D, J, L = cv.descriptors(torch.tensor(pos, device=DEV), torch.tensor(types, device=DEV), box)
+    Do, Jo, Lo = O.descriptors(pos, types, box)
     assert np.allclose(D.cpu().numpy(), Do, rtol=2e-5, atol=2e-6 * np.abs(Do).max())
     assert np.allclose(J.cpu().numpy(), Jo, rtol=2e-5, atol=2e-6 * np.abs(Jo).max())
+    assert np.allclose(L.cpu().numpy(), Lo, rtol=2e-5, atol=2e-6 * np.abs(Lo).max())
+
+
+def test_driver_descriptors_match_the_oracle():
+    """The list-based descriptors of a 500-atom brass cell after some steps,
+    against the brute-force oracle on the driver's own positions."""
+    print("driver descriptors: brass 500 atoms", flush=True)
+    drv, sites, types = cv.brass_driver(edge=5)
+    drv.run(300, sample_stride=300)
+    D, J, L = cv.driver_descriptors(drv)
+    pos = drv.positions_array() / cv.U.angstrom
+    box = np.diag(drv.box.h) / cv.U.angstrom
+    Do, Jo, Lo = O.descriptors(pos, types, box)
+    assert np.allclose(D.cpu().numpy(), Do, rtol=2e-5, atol=2e-6 * np.abs(Do).max())
+    assert np.allclose(J.cpu().numpy(), Jo, rtol=2e-5, atol=2e-6 * np.abs(Jo).max())
+    assert np.allclose(L.cpu().numpy(), Lo, rtol=2e-5, atol=2e-6 * np.abs(Lo).max())
+    F = cv.driver_atoms(drv, "forces").cpu().numpy()
+    assert np.array_equal(F, drv.forces_array())
+    P = cv.driver_atoms(drv, "positions").cpu().numpy()
+    assert np.array_equal(P, drv.positions_array())
 
 
 def test_descriptors_refuse_a_cutoff_beyond_half_the_box():
@@ -71,7 +91,7 @@ def test_gemm_matches_bf16_operands_in_fp64(R, K, N, transB):
 
 
 @pytest.mark.parametrize("R,rows", [(1000, 512), (48, 16), (33, 32)])
-def test_gemm_tn_and_colsum(R, rows):
+def test_gemm_tn(R, rows):
     rs = np.random.RandomState(R)
     A = rs.normal(size=(R, 64)).astype(np.float32)
     B = rs.normal(size=(R, 32)).astype(np.float32)
@@ -80,49 +100,58 @@ def test_gemm_tn_and_colsum(R, rows):
     ref = O.bf16_round(A).T @ O.bf16_round(B)
     scale = np.abs(O.bf16_round(A)).T @ np.abs(O.bf16_round(B))
     assert np.all(np.abs(C.cpu().numpy() - ref) <= 1e-6 * scale + 1e-6)
-    out = torch.empty(32, device=DEV)
-    cv._check(cv.nat.lib().mdl_cv_colsum(cv._p(torch.tensor(B, device=DEV)), cv._p(out), R, 32,
-                                         cv._stream()))
-    assert np.allclose(out.cpu().numpy(), B.astype(np.float64).sum(0), rtol=1e-5, atol=1e-4)
+
+
+@pytest.mark.parametrize("R,N", [(5000, 64), (33, 384), (1024, 16), (1, 48)])
+def test_colsum(R, N):
+    B = np.random.RandomState(R).normal(size=(R, N)).astype(np.float32)
+    print(f"colsum R={R} N={N}", flush=True)
+    out = cv._colsum(torch.tensor(B, device=DEV)).cpu().numpy()
+    assert np.allclose(out, B.astype(np.float64).sum(0), rtol=1e-5,
+                       atol=1e-6 * np.abs(B).sum(0).max())
 
 
 def _net(seed=3):
     net = cv.SteinNet(seed=seed, device=DEV)
     with torch.no_grad():
-        net.W3.mul_(30.0)            # outputs of order one
+        net.W3.mul_(30.0)            # weights of order one
         net.b1.normal_(0, 0.1)
-        net.b3[:9].normal_(0, 0.1)
+        net.b3.normal_(0, 0.1)
     return net
 
 
-def test_network_forward_and_tangents_match_the_oracle():
+def test_network_and_field_sums_match_the_oracle():
     _, pos, types, box = _cell(seed=1)
-    D, J = O.descriptors(pos, types, box)
+    D, J, L = O.descriptors(pos, types, box)
     net = _net()
+    t = [torch.tensor(x, dtype=torch.float32, device=DEV) for x in (D, J, L)]
     print("network forward: 256 atoms", flush=True)
-    g, gt = net.forward(torch.tensor(D, dtype=torch.float32, device=DEV),
-                        torch.tensor(J, dtype=torch.float32, device=DEV))
-    go, gto = O.net_forward(net.numpy_params(), D.astype(np.float32), J.astype(np.float32),
+    w, wt = net.forward(t[0], t[1])
+    wo, wto = O.net_forward(net.numpy_params(), D.astype(np.float32), J.astype(np.float32),
                             emulate_bf16=True)
-    assert _rel(g.detach().cpu().numpy(), go) < 1e-4
-    assert _rel(gt.detach().cpu().numpy(), gto) < 1e-4
-    # and the stein sums
+    assert _rel(w.detach().cpu().numpy(), wo) < 1e-4
+    assert _rel(wt.detach().cpu().numpy(), wto) < 1e-4
     F = np.random.RandomState(2).normal(size=(len(pos), 3))
-    s = net.stein(torch.tensor(D, dtype=torch.float32, device=DEV),
-                  torch.tensor(J, dtype=torch.float32, device=DEV),
-                  torch.tensor(F, dtype=torch.float32, device=DEV), 38.7, 1)
-    so = O.local_stein(go, gto, F, 38.7, net.out_scale)
-    assert _rel(s.detach().cpu().numpy()[0], so) < 1e-3
+    A, Cs = net.field_sums(t[0], t[1], t[2], torch.tensor(F, dtype=torch.float32, device=DEV),
+                           38.7, 1)
+    c, div = O.fields(wo, wto, J.astype(np.float32), L.astype(np.float32))
+    Ao, Co = O.field_sums(c, div, F, 38.7)
+    assert _rel(A.detach().cpu().numpy()[0], net.out_scale * Ao) < 1e-3
+    assert _rel(Cs.detach().cpu().numpy()[0], net.out_scale * Co) < 1e-3
+    nx = cv.net_features(A, Cs, torch.tensor([0.7], dtype=torch.float64, device=DEV))
+    assert np.allclose(nx.detach().cpu().numpy()[0],
+                       O.net_features(Ao, Co, 0.7, net.out_scale), rtol=1e-3,
+                       atol=1e-3 * net.out_scale * np.abs(Ao).max())
 
 
-def _torch_reference(params, D, J, F, beta, out_scale, scale=0.1):
-    """FP64 autograd reference of the Stein sums (CPU, no rounding)."""
+def _torch_reference(params, D, J, L, F, beta, out_scale, scale=0.1):
+    """FP64 autograd reference of the field sums (CPU, no rounding)."""
     W1, b1, W2, b2, W3, b3 = params
     X, T = D * scale, J * scale
 
     def layer(X, T, W, b, act):
         Z = X @ W + b
-        Zt = torch.einsum("amk,kn->amn", T, W)
+        Zt = torch.stack([T[a] @ W for a in range(3)])
         if act == 0:
             Y = torch.tanh(Z)
             return Y, (1 - Y ** 2)[None] * Zt
@@ -131,65 +160,67 @@ def _torch_reference(params, D, J, F, beta, out_scale, scale=0.1):
     h, ht = layer(X, T, W1, b1, 0)
     h, ht = layer(h, ht, W2, b2, 0)
     o, ot = layer(h, ht, W3, b3, 1)
-    g, gt = o[:, :9], ot[:, :, :9]
-    s = []
-    for c in range(3):
-        div = gt[0, :, 3 * c] + gt[1, :, 3 * c + 1] + gt[2, :, 3 * c + 2]
-        s.append(out_scale * (div.sum() + beta * (g[:, 3 * c:3 * c + 3] * F).sum()))
-    return torch.stack(s)
+    M = D.shape[0]
+    w, wt = o.reshape(M, 6, 64), ot.reshape(3, M, 6, 64)
+    c = (w[None] * J[:, :, None, :]).sum(3)                      # [3, M, 6]
+    div = (w * L[:, None, :]).sum(2) + (wt * J[:, :, None, :]).sum((0, 3))
+    cF = (c * F.T[:, :, None]).sum(0)
+    return out_scale * (div + beta * cF).sum(0), out_scale * cF.sum(0)
 
 
 def test_backward_matches_fp64_autograd():
     _, pos, types, box = _cell(seed=4)
-    D, J = O.descriptors(pos, types, box)
+    D, J, L = O.descriptors(pos, types, box)
     F = np.random.RandomState(5).normal(size=(len(pos), 3))
     net = _net(seed=6)
-    wts = torch.tensor([0.7, -1.3, 0.4], dtype=torch.float64)
+    wts = torch.tensor(np.random.RandomState(7).normal(size=12))
     print("network backward: 256 atoms", flush=True)
-    s = net.stein(torch.tensor(D, dtype=torch.float32, device=DEV),
-                  torch.tensor(J, dtype=torch.float32, device=DEV),
-                  torch.tensor(F, dtype=torch.float32, device=DEV), 38.7, 1)[0]
-    loss = (s.double().cpu() * wts).sum() + (s.double() ** 2).sum().cpu()
+    t = [torch.tensor(x, dtype=torch.float32, device=DEV) for x in (D, J, L)]
+    A, Cs = net.field_sums(t[0], t[1], t[2], torch.tensor(F, dtype=torch.float32, device=DEV),
+                           38.7, 1)
+    out = torch.cat([A[0], Cs[0]]).double().cpu()
+    loss = (out * wts).sum() + (out ** 2).sum() * 1e-3
     loss.backward()
     grads = [p.grad.detach().double().cpu().numpy() for p in net.parameters()]
     ref_p = [torch.tensor(O.bf16_round(p) if p.ndim == 2 else p, requires_grad=True)
              for p in net.numpy_params()]
-    sr = _torch_reference(ref_p, torch.tensor(D), torch.tensor(J), torch.tensor(F), 38.7,
-                          net.out_scale)
-    lr = (sr * wts).sum() + (sr ** 2).sum()
+    Ar, Cr = _torch_reference(ref_p, torch.tensor(D), torch.tensor(J), torch.tensor(L),
+                              torch.tensor(F), 38.7, net.out_scale)
+    outr = torch.cat([Ar, Cr])
+    lr = (outr * wts).sum() + (outr ** 2).sum() * 1e-3
     lr.backward()
+    assert _rel(out.detach().numpy(), outr.detach().numpy()) < 2e-2
     for k, (g, p) in enumerate(zip(grads, ref_p)):
-        ref = p.grad.numpy()
-        if k == 5:                      # padded outputs have no gradient
-            assert np.all(g[9:] == 0)
-            g, ref = g[:9], ref[:9]
-        if k == 4:
-            assert np.all(g[:, 9:] == 0)
-            g, ref = g[:, :9], ref[:, :9]
-        assert _rel(g, ref) < 2e-2, k
+        assert _rel(g, p.grad.numpy()) < 2e-2, k
 
 
 def test_stein_identity_and_fallback_on_md_frames():
     """500-atom alpha-brass NVT: the trained control variate keeps a zero mean
-    on a fresh trajectory (Stein), switched-off heads reproduce the plain
-    estimate exactly, and the variance of no enabled head grows."""
+    on a fresh trajectory (Stein), computed from the driver's device state in
+    the loop and from stored frames alike; switched-off heads reproduce the
+    plain estimate exactly."""
     print("brass 500 atoms: frames", flush=True)
     drv, sites, types = cv.brass_driver(edge=5)
     drv.run(4000, sample_stride=4000)
-    fr = cv.collect_frames(drv, 500, 10, 300.0)
+    fr = cv.collect_frames(drv, 500, 20, 300.0, sites=sites)
     train, valid = cv.subset(fr, slice(0, 400)), cv.subset(fr, slice(400, None))
     cvm = cv.ControlVariate(sites, fr.box, 300.0, device=DEV)
-    cvm.set_types(types)
-    cvm.cache(train)
     print("brass 500 atoms: training", flush=True)
-    cvm.fit(train, valid, epochs=3)
+    cvm.fit(train, valid, epochs=2)
     d, _, _ = cv.brass_driver(edge=5, vel_seed=999)
     d.run(4000, sample_stride=4000)
-    ev = cv.collect_frames(d, 1280, 10, 300.0)
+    ev = cv.collect_frames(d, 1280, 20, 300.0, sites=sites)
     s_all = cvm.controls(ev, enabled_only=False)
     se = np.array([cv.block_bootstrap_se(s_all[:, h]) for h in range(3)])
     print("mean s", s_all.mean(0), "se", se, flush=True)
     assert np.all(np.abs(s_all.mean(0)) < 4.5 * se)
+    # the in-loop path gives the stored path's controls
+    rec = d.run(20, sample_stride=20)[-1]
+    ep = cv._record_scalars(rec, d.N)[0]
+    s1 = cvm.control_of_driver(d, ep, enabled_only=False)
+    X = cvm._x(*[v[None] for v in cv.frame_inputs(d, cvm.sites, cvm.box)],
+               torch.tensor([ep], dtype=torch.float64, device=DEV))[0].cpu().numpy()
+    assert np.allclose(s1, (X * cvm.coef).sum(1), rtol=1e-12, atol=1e-12)
     cvm.enabled[:] = False
     Cp, sep = cv.estimate(ev)
     Cc, sec = cv.estimate(ev, cvm.controls(ev))
