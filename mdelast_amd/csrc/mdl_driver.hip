@@ -1022,15 +1022,19 @@ mdl_status mdl_driver_get_atoms(mdl_driver *p, int32_t what, double *out) {
 }
 
 mdl_status mdl_driver_cv_descriptors(mdl_driver *p, float rc, float mu0, float dmu, float width,
-                                     float *Dp, float *Jp, float *Lp) {
+                                     float *Dp, float *Jp, float *Lp, double *pos_out,
+                                     double *force_out) {
     Driver *d = D(p);
     if (!d || !Dp || !Jp || !(rc > 0.f) || !(width > 0.f))
         return bad("mdl_driver_cv_descriptors: bad arguments");
     if (d->ntypes > 2)
         return bad("mdl_driver_cv_descriptors: the descriptors have two species channels");
     // lengths of the driver are in cm; the list holds every pair within hs.rc
-    if (!((double)rc * 1e-8 <= d->hs.rc))
+    // (a float rc equal to the potential's cut-off rounds up by < 1e-7)
+    if (!((double)rc * 1e-8 <= d->hs.rc * (1.0 + 1e-6)))
         return bad("mdl_driver_cv_descriptors: rc exceeds the neighbour list's cut-off");
+    if (d->cycle_host == 0 && pos_out)
+        return bad("mdl_driver_cv_descriptors: no step taken yet");
     MDL_HIP(hipSetDevice(d->device));
     if (d->failed)
         return bad("the driver stopped on an earlier error");
@@ -1038,6 +1042,15 @@ mdl_status mdl_driver_cv_descriptors(mdl_driver *p, float rc, float mu0, float d
     k_cv_desc_list<<<(waves * 64 + 255) / 256, 256, 0, d->stream>>>(
         d->a, d->l, d->N, d->gs, 1e8, cv_basis(rc, mu0, dmu, width), Dp, Jp, Lp);
     MDL_HIP(hipGetLastError());
+    unsigned const g = (d->N + 255) / 256;
+    if (pos_out) {
+        k_materialize<<<g, 256, 0, d->stream>>>(d->a, d->N, d->gs, MDL_A_POSITIONS, pos_out);
+        MDL_HIP(hipGetLastError());
+    }
+    if (force_out) {
+        k_materialize<<<g, 256, 0, d->stream>>>(d->a, d->N, d->gs, MDL_A_FORCES, force_out);
+        MDL_HIP(hipGetLastError());
+    }
     MDL_HIP(hipStreamSynchronize(d->stream));
     return MDL_OK;
 }

@@ -55,11 +55,11 @@ from . import units as U
 NB = 32            # radial functions per neighbour species
 NF = 2 * NB        # descriptor length (two species)
 HIDDEN = 64
-NFIELD = 6         # per head: one field as is, one weighted by U - U0
+NFIELD = 9         # per head: one field as is, one weighted by U - U0, one by h - h0
 NOUT = NFIELD * NF
 HEADS = ("C11", "C12", "C44")
 NGLOBAL = 27       # global_features: S, T, Q
-NGLOBAL_USED = 18  # S and T enter the controls
+NGLOBAL_USED = 27  # the controls use S, T and the h-weighted sums
 
 
 def _torch():
@@ -104,25 +104,25 @@ def descriptors(pos_A, types, box_A, basis: Basis = Basis()):
     return D, J, L
 
 
-def driver_descriptors(drv, basis: Basis = Basis(), device="cuda:0"):
-    """D, J, L of the driver's current configuration from its neighbour list."""
+def driver_descriptors(drv, basis: Basis = Basis(), device="cuda:0", out=None):
+    """D, J, L of the driver's current configuration from its neighbour list,
+    and its positions and forces (CGS, [N, 3]); ``out`` = (D, J, L, pos, F)
+    device buffers to fill, else new tensors."""
     torch = _torch()
     N = drv.N
-    D = torch.empty((N, NF), device=device, dtype=torch.float32)
-    J = torch.empty((3, N, NF), device=device, dtype=torch.float32)
-    L = torch.empty((N, NF), device=device, dtype=torch.float32)
-    torch.cuda.current_stream().synchronize()
+    if out is None:
+        out = (torch.empty((N, NF), device=device, dtype=torch.float32),
+               torch.empty((3, N, NF), device=device, dtype=torch.float32),
+               torch.empty((N, NF), device=device, dtype=torch.float32),
+               torch.empty((N, 3), device=device, dtype=torch.float64),
+               torch.empty((N, 3), device=device, dtype=torch.float64))
+    D, J, L, pos, F = out
+    for t in out:
+        if not t.is_contiguous():
+            raise ValueError("driver_descriptors: output buffers must be contiguous")
     _check(nat.lib().mdl_driver_cv_descriptors(drv.handle, basis.rc, basis.mu0, basis.dmu,
-                                               basis.width, _p(D), _p(J), _p(L)))
-    return D, J, L
-
-
-def driver_atoms(drv, what, device="cuda:0"):
-    """A per-atom [N, 3] array of the driver (MDL_A_* layout, CGS) copied on the device."""
-    torch = _torch()
-    out = torch.empty((drv.N, 3), device=device, dtype=torch.float64)
-    torch.cuda.current_stream().synchronize()
-    _check(nat.lib().mdl_driver_copy_atoms(drv.handle, nat.A_INDEX[what], _p(out)))
+                                               basis.width, _p(D), _p(J), _p(L), _p(pos),
+                                               _p(F)))
     return out
 
 
@@ -247,26 +247,31 @@ class SteinNet:
         M = D.shape[0]
         return o.view(M, NFIELD, NF), ot.view(3, M, NFIELD, NF)
 
-    def field_sums(self, D, J, L, F, beta, nframes):
-        """Per frame and field: A = sum_i div_i c_i + beta c_i . F_i and
-        C = sum_i c_i . F_i, both times out_scale ([nframes, 6] each), for
-        D [B*N, 64], J [3, B*N, 64], L [B*N, 64], F [B*N, 3] (eV/A), beta in 1/eV."""
+    def field_sums(self, D, J, L, F, G, beta, nframes):
+        """Per frame and field: A = sum_i div_i c_i + beta c_i . F_i (the Stein
+        sum), C = sum_i c_i . F_i and E = sum_i c_i . G_i, all times out_scale
+        ([nframes, 9] each), for D [B*N, 64], J [3, B*N, 64], L [B*N, 64],
+        F and G [B*N, 3] (F in eV/A), beta in 1/eV."""
         torch = _torch()
         w, wt = self.forward(D, J)
         c = torch.einsum("mfk,amk->mfa", w, J)
         div = torch.einsum("mfk,mk->mf", w, L) + torch.einsum("amfk,amk->mf", wt, J)
         cF = torch.einsum("mfa,ma->mf", c, F)
+        cG = torch.einsum("mfa,ma->mf", c, G)
         A = (div + beta * cF).view(nframes, -1, NFIELD).sum(1)
         Cs = cF.view(nframes, -1, NFIELD).sum(1)
-        return self.out_scale * A, self.out_scale * Cs
+        E = cG.view(nframes, -1, NFIELD).sum(1)
+        return self.out_scale * A, self.out_scale * Cs, self.out_scale * E
 
 
-def net_features(A, Cs, dU):
-    """[B, 3, 2] per head: the Stein sum of its plain field and that of its
-    U-weighted field, dU A - C."""
+def net_features(A, Cs, E, dU, dh):
+    """[B, 3, 3] per head: the Stein sums of its plain field, of its field
+    weighted by U - U0 (dU A - C) and of its field weighted by h - h0
+    (dh A + E, E = sum_i c_i . grad_i h)."""
     torch = _torch()
-    A, Cs = A.double(), Cs.double()
-    return torch.stack([A[:, 0::2], dU[:, None] * A[:, 1::2] - Cs[:, 1::2]], dim=2)
+    A, Cs, E = A.double(), Cs.double(), E.double()
+    return torch.stack([A[:, 0::3], dU[:, None] * A[:, 1::3] - Cs[:, 1::3],
+                        dh[:, None] * A[:, 2::3] + E[:, 2::3]], dim=2)
 
 
 # ---------------------------------------------------------------------------
@@ -304,6 +309,23 @@ def displacements(pos, sites, box):
     d = pos - sites
     L = torch.as_tensor(box, dtype=d.dtype, device=d.device)
     return d - L * torch.round(d / L)
+
+
+def proxy_weights(V):
+    """Gradient weights of h = sum_i V[t_i] . D_i: grad_k h = sum_q Wg[t_k, q] J_k,q
+    with Wg[c, t*32 + m] = V[c, t*32 + m] + V[t, c*32 + m] (atom k's own
+    descriptors, and its neighbours' channel of k's species)."""
+    Wg = V.clone()
+    for c in range(2):
+        for t in range(2):
+            Wg[c, t * NB:(t + 1) * NB] += V[t, c * NB:(c + 1) * NB]
+    return Wg
+
+
+def h_features(S, dh, gh, u):
+    """Stein sums of the fields (h - h0) e_a u_ib: dh S_ab + sum_k (grad_k h)_a u_kb, [B, 9]."""
+    torch = _torch()
+    return (dh[:, None, None] * S + torch.einsum("bna,bnc->bac", gh, u)).reshape(-1, 9)
 
 
 # ---------------------------------------------------------------------------
@@ -392,15 +414,22 @@ def _record_scalars(r, N):
             r[nat.S_BORN_ELAST:nat.S_BORN_ELAST + 36].reshape(6, 6) / U.GPa)
 
 
-def frame_inputs(drv, sites, box, basis: Basis = Basis(), device="cuda:0"):
-    """The per-atom inputs of the controls for the driver's current step, all on
-    the device: descriptors, displacements from the sites (A) and forces (eV/A)."""
+def frame_inputs(drv, sites, box, basis: Basis = Basis(), device="cuda:0", out=None):
+    """The per-atom inputs of the controls for the driver's current step, on
+    the device: descriptors D, J, L, displacements from the sites (A) and forces
+    (eV/A). ``out`` = (D, J, L, u, F) buffers to fill."""
     torch = _torch()
-    D, J, L = driver_descriptors(drv, basis, device)
-    pos = driver_atoms(drv, "positions", device) / U.angstrom
-    F = driver_atoms(drv, "forces", device) * (U.angstrom / U.ev)
-    u = displacements(pos, sites, box)
-    return D, J, L, u, F
+    N = drv.N
+    if out is None:
+        out = (torch.empty((N, NF), device=device), torch.empty((3, N, NF), device=device),
+               torch.empty((N, NF), device=device),
+               torch.empty((N, 3), device=device, dtype=torch.float64),
+               torch.empty((N, 3), device=device, dtype=torch.float64))
+    D, J, L, u, F = out
+    driver_descriptors(drv, basis, device, out=(D, J, L, u, F))
+    u.copy_(displacements(u / U.angstrom, sites, box))
+    F.mul_(U.angstrom / U.ev)
+    return out
 
 
 def collect_frames(drv, nframes, stride, T, sites=None, basis: Basis = Basis(),
@@ -425,9 +454,8 @@ def collect_frames(drv, nframes, stride, T, sites=None, basis: Basis = Basis(),
         r = drv.run(stride, sample_stride=stride)[-1]
         ep[k], sb[k], cb[k] = _record_scalars(r, N)
         if store is not None:
-            D, J, L, u, F = frame_inputs(drv, st, box, basis, device)
-            store["D"][k], store["J"][k], store["L"][k] = D, J, L
-            store["u"][k], store["F"][k] = u, F
+            frame_inputs(drv, st, box, basis, device,
+                         out=tuple(store[q][k] for q in ("D", "J", "L", "u", "F")))
     return Frames(ep, sb, cb, box, T, N / float(np.prod(box)), store)
 
 
@@ -435,18 +463,22 @@ def collect_frames(drv, nframes, stride, T, sites=None, basis: Basis = Basis(),
 # the control variate
 # ---------------------------------------------------------------------------
 class ControlVariate:
-    """s_h(x) = theta_h . X_h(x) per head h, X_h: the 18 global Stein sums (scaled
-    by their training spread) and the two learned-field sums of the head.
+    """s_h(x) = theta_h . X_h(x) per head h. X_h: 27 global Stein sums (the
+    displacement fields e_a u_ib as they are, weighted by U - U0 and weighted by
+    h - h0, scaled by their training spread) and the sums of the head's three
+    learned fields (plain, U-weighted, h-weighted). h = sum_i V[t_i] . D_i is a
+    linear function of the descriptors fitted to the instantaneous pressure.
     Heads whose held-out variance the variate does not cut are switched off
     (s_h = 0), so the controlled estimate falls back to the plain one."""
 
-    NX = NGLOBAL_USED + 2
+    NX = NGLOBAL_USED + 3
 
-    def __init__(self, sites, box, T, device="cuda:0", seed=0, basis: Basis = Basis(),
+    def __init__(self, sites, box, T, types, device="cuda:0", seed=0, basis: Basis = Basis(),
                  use_net=True):
         torch = _torch()
         self.device = device
         self.sites = torch.as_tensor(np.asarray(sites), dtype=torch.float64, device=device)
+        self.types = torch.as_tensor(np.asarray(types), dtype=torch.long, device=device)
         self.box = np.asarray(box, dtype=float)
         self.beta = 1.0 / (U.k_boltz * T / U.ev)
         self.basis = basis
@@ -454,24 +486,58 @@ class ControlVariate:
         self.use_net = use_net
         self.t_scale = np.ones(NGLOBAL_USED)
         self.U0 = 0.0
+        self.h0 = 0.0
+        self.V = torch.zeros((2, NF), dtype=torch.float64, device=device)
         self.coef = np.zeros((3, self.NX))
         self.enabled = np.ones(3, dtype=bool)
 
+    # -- the pressure proxy h ----------------------------------------------------
+    def _species_sums(self, D):
+        """[B, 2, 64]: sum of the descriptors over the atoms of each species."""
+        torch = _torch()
+        oh = torch.nn.functional.one_hot(self.types, 2).to(D.dtype)      # [N, 2]
+        return torch.einsum("bnk,nc->bck", D, oh)
+
+    def fit_proxy(self, fr: Frames, batch=64, ridge=1e-8):
+        """V by least squares of the pressure -(s_1 + s_2 + s_3)/3 (GPa) on the
+        species sums of the descriptors; returns R^2 on ``fr``."""
+        torch = _torch()
+        P = np.concatenate([self._species_sums(fr.store["D"][k:k + batch]).double().cpu().numpy()
+                            for k in range(0, fr.n, batch)]).reshape(fr.n, -1)
+        p = -fr.born_stress[:, :3].mean(1)
+        Pm, pm = P.mean(0), p.mean()
+        A = P - Pm
+        G = A.T @ A
+        G += ridge * np.trace(G) / len(G) * np.eye(len(G))
+        v = np.linalg.solve(G, A.T @ (p - pm))
+        self.V = torch.as_tensor(v.reshape(2, NF), dtype=torch.float64, device=self.device)
+        self.h0 = float(Pm @ v)
+        r = p - pm - A @ v
+        return 1.0 - r.var() / p.var()
+
+    # -- features ----------------------------------------------------------------
     def _x(self, D, J, L, u, F, Ut, grad=False):
         """X [B, 3, NX] (torch, FP64) for B frames of per-atom inputs."""
         torch = _torch()
         B, N, _ = u.shape
-        t = global_features(u, F, Ut, self.U0, self.beta, N - 1)[:, :NGLOBAL_USED]
+        M = torch.einsum("bna,bnc->bac", F, u)
+        S = (N - 1) * torch.eye(3, dtype=M.dtype, device=M.device) + self.beta * M
+        dU = Ut - self.U0
+        dh = (self._species_sums(D.double()) * self.V[None]).sum((1, 2)) - self.h0
+        Wg = proxy_weights(self.V)[self.types]                              # [N, 64]
+        gh = torch.einsum("bank,nk->bna", J.double(), Wg)                   # [B, N, 3]
+        t = torch.cat([S.reshape(B, 9), (dU[:, None, None] * S - M).reshape(B, 9),
+                       h_features(S, dh, gh, u)], dim=1)
         t = t / torch.as_tensor(self.t_scale, dtype=t.dtype, device=t.device)
         if self.use_net:
             with torch.set_grad_enabled(grad):
-                A, Cs = self.net.field_sums(D.reshape(B * N, NF),
-                                            J.permute(1, 0, 2, 3).reshape(3, B * N, NF),
-                                            L.reshape(B * N, NF), F.reshape(B * N, 3).float(),
-                                            self.beta, B)
-                nx = net_features(A, Cs, Ut - self.U0)
+                A, Cs, E = self.net.field_sums(
+                    D.reshape(B * N, NF), J.permute(1, 0, 2, 3).reshape(3, B * N, NF),
+                    L.reshape(B * N, NF), F.reshape(B * N, 3).float(),
+                    gh.reshape(B * N, 3).float(), self.beta, B)
+                nx = net_features(A, Cs, E, dU, dh)
         else:
-            nx = torch.zeros((B, 3, 2), dtype=t.dtype, device=t.device)
+            nx = torch.zeros((B, 3, 3), dtype=t.dtype, device=t.device)
         return torch.cat([t[:, None, :].expand(B, 3, NGLOBAL_USED), nx], dim=2)
 
     def _batch(self, fr: Frames, sl, grad=False):
@@ -497,12 +563,16 @@ class ControlVariate:
             coef[h] = np.linalg.solve(G, A.T @ yc[:, h])
         return coef
 
-    def fit(self, train: Frames, valid: Frames, epochs=20, batch=8, lr=2e-3, log=None):
-        """Least squares for the global coefficients, then Adam on the network
-        for the residual variance, then least squares for all coefficients;
-        switch off the heads whose variance on ``valid`` does not drop."""
+    def fit(self, train: Frames, valid: Frames, epochs=30, batch=8, lr=1e-3, log=None):
+        """The pressure proxy, then least squares for the global coefficients,
+        then Adam on the network for the residual variance (the parameters of the
+        best epoch are kept), then least squares for all coefficients; switch off
+        the heads whose variance on ``valid`` does not drop."""
         torch = _torch()
+        say = log or (lambda *_: None)
         self.U0 = float(train.epot.mean())
+        r2 = self.fit_proxy(train)
+        say(f"pressure proxy: R^2 {r2:.4f} on the training frames")
         self.t_scale = np.ones(NGLOBAL_USED)
         use_net, self.use_net = self.use_net, False
         X = self.features(train)
@@ -511,16 +581,19 @@ class ControlVariate:
         self.use_net = use_net
         y = train.y()
         yc = y - y.mean(0)
-        self.coef = self._fit_linear(X[:, :, :NGLOBAL_USED], yc)
-        self.coef = np.concatenate([self.coef, np.ones((3, 2))], axis=1)
+        glob = self._fit_linear(X[:, :, :NGLOBAL_USED], yc)
         history = []
         if self.use_net and epochs > 0:
-            opt = torch.optim.Adam(self.net.parameters(), lr=lr)
+            self.coef = np.concatenate([glob, np.ones((3, 3))], axis=1)
+            params = self.net.parameters()
+            opt = torch.optim.Adam(params, lr=lr)
+            sched = torch.optim.lr_scheduler.ExponentialLR(opt, gamma=0.1 ** (1.0 / epochs))
             co = torch.as_tensor(self.coef, dtype=torch.float64, device=self.device)
             wt = torch.as_tensor(1.0 / np.maximum(yc.var(0), 1e-30), dtype=torch.float64,
                                  device=self.device)
             ytc = torch.as_tensor(yc, dtype=torch.float64, device=self.device)
             rs = np.random.RandomState(1234)
+            best, best_state = np.inf, [p.detach().clone() for p in params]
             for ep in range(epochs):
                 tot = 0.0
                 order = rs.permutation(train.n // batch)
@@ -531,35 +604,74 @@ class ControlVariate:
                     loss = ((r ** 2) * wt).mean()
                     opt.zero_grad()
                     loss.backward()
+                    torch.nn.utils.clip_grad_norm_(params, 1.0)
                     opt.step()
                     tot += float(loss.detach())
+                sched.step()
                 history.append(tot / len(order))
-                if log:
-                    log(f"epoch {ep}: residual variance / plain {history[-1]:.4f}")
+                if history[-1] < best:
+                    best = history[-1]
+                    best_state = [p.detach().clone() for p in params]
+                say(f"epoch {ep}: residual variance / plain {history[-1]:.4f}")
+            with torch.no_grad():
+                for p, q in zip(params, best_state):
+                    p.copy_(q)
         X = self.features(train)
-        self.coef = self._fit_linear(X if self.use_net else X[:, :, :NGLOBAL_USED], yc)
-        if not self.use_net:
-            self.coef = np.concatenate([self.coef, np.zeros((3, 2))], axis=1)
+        if self.use_net:
+            self.coef = self._fit_linear(X, yc)
+        else:
+            self.coef = np.concatenate([glob, np.zeros((3, 3))], axis=1)
         yv = valid.y()
         sv = self.controls(valid, enabled_only=False)
         var_plain, var_cv = yv.var(0), (yv - sv).var(0)
         self.enabled = var_cv < var_plain
-        return dict(var_plain_valid=var_plain, var_cv_valid=var_cv, enabled=self.enabled.copy(),
-                    history=history)
+        return dict(proxy_r2=r2, var_plain_valid=var_plain, var_cv_valid=var_cv,
+                    enabled=self.enabled.copy(), history=history)
 
     def controls(self, fr: Frames, enabled_only=True):
         """s [n, 3] of stored frames (zero for switched-off heads)."""
         s = np.einsum("nhq,hq->nh", self.features(fr), self.coef)
         return s * self.enabled[None, :] if enabled_only else s
 
-    def control_of_driver(self, drv, epot, enabled_only=True):
-        """s [3] of the driver's current step (epot: its configurational energy in eV)."""
+
+class ControlBatch:
+    """Controls of a running driver, gathered frame by frame into device
+    buffers and evaluated B frames at a time (one network pass per batch)."""
+
+    def __init__(self, cvm: ControlVariate, N, B=16):
         torch = _torch()
-        D, J, L, u, F = frame_inputs(drv, self.sites, self.box, self.basis, self.device)
-        Ut = torch.tensor([epot], dtype=torch.float64, device=self.device)
-        X = self._x(D[None], J[None], L[None], u[None], F[None], Ut)[0].detach().cpu().numpy()
-        s = (X * self.coef).sum(1)
-        return s * self.enabled if enabled_only else s
+        dev = cvm.device
+        self.cvm, self.B, self.k = cvm, B, 0
+        self.buf = (torch.empty((B, N, NF), device=dev), torch.empty((B, 3, N, NF), device=dev),
+                    torch.empty((B, N, NF), device=dev),
+                    torch.empty((B, N, 3), device=dev, dtype=torch.float64),
+                    torch.empty((B, N, 3), device=dev, dtype=torch.float64))
+        self.U = np.zeros(B)
+        self.out = []
+
+    def push(self, drv, epot):
+        """Take the driver's current step (epot: configurational energy, eV)."""
+        frame_inputs(drv, self.cvm.sites, self.cvm.box, self.cvm.basis, self.cvm.device,
+                     out=tuple(b[self.k] for b in self.buf))
+        self.U[self.k] = epot
+        self.k += 1
+        if self.k == self.B:
+            self.flush()
+
+    def flush(self):
+        torch = _torch()
+        if self.k == 0:
+            return
+        k = self.k
+        Ut = torch.as_tensor(self.U[:k], dtype=torch.float64, device=self.cvm.device)
+        X = self.cvm._x(*[b[:k] for b in self.buf], Ut).detach().cpu().numpy()
+        self.out.append((X * self.cvm.coef[None]).sum(2))
+        self.k = 0
+
+    def result(self):
+        """s [n, 3] of every frame pushed so far (all heads)."""
+        self.flush()
+        return np.concatenate(self.out) if self.out else np.zeros((0, 3))
 
 
 def estimate(fr: Frames, s=None, nblocks=64):
@@ -621,7 +733,7 @@ def brass_experiment(n_train=2000, n_valid=400, n_eval=4000, eval_runs=2, stride
     say(f"training run: {n_train + n_valid} frames every {stride} steps")
     fr = collect_frames(drv, n_train + n_valid, stride, T, sites=sites)
     train, valid = subset(fr, slice(0, n_train)), subset(fr, slice(n_train, None))
-    cvm = ControlVariate(sites, fr.box, T, use_net=use_net)
+    cvm = ControlVariate(sites, fr.box, T, types, use_net=use_net)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     fit = cvm.fit(train, valid, epochs=epochs, log=say)
@@ -631,22 +743,25 @@ def brass_experiment(n_train=2000, n_valid=400, n_eval=4000, eval_runs=2, stride
     t_step = time_steps(drv, 4000)
     say(f"step {t_step * 1e6:.1f} us; training {train_s:.1f} s; enabled {cvm.enabled}")
     runs = []
-    t_ctrl = []
+    t_ctrl = 0.0
     for r in range(eval_runs):
         d, _, _ = brass_driver(edge=edge, T=T, vel_seed=300 + r, precision=precision)
         d.run(equil, sample_stride=equil)
         ep = np.zeros(n_eval)
         sb = np.zeros((n_eval, 6))
         cb = np.zeros((n_eval, 6, 6))
-        s = np.zeros((n_eval, 3))
+        cbat = ControlBatch(cvm, d.N)
         for k in range(n_eval):
             rec = d.run(stride, sample_stride=stride)[-1]
             ep[k], sb[k], cb[k] = _record_scalars(rec, d.N)
             t0 = time.perf_counter()
-            s[k] = cvm.control_of_driver(d, ep[k], enabled_only=False)
-            t_ctrl.append(time.perf_counter() - t0)
-        ev = Frames(ep, sb, cb, np.diag(d.box.h) / U.angstrom, T, d.N / np.prod(
-            np.diag(d.box.h) / U.angstrom))
+            cbat.push(d, ep[k])
+            t_ctrl += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        s = cbat.result()
+        t_ctrl += time.perf_counter() - t0
+        box = np.diag(d.box.h) / U.angstrom
+        ev = Frames(ep, sb, cb, box, T, d.N / np.prod(box))
         sc = s * cvm.enabled[None, :]
         Cp, sep = estimate(ev)
         Cc, sec = estimate(ev, sc)
@@ -657,10 +772,10 @@ def brass_experiment(n_train=2000, n_valid=400, n_eval=4000, eval_runs=2, stride
                          se_mean_s=np.array([block_bootstrap_se(s[:, h]) for h in range(3)])))
         say(f"eval run {r}: plain {np.round(Cp, 2)} +- {np.round(sep, 3)}; "
             f"controlled {np.round(Cc, 2)} +- {np.round(sec, 3)}")
-        del d
+        del d, cbat
     se_p = np.sqrt(np.mean([x["se_plain"] ** 2 for x in runs], 0))
     se_c = np.sqrt(np.mean([x["se_cv"] ** 2 for x in runs], 0))
-    t_cv = float(np.median(t_ctrl))
+    t_cv = t_ctrl / (eval_runs * n_eval)
     cost_p = stride * t_step
     cost_c = stride * t_step + t_cv
     gpu_s_plain = frames_needed(se_p, n_eval) * cost_p

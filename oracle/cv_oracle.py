@@ -104,9 +104,9 @@ def dense_forward(X, T, W, b, act, emulate_bf16=False):
 
 
 def net_forward(params, D, J, scale=0.1, emulate_bf16=False):
-    """The control variate's network: 64 -> 64 -> 64 -> 6 x 64 (tanh, tanh,
-    identity); returns the weights w [M, 6, 64] of the six fields and their
-    tangents wt [3, M, 6, 64] along each atom's own coordinates."""
+    """The control variate's network: 64 -> 64 -> 64 -> F x 64 (tanh, tanh,
+    identity); returns the weights w [M, F, 64] of the F fields and their
+    tangents wt [3, M, F, 64] along each atom's own coordinates."""
     W1, b1, W2, b2, W3, b3 = params
     h, ht = dense_forward(D * scale, J * scale, W1, b1, 0, emulate_bf16)
     if emulate_bf16:
@@ -124,24 +124,52 @@ def fields(w, wt, J, L):
     along the bonds of atom i, equivariant) and their divergences along each
     atom's own coordinates:
       div_i c_{f,i} = sum_k [ w_{f,ik} L_ik + sum_a (d w_{f,ik} / d x_{i,a}) J_{a,ik} ].
-    Returns c [M, 6, 3] and div [M, 6]."""
+    Returns c [M, F, 3] and div [M, F]."""
     c = np.einsum("mfk,amk->mfa", w, J)
     div = np.einsum("mfk,mk->mf", w, L) + np.einsum("amfk,amk->mf", wt, J)
     return c, div
 
 
-def field_sums(c, div, F, beta):
+def field_sums(c, div, F, beta, G=None):
     """Per field f of one frame: A_f = sum_i div_i c_f,i + beta c_f,i . F_i (a
-    Stein sum) and C_f = sum_i c_f,i . F_i."""
+    Stein sum), C_f = sum_i c_f,i . F_i and E_f = sum_i c_f,i . G_i."""
     cF = np.einsum("mfa,ma->mf", c, F)
-    return (div + beta * cF).sum(0), cF.sum(0)
+    E = np.einsum("mfa,ma->mf", c, G).sum(0) if G is not None else np.zeros(c.shape[1])
+    return (div + beta * cF).sum(0), cF.sum(0), E
 
 
-def net_features(A, C, dU, out_scale=1.0):
-    """Per head h: the Stein sum of field 2h, and that of the field (U - U0) c
-    of field 2h + 1, dU A - C (grad_i U = -F_i). Returns [3, 2]."""
-    A, C = out_scale * np.asarray(A), out_scale * np.asarray(C)
-    return np.stack([A[0::2], dU * A[1::2] - C[1::2]], axis=1)
+def net_features(A, C, E, dU, dh, out_scale=1.0):
+    """Per head j (fields 3j, 3j+1, 3j+2): the Stein sums of the plain field,
+    of the field times (U - U0), dU A - C (grad_i U = -F_i), and of the field
+    times (h - h0), dh A + E (E = sum_i c_i . grad_i h). Returns [3, 3]."""
+    A, C, E = (out_scale * np.asarray(x) for x in (A, C, E))
+    return np.stack([A[0::3], dU * A[1::3] - C[1::3], dh * A[2::3] + E[2::3]], axis=1)
+
+
+def proxy(D, types, V):
+    """h = sum_i V[t_i] . D_i: a pair sum sum_{i != j} v_{t_i, t_j}(r_ij) with
+    v_{c,t}(r) = sum_m V[c, t*32 + m] phi_m(r)."""
+    return float(np.einsum("ik,ik->", D, V[np.asarray(types)]))
+
+
+def proxy_grad(J, types, V):
+    """grad_k h [N, 3]: atom k's own descriptors give V[t_k] . J_k; each
+    neighbour j's channel of k's species gives V[t_j, t_k*32 + m] phi_m'(r) r_hat,
+    which, summed over the neighbours of one species t, is V[t, t_k*32 + m]
+    times atom k's channel t of J."""
+    t = np.asarray(types)
+    Wg = V.copy()
+    for c in range(2):
+        for u in range(2):
+            Wg[c, u * NB:(u + 1) * NB] += V[u, c * NB:(c + 1) * NB]
+    return np.einsum("amk,mk->ma", J, Wg[t])
+
+
+def h_stein(u, F, h, h0, gh, beta, ndiv):
+    """Stein sums of the fields (h - h0) e_a u_ib, [9]:
+    (h - h0) S_ab + sum_k (grad_k h)_a u_kb."""
+    S = ndiv * np.eye(3) + beta * np.einsum("ia,ib->ab", F, u)
+    return ((h - h0) * S + np.einsum("ka,kb->ab", gh, u)).ravel()
 
 
 def global_stein(u, F, U, U0, beta, ndiv=None):

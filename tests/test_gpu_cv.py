@@ -132,19 +132,23 @@ def test_network_and_field_sums_match_the_oracle():
     assert _rel(w.detach().cpu().numpy(), wo) < 1e-4
     assert _rel(wt.detach().cpu().numpy(), wto) < 1e-4
     F = np.random.RandomState(2).normal(size=(len(pos), 3))
-    A, Cs = net.field_sums(t[0], t[1], t[2], torch.tensor(F, dtype=torch.float32, device=DEV),
-                           38.7, 1)
+    G = np.random.RandomState(3).normal(size=(len(pos), 3))
+    A, Cs, E = net.field_sums(t[0], t[1], t[2],
+                              torch.tensor(F, dtype=torch.float32, device=DEV),
+                              torch.tensor(G, dtype=torch.float32, device=DEV), 38.7, 1)
     c, div = O.fields(wo, wto, J.astype(np.float32), L.astype(np.float32))
-    Ao, Co = O.field_sums(c, div, F, 38.7)
+    Ao, Co, Eo = O.field_sums(c, div, F, 38.7, G)
     assert _rel(A.detach().cpu().numpy()[0], net.out_scale * Ao) < 1e-3
     assert _rel(Cs.detach().cpu().numpy()[0], net.out_scale * Co) < 1e-3
-    nx = cv.net_features(A, Cs, torch.tensor([0.7], dtype=torch.float64, device=DEV))
-    assert np.allclose(nx.detach().cpu().numpy()[0],
-                       O.net_features(Ao, Co, 0.7, net.out_scale), rtol=1e-3,
-                       atol=1e-3 * net.out_scale * np.abs(Ao).max())
+    assert _rel(E.detach().cpu().numpy()[0], net.out_scale * Eo) < 1e-3
+    one = torch.tensor([0.7], dtype=torch.float64, device=DEV)
+    nx = cv.net_features(A, Cs, E, one, -one)
+    ref = O.net_features(Ao, Co, Eo, 0.7, -0.7, net.out_scale)
+    assert np.allclose(nx.detach().cpu().numpy()[0], ref, rtol=1e-3,
+                       atol=1e-3 * np.abs(ref).max())
 
 
-def _torch_reference(params, D, J, L, F, beta, out_scale, scale=0.1):
+def _torch_reference(params, D, J, L, F, G, beta, out_scale, scale=0.1):
     """FP64 autograd reference of the field sums (CPU, no rounding)."""
     W1, b1, W2, b2, W3, b3 = params
     X, T = D * scale, J * scale
@@ -161,32 +165,36 @@ def _torch_reference(params, D, J, L, F, beta, out_scale, scale=0.1):
     h, ht = layer(h, ht, W2, b2, 0)
     o, ot = layer(h, ht, W3, b3, 1)
     M = D.shape[0]
-    w, wt = o.reshape(M, 6, 64), ot.reshape(3, M, 6, 64)
-    c = (w[None] * J[:, :, None, :]).sum(3)                      # [3, M, 6]
+    w, wt = o.reshape(M, 9, 64), ot.reshape(3, M, 9, 64)
+    c = (w[None] * J[:, :, None, :]).sum(3)                      # [3, M, 9]
     div = (w * L[:, None, :]).sum(2) + (wt * J[:, :, None, :]).sum((0, 3))
     cF = (c * F.T[:, :, None]).sum(0)
-    return out_scale * (div + beta * cF).sum(0), out_scale * cF.sum(0)
+    cG = (c * G.T[:, :, None]).sum(0)
+    return (out_scale * (div + beta * cF).sum(0), out_scale * cF.sum(0),
+            out_scale * cG.sum(0))
 
 
 def test_backward_matches_fp64_autograd():
     _, pos, types, box = _cell(seed=4)
     D, J, L = O.descriptors(pos, types, box)
     F = np.random.RandomState(5).normal(size=(len(pos), 3))
+    G = np.random.RandomState(8).normal(size=(len(pos), 3))
     net = _net(seed=6)
-    wts = torch.tensor(np.random.RandomState(7).normal(size=12))
+    wts = torch.tensor(np.random.RandomState(7).normal(size=27))
     print("network backward: 256 atoms", flush=True)
     t = [torch.tensor(x, dtype=torch.float32, device=DEV) for x in (D, J, L)]
-    A, Cs = net.field_sums(t[0], t[1], t[2], torch.tensor(F, dtype=torch.float32, device=DEV),
-                           38.7, 1)
-    out = torch.cat([A[0], Cs[0]]).double().cpu()
+    A, Cs, E = net.field_sums(t[0], t[1], t[2],
+                              torch.tensor(F, dtype=torch.float32, device=DEV),
+                              torch.tensor(G, dtype=torch.float32, device=DEV), 38.7, 1)
+    out = torch.cat([A[0], Cs[0], E[0]]).double().cpu()
     loss = (out * wts).sum() + (out ** 2).sum() * 1e-3
     loss.backward()
     grads = [p.grad.detach().double().cpu().numpy() for p in net.parameters()]
     ref_p = [torch.tensor(O.bf16_round(p) if p.ndim == 2 else p, requires_grad=True)
              for p in net.numpy_params()]
-    Ar, Cr = _torch_reference(ref_p, torch.tensor(D), torch.tensor(J), torch.tensor(L),
-                              torch.tensor(F), 38.7, net.out_scale)
-    outr = torch.cat([Ar, Cr])
+    Ar, Cr, Er = _torch_reference(ref_p, torch.tensor(D), torch.tensor(J), torch.tensor(L),
+                                  torch.tensor(F), torch.tensor(G), 38.7, net.out_scale)
+    outr = torch.cat([Ar, Cr, Er])
     lr = (outr * wts).sum() + (outr ** 2).sum() * 1e-3
     lr.backward()
     assert _rel(out.detach().numpy(), outr.detach().numpy()) < 2e-2
@@ -196,17 +204,18 @@ def test_backward_matches_fp64_autograd():
 
 def test_stein_identity_and_fallback_on_md_frames():
     """500-atom alpha-brass NVT: the trained control variate keeps a zero mean
-    on a fresh trajectory (Stein), computed from the driver's device state in
-    the loop and from stored frames alike; switched-off heads reproduce the
-    plain estimate exactly."""
+    on a fresh trajectory (Stein); the batched in-loop evaluation from the
+    driver's device state gives the stored path's controls; switched-off heads
+    reproduce the plain estimate exactly."""
     print("brass 500 atoms: frames", flush=True)
     drv, sites, types = cv.brass_driver(edge=5)
     drv.run(4000, sample_stride=4000)
     fr = cv.collect_frames(drv, 500, 20, 300.0, sites=sites)
     train, valid = cv.subset(fr, slice(0, 400)), cv.subset(fr, slice(400, None))
-    cvm = cv.ControlVariate(sites, fr.box, 300.0, device=DEV)
+    cvm = cv.ControlVariate(sites, fr.box, 300.0, types, device=DEV)
     print("brass 500 atoms: training", flush=True)
-    cvm.fit(train, valid, epochs=2)
+    info = cvm.fit(train, valid, epochs=2)
+    assert info["proxy_r2"] > 0.5
     d, _, _ = cv.brass_driver(edge=5, vel_seed=999)
     d.run(4000, sample_stride=4000)
     ev = cv.collect_frames(d, 1280, 20, 300.0, sites=sites)
@@ -214,13 +223,23 @@ def test_stein_identity_and_fallback_on_md_frames():
     se = np.array([cv.block_bootstrap_se(s_all[:, h]) for h in range(3)])
     print("mean s", s_all.mean(0), "se", se, flush=True)
     assert np.all(np.abs(s_all.mean(0)) < 4.5 * se)
-    # the in-loop path gives the stored path's controls
-    rec = d.run(20, sample_stride=20)[-1]
-    ep = cv._record_scalars(rec, d.N)[0]
-    s1 = cvm.control_of_driver(d, ep, enabled_only=False)
-    X = cvm._x(*[v[None] for v in cv.frame_inputs(d, cvm.sites, cvm.box)],
-               torch.tensor([ep], dtype=torch.float64, device=DEV))[0].cpu().numpy()
-    assert np.allclose(s1, (X * cvm.coef).sum(1), rtol=1e-12, atol=1e-12)
+    # batched in-loop evaluation == stored frames
+    batch = cv.ControlBatch(cvm, d.N, B=4)
+    ref = cv.collect_frames(d, 0, 20, 300.0)
+    eps = []
+    st = {q: [] for q in ("D", "J", "L", "u", "F")}
+    for k in range(6):
+        rec = d.run(20, sample_stride=20)[-1]
+        eps.append(cv._record_scalars(rec, d.N)[0])
+        batch.push(d, eps[-1])
+        for q, v in zip(("D", "J", "L", "u", "F"), cv.frame_inputs(d, cvm.sites, cvm.box)):
+            st[q].append(v)
+    s_loop = batch.result()
+    ref.store = {q: torch.stack(v) for q, v in st.items()}
+    ref.epot = np.asarray(eps)
+    s_ref = cvm.controls(ref, enabled_only=False)
+    assert s_loop.shape == (6, 3)
+    assert np.allclose(s_loop, s_ref, rtol=1e-9, atol=1e-9 * np.abs(s_ref).max())
     cvm.enabled[:] = False
     Cp, sep = cv.estimate(ev)
     Cc, sec = cv.estimate(ev, cvm.controls(ev))
