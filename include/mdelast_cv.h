@@ -71,6 +71,19 @@ mdl_status mdl_cv_act_forward(const float *Z, const float *Zt, float *Y, float *
 mdl_status mdl_cv_act_backward(const float *Z, const float *Zt, const float *gY, const float *gYt,
                                float *gZ, float *gZt, int64_t MN, int32_t act, void *stream);
 
+/* Fields from their weights, per atom m and field f (w: [M][nfield][64],
+ * wt: [3][M][nfield][64], J: [3][M][64], L: [M][64], F and G: [M][3]):
+ *   c = sum_k w_k J[:][m][k],  div = sum_k (w_k L[m][k] + sum_a wt_a,k J[a][m][k]),
+ *   a[m][f] = div + beta c . F[m],  cF[m][f] = c . F[m],  cG[m][f] = c . G[m]. */
+mdl_status mdl_cv_field_contract(const float *w, const float *wt, const float *J, const float *L,
+                                 const float *F, const float *G, float beta, int64_t M,
+                                 int32_t nfield, float *a, float *cF, float *cG, void *stream);
+/* Its backward with respect to w and wt, given ga, gcF, gcG ([M][nfield]). */
+mdl_status mdl_cv_field_contract_backward(const float *J, const float *L, const float *F,
+                                          const float *G, float beta, const float *ga,
+                                          const float *gcF, const float *gcG, int64_t M,
+                                          int32_t nfield, float *gw, float *gwt, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -106,6 +106,10 @@ SIGNATURES = {
                                    C.c_int32, _vp]),
     "mdl_cv_colsum": (C.c_int32, [_vp, _vp, _vp, C.c_int64, C.c_int32, _vp]),
     "mdl_cv_act_forward": (C.c_int32, [_vp, _vp, _vp, _vp, C.c_int64, C.c_int32, _vp]),
+    "mdl_cv_field_contract": (C.c_int32, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_float, C.c_int64,
+                                          C.c_int32, _vp, _vp, _vp, _vp]),
+    "mdl_cv_field_contract_backward": (C.c_int32, [_vp, _vp, _vp, _vp, C.c_float, _vp, _vp, _vp,
+                                                   C.c_int64, C.c_int32, _vp, _vp, _vp]),
     "mdl_cv_act_backward": (C.c_int32, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int64, C.c_int32,
                                         _vp]),
 }

@@ -215,6 +215,84 @@ __global__ void k_act_bwd(const float *__restrict__ Z, const float *__restrict__
     gZ[e] = g;
 }
 
+// ---------------------------------------------------------------------------
+// Fields from their weights. For atom m and field f (weights w[m][f][k] and
+// their tangents wt[a][m][f][k], k = 0..63):
+//   c_a   = sum_k w_k J[a][m][k]                      (the field, 3-vector)
+//   div   = sum_k (w_k L[m][k] + sum_a wt_a,k J[a][m][k])
+//   a     = div + beta c . F[m],  cF = c . F[m],  cG = c . G[m]
+// One wave per atom, lane = k; the k-sums are wave reductions in a fixed
+// order. Memory-bound: w and wt are read once.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_field_contract(
+    const float *__restrict__ w, const float *__restrict__ wt, const float *__restrict__ J,
+    const float *__restrict__ L, const float *__restrict__ F, const float *__restrict__ G,
+    float beta, int M, int nfield, float *__restrict__ a_out, float *__restrict__ cf_out,
+    float *__restrict__ cg_out) {
+    int const k = threadIdx.x & 63;
+    int const m = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (m >= M)
+        return;   // whole wave
+    size_t const NFK = (size_t)nfield * 64;
+    size_t const jm = (size_t)m * 64 + k;
+    size_t const jplane = (size_t)M * 64;
+    float const jx = J[jm], jy = J[jplane + jm], jz = J[2 * jplane + jm], l = L[jm];
+    float const fx = F[3 * m], fy = F[3 * m + 1], fz = F[3 * m + 2];
+    float const gx = G[3 * m], gy = G[3 * m + 1], gz = G[3 * m + 2];
+    size_t const wplane = (size_t)M * NFK;
+    for (int f = 0; f < nfield; ++f) {
+        size_t const o = (size_t)m * NFK + (size_t)f * 64 + k;
+        float const wk = w[o];
+        float const d = wk * l + (wt[o] * jx + wt[wplane + o] * jy + wt[2 * wplane + o] * jz);
+        float const cx = wave_sum(wk * jx), cy = wave_sum(wk * jy), cz = wave_sum(wk * jz);
+        float const dv = wave_sum(d);
+        if (k == 0) {
+            float const cF = cx * fx + cy * fy + cz * fz;
+            a_out[(size_t)m * nfield + f] = dv + beta * cF;
+            cf_out[(size_t)m * nfield + f] = cF;
+            cg_out[(size_t)m * nfield + f] = cx * gx + cy * gy + cz * gz;
+        }
+    }
+}
+
+// Its backward with respect to the weights (J, L, F, G are inputs):
+//   gw[m][f][k]     = ga (L_k + beta JF_k) + gcF JF_k + gcG JG_k
+//   gwt[a][m][f][k] = ga J[a][m][k]
+// with JF_k = sum_a J[a][m][k] F_a, JG_k = sum_a J[a][m][k] G_a.
+__global__ __launch_bounds__(256) void k_field_contract_bwd(
+    const float *__restrict__ J, const float *__restrict__ L, const float *__restrict__ F,
+    const float *__restrict__ G, float beta, const float *__restrict__ ga,
+    const float *__restrict__ gcf, const float *__restrict__ gcg, int M, int nfield,
+    float *__restrict__ gw, float *__restrict__ gwt) {
+    int const k = threadIdx.x & 63;
+    int const m = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (m >= M)
+        return;
+    size_t const NFK = (size_t)nfield * 64;
+    size_t const jm = (size_t)m * 64 + k;
+    size_t const jplane = (size_t)M * 64;
+    float const jx = J[jm], jy = J[jplane + jm], jz = J[2 * jplane + jm], l = L[jm];
+    float const JF = jx * F[3 * m] + jy * F[3 * m + 1] + jz * F[3 * m + 2];
+    float const JG = jx * G[3 * m] + jy * G[3 * m + 1] + jz * G[3 * m + 2];
+    size_t const wplane = (size_t)M * NFK;
+    for (int f = 0; f < nfield; ++f) {
+        size_t const o = (size_t)m * NFK + (size_t)f * 64 + k;
+        size_t const q = (size_t)m * nfield + f;
+        float const a = ga[q];
+        gw[o] = a * (l + beta * JF) + gcf[q] * JF + gcg[q] * JG;
+        gwt[o] = a * jx;
+        gwt[wplane + o] = a * jy;
+        gwt[2 * wplane + o] = a * jz;
+    }
+}
+
 mdl_status launch_ok(const char *what) {
     hipError_t const e = hipGetLastError();
     if (e != hipSuccess)
@@ -318,6 +396,37 @@ mdl_status mdl_cv_act_backward(const float *Z, const float *Zt, const float *gY,
     k_act_bwd<<<(unsigned)((MN + 255) / 256), 256, 0, (hipStream_t)stream>>>(Z, Zt, gY, gYt, gZ,
                                                                              gZt, MN, act);
     return launch_ok("mdl_cv_act_backward");
+}
+
+mdl_status mdl_cv_field_contract(const float *w, const float *wt, const float *J, const float *L,
+                                 const float *F, const float *G, float beta, int64_t M,
+                                 int32_t nfield, float *a, float *cF, float *cG, void *stream) {
+    if (!w || !wt || !J || !L || !F || !G || !a || !cF || !cG || M < 0 ||
+        M > (1ll << 31) / 64 - 1 || nfield < 1) {
+        set_error("mdl_cv_field_contract: bad arguments");
+        return MDL_ERR_ARG;
+    }
+    if (M == 0)
+        return MDL_OK;
+    k_field_contract<<<(unsigned)((M * 64 + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+        w, wt, J, L, F, G, beta, (int)M, nfield, a, cF, cG);
+    return launch_ok("mdl_cv_field_contract");
+}
+
+mdl_status mdl_cv_field_contract_backward(const float *J, const float *L, const float *F,
+                                          const float *G, float beta, const float *ga,
+                                          const float *gcF, const float *gcG, int64_t M,
+                                          int32_t nfield, float *gw, float *gwt, void *stream) {
+    if (!J || !L || !F || !G || !ga || !gcF || !gcG || !gw || !gwt || M < 0 ||
+        M > (1ll << 31) / 64 - 1 || nfield < 1) {
+        set_error("mdl_cv_field_contract_backward: bad arguments");
+        return MDL_ERR_ARG;
+    }
+    if (M == 0)
+        return MDL_OK;
+    k_field_contract_bwd<<<(unsigned)((M * 64 + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+        J, L, F, G, beta, ga, gcF, gcG, (int)M, nfield, gw, gwt);
+    return launch_ok("mdl_cv_field_contract_backward");
 }
 
 }  // extern "C"

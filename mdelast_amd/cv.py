@@ -12,12 +12,12 @@ the plain mean(y) for any fields g_q and fixed coefficients theta.
 
 The per-frame integrand of C11 or C12 is dominated by (V/kT) ds_a ds_b, a
 product of two global stress fluctuations; a sum of local terms cannot
-follow it. The fields therefore include products with a global scalar whose
-gradient is known, h(x) = U(x) - U0 (grad_i U = -F_i):
-s[h g] = h s[g] - sum_i F_i . g_i. The fields are
+follow it. The fields therefore include products with global scalars H whose
+gradients are known: s[H g] = H s[g] + sum_i grad_i H . g_i. The fields are
 
-* the displacements from the lattice sites, e_a u_ib and (U - U0) e_a u_ib
-  (their sums need only u, F and U: ``global_features``);
+* the displacements from the lattice sites, e_a u_ib, as they are and times
+  U - U0 (grad_i U = -F_i) and h - h0 (below); their sums need only u, F, U,
+  h and grad h (``global_features``, ``h_features``);
 * learned fields c_i = sum_k w_k(D_i) grad_i D_ik: a per-atom network on the
   radial-basis descriptors D_i (32 Gaussians per neighbour species, two hidden
   tanh layers of width 64) weights the descriptor gradients, which gives
@@ -25,7 +25,9 @@ s[h g] = h s[g] - sum_i F_i . g_i. The fields are
   sum_k [w_k L_k + grad_i w_k . grad_i D_k] with L the descriptor Laplacian,
   needs the derivatives of the weights along each atom's own coordinates,
   which the network carries forward. Six fields: per head (C11, C12, C44) one
-  used as is and one weighted by U - U0.
+  used as is and one weighted by h - h0, where h = sum_i V[t_i] . D_i is a
+  linear function of the descriptors fitted to the instantaneous pressure
+  (grad h follows from the descriptor derivatives alone).
 
 The coefficients of each head are fitted by least squares on frames of a
 plain run after the network has been trained on them (Adam on the residual
@@ -55,11 +57,11 @@ from . import units as U
 NB = 32            # radial functions per neighbour species
 NF = 2 * NB        # descriptor length (two species)
 HIDDEN = 64
-NFIELD = 9         # per head: one field as is, one weighted by U - U0, one by h - h0
+NFIELD = 6         # per head: one field as is, one weighted by h - h0
 NOUT = NFIELD * NF
 HEADS = ("C11", "C12", "C44")
 NGLOBAL = 27       # global_features: S, T, Q
-NGLOBAL_USED = 27  # the controls use S, T and the h-weighted sums
+NGLOBAL_USED = 27  # the controls use S, T and the h-weighted displacement sums
 
 
 def _torch():
@@ -123,6 +125,14 @@ def driver_descriptors(drv, basis: Basis = Basis(), device="cuda:0", out=None):
     _check(nat.lib().mdl_driver_cv_descriptors(drv.handle, basis.rc, basis.mu0, basis.dmu,
                                                basis.width, _p(D), _p(J), _p(L), _p(pos),
                                                _p(F)))
+    return out
+
+
+def driver_atoms(drv, what, device="cuda:0"):
+    """A per-atom [N, 3] array of the driver (MDL_A_* layout, CGS) copied on the device."""
+    torch = _torch()
+    out = torch.empty((drv.N, 3), device=device, dtype=torch.float64)
+    _check(nat.lib().mdl_driver_copy_atoms(drv.handle, nat.A_INDEX[what], _p(out)))
     return out
 
 
@@ -210,6 +220,53 @@ class TangentDense:
         return cls._cached.apply(X, T, W, b, act)
 
 
+class FieldContract:
+    """(a, cF, cG) per atom and field from the field weights w [M, F*64] and
+    their tangents wt [3, M, F*64] (mdl_cv_field_contract), with the backward
+    with respect to w and wt (mdl_cv_field_contract_backward)."""
+
+    _cached = None
+
+    @staticmethod
+    def _fn():
+        torch = _torch()
+
+        class _F(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, w, wt, J, L, F, G, beta):
+                M = J.shape[1]
+                nf = w.shape[1] // NF
+                outs = [torch.empty((M, nf), device=w.device, dtype=torch.float32)
+                        for _ in range(3)]
+                args = [x.contiguous() for x in (w, wt, J, L, F, G)]
+                _check(nat.lib().mdl_cv_field_contract(*[_p(x) for x in args], float(beta), M, nf,
+                                                       *[_p(o) for o in outs], _stream()))
+                ctx.save_for_backward(*args[2:])
+                ctx.beta, ctx.nf = float(beta), nf
+                return tuple(outs)
+
+            @staticmethod
+            def backward(ctx, ga, gcf, gcg):
+                J, L, F, G = ctx.saved_tensors
+                M, nf = J.shape[1], ctx.nf
+                g = [torch.zeros((M, nf), device=J.device) if x is None else x.contiguous()
+                     for x in (ga, gcf, gcg)]
+                gw = torch.empty((M, nf * NF), device=J.device)
+                gwt = torch.empty((3, M, nf * NF), device=J.device)
+                _check(nat.lib().mdl_cv_field_contract_backward(
+                    _p(J), _p(L), _p(F), _p(G), ctx.beta, *[_p(x) for x in g], M, nf, _p(gw),
+                    _p(gwt), _stream()))
+                return gw, gwt, None, None, None, None, None
+
+        return _F
+
+    @classmethod
+    def apply(cls, w, wt, J, L, F, G, beta):
+        if cls._cached is None:
+            cls._cached = cls._fn()
+        return cls._cached.apply(w, wt, J, L, F, G, beta)
+
+
 class SteinNet:
     """Weights of the learned fields: descriptors -> 6 fields x 64 weights per
     atom, with their tangents along each atom's own coordinates. Parameters in
@@ -238,40 +295,38 @@ class SteinNet:
     def numpy_params(self):
         return [p.detach().double().cpu().numpy() for p in self.parameters()]
 
-    def forward(self, D, J):
-        """w [M, 6, 64] and its tangents wt [3, M, 6, 64]."""
+    def weights(self, D, J):
+        """The field weights w [M, 6*64] and their tangents wt [3, M, 6*64]."""
         X, T = D * self.scale, J * self.scale
         h, ht = TangentDense.apply(X, T, self.W1, self.b1, 0)
         h, ht = TangentDense.apply(h, ht, self.W2, self.b2, 0)
-        o, ot = TangentDense.apply(h, ht, self.W3, self.b3, 1)
+        return TangentDense.apply(h, ht, self.W3, self.b3, 1)
+
+    def forward(self, D, J):
+        """w [M, 6, 64] and its tangents wt [3, M, 6, 64]."""
+        o, ot = self.weights(D, J)
         M = D.shape[0]
         return o.view(M, NFIELD, NF), ot.view(3, M, NFIELD, NF)
 
     def field_sums(self, D, J, L, F, G, beta, nframes):
         """Per frame and field: A = sum_i div_i c_i + beta c_i . F_i (the Stein
         sum), C = sum_i c_i . F_i and E = sum_i c_i . G_i, all times out_scale
-        ([nframes, 9] each), for D [B*N, 64], J [3, B*N, 64], L [B*N, 64],
+        ([nframes, 6] each), for D [B*N, 64], J [3, B*N, 64], L [B*N, 64],
         F and G [B*N, 3] (F in eV/A), beta in 1/eV."""
-        torch = _torch()
-        w, wt = self.forward(D, J)
-        c = torch.einsum("mfk,amk->mfa", w, J)
-        div = torch.einsum("mfk,mk->mf", w, L) + torch.einsum("amfk,amk->mf", wt, J)
-        cF = torch.einsum("mfa,ma->mf", c, F)
-        cG = torch.einsum("mfa,ma->mf", c, G)
-        A = (div + beta * cF).view(nframes, -1, NFIELD).sum(1)
+        o, ot = self.weights(D, J)
+        a, cF, cG = FieldContract.apply(o, ot, J, L, F, G, beta)
+        A = a.view(nframes, -1, NFIELD).sum(1)
         Cs = cF.view(nframes, -1, NFIELD).sum(1)
         E = cG.view(nframes, -1, NFIELD).sum(1)
         return self.out_scale * A, self.out_scale * Cs, self.out_scale * E
 
 
-def net_features(A, Cs, E, dU, dh):
-    """[B, 3, 3] per head: the Stein sums of its plain field, of its field
-    weighted by U - U0 (dU A - C) and of its field weighted by h - h0
-    (dh A + E, E = sum_i c_i . grad_i h)."""
+def net_features(A, Cs, E, dh):
+    """[B, 3, 2] per head: the Stein sums of its plain field and of its field
+    weighted by h - h0 (dh A + E, E = sum_i c_i . grad_i h)."""
     torch = _torch()
-    A, Cs, E = A.double(), Cs.double(), E.double()
-    return torch.stack([A[:, 0::3], dU[:, None] * A[:, 1::3] - Cs[:, 1::3],
-                        dh[:, None] * A[:, 2::3] + E[:, 2::3]], dim=2)
+    A, E = A.double(), E.double()
+    return torch.stack([A[:, 0::2], dh[:, None] * A[:, 1::2] + E[:, 1::2]], dim=2)
 
 
 # ---------------------------------------------------------------------------
@@ -414,28 +469,18 @@ def _record_scalars(r, N):
             r[nat.S_BORN_ELAST:nat.S_BORN_ELAST + 36].reshape(6, 6) / U.GPa)
 
 
-def frame_inputs(drv, sites, box, basis: Basis = Basis(), device="cuda:0", out=None):
+def frame_inputs(drv, basis: Basis = Basis(), device="cuda:0", out=None):
     """The per-atom inputs of the controls for the driver's current step, on
-    the device: descriptors D, J, L, displacements from the sites (A) and forces
-    (eV/A). ``out`` = (D, J, L, u, F) buffers to fill."""
-    torch = _torch()
-    N = drv.N
-    if out is None:
-        out = (torch.empty((N, NF), device=device), torch.empty((3, N, NF), device=device),
-               torch.empty((N, NF), device=device),
-               torch.empty((N, 3), device=device, dtype=torch.float64),
-               torch.empty((N, 3), device=device, dtype=torch.float64))
-    D, J, L, u, F = out
-    driver_descriptors(drv, basis, device, out=(D, J, L, u, F))
-    u.copy_(displacements(u / U.angstrom, sites, box))
-    F.mul_(U.angstrom / U.ev)
-    return out
+    the device: descriptors D, J, L and the positions and forces (CGS).
+    ``out`` = (D, J, L, pos, F) buffers to fill."""
+    return driver_descriptors(drv, basis, device, out=out)
 
 
 def collect_frames(drv, nframes, stride, T, sites=None, basis: Basis = Basis(),
                    device="cuda:0"):
     """Advance the driver nframes x stride steps and keep the scalars of every
-    stride-th step, and with ``sites`` its per-atom inputs on the device."""
+    stride-th step, and with ``sites`` (a flag: any value but None) its per-atom
+    inputs on the device (D, J, L, and positions and forces in CGS)."""
     torch = _torch()
     N = drv.N
     ep = np.zeros(nframes)
@@ -444,18 +489,17 @@ def collect_frames(drv, nframes, stride, T, sites=None, basis: Basis = Basis(),
     box = np.diag(drv.box.h) / U.angstrom
     store = None
     if sites is not None:
-        st = torch.as_tensor(np.asarray(sites), dtype=torch.float64, device=device)
         store = dict(D=torch.empty((nframes, N, NF), device=device),
                      J=torch.empty((nframes, 3, N, NF), device=device),
                      L=torch.empty((nframes, N, NF), device=device),
-                     u=torch.empty((nframes, N, 3), device=device, dtype=torch.float64),
+                     pos=torch.empty((nframes, N, 3), device=device, dtype=torch.float64),
                      F=torch.empty((nframes, N, 3), device=device, dtype=torch.float64))
     for k in range(nframes):
         r = drv.run(stride, sample_stride=stride)[-1]
         ep[k], sb[k], cb[k] = _record_scalars(r, N)
         if store is not None:
-            frame_inputs(drv, st, box, basis, device,
-                         out=tuple(store[q][k] for q in ("D", "J", "L", "u", "F")))
+            frame_inputs(drv, basis, device,
+                         out=tuple(store[q][k] for q in ("D", "J", "L", "pos", "F")))
     return Frames(ep, sb, cb, box, T, N / float(np.prod(box)), store)
 
 
@@ -465,13 +509,13 @@ def collect_frames(drv, nframes, stride, T, sites=None, basis: Basis = Basis(),
 class ControlVariate:
     """s_h(x) = theta_h . X_h(x) per head h. X_h: 27 global Stein sums (the
     displacement fields e_a u_ib as they are, weighted by U - U0 and weighted by
-    h - h0, scaled by their training spread) and the sums of the head's three
-    learned fields (plain, U-weighted, h-weighted). h = sum_i V[t_i] . D_i is a
+    h - h0, scaled by their training spread) and the sums of the head's two
+    learned fields (plain, h-weighted). h = sum_i V[t_i] . D_i is a
     linear function of the descriptors fitted to the instantaneous pressure.
     Heads whose held-out variance the variate does not cut are switched off
     (s_h = 0), so the controlled estimate falls back to the plain one."""
 
-    NX = NGLOBAL_USED + 3
+    NX = NGLOBAL_USED + 2
 
     def __init__(self, sites, box, T, types, device="cuda:0", seed=0, basis: Basis = Basis(),
                  use_net=True):
@@ -516,16 +560,19 @@ class ControlVariate:
         return 1.0 - r.var() / p.var()
 
     # -- features ----------------------------------------------------------------
-    def _x(self, D, J, L, u, F, Ut, grad=False):
-        """X [B, 3, NX] (torch, FP64) for B frames of per-atom inputs."""
+    def _x(self, D, J, L, pos, Fc, Ut, grad=False):
+        """X [B, 3, NX] (torch, FP64) for B frames of per-atom inputs (positions
+        and forces in CGS)."""
         torch = _torch()
-        B, N, _ = u.shape
+        B, N, _ = pos.shape
+        u = displacements(pos / U.angstrom, self.sites, self.box)
+        F = Fc * (U.angstrom / U.ev)
         M = torch.einsum("bna,bnc->bac", F, u)
         S = (N - 1) * torch.eye(3, dtype=M.dtype, device=M.device) + self.beta * M
         dU = Ut - self.U0
         dh = (self._species_sums(D.double()) * self.V[None]).sum((1, 2)) - self.h0
-        Wg = proxy_weights(self.V)[self.types]                              # [N, 64]
-        gh = torch.einsum("bank,nk->bna", J.double(), Wg)                   # [B, N, 3]
+        Wg = proxy_weights(self.V)[self.types].float()                      # [N, 64]
+        gh = torch.einsum("bank,nk->bna", J, Wg).double()                   # [B, N, 3]
         t = torch.cat([S.reshape(B, 9), (dU[:, None, None] * S - M).reshape(B, 9),
                        h_features(S, dh, gh, u)], dim=1)
         t = t / torch.as_tensor(self.t_scale, dtype=t.dtype, device=t.device)
@@ -535,16 +582,17 @@ class ControlVariate:
                     D.reshape(B * N, NF), J.permute(1, 0, 2, 3).reshape(3, B * N, NF),
                     L.reshape(B * N, NF), F.reshape(B * N, 3).float(),
                     gh.reshape(B * N, 3).float(), self.beta, B)
-                nx = net_features(A, Cs, E, dU, dh)
+                nx = net_features(A, Cs, E, dh)
         else:
-            nx = torch.zeros((B, 3, 3), dtype=t.dtype, device=t.device)
+            nx = torch.zeros((B, 3, 2), dtype=t.dtype, device=t.device)
         return torch.cat([t[:, None, :].expand(B, 3, NGLOBAL_USED), nx], dim=2)
 
     def _batch(self, fr: Frames, sl, grad=False):
         torch = _torch()
         st = fr.store
         Ut = torch.as_tensor(fr.epot[sl], dtype=torch.float64, device=self.device)
-        return self._x(st["D"][sl], st["J"][sl], st["L"][sl], st["u"][sl], st["F"][sl], Ut, grad)
+        return self._x(st["D"][sl], st["J"][sl], st["L"][sl], st["pos"][sl], st["F"][sl], Ut,
+                       grad)
 
     def features(self, fr: Frames, batch=8):
         out = np.zeros((fr.n, 3, self.NX))
@@ -584,7 +632,7 @@ class ControlVariate:
         glob = self._fit_linear(X[:, :, :NGLOBAL_USED], yc)
         history = []
         if self.use_net and epochs > 0:
-            self.coef = np.concatenate([glob, np.ones((3, 3))], axis=1)
+            self.coef = np.concatenate([glob, np.ones((3, 2))], axis=1)
             params = self.net.parameters()
             opt = torch.optim.Adam(params, lr=lr)
             sched = torch.optim.lr_scheduler.ExponentialLR(opt, gamma=0.1 ** (1.0 / epochs))
@@ -620,7 +668,7 @@ class ControlVariate:
         if self.use_net:
             self.coef = self._fit_linear(X, yc)
         else:
-            self.coef = np.concatenate([glob, np.zeros((3, 3))], axis=1)
+            self.coef = np.concatenate([glob, np.zeros((3, 2))], axis=1)
         yv = valid.y()
         sv = self.controls(valid, enabled_only=False)
         var_plain, var_cv = yv.var(0), (yv - sv).var(0)
@@ -651,7 +699,7 @@ class ControlBatch:
 
     def push(self, drv, epot):
         """Take the driver's current step (epot: configurational energy, eV)."""
-        frame_inputs(drv, self.cvm.sites, self.cvm.box, self.cvm.basis, self.cvm.device,
+        frame_inputs(drv, self.cvm.basis, self.cvm.device,
                      out=tuple(b[self.k] for b in self.buf))
         self.U[self.k] = epot
         self.k += 1

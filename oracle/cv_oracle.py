@@ -138,12 +138,12 @@ def field_sums(c, div, F, beta, G=None):
     return (div + beta * cF).sum(0), cF.sum(0), E
 
 
-def net_features(A, C, E, dU, dh, out_scale=1.0):
-    """Per head j (fields 3j, 3j+1, 3j+2): the Stein sums of the plain field,
-    of the field times (U - U0), dU A - C (grad_i U = -F_i), and of the field
-    times (h - h0), dh A + E (E = sum_i c_i . grad_i h). Returns [3, 3]."""
-    A, C, E = (out_scale * np.asarray(x) for x in (A, C, E))
-    return np.stack([A[0::3], dU * A[1::3] - C[1::3], dh * A[2::3] + E[2::3]], axis=1)
+def net_features(A, C, E, dh, out_scale=1.0):
+    """Per head j (fields 2j, 2j+1): the Stein sums of the plain field and of
+    the field times (h - h0), dh A + E (E = sum_i c_i . grad_i h). The sums of
+    fields times (U - U0) are dU A - C (grad_i U = -F_i). Returns [3, 2]."""
+    A, E = (out_scale * np.asarray(x) for x in (A, E))
+    return np.stack([A[0::2], dh * A[1::2] + E[1::2]], axis=1)
 
 
 def proxy(D, types, V):

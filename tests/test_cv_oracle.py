@@ -41,7 +41,7 @@ def _params(seed=1, w3=0.3):
     rs = np.random.RandomState(seed)
     return [rs.normal(size=(64, 64)) / 8, rs.normal(size=64) * 0.1,
             rs.normal(size=(64, 64)) / 8, rs.normal(size=64) * 0.1,
-            rs.normal(size=(64, 576)) * w3, rs.normal(size=576) * 0.1]
+            rs.normal(size=(64, 384)) * w3, rs.normal(size=384) * 0.1]
 
 
 def _fd(f, pos, i, a, h=1e-5):
@@ -75,10 +75,10 @@ def test_network_tangents_and_field_divergence_are_finite_differences():
     def field(p, i):
         Dp, Jp, Lp = O.descriptors(p, types, box)
         wp, wtp = O.net_forward(P, Dp, Jp)
-        return O.fields(wp, wtp, Jp, Lp)[0][i]          # [9, 3]
+        return O.fields(wp, wtp, Jp, Lp)[0][i]          # [6, 3]
 
     for i in (3, 50):
-        fd_div = np.zeros(9)
+        fd_div = np.zeros(6)
         for a in range(3):
             fw = _fd(lambda p: O.net_forward(P, *O.descriptors(p, types, box)[:2])[0][i], pos, i, a)
             assert np.allclose(wt[a, i], fw, rtol=1e-6, atol=1e-8 * np.abs(wt).max())
@@ -135,7 +135,7 @@ def test_stein_identity_of_the_learned_fields():
     D0, J0, L0 = O.descriptors(sites, types, box)
     # output biases aligned with the descriptor Laplacians at the sites give
     # divergences with a large mean, so that the checks below have power
-    P[5] = 0.05 * np.tile(np.sign(L0.sum(0)), 9) * np.repeat([1, -1, 2, -2, 1, 3, -1, 2, 1], 64)
+    P[5] = 0.05 * np.tile(np.sign(L0.sum(0)), 6) * np.repeat([1, -1, 2, -2, 1, 3], 64)
     # the fields at the sites form a constant field (zero divergence) whose
     # Stein sum beta c0 . F has zero mean on its own; taking it out of the
     # additive sums leaves a quieter statistic for the power checks
@@ -150,7 +150,8 @@ def test_stein_identity_of_the_learned_fields():
         U = 0.5 * k * (u ** 2).sum()
         h, gh = O.proxy(D, types, V), O.proxy_grad(J, types, V)
         A, C, E = O.field_sums(c, div, F, beta, gh)
-        feats.append(O.net_features(A, C, E, U - 1.0, h - 40.0).ravel())
+        feats.append(np.concatenate([O.net_features(A, C, E, h - 40.0).ravel(),
+                                     (U - 1.0) * A - C]))
         hs.append(O.h_stein(u, F, h, 40.0, gh, beta, len(u)))
         quiet.append(A - beta * np.einsum("mfa,ma->f", c0, F))
         divs.append(div.sum(0))

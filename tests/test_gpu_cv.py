@@ -55,7 +55,7 @@ def test_driver_descriptors_match_the_oracle():
     print("driver descriptors: brass 500 atoms", flush=True)
     drv, sites, types = cv.brass_driver(edge=5)
     drv.run(300, sample_stride=300)
-    D, J, L = cv.driver_descriptors(drv)
+    D, J, L, P2, F2 = cv.driver_descriptors(drv)
     pos = drv.positions_array() / cv.U.angstrom
     box = np.diag(drv.box.h) / cv.U.angstrom
     Do, Jo, Lo = O.descriptors(pos, types, box)
@@ -66,6 +66,7 @@ def test_driver_descriptors_match_the_oracle():
     assert np.array_equal(F, drv.forces_array())
     P = cv.driver_atoms(drv, "positions").cpu().numpy()
     assert np.array_equal(P, drv.positions_array())
+    assert np.array_equal(P2.cpu().numpy(), P) and np.array_equal(F2.cpu().numpy(), F)
 
 
 def test_descriptors_refuse_a_cutoff_beyond_half_the_box():
@@ -142,8 +143,8 @@ def test_network_and_field_sums_match_the_oracle():
     assert _rel(Cs.detach().cpu().numpy()[0], net.out_scale * Co) < 1e-3
     assert _rel(E.detach().cpu().numpy()[0], net.out_scale * Eo) < 1e-3
     one = torch.tensor([0.7], dtype=torch.float64, device=DEV)
-    nx = cv.net_features(A, Cs, E, one, -one)
-    ref = O.net_features(Ao, Co, Eo, 0.7, -0.7, net.out_scale)
+    nx = cv.net_features(A, Cs, E, -one)
+    ref = O.net_features(Ao, Co, Eo, -0.7, net.out_scale)
     assert np.allclose(nx.detach().cpu().numpy()[0], ref, rtol=1e-3,
                        atol=1e-3 * np.abs(ref).max())
 
@@ -165,8 +166,8 @@ def _torch_reference(params, D, J, L, F, G, beta, out_scale, scale=0.1):
     h, ht = layer(h, ht, W2, b2, 0)
     o, ot = layer(h, ht, W3, b3, 1)
     M = D.shape[0]
-    w, wt = o.reshape(M, 9, 64), ot.reshape(3, M, 9, 64)
-    c = (w[None] * J[:, :, None, :]).sum(3)                      # [3, M, 9]
+    w, wt = o.reshape(M, 6, 64), ot.reshape(3, M, 6, 64)
+    c = (w[None] * J[:, :, None, :]).sum(3)                      # [3, M, 6]
     div = (w * L[:, None, :]).sum(2) + (wt * J[:, :, None, :]).sum((0, 3))
     cF = (c * F.T[:, :, None]).sum(0)
     cG = (c * G.T[:, :, None]).sum(0)
@@ -180,7 +181,7 @@ def test_backward_matches_fp64_autograd():
     F = np.random.RandomState(5).normal(size=(len(pos), 3))
     G = np.random.RandomState(8).normal(size=(len(pos), 3))
     net = _net(seed=6)
-    wts = torch.tensor(np.random.RandomState(7).normal(size=27))
+    wts = torch.tensor(np.random.RandomState(7).normal(size=18))
     print("network backward: 256 atoms", flush=True)
     t = [torch.tensor(x, dtype=torch.float32, device=DEV) for x in (D, J, L)]
     A, Cs, E = net.field_sums(t[0], t[1], t[2],
@@ -227,19 +228,22 @@ def test_stein_identity_and_fallback_on_md_frames():
     batch = cv.ControlBatch(cvm, d.N, B=4)
     ref = cv.collect_frames(d, 0, 20, 300.0)
     eps = []
-    st = {q: [] for q in ("D", "J", "L", "u", "F")}
+    st = {q: [] for q in ("D", "J", "L", "pos", "F")}
     for k in range(6):
         rec = d.run(20, sample_stride=20)[-1]
         eps.append(cv._record_scalars(rec, d.N)[0])
         batch.push(d, eps[-1])
-        for q, v in zip(("D", "J", "L", "u", "F"), cv.frame_inputs(d, cvm.sites, cvm.box)):
+        for q, v in zip(("D", "J", "L", "pos", "F"), cv.frame_inputs(d)):
             st[q].append(v)
     s_loop = batch.result()
     ref.store = {q: torch.stack(v) for q, v in st.items()}
     ref.epot = np.asarray(eps)
     s_ref = cvm.controls(ref, enabled_only=False)
     assert s_loop.shape == (6, 3)
-    assert np.allclose(s_loop, s_ref, rtol=1e-9, atol=1e-9 * np.abs(s_ref).max())
+    # the same inputs; FP32 sums over the atoms of a frame may round in
+    # another order for another batch shape
+    print("in-loop", s_loop.ravel(), "stored", s_ref.ravel(), flush=True)
+    assert _rel(s_loop, s_ref) < 1e-5
     cvm.enabled[:] = False
     Cp, sep = cv.estimate(ev)
     Cc, sec = cv.estimate(ev, cvm.controls(ev))
