@@ -13,6 +13,7 @@
 // layer is three more products of the same kinds.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 
@@ -102,6 +103,59 @@ __global__ __launch_bounds__(256) void k_gemm_nn(const float *__restrict__ A,
         int const rr = r0 + kq + q;
         if (rr < R)
             C[(size_t)rr * N + col] = acc[q] + (bias ? bias[col] : 0.f);
+    }
+}
+
+// The same product with the B operand staged once per block in LDS as bf16,
+// transposed to [N][K + 4] (so a lane's four k values are one 8-byte read and
+// the 16 columns of a tile fall in distinct banks), and each wave's A
+// fragments (16 rows x K) held in registers across all N/16 column tiles.
+// Waves stride over 16-row tiles; KT = K / 16.
+template <int KT>
+__global__ __launch_bounds__(256) void k_gemm_lds(const float *__restrict__ A,
+                                                  const float *__restrict__ W, int transB,
+                                                  const float *__restrict__ bias,
+                                                  float *__restrict__ C, int R, int N) {
+    extern __shared__ short wl[];
+    constexpr int K = KT * 16, KP = K + 4;
+    for (int e = threadIdx.x; e < K * N; e += blockDim.x) {
+        int k, n;
+        if (transB) {
+            n = e / K;
+            k = e - n * K;
+        } else {
+            k = e / N;
+            n = e - k * N;
+        }
+        wl[n * KP + k] = bf16_rne(W[e]);
+    }
+    __syncthreads();
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int const kq = 4 * (lane >> 4), c16 = lane & 15;
+    for (int r0 = (blockIdx.x * 4 + wave) * 16; r0 < R; r0 += gridDim.x * 64) {
+        int const row = r0 + c16;
+        bf16x4 af[KT];
+#pragma unroll
+        for (int t = 0; t < KT; ++t)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                af[t][q] = bf16_rne(row < R ? A[(size_t)row * K + 16 * t + kq + q] : 0.f);
+        for (int n0 = 0; n0 < N; n0 += 16) {
+            int const col = n0 + c16;
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int t = 0; t < KT; ++t) {
+                bf16x4 const b = *reinterpret_cast<const bf16x4 *>(&wl[col * KP + 16 * t + kq]);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[t], b, acc, 0, 0, 0);
+            }
+            float const bb = bias ? bias[col] : 0.f;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                int const rr = r0 + kq + q;
+                if (rr < R)
+                    C[(size_t)rr * N + col] = acc[q] + bb;
+            }
+        }
     }
 }
 
@@ -332,10 +386,25 @@ mdl_status mdl_cv_gemm(const float *A, const float *W, int32_t transB, const flo
     }
     if (R == 0)
         return MDL_OK;
+    hipStream_t const st = (hipStream_t)stream;
+    // staged B: N (K + 4) bf16 within 64 KB of LDS
+    if ((int64_t)N * (K + 4) <= 32768 && (K == 16 || K == 32 || K == 64 || K == 128 ||
+                                           K == 256 || K == 384)) {
+        unsigned const blocks = (unsigned)std::min<int64_t>((R + 63) / 64, 512);
+        size_t const lds = (size_t)N * (K + 4) * sizeof(short);
+        switch (K / 16) {
+        case 1: k_gemm_lds<1><<<blocks, 256, lds, st>>>(A, W, transB, bias, C, (int)R, N); break;
+        case 2: k_gemm_lds<2><<<blocks, 256, lds, st>>>(A, W, transB, bias, C, (int)R, N); break;
+        case 4: k_gemm_lds<4><<<blocks, 256, lds, st>>>(A, W, transB, bias, C, (int)R, N); break;
+        case 8: k_gemm_lds<8><<<blocks, 256, lds, st>>>(A, W, transB, bias, C, (int)R, N); break;
+        case 16: k_gemm_lds<16><<<blocks, 256, lds, st>>>(A, W, transB, bias, C, (int)R, N); break;
+        default: k_gemm_lds<24><<<blocks, 256, lds, st>>>(A, W, transB, bias, C, (int)R, N); break;
+        }
+        return launch_ok("mdl_cv_gemm");
+    }
     int64_t const tiles = ((R + 15) / 16) * (N / 16);
     int64_t const blocks = (tiles * 64 + 255) / 256;
-    k_gemm_nn<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(A, W, transB, bias, C, (int)R,
-                                                                 K, N);
+    k_gemm_nn<<<(unsigned)blocks, 256, 0, st>>>(A, W, transB, bias, C, (int)R, K, N);
     return launch_ok("mdl_cv_gemm");
 }
 

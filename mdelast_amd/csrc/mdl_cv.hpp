@@ -36,7 +36,7 @@ __device__ __forceinline__ void cv_phi(const CvBasis &b, int k, float r, float &
     float const u = r - (b.mu0 + b.dmu * (float)k);
     float const g = __expf(-0.5f * u * u * b.iw2);
     float sn, c;
-    sincosf(b.pirc * r, &sn, &c);
+    __sincosf(b.pirc * r, &sn, &c);   // argument in [0, pi]
     float const fc = 0.5f * (c + 1.f);
     float const dfc = -0.5f * b.pirc * sn;
     float const d2fc = -0.5f * b.pirc * b.pirc * c;
@@ -44,7 +44,7 @@ __device__ __forceinline__ void cv_phi(const CvBasis &b, int k, float r, float &
     phi = g * fc;
     float const d1 = g * (dfc - ui * fc);
     float const d2 = g * ((ui * ui - b.iw2) * fc - 2.f * ui * dfc + d2fc);
-    float const ir = 1.f / r;
+    float const ir = __builtin_amdgcn_rcpf(r);
     w = d1 * ir;
     lap = d2 + 2.f * d1 * ir;
 }

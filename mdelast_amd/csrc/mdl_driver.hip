@@ -714,12 +714,13 @@ static mdl_status create(const mdl_config *c, Driver &d) {
 
 using namespace mdl;
 
-// Control-variate descriptors of the current configuration from the full
-// neighbour list (the list holds every pair within its guaranteed cut-off).
+// Control-variate descriptors of the current configuration from the pairs
+// that pass A of the last force evaluation kept within the potential's
+// cut-off (cnbr; use_full: the full list, before any force evaluation).
 // One wave per two atoms, lane k%32 = radial function k; lengths in A.
 __global__ __launch_bounds__(256) void k_cv_desc_list(AtomArrays a, ListArrays l, int N,
                                                       const DevState *__restrict__ gs,
-                                                      double to_A, CvBasis basis,
+                                                      double to_A, CvBasis basis, int use_full,
                                                       float *__restrict__ D,
                                                       float *__restrict__ J,
                                                       float *__restrict__ Lp) {
@@ -729,8 +730,8 @@ __global__ __launch_bounds__(256) void k_cv_desc_list(AtomArrays a, ListArrays l
     int const ia = active ? i : 0;
     double const L[3] = {gs->L[0], gs->L[1], gs->L[2]};
     double4 const ci = a.pos[ia];
-    int const cnt = active ? l.nnbr[ia] : 0;
-    const int *nb = l.nbr + (size_t)ia * l.stride;
+    int const cnt = active ? (use_full ? l.nnbr[ia] : l.ccnt[ia]) : 0;
+    const int *nb = (use_full ? l.nbr : l.cnbr) + (size_t)ia * l.stride;
     CvAcc acc;
     for (int q = 0; q < cnt; ++q) {
         double4 const cj = a.pos[nb[q]];
@@ -740,7 +741,7 @@ __global__ __launch_bounds__(256) void k_cv_desc_list(AtomArrays a, ListArrays l
         float const r2 = x * x + y * y + z * z;
         if (!(r2 < basis.rc2))
             continue;
-        acc.add(basis, k, (int)cj.w, x, y, z, sqrtf(r2));
+        acc.add(basis, k, (int)cj.w, x, y, z, __builtin_amdgcn_sqrtf(r2));
     }
     if (active)
         acc.store(i, k, N, D, J, Lp);
@@ -1038,9 +1039,13 @@ mdl_status mdl_driver_cv_descriptors(mdl_driver *p, float rc, float mu0, float d
     MDL_HIP(hipSetDevice(d->device));
     if (d->failed)
         return bad("the driver stopped on an earlier error");
+    // the compacted pairs lie within the potential's cut-off rc2 (cm^2); a float
+    // rc equal to it rounds up by < 1e-7
+    int const use_full =
+        (d->cycle_host == 0 || (double)rc * rc * 1e-16 > d->rc2 * (1.0 + 1e-6)) ? 1 : 0;
     int const waves = (d->N + 1) / 2;
     k_cv_desc_list<<<(waves * 64 + 255) / 256, 256, 0, d->stream>>>(
-        d->a, d->l, d->N, d->gs, 1e8, cv_basis(rc, mu0, dmu, width), Dp, Jp, Lp);
+        d->a, d->l, d->N, d->gs, 1e8, cv_basis(rc, mu0, dmu, width), use_full, Dp, Jp, Lp);
     MDL_HIP(hipGetLastError());
     unsigned const g = (d->N + 255) / 256;
     if (pos_out) {

@@ -184,6 +184,10 @@ class TangentDense:
                 N = W.shape[1]
                 Z = _gemm(X, W, b)
                 Zt = _gemm(T.reshape(3 * M, K), W).reshape(3, M, N)
+                if act == 1:            # identity: the products are the outputs
+                    ctx.save_for_backward(X, T, W, Z, Zt)
+                    ctx.act = act
+                    return Z, Zt
                 Y = torch.empty_like(Z)
                 Yt = torch.empty_like(Zt)
                 _check(nat.lib().mdl_cv_act_forward(_p(Z), _p(Zt), _p(Y), _p(Yt), M * N, act,
@@ -199,10 +203,14 @@ class TangentDense:
                 N = W.shape[1]
                 gY = gY.contiguous() if gY is not None else torch.zeros_like(Z)
                 gYt = gYt.contiguous() if gYt is not None else torch.zeros_like(Zt)
-                gZ = torch.empty_like(Z)
-                gZt = torch.empty_like(Zt)
-                _check(nat.lib().mdl_cv_act_backward(_p(Z), _p(Zt), _p(gY), _p(gYt), _p(gZ),
-                                                     _p(gZt), M * N, ctx.act, _stream()))
+                if ctx.act == 1:
+                    gZ, gZt = gY, gYt
+                else:
+                    gZ = torch.empty_like(Z)
+                    gZt = torch.empty_like(Zt)
+                    _check(nat.lib().mdl_cv_act_backward(_p(Z), _p(Zt), _p(gY), _p(gYt),
+                                                         _p(gZ), _p(gZt), M * N, ctx.act,
+                                                         _stream()))
                 gX = _gemm(gZ, W, transB=True)
                 gT = _gemm(gZt.reshape(3 * M, N), W, transB=True).reshape(3, M, K)
                 gW = _gemm_tn(torch.cat([X, T.reshape(3 * M, K)]),

@@ -76,7 +76,8 @@ def test_descriptors_refuse_a_cutoff_beyond_half_the_box():
 
 
 @pytest.mark.parametrize("R,K,N,transB", [(100, 64, 48, 0), (37, 64, 64, 1), (16, 16, 16, 0),
-                                          (2049, 64, 16, 1)])
+                                          (2049, 64, 16, 1), (300, 384, 64, 1),
+                                          (70000, 64, 384, 0), (40, 48, 32, 0)])
 def test_gemm_matches_bf16_operands_in_fp64(R, K, N, transB):
     rs = np.random.RandomState(R)
     A = rs.normal(size=(R, K)).astype(np.float32)
