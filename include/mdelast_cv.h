@@ -36,10 +36,13 @@ mdl_status mdl_cv_descriptors(const double *pos, const int32_t *types, int32_t N
  * unless NULL, its positions and forces (device [N][3] FP64, the layout and
  * CGS units of mdl_driver_get_atoms; positions need a step taken). rc must not
  * exceed the list's guaranteed cut-off and the driver must have at most two
- * species. Runs on the driver's stream and returns when done. */
+ * species. With a stream (hipStream_t) the kernels run there, ordered after
+ * the driver's queued work, and the driver's later work waits for them (no
+ * host synchronisation); with NULL they run on the driver's stream and the
+ * call returns when they are done. */
 mdl_status mdl_driver_cv_descriptors(mdl_driver *d, float rc, float mu0, float dmu, float width,
                                      float *D, float *J, float *L, double *pos_out,
-                                     double *force_out);
+                                     double *force_out, void *stream);
 
 /* Device-to-device copy of a per-atom array of the driver (MDL_A_*, the
  * layout and units of mdl_driver_get_atoms) into device memory `out`;

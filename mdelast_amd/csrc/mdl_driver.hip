@@ -123,6 +123,7 @@ struct Driver {
     std::vector<void *> allocs;
     // timing of the pair passes
     std::vector<hipEvent_t> ev;
+    hipEvent_t ev_cv[2] = {nullptr, nullptr};   // ordering with a caller's stream
     double t_dens = 0, t_force = 0, t_step = 0;
 
     template <class T> mdl_status alloc(T **p, size_t n) {
@@ -135,6 +136,9 @@ struct Driver {
     ~Driver() {
         for (auto e : ev)
             (void)hipEventDestroy(e);
+        for (auto e : ev_cv)
+            if (e)
+                (void)hipEventDestroy(e);
         for (void *p : allocs)
             (void)hipFree(p);
         if (stream)
@@ -1024,7 +1028,7 @@ mdl_status mdl_driver_get_atoms(mdl_driver *p, int32_t what, double *out) {
 
 mdl_status mdl_driver_cv_descriptors(mdl_driver *p, float rc, float mu0, float dmu, float width,
                                      float *Dp, float *Jp, float *Lp, double *pos_out,
-                                     double *force_out) {
+                                     double *force_out, void *stream) {
     Driver *d = D(p);
     if (!d || !Dp || !Jp || !(rc > 0.f) || !(width > 0.f))
         return bad("mdl_driver_cv_descriptors: bad arguments");
@@ -1039,24 +1043,39 @@ mdl_status mdl_driver_cv_descriptors(mdl_driver *p, float rc, float mu0, float d
     MDL_HIP(hipSetDevice(d->device));
     if (d->failed)
         return bad("the driver stopped on an earlier error");
+    // on the caller's stream: after the driver's queued work, and the driver's
+    // later work after these reads; else on the driver's stream, synchronously
+    hipStream_t const st = stream ? (hipStream_t)stream : d->stream;
+    if (stream) {
+        for (auto &e : d->ev_cv)
+            if (!e)
+                MDL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        MDL_HIP(hipEventRecord(d->ev_cv[0], d->stream));
+        MDL_HIP(hipStreamWaitEvent(st, d->ev_cv[0], 0));
+    }
     // the compacted pairs lie within the potential's cut-off rc2 (cm^2); a float
     // rc equal to it rounds up by < 1e-7
     int const use_full =
         (d->cycle_host == 0 || (double)rc * rc * 1e-16 > d->rc2 * (1.0 + 1e-6)) ? 1 : 0;
     int const waves = (d->N + 1) / 2;
-    k_cv_desc_list<<<(waves * 64 + 255) / 256, 256, 0, d->stream>>>(
+    k_cv_desc_list<<<(waves * 64 + 255) / 256, 256, 0, st>>>(
         d->a, d->l, d->N, d->gs, 1e8, cv_basis(rc, mu0, dmu, width), use_full, Dp, Jp, Lp);
     MDL_HIP(hipGetLastError());
     unsigned const g = (d->N + 255) / 256;
     if (pos_out) {
-        k_materialize<<<g, 256, 0, d->stream>>>(d->a, d->N, d->gs, MDL_A_POSITIONS, pos_out);
+        k_materialize<<<g, 256, 0, st>>>(d->a, d->N, d->gs, MDL_A_POSITIONS, pos_out);
         MDL_HIP(hipGetLastError());
     }
     if (force_out) {
-        k_materialize<<<g, 256, 0, d->stream>>>(d->a, d->N, d->gs, MDL_A_FORCES, force_out);
+        k_materialize<<<g, 256, 0, st>>>(d->a, d->N, d->gs, MDL_A_FORCES, force_out);
         MDL_HIP(hipGetLastError());
     }
-    MDL_HIP(hipStreamSynchronize(d->stream));
+    if (stream) {
+        MDL_HIP(hipEventRecord(d->ev_cv[1], st));
+        MDL_HIP(hipStreamWaitEvent(d->stream, d->ev_cv[1], 0));
+    } else {
+        MDL_HIP(hipStreamSynchronize(d->stream));
+    }
     return MDL_OK;
 }
 

@@ -122,9 +122,10 @@ def driver_descriptors(drv, basis: Basis = Basis(), device="cuda:0", out=None):
     for t in out:
         if not t.is_contiguous():
             raise ValueError("driver_descriptors: output buffers must be contiguous")
+    # on torch's current stream, ordered against the driver's stream by events
     _check(nat.lib().mdl_driver_cv_descriptors(drv.handle, basis.rc, basis.mu0, basis.dmu,
                                                basis.width, _p(D), _p(J), _p(L), _p(pos),
-                                               _p(F)))
+                                               _p(F), _stream()))
     return out
 
 
@@ -367,10 +368,11 @@ def global_features(u, F, U, U0, beta, ndiv):
 
 
 def displacements(pos, sites, box):
-    """Minimum-image displacement of every atom from its lattice site (A)."""
+    """Minimum-image displacement of every atom from its lattice site (A); box
+    a sequence or a device tensor of the three edges."""
     torch = _torch()
     d = pos - sites
-    L = torch.as_tensor(box, dtype=d.dtype, device=d.device)
+    L = box if torch.is_tensor(box) else torch.as_tensor(box, dtype=d.dtype, device=d.device)
     return d - L * torch.round(d / L)
 
 
@@ -567,23 +569,46 @@ class ControlVariate:
         r = p - pm - A @ v
         return 1.0 - r.var() / p.var()
 
+    def _device_consts(self):
+        """The box, the feature scales and the coefficients on the device,
+        refreshed when fit() replaces them (no host copies inside the loop)."""
+        torch = _torch()
+        key = (self.t_scale, self.coef)
+        if getattr(self, "_consts_of", None) is None or any(
+                a is not b for a, b in zip(self._consts_of, key)):
+            dev = self.device
+            self._consts = (torch.as_tensor(self.box, dtype=torch.float64, device=dev),
+                            torch.as_tensor(self.t_scale, dtype=torch.float64, device=dev),
+                            torch.as_tensor(self.coef, dtype=torch.float64, device=dev))
+            self._consts_of = key
+        return self._consts
+
+    def _proxy_tables(self):
+        """Per-atom rows of V (FP64) and of the gradient weights (FP32), cached."""
+        if getattr(self, "_tables_of", None) is not self.V:
+            self._tables = (self.V[self.types], proxy_weights(self.V)[self.types].float())
+            self._tables_of = self.V
+        return self._tables
+
     # -- features ----------------------------------------------------------------
     def _x(self, D, J, L, pos, Fc, Ut, grad=False):
         """X [B, 3, NX] (torch, FP64) for B frames of per-atom inputs (positions
         and forces in CGS)."""
         torch = _torch()
         B, N, _ = pos.shape
-        u = displacements(pos / U.angstrom, self.sites, self.box)
+        box_d, tsc, _ = self._device_consts()
+        u = displacements(pos / U.angstrom, self.sites, box_d)
         F = Fc * (U.angstrom / U.ev)
-        M = torch.einsum("bna,bnc->bac", F, u)
+        M = (F.unsqueeze(3) * u.unsqueeze(2)).sum(1)                        # [B, 3, 3]
         S = (N - 1) * torch.eye(3, dtype=M.dtype, device=M.device) + self.beta * M
         dU = Ut - self.U0
-        dh = (self._species_sums(D.double()) * self.V[None]).sum((1, 2)) - self.h0
-        Wg = proxy_weights(self.V)[self.types].float()                      # [N, 64]
-        gh = torch.einsum("bank,nk->bna", J, Wg).double()                   # [B, N, 3]
+        Vt, Wg = self._proxy_tables()
+        dh = (D.double() * Vt).sum((1, 2)) - self.h0
+        gh = (J * Wg).sum(3).transpose(1, 2).double()                       # [B, N, 3]
+        Th = dh[:, None, None] * S + (gh.unsqueeze(3) * u.unsqueeze(2)).sum(1)
         t = torch.cat([S.reshape(B, 9), (dU[:, None, None] * S - M).reshape(B, 9),
-                       h_features(S, dh, gh, u)], dim=1)
-        t = t / torch.as_tensor(self.t_scale, dtype=t.dtype, device=t.device)
+                       Th.reshape(B, 9)], dim=1)
+        t = t / tsc
         if self.use_net:
             with torch.set_grad_enabled(grad):
                 A, Cs, E = self.net.field_sums(
@@ -715,19 +740,24 @@ class ControlBatch:
             self.flush()
 
     def flush(self):
+        """Queue the controls of the gathered frames (no host synchronisation)."""
         torch = _torch()
         if self.k == 0:
             return
         k = self.k
-        Ut = torch.as_tensor(self.U[:k], dtype=torch.float64, device=self.cvm.device)
-        X = self.cvm._x(*[b[:k] for b in self.buf], Ut).detach().cpu().numpy()
-        self.out.append((X * self.cvm.coef[None]).sum(2))
+        # a fresh pinned copy per batch: the asynchronous upload reads it later
+        Ut = torch.tensor(self.U[:k], dtype=torch.float64).pin_memory().to(
+            self.cvm.device, non_blocking=True)
+        X = self.cvm._x(*[b[:k] for b in self.buf], Ut)
+        co = self.cvm._device_consts()[2]
+        self.out.append((X * co[None]).sum(2))
         self.k = 0
 
     def result(self):
         """s [n, 3] of every frame pushed so far (all heads)."""
+        torch = _torch()
         self.flush()
-        return np.concatenate(self.out) if self.out else np.zeros((0, 3))
+        return torch.cat(self.out).cpu().numpy() if self.out else np.zeros((0, 3))
 
 
 def estimate(fr: Frames, s=None, nblocks=64):
@@ -774,6 +804,22 @@ def time_steps(drv, nsteps):
     return (time.perf_counter() - t0) / nsteps
 
 
+def frame_loop_time(drv, nframes, stride, cbat=None):
+    """Seconds per frame of a loop of run(stride) calls reading each frame's
+    record, and with ``cbat`` taking the controls of every frame."""
+    torch = _torch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(nframes):
+        rec = drv.run(stride, sample_stride=stride)[-1]
+        if cbat is not None:
+            cbat.push(drv, _record_scalars(rec, drv.N)[0])
+    if cbat is not None:
+        cbat.result()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / nframes
+
+
 def brass_experiment(n_train=2000, n_valid=400, n_eval=4000, eval_runs=2, stride=20,
                      equil=20_000, edge=10, epochs=20, T=300.0, use_net=True, log=None,
                      precision="mixed"):
@@ -797,9 +843,15 @@ def brass_experiment(n_train=2000, n_valid=400, n_eval=4000, eval_runs=2, stride
     train_s = time.perf_counter() - t0
     del fr, train, valid
     t_step = time_steps(drv, 4000)
-    say(f"step {t_step * 1e6:.1f} us; training {train_s:.1f} s; enabled {cvm.enabled}")
+    # marginal device cost of the controls: the same per-frame loop with and
+    # without them (the controls run on another stream and overlap the steps)
+    loop_plain = frame_loop_time(drv, 400, stride)
+    loop_cv = frame_loop_time(drv, 400, stride, ControlBatch(cvm, drv.N))
+    t_cv = max(loop_cv - loop_plain, 0.0)
+    say(f"step {t_step * 1e6:.1f} us; frame loop {loop_plain * 1e6:.0f} us plain, "
+        f"{loop_cv * 1e6:.0f} us with controls; training {train_s:.1f} s; "
+        f"enabled {cvm.enabled}")
     runs = []
-    t_ctrl = 0.0
     for r in range(eval_runs):
         d, _, _ = brass_driver(edge=edge, T=T, vel_seed=300 + r, precision=precision)
         d.run(equil, sample_stride=equil)
@@ -810,12 +862,8 @@ def brass_experiment(n_train=2000, n_valid=400, n_eval=4000, eval_runs=2, stride
         for k in range(n_eval):
             rec = d.run(stride, sample_stride=stride)[-1]
             ep[k], sb[k], cb[k] = _record_scalars(rec, d.N)
-            t0 = time.perf_counter()
             cbat.push(d, ep[k])
-            t_ctrl += time.perf_counter() - t0
-        t0 = time.perf_counter()
         s = cbat.result()
-        t_ctrl += time.perf_counter() - t0
         box = np.diag(d.box.h) / U.angstrom
         ev = Frames(ep, sb, cb, box, T, d.N / np.prod(box))
         sc = s * cvm.enabled[None, :]
@@ -831,13 +879,13 @@ def brass_experiment(n_train=2000, n_valid=400, n_eval=4000, eval_runs=2, stride
         del d, cbat
     se_p = np.sqrt(np.mean([x["se_plain"] ** 2 for x in runs], 0))
     se_c = np.sqrt(np.mean([x["se_cv"] ** 2 for x in runs], 0))
-    t_cv = t_ctrl / (eval_runs * n_eval)
     cost_p = stride * t_step
     cost_c = stride * t_step + t_cv
     gpu_s_plain = frames_needed(se_p, n_eval) * cost_p
     gpu_s_cv = frames_needed(se_c, n_eval) * cost_c
     out = dict(N=4 * edge ** 3, stride=stride, n_train=n_train, n_valid=n_valid, n_eval=n_eval,
                eval_runs=eval_runs, t_step_s=t_step, t_controls_s=t_cv, train_s=train_s,
+               frame_loop_s=dict(plain=loop_plain, controlled=loop_cv),
                enabled=cvm.enabled.tolist(), fit=fit, runs=runs, se_plain=se_p, se_cv=se_c,
                gpu_s_plain=gpu_s_plain, gpu_s_cv=gpu_s_cv,
                gpu_s_plain_worst=float(gpu_s_plain.max()),
