@@ -644,11 +644,19 @@ class ControlVariate:
             coef[h] = np.linalg.solve(G, A.T @ yc[:, h])
         return coef
 
-    def fit(self, train: Frames, valid: Frames, epochs=30, batch=8, lr=1e-3, log=None):
-        """The pressure proxy, then least squares for the global coefficients,
-        then Adam on the network for the residual variance (the parameters of the
-        best epoch are kept), then least squares for all coefficients; switch off
-        the heads whose variance on ``valid`` does not drop."""
+    def fit(self, train: Frames, valid: Frames, epochs=30, batch=8, lr=1e-3, weight_decay=1e-4,
+            stop_fraction=0.2, log=None):
+        """Fit the control variate on ``train`` and decide on ``valid``.
+
+        1. the pressure proxy h (least squares);
+        2. the global coefficients (least squares, 27 sums per head);
+        3. the network (Adam on the residual variance of the heads that step 2
+           helps on ``valid``) on the first part of ``train``, the parameters of
+           the epoch with the lowest residual on the last ``stop_fraction`` kept;
+        4. all coefficients again (least squares on ``train``);
+        5. per head, the controls with the learned fields are kept where they
+           beat the global ones on ``valid``, the global ones where those beat
+           plain, and the head is switched off otherwise."""
         torch = _torch()
         say = log or (lambda *_: None)
         self.U0 = float(train.epot.mean())
@@ -659,55 +667,74 @@ class ControlVariate:
         X = self.features(train)
         self.t_scale = np.maximum(X[:, 0, :NGLOBAL_USED].std(0), 1e-30)
         X[:, :, :NGLOBAL_USED] /= self.t_scale
-        self.use_net = use_net
         y = train.y()
         yc = y - y.mean(0)
-        glob = self._fit_linear(X[:, :, :NGLOBAL_USED], yc)
-        history = []
-        if self.use_net and epochs > 0:
-            self.coef = np.concatenate([glob, np.ones((3, 2))], axis=1)
+        glob = np.concatenate([self._fit_linear(X[:, :, :NGLOBAL_USED], yc), np.zeros((3, 2))],
+                              axis=1)
+        self.coef = glob
+        yv = valid.y()
+        var_plain = yv.var(0)
+        var_glob = (yv - self.controls(valid, enabled_only=False)).var(0)
+        say(f"global fields: held-out variance / plain {np.round(var_glob / var_plain, 4)}")
+        self.use_net = use_net
+        history, var_net = [], np.full(3, np.inf)
+        heads = var_glob < var_plain
+        if self.use_net and epochs > 0 and heads.any():
+            n_fit = int(round(train.n * (1.0 - stop_fraction)))
+            fit_part, stop_part = subset(train, slice(0, n_fit)), subset(train, slice(n_fit, None))
+            self.coef = np.concatenate([glob[:, :NGLOBAL_USED], np.ones((3, 2))], axis=1)
             params = self.net.parameters()
-            opt = torch.optim.Adam(params, lr=lr)
+            opt = torch.optim.AdamW(params, lr=lr, weight_decay=weight_decay)
             sched = torch.optim.lr_scheduler.ExponentialLR(opt, gamma=0.1 ** (1.0 / epochs))
             co = torch.as_tensor(self.coef, dtype=torch.float64, device=self.device)
-            wt = torch.as_tensor(1.0 / np.maximum(yc.var(0), 1e-30), dtype=torch.float64,
-                                 device=self.device)
+            w = heads / np.maximum(yc.var(0), 1e-30)
+            wt = torch.as_tensor(w / w.sum(), dtype=torch.float64, device=self.device)
             ytc = torch.as_tensor(yc, dtype=torch.float64, device=self.device)
+            ys = yc[n_fit:]
             rs = np.random.RandomState(1234)
             best, best_state = np.inf, [p.detach().clone() for p in params]
             for ep in range(epochs):
                 tot = 0.0
-                order = rs.permutation(train.n // batch)
+                order = rs.permutation(fit_part.n // batch)
                 for bi in order:
                     sl = slice(bi * batch, (bi + 1) * batch)
-                    Xb = self._batch(train, sl, grad=True)
+                    Xb = self._batch(fit_part, sl, grad=True)
                     r = ytc[sl] - (Xb * co[None]).sum(2)
-                    loss = ((r ** 2) * wt).mean()
+                    loss = ((r ** 2) * wt).sum(1).mean()
                     opt.zero_grad()
                     loss.backward()
                     torch.nn.utils.clip_grad_norm_(params, 1.0)
                     opt.step()
                     tot += float(loss.detach())
                 sched.step()
-                history.append(tot / len(order))
-                if history[-1] < best:
-                    best = history[-1]
+                stop = float(((((ys - self.controls(stop_part, enabled_only=False)) ** 2)
+                               * w / w.sum()).sum(1)).mean())
+                history.append(dict(fit=tot / len(order), stop=stop))
+                if stop < best:
+                    best = stop
                     best_state = [p.detach().clone() for p in params]
-                say(f"epoch {ep}: residual variance / plain {history[-1]:.4f}")
+                say(f"epoch {ep}: residual variance / plain {tot / len(order):.4f} "
+                    f"(held-back {stop:.4f})")
             with torch.no_grad():
                 for p, q in zip(params, best_state):
                     p.copy_(q)
-        X = self.features(train)
-        if self.use_net:
+            X = self.features(train)
             self.coef = self._fit_linear(X, yc)
-        else:
-            self.coef = np.concatenate([glob, np.zeros((3, 2))], axis=1)
-        yv = valid.y()
-        sv = self.controls(valid, enabled_only=False)
-        var_plain, var_cv = yv.var(0), (yv - sv).var(0)
+            var_net = (yv - self.controls(valid, enabled_only=False)).var(0)
+            say(f"learned fields: held-out variance / plain {np.round(var_net / var_plain, 4)}")
+        use_fields = var_net < var_glob
+        coef = self.coef.copy()
+        for h in range(3):
+            if not use_fields[h]:
+                coef[h] = glob[h]
+        self.coef = coef          # a new array: the device copy is refreshed
+        var_cv = np.minimum(var_net, var_glob)
         self.enabled = var_cv < var_plain
-        return dict(proxy_r2=r2, var_plain_valid=var_plain, var_cv_valid=var_cv,
-                    enabled=self.enabled.copy(), history=history)
+        self.uses_fields = use_fields & self.enabled
+        return dict(proxy_r2=r2, var_plain_valid=var_plain, var_global_valid=var_glob,
+                    var_fields_valid=var_net, var_cv_valid=var_cv,
+                    enabled=self.enabled.copy(), uses_learned_fields=self.uses_fields.copy(),
+                    history=history)
 
     def controls(self, fr: Frames, enabled_only=True):
         """s [n, 3] of stored frames (zero for switched-off heads)."""
