@@ -212,6 +212,51 @@ def cpu_baseline(w, nsteps=2, rebuild_every=20):
             "ms_per_step": 1e3 * per_step}
 
 
+def learned_cv(n_eval):
+    """GPU-seconds to C11, C12 and C44 at 2 SE <= 1 GPa, plain and with the
+    learned Stein control variate: 4,000-atom alpha-brass (6 at.% Zn random
+    fcc, 10^3 cells, a = 3.667 A), NVT 300 K, dt 1 fs, frames every 20 steps;
+    trained on 2,000 frames of a plain run (400 more held out), evaluated on
+    two fresh trajectories (new velocity seeds). GPU-s = frames needed x the
+    device time per frame (20 MD steps of the device loop, plus the marginal
+    time of the controls for the controlled estimate)."""
+    from mdelast_amd import cv
+    try:
+        r = cv.brass_experiment(n_train=2000, n_valid=400, n_eval=n_eval, eval_runs=2,
+                                stride=20, epochs=20)
+    except Exception as e:        # reported, never hides the throughput line
+        return {"error": f"{type(e).__name__}: {e}"}
+
+    def f(x):
+        return [round(float(v), 6) for v in np.ravel(x)]
+
+    runs = r["runs"]
+    return {
+        "workload": "CuZn 6 at.% Zn random fcc 10^3 cells, 4000 atoms (240 Zn), a=3.667 A, "
+                    "NVT 300 K, dt 1 fs, NHC M=10, frames every 20 steps",
+        "heads": ["C11", "C12", "C44"],
+        "gpu_s_plain": f(r["gpu_s_plain"]),
+        "gpu_s_cv": f(r["gpu_s_cv"]),
+        "gpu_s_plain_worst": round(r["gpu_s_plain_worst"], 6),
+        "gpu_s_cv_worst": round(r["gpu_s_cv_worst"], 6),
+        "speedup": round(r["speedup"], 4),
+        "training_gpu_s": round(r["training_gpu_s"], 3),
+        "C_plain_GPa": [f(x["C_plain"]) for x in runs],
+        "C_cv_GPa": [f(x["C_cv"]) for x in runs],
+        "se_plain_GPa": f(r["se_plain"]),
+        "se_cv_GPa": f(r["se_cv"]),
+        "mean_control_over_se": [f(x["mean_s"] / x["se_mean_s"]) for x in runs],
+        "enabled": [bool(b) for b in r["enabled"]],
+        "uses_learned_fields": [bool(b) for b in r["fit"]["uses_learned_fields"]],
+        "pressure_proxy_r2": round(float(r["fit"]["proxy_r2"]), 5),
+        "heldout_var_ratio": f(np.asarray(r["fit"]["var_cv_valid"]) /
+                               np.asarray(r["fit"]["var_plain_valid"])),
+        "t_step_us": round(r["t_step_s"] * 1e6, 3),
+        "t_controls_us_per_frame": round(r["t_controls_s"] * 1e6, 3),
+        "frames_eval": r["n_eval"] * r["eval_runs"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,6 +273,9 @@ def main():
     ap.add_argument("--replicas", type=int, default=64)
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="hardware queues of the process (GPU_MAX_HW_QUEUES, at most 32)")
+    ap.add_argument("--no-learned-cv", action="store_true",
+                    help="skip the learned control-variate comparison (4,000-atom alpha-brass)")
+    ap.add_argument("--cv-eval-frames", type=int, default=4000)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB three times "
                          "(a kernel of known HBM traffic, to scale rocprofv3 --pmc "
@@ -302,6 +350,9 @@ def main():
         out["cpu_baseline"] = cpu_baseline(w)
     if args.copy_calibration_mib > 0:
         out["copy_calibration"] = copy_calibration(args.copy_calibration_mib)
+    if not args.no_learned_cv:
+        del d, keep
+        out["learned_cv"] = learned_cv(args.cv_eval_frames)
     print(json.dumps(out), flush=True)
 
 
