@@ -117,12 +117,27 @@ SIGNATURES = {
 _lib = None
 
 
+def _one_hip_runtime():
+    """Load torch's HIP runtime before this library when torch is installed.
+
+    PyTorch ships its own libamdhip64. If this library is loaded first, the
+    process maps the system runtime, torch later maps its own, and the second
+    runtime finds no device ("No HIP GPUs are available"). Loading torch first
+    makes this library bind to the runtime torch already mapped, so the
+    control variate's torch tensors and the driver share one runtime."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """The loaded library; raises if it is not built or not loadable."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise MdlError(11, f"{LIB_PATH} is not built (run `python -m mdelast_amd.build`)")
+        _one_hip_runtime()
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

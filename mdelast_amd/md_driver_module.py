@@ -119,8 +119,10 @@ class _Table:
         self.handle = h_
 
     def __del__(self):
-        if getattr(self, "handle", None) and nat._lib is not None:
-            nat._lib.mdl_table_destroy(self.handle)
+        # at interpreter exit the module globals may already be gone
+        lib_ = getattr(nat, "_lib", None) if nat is not None else None
+        if getattr(self, "handle", None) and lib_ is not None:
+            lib_.mdl_table_destroy(self.handle)
             self.handle = None
 
     def coefficients(self) -> np.ndarray:
@@ -456,8 +458,10 @@ class MDDriver:
                   "###############")
 
     def __del__(self):
-        if getattr(self, "handle", None) and nat._lib is not None:
-            nat._lib.mdl_driver_destroy(self.handle)
+        # at interpreter exit the module globals may already be gone
+        lib_ = getattr(nat, "_lib", None) if nat is not None else None
+        if getattr(self, "handle", None) and lib_ is not None:
+            lib_.mdl_driver_destroy(self.handle)
             self.handle = None
 
     # -- calls -------------------------------------------------------------
