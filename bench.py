@@ -266,10 +266,10 @@ def main():
     ap.add_argument("--nside", type=int, default=30)
     ap.add_argument("--lanes-per-atom", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="nialco108k", choices=["nialco108k", "replicas"],
+    ap.add_argument("--workload", default="nialco108k", choices=["nialco108k", "replicas", "cv"],
                     help="nialco108k: the single 108,000-atom instance (default); replicas: "
                          "independent NVT Cu / Cu-Zn replicas of 500-32,000 atoms advanced side "
-                         "by side")
+                         "by side; cv: only the learned control-variate comparison")
     ap.add_argument("--replicas", type=int, default=64)
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="hardware queues of the process (GPU_MAX_HW_QUEUES, at most 32)")
@@ -291,6 +291,12 @@ def main():
         os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(1, args.hw_queues)))
     if args.workload == "replicas":
         return bench_replicas(args)
+    if args.workload == "cv":
+        r = learned_cv(args.cv_eval_frames)
+        print(json.dumps({"metric": "GPU-s to C11/C12/C44 at +/-1 GPa (worst head)",
+                          "value": r.get("gpu_s_cv_worst"), "unit": "s",
+                          "higher_is_better": False, "n_gpus": 1, "learned_cv": r}), flush=True)
+        return
     w = build_workload(nside=args.nside)
     d, keep = make_driver(w, args.precision, args.lanes_per_atom)
     d.run(args.warmup, 1, 20, records=False)
