@@ -120,6 +120,8 @@ def bench_replicas(args):
     specs = R.batched_replica_specs(args.replicas)
     drivers = [R.make_replica_driver(s, pots, eams, masses, precision=args.precision)
                for s in specs]
+    for d in drivers:
+        d.graph_steps = args.graph_steps
     natoms = sum(s["N"] for s in specs)
     R.run_replicas(drivers, args.warmup, 1, 20, records=False)
     t0 = time.perf_counter()
@@ -149,6 +151,7 @@ def bench_replicas(args):
             "atoms_max": max(s["N"] for s in specs),
             "precision": args.precision,
             "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "default"),
+            "graph_steps": args.graph_steps,
             "T_kin_final_minmax": [min(Tk), max(Tk)],
         },
     }
@@ -271,6 +274,9 @@ def main():
                          "independent NVT Cu / Cu-Zn replicas of 500-32,000 atoms advanced side "
                          "by side; cv: only the learned control-variate comparison")
     ap.add_argument("--replicas", type=int, default=64)
+    ap.add_argument("--graph-steps", type=int, default=20,
+                    help="replicas: replay chunks of this many steps as one captured HIP graph "
+                         "per replica (0: launch step by step)")
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="hardware queues of the process (GPU_MAX_HW_QUEUES, at most 32)")
     ap.add_argument("--no-learned-cv", action="store_true",

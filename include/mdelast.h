@@ -148,6 +148,15 @@ mdl_status mdl_driver_run(mdl_driver *d, int64_t nsteps, int64_t sample_stride,
 mdl_status mdl_drivers_run(mdl_driver *const *drivers, int32_t n, int64_t nsteps,
                            int64_t sample_stride, int64_t rebuild_every, double *const *records,
                            int64_t *n_records);
+/* mdl_driver_run and mdl_drivers_run replay chunks of `steps` consecutive
+ * steps as one captured HIP graph (0, the default: launch step by step). A
+ * chunk is used where its first cycle is a multiple of `steps`, `steps` is a
+ * multiple of sample_stride and of rebuild_every (when > 0), no rebuild was
+ * requested and the run's final step is not in it; other steps launch as
+ * usual. The arithmetic and its order are those of the step-by-step launches.
+ * No counterpart in the reference (a launch-overhead measure for small
+ * systems and many replicas). */
+mdl_status mdl_driver_set_graph_steps(mdl_driver *d, int32_t steps);
 /* Force a rebuild before the next force evaluation (md_driver.cpp:279-298). */
 mdl_status mdl_driver_calculate_neighbors(mdl_driver *d);
 /* MDDriver::sample_observables (md_driver.hpp:163-167) */

@@ -94,6 +94,7 @@ SIGNATURES = {
     "mdl_driver_get_atoms": (C.c_int32, [_vp, C.c_int32, _dp]),
     "mdl_driver_get_chain": (C.c_int32, [_vp, C.c_int32, _dp, _dp]),
     "mdl_driver_kernel_times": (C.c_int32, [_vp, _dp]),
+    "mdl_driver_set_graph_steps": (C.c_int32, [_vp, C.c_int32]),
     # learned control variate (include/mdelast_cv.h); device pointers as c_void_p
     "mdl_cv_descriptors": (C.c_int32, [_vp, _vp, C.c_int32, _dp, C.c_float, C.c_float,
                                        C.c_float, C.c_float, _vp, _vp, _vp, _vp]),

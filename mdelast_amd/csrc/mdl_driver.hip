@@ -124,6 +124,11 @@ struct Driver {
     // timing of the pair passes
     std::vector<hipEvent_t> ev;
     hipEvent_t ev_cv[2] = {nullptr, nullptr};   // ordering with a caller's stream
+    // run() replays graph_steps consecutive steps as one captured graph (0: off)
+    int graph_steps = 0;
+    hipGraphExec_t gexec = nullptr;
+    int64_t g_stride = 0, g_rebuild = 0, g_nrec = 0, g_cap = 0;
+    double *g_rec = nullptr;
     double t_dens = 0, t_force = 0, t_step = 0;
 
     template <class T> mdl_status alloc(T **p, size_t n) {
@@ -139,6 +144,8 @@ struct Driver {
         for (auto e : ev_cv)
             if (e)
                 (void)hipEventDestroy(e);
+        if (gexec)
+            (void)hipGraphExecDestroy(gexec);
         for (void *p : allocs)
             (void)hipFree(p);
         if (stream)
@@ -231,6 +238,58 @@ struct Driver {
         launch_finish(1, rec_out, next_sample);
         ++cycle_host;
         return hipGetLastError();
+    }
+
+    void drop_graph() {
+        if (gexec)
+            (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+    }
+
+    // Capture graph_steps steps from the current cycle (a multiple of
+    // graph_steps) with the given sampling stride and build interval, as
+    // launch_step would issue them; every step continues into the next one.
+    // Sampled records go to g_rec in order.
+    mdl_status capture_graph(int64_t stride, int64_t rebuild) {
+        drop_graph();
+        int64_t const G = graph_steps, c0 = cycle_host;
+        int64_t nrec = 0;
+        for (int64_t j = 0; j < G; ++j)
+            if ((c0 + j + 1) % stride == 0)
+                ++nrec;
+        if (nrec > g_cap) {
+            if (mdl_status st = alloc(&g_rec, (size_t)(MDL_NRECORD * nrec)))
+                return st;
+            g_cap = nrec;
+        }
+        hipGraph_t graph = nullptr;
+        MDL_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        int64_t irec = 0;
+        for (int64_t j = 0; j < G && e == hipSuccess; ++j) {
+            int64_t const cyc = c0 + j;
+            bool const smp = (cyc + 1) % stride == 0;
+            bool const build = rebuild > 0 && cyc % rebuild == 0;
+            double *ro = smp ? g_rec + MDL_NRECORD * irec++ : nullptr;
+            e = launch_step(smp, ro, nullptr, build, (cyc + 2) % stride == 0 ? 1 : 0);
+        }
+        hipError_t const e2 = hipStreamEndCapture(stream, &graph);
+        cycle_host = c0;   // launch_step counted the captured steps
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (graph)
+                (void)hipGraphDestroy(graph);
+            return hip_fail(e != hipSuccess ? e : e2, "step graph capture");
+        }
+        e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            gexec = nullptr;
+            return hip_fail(e, "step graph instantiation");
+        }
+        g_stride = stride;
+        g_rebuild = rebuild;
+        g_nrec = nrec;
+        return MDL_OK;
     }
 
     mdl_status sync_state() {
@@ -845,8 +904,34 @@ struct RunPlan {
         return MDL_OK;
     }
 
-    // Launch step k (asynchronous).
+    // graph_steps steps from here as one graph replay: the cycle is aligned, no
+    // build is pending, the build interval and stride fit the chunk, and the
+    // final step of the run stays out (it ends without the next first half)
+    bool graph_fits() const {
+        int64_t const G = d->graph_steps;
+        return G > 0 && k + G < nsteps && d->cycle_host % G == 0 && !d->build_pending &&
+               G % stride == 0 && (rebuild <= 0 || G % rebuild == 0);
+    }
+
+    mdl_status graph_step() {
+        if (!d->gexec || d->g_stride != stride || d->g_rebuild != rebuild)
+            if (mdl_status s = d->capture_graph(stride, rebuild))
+                return s;
+        MDL_HIP(hipGraphLaunch(d->gexec, d->stream));
+        if (drec && d->g_nrec > 0)
+            MDL_HIP(hipMemcpyAsync(drec + MDL_NRECORD * irec, d->g_rec,
+                                   sizeof(double) * MDL_NRECORD * d->g_nrec,
+                                   hipMemcpyDeviceToDevice, d->stream));
+        irec += d->g_nrec;
+        k += d->graph_steps;
+        d->cycle_host += d->graph_steps;
+        return MDL_OK;
+    }
+
+    // Launch step k (asynchronous), or a captured chunk of steps.
     mdl_status step() {
+        if (graph_fits())
+            return graph_step();
         int64_t const cyc = d->cycle_host;
         bool const s = sampled(cyc);
         double *ro = (drec && s) ? drec + MDL_NRECORD * irec : nullptr;
@@ -920,7 +1005,7 @@ mdl_status mdl_driver_run(mdl_driver *p, int64_t nsteps, int64_t sample_stride,
     r.n_records = n_records;
     if (mdl_status s = r.begin())
         return s;
-    for (int64_t k = 0; k < nsteps; ++k)
+    while (r.k < nsteps)
         if (mdl_status s = r.step())
             return s;
     return r.end();
@@ -946,13 +1031,19 @@ mdl_status mdl_drivers_run(mdl_driver *const *drivers, int32_t n, int64_t nsteps
         if (mdl_status s = r.begin())
             return s;
     }
-    // step k of every driver, then step k + 1: the streams run side by side
-    for (int64_t k = 0; k < nsteps; ++k)
+    // step (or captured chunk) k of every driver, then the next: the streams
+    // run side by side
+    for (bool more = true; more;) {
+        more = false;
         for (RunPlan &r : plans) {
+            if (r.k >= nsteps)
+                continue;
+            more = true;
             MDL_HIP(hipSetDevice(r.d->device));
             if (mdl_status s = r.step())
                 return s;
         }
+    }
     mdl_status first = MDL_OK;
     for (RunPlan &r : plans) {
         mdl_status s = r.end();
@@ -960,6 +1051,16 @@ mdl_status mdl_drivers_run(mdl_driver *const *drivers, int32_t n, int64_t nsteps
             first = s;
     }
     return first;
+}
+
+mdl_status mdl_driver_set_graph_steps(mdl_driver *p, int32_t steps) {
+    Driver *d = D(p);
+    if (!d || steps < 0)
+        return bad("mdl_driver_set_graph_steps: bad arguments");
+    MDL_HIP(hipSetDevice(d->device));
+    d->drop_graph();
+    d->graph_steps = steps;
+    return MDL_OK;
 }
 
 mdl_status mdl_driver_calculate_neighbors(mdl_driver *p) {

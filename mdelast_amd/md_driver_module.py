@@ -502,6 +502,16 @@ class MDDriver:
             self._cache = None
         return buf[:n.value]
 
+    @property
+    def graph_steps(self) -> int:
+        """Chunks of this many steps replayed as one captured graph in run() (0: off)."""
+        return getattr(self, "_graph_steps", 0)
+
+    @graph_steps.setter
+    def graph_steps(self, steps: int):
+        self._call(nat.lib().mdl_driver_set_graph_steps(self.handle, int(steps)))
+        self._graph_steps = int(steps)
+
     def kernel_times(self):
         """Average ms of the density pass, force pass and whole step over the last run()."""
         out = np.zeros(3)

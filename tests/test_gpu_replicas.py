@@ -32,3 +32,32 @@ def test_side_by_side_equals_one_at_a_time(precision):
     # temperatures are those of three different systems
     T = [r[-1][11] for r in ra]
     assert len(set(np.round(T, 6))) == 3
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replay_equals_step_launches(precision):
+    """Chunks of 20 steps replayed as captured graphs give the step-by-step
+    launches' records and state bit for bit, with a sampled stride, forced
+    rebuilds, a requested rebuild and a run whose length is not a multiple of
+    the chunk."""
+    elements, pots, eams = get_potential.get_EAM_alloy_file("ZnCu.eam.alloy")
+    masses = [e[2] / U.Navogadro for e in elements]
+    specs = [dict(edge=5, N=500, x_zn=0.06, T=300.0, place_seed=1, vel_seed=2),
+             dict(edge=6, N=864, x_zn=0.0, T=800.0, place_seed=3, vel_seed=4)]
+    a = [R.make_replica_driver(s, pots, eams, masses, precision=precision) for s in specs]
+    b = [R.make_replica_driver(s, pots, eams, masses, precision=precision) for s in specs]
+    for d in a:
+        d.graph_steps = 20
+    for da, db in zip(a, b):
+        assert np.array_equal(da.run(45, 5, 20), db.run(45, 5, 20))
+        da.calculate_neighbors()
+        db.calculate_neighbors()
+        assert np.array_equal(da.run(70, 5, 10), db.run(70, 5, 10))
+    ra = R.run_replicas(a, 101, 1, 20)
+    rb = R.run_replicas(b, 101, 1, 20)
+    for x, y in zip(ra, rb):
+        assert x.shape == (101, 72)
+        assert np.array_equal(x, y)
+    for da, db in zip(a, b):
+        assert np.array_equal(da.forces_array(), db.forces_array())
+        assert np.array_equal(da.positions_array(), db.positions_array())
