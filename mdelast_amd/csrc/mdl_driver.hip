@@ -873,6 +873,7 @@ struct RunPlan {
     Driver *d = nullptr;
     int64_t nsteps = 0, stride = 1, rebuild = 0, k = 0, irec = 0, t0 = 0;
     int ntime = 0;
+    std::vector<char> timed;   // timing slots whose events this run recorded
     double *records = nullptr, *drec = nullptr;
     int64_t *n_records = nullptr;
 
@@ -898,6 +899,7 @@ struct RunPlan {
             d->ev.push_back(e);
         }
         t0 = nsteps - ntime;   // time the last steps (after any warm-up)
+        timed.assign((size_t)ntime, 0);
         if (nsteps > 0)
             k_pre<<<1, BLOCK, 0, d->stream>>>(d->gs, sampled(d->cycle_host) ? 1 : 0);
         MDL_HIP(hipGetLastError());
@@ -941,6 +943,7 @@ struct RunPlan {
         if (k >= t0) {
             tev = &d->ev[4 * (k - t0)];
             (void)hipEventRecord(tev[3], d->stream);
+            timed[(size_t)(k - t0)] = 1;
         }
         int const ns = k + 1 < nsteps ? (sampled(cyc + 1) ? 1 : 0) : -1;
         bool const build = d->build_pending || forced(cyc);
@@ -968,7 +971,11 @@ struct RunPlan {
         // pass timings: [3] step start, [0] before A, [1] after A, [2] after B
         double ta = 0, tb = 0, tsum = 0;
         int nt = 0;
+        // steps replayed from a graph recorded no events: only recorded pairs
+        int nstep = 0;
         for (int j = 0; j < ntime; ++j) {
+            if (!timed[(size_t)j])
+                continue;
             hipEvent_t *e = &d->ev[4 * j];
             float x = 0, y = 0;
             if (hipEventElapsedTime(&x, e[0], e[1]) == hipSuccess &&
@@ -977,17 +984,21 @@ struct RunPlan {
                 tb += y;
                 ++nt;
             }
-            if (j + 1 < ntime) {
+            if (j + 1 < ntime && timed[(size_t)j + 1]) {
                 float z = 0;
-                if (hipEventElapsedTime(&z, e[3], d->ev[4 * (j + 1) + 3]) == hipSuccess)
+                if (hipEventElapsedTime(&z, e[3], d->ev[4 * (j + 1) + 3]) == hipSuccess) {
                     tsum += z;
+                    ++nstep;
+                }
             }
         }
+        (void)hipGetLastError();   // a failed elapsed-time query must not linger
         if (nt > 0) {
             d->t_dens = ta / nt;
             d->t_force = tb / nt;
-            d->t_step = ntime > 1 ? tsum / (ntime - 1) : 0.;
         }
+        if (nstep > 0)
+            d->t_step = tsum / nstep;
         return d->check_errors();
     }
 };
