@@ -316,11 +316,34 @@ __device__ __forceinline__ bool nhc_prop(double (&p)[M], double (&eta)[M], const
                 kin *= f * f;
             }
         }
-        // U4, outward
+        // U4, outward: the M - 1 factors depend only on the momenta after U3,
+        // so lane j of the (fully active, identically loaded) wave evaluates
+        // factor j and the others read it back; the same operations as one
+        // lane evaluating all of them
         double E[M], O[M];
+        {
+            int const lane = threadIdx.x & 63;
+            double xl = 0.;
 #pragma unroll
-        for (int j = 0; j + 1 < M; ++j)
-            expx_flag(c[j + 1] * p[j + 1], E[j], O[j], general);
+            for (int j = 0; j + 1 < M; ++j)
+                xl = lane == j ? c[j + 1] * p[j + 1] : xl;
+            double el, ol;
+            bool const big = !(fabs(xl) <= EXPX_SMALL);
+            if (big) {
+                el = exp_neg_general(xl);
+                ol = (1.0 - el) / xl;
+            } else {
+                ol = op_series(xl);
+                el = fma(-xl, ol, 1.);
+            }
+            if (__ballot(big && lane + 1 < M) != 0ull)
+                general = true;
+#pragma unroll
+            for (int j = 0; j + 1 < M; ++j) {
+                E[j] = __shfl(el, j, 64);
+                O[j] = __shfl(ol, j, 64);
+            }
+        }
         double const e = BARO ? 0.5 * (kin * kin) * inv_W : cw * kin + c0;
         double const tG0 = dt4 * (2. * e - Nf * kT);
         if (M > 1)
@@ -469,7 +492,9 @@ __global__ __launch_bounds__(BLOCK) void k_pre(DevState *__restrict__ gs, int sa
         return;
     __shared__ DevState s;
     state_load(s, gs);
-    if (threadIdx.x == 0) {
+    // the whole first wave runs the scalar step with identical values (the
+    // chain sweeps spread independent factors over its lanes)
+    if (threadIdx.x < 64) {
         pre_step(s, sample);
     }
     state_store(gs, s);
@@ -1435,11 +1460,14 @@ __global__ __launch_bounds__(BLOCK) void k_finish(DevState *__restrict__ gs,
         tot[c] = __hip_atomic_load(&totals[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __shared__ DevState s;
     state_load(s, gs);
-    if (threadIdx.x == 0) {
+    // the whole first wave runs the scalar step with identical values (the
+    // chain sweeps spread independent factors over its lanes)
+    if (threadIdx.x < 64) {
         post_step(s, tot, tot + PA_N, tot + PA_N + PB_N, mode, rec_out);
         if (next_sample >= 0 && !s.err)
             pre_step(s, next_sample);
-        *ticket = 0u;
+        if (threadIdx.x == 0)
+            *ticket = 0u;
     }
     state_store(gs, s);
 }
