@@ -796,15 +796,33 @@ __global__ __launch_bounds__(256) void k_cv_desc_list(AtomArrays a, ListArrays l
     int const cnt = active ? (use_full ? l.nnbr[ia] : l.ccnt[ia]) : 0;
     const int *nb = (use_full ? l.nbr : l.cnbr) + (size_t)ia * l.stride;
     CvAcc acc;
-    for (int q = 0; q < cnt; ++q) {
-        double4 const cj = a.pos[nb[q]];
-        double dx, dy, dz;
-        min_image(ci, cj, L, dx, dy, dz);
-        float const x = (float)(dx * to_A), y = (float)(dy * to_A), z = (float)(dz * to_A);
-        float const r2 = x * x + y * y + z * z;
-        if (!(r2 < basis.rc2))
-            continue;
-        acc.add(basis, k, (int)cj.w, x, y, z, __builtin_amdgcn_sqrtf(r2));
+    // the 32 lanes of an atom gather 32 pairs at once (one load round trip per
+    // 32 pairs), then take them one by one from the lane that holds each, in
+    // list order
+    int const half = lane & 32;
+    for (int q0 = 0; q0 < cnt; q0 += 32) {
+        int const q = q0 + k;
+        float x = 0.f, y = 0.f, z = 0.f, r = 1.f;
+        int sp = 0, ok = 0;
+        if (q < cnt) {
+            double4 const cj = a.pos[nb[q]];
+            double dx, dy, dz;
+            min_image(ci, cj, L, dx, dy, dz);
+            x = (float)(dx * to_A);
+            y = (float)(dy * to_A);
+            z = (float)(dz * to_A);
+            float const r2 = x * x + y * y + z * z;
+            ok = r2 < basis.rc2 ? 1 : 0;
+            r = ok ? __builtin_amdgcn_sqrtf(r2) : 1.f;
+            sp = (int)cj.w;
+        }
+        int const m = min(32, cnt - q0);
+        for (int t = 0; t < m; ++t) {
+            int const src = half + t;
+            if (__shfl(ok, src, 64))
+                acc.add(basis, k, __shfl(sp, src, 64), __shfl(x, src, 64), __shfl(y, src, 64),
+                        __shfl(z, src, 64), __shfl(r, src, 64));
+        }
     }
     if (active)
         acc.store(i, k, N, D, J, Lp);
