@@ -129,6 +129,9 @@ struct Driver {
     hipGraphExec_t gexec = nullptr;
     int64_t g_stride = 0, g_rebuild = 0, g_nrec = 0, g_cap = 0;
     double *g_rec = nullptr;
+    // device staging of run()'s records, kept between calls (grown on demand)
+    double *run_rec = nullptr;
+    int64_t run_rec_cap = 0;
     double t_dens = 0, t_force = 0, t_step = 0;
 
     template <class T> mdl_status alloc(T **p, size_t n) {
@@ -907,8 +910,14 @@ struct RunPlan {
         for (int64_t j = 0; j < nsteps; ++j)
             if ((d->cycle_host + j + 1) % stride == 0)
                 ++nsamp;
-        if (records && nsamp > 0)
-            MDL_HIP(hipMalloc(&drec, sizeof(double) * MDL_NRECORD * (size_t)nsamp));
+        if (records && nsamp > 0) {
+            if (nsamp > d->run_rec_cap) {
+                if (mdl_status st = d->alloc(&d->run_rec, (size_t)(MDL_NRECORD * nsamp)))
+                    return st;
+                d->run_rec_cap = nsamp;
+            }
+            drec = d->run_rec;
+        }
         // events around the passes of up to 64 steps
         ntime = (int)std::min<int64_t>(nsteps, 64);
         while ((int)d->ev.size() < 4 * ntime) {
@@ -977,8 +986,7 @@ struct RunPlan {
                                           hipMemcpyDeviceToHost, d->stream);
             if (e == hipSuccess)
                 e = hipStreamSynchronize(d->stream);
-            (void)hipFree(drec);
-            drec = nullptr;
+            drec = nullptr;   // the driver's buffer, reused by the next run
             if (e != hipSuccess)
                 return hip_fail(e, "mdl_driver_run records");
         }
