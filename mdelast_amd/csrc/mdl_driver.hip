@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <tuple>
 
@@ -129,6 +130,7 @@ struct Driver {
     hipGraphExec_t gexec = nullptr;
     int64_t g_stride = 0, g_rebuild = 0, g_nrec = 0, g_cap = 0;
     double *g_rec = nullptr;
+    uint64_t uid = 0;   // identifies the driver in the joint-graph cache
     // device staging of run()'s records, kept between calls (grown on demand)
     double *run_rec = nullptr;
     int64_t run_rec_cap = 0;
@@ -249,35 +251,51 @@ struct Driver {
         gexec = nullptr;
     }
 
-    // Capture graph_steps steps from the current cycle (a multiple of
-    // graph_steps) with the given sampling stride and build interval, as
-    // launch_step would issue them; every step continues into the next one.
-    // Sampled records go to g_rec in order.
-    mdl_status capture_graph(int64_t stride, int64_t rebuild) {
-        drop_graph();
-        int64_t const G = graph_steps, c0 = cycle_host;
-        int64_t nrec = 0;
-        for (int64_t j = 0; j < G; ++j)
-            if ((c0 + j + 1) % stride == 0)
+    // Records of one captured chunk: count them and make room in g_rec.
+    mdl_status chunk_records(int64_t stride, int64_t &nrec) {
+        nrec = 0;
+        for (int64_t j = 0; j < graph_steps; ++j)
+            if ((cycle_host + j + 1) % stride == 0)
                 ++nrec;
         if (nrec > g_cap) {
+            drop_graph();   // it writes to the previous buffer
             if (mdl_status st = alloc(&g_rec, (size_t)(MDL_NRECORD * nrec)))
                 return st;
             g_cap = nrec;
         }
-        hipGraph_t graph = nullptr;
-        MDL_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        return MDL_OK;
+    }
+
+    // The launches of graph_steps steps from the current cycle (a multiple of
+    // graph_steps) with the given sampling stride and build interval, as
+    // launch_step would issue them, every step continuing into the next one;
+    // sampled records go to g_rec in order. Used under stream capture: the
+    // cycle counter is restored.
+    hipError_t chunk_launches(int64_t stride, int64_t rebuild) {
+        int64_t const c0 = cycle_host;
         hipError_t e = hipSuccess;
         int64_t irec = 0;
-        for (int64_t j = 0; j < G && e == hipSuccess; ++j) {
+        for (int64_t j = 0; j < graph_steps && e == hipSuccess; ++j) {
             int64_t const cyc = c0 + j;
             bool const smp = (cyc + 1) % stride == 0;
             bool const build = rebuild > 0 && cyc % rebuild == 0;
             double *ro = smp ? g_rec + MDL_NRECORD * irec++ : nullptr;
             e = launch_step(smp, ro, nullptr, build, (cyc + 2) % stride == 0 ? 1 : 0);
         }
+        cycle_host = c0;
+        return e;
+    }
+
+    // Capture one chunk of this driver alone as gexec.
+    mdl_status capture_graph(int64_t stride, int64_t rebuild) {
+        drop_graph();
+        int64_t nrec = 0;
+        if (mdl_status st = chunk_records(stride, nrec))
+            return st;
+        hipGraph_t graph = nullptr;
+        MDL_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = chunk_launches(stride, rebuild);
         hipError_t const e2 = hipStreamEndCapture(stream, &graph);
-        cycle_host = c0;   // launch_step counted the captured steps
         if (e != hipSuccess || e2 != hipSuccess) {
             if (graph)
                 (void)hipGraphDestroy(graph);
@@ -831,6 +849,55 @@ __global__ __launch_bounds__(256) void k_cv_desc_list(AtomArrays a, ListArrays l
         acc.store(i, k, N, D, J, Lp);
 }
 
+// ---------------------------------------------------------------------------
+// One captured graph for a chunk of steps of several drivers: each driver's
+// launches form a branch of its own (its stream joins the capture through a
+// fork event), so one launch advances every replica by graph_steps steps.
+// Cached per (drivers, stride, build interval, chunk length); a driver's
+// destruction or a change of its chunk length drops the entries with it.
+// ---------------------------------------------------------------------------
+struct JointGraph {
+    std::vector<uint64_t> ids;
+    int64_t stride = 0, rebuild = 0, steps = 0;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t fork = nullptr, after = nullptr;
+    std::vector<hipEvent_t> joins;
+    std::vector<double *> rec;       // each driver's records of one chunk
+    std::vector<int64_t> nrec;
+
+    void release() {
+        if (exec)
+            (void)hipGraphExecDestroy(exec);
+        for (hipEvent_t e : joins)
+            if (e)
+                (void)hipEventDestroy(e);
+        if (fork)
+            (void)hipEventDestroy(fork);
+        if (after)
+            (void)hipEventDestroy(after);
+        exec = nullptr;
+        fork = after = nullptr;
+        joins.clear();
+    }
+};
+
+std::mutex g_joint_mutex;
+std::vector<JointGraph> g_joint;
+uint64_t g_next_uid = 1;
+
+void joint_forget(uint64_t uid) {
+    std::lock_guard<std::mutex> lock(g_joint_mutex);
+    for (size_t k = 0; k < g_joint.size();) {
+        JointGraph &j = g_joint[k];
+        if (std::find(j.ids.begin(), j.ids.end(), uid) != j.ids.end()) {
+            j.release();
+            g_joint.erase(g_joint.begin() + (long)k);
+        } else {
+            ++k;
+        }
+    }
+}
+
 static Driver *D(mdl_driver *p) { return reinterpret_cast<Driver *>(p); }
 
 extern "C" {
@@ -843,6 +910,10 @@ mdl_status mdl_driver_create(const mdl_config *cfg, mdl_driver **out) {
     mdl_status s = create(cfg, *d);
     if (s != MDL_OK)
         return s;
+    {
+        std::lock_guard<std::mutex> lock(g_joint_mutex);
+        d->uid = g_next_uid++;
+    }
     *out = reinterpret_cast<mdl_driver *>(d.release());
     return MDL_OK;
 }
@@ -853,6 +924,7 @@ mdl_status mdl_driver_destroy(mdl_driver *p) {
         return MDL_OK;
     (void)hipSetDevice(d->device);
     (void)hipStreamSynchronize(d->stream);
+    joint_forget(d->uid);
     delete d;
     return MDL_OK;
 }
@@ -1029,6 +1101,107 @@ struct RunPlan {
     }
 };
 
+// Every plan can take the same graph chunk now (same chunk length, device,
+// position in the run, and each fits).
+static bool joint_fits(std::vector<RunPlan> &plans) {
+    RunPlan const &p0 = plans[0];
+    for (RunPlan const &r : plans)
+        if (!r.graph_fits() || r.k != p0.k || r.d->graph_steps != p0.d->graph_steps ||
+            r.d->device != p0.d->device)
+            return false;
+    return true;
+}
+
+static mdl_status joint_capture(std::vector<RunPlan> &plans, JointGraph &j) {
+    size_t const n = plans.size();
+    Driver *d0 = plans[0].d;
+    j.stride = plans[0].stride;
+    j.rebuild = plans[0].rebuild;
+    j.steps = d0->graph_steps;
+    j.rec.assign(n, nullptr);
+    j.nrec.assign(n, 0);
+    j.joins.assign(n, nullptr);
+    MDL_HIP(hipEventCreateWithFlags(&j.fork, hipEventDisableTiming));
+    MDL_HIP(hipEventCreateWithFlags(&j.after, hipEventDisableTiming));
+    for (size_t i = 0; i < n; ++i) {
+        j.ids.push_back(plans[i].d->uid);
+        MDL_HIP(hipEventCreateWithFlags(&j.joins[i], hipEventDisableTiming));
+        if (mdl_status st = plans[i].d->chunk_records(j.stride, j.nrec[i]))
+            return st;
+        j.rec[i] = plans[i].d->g_rec;
+    }
+    hipGraph_t graph = nullptr;
+    MDL_HIP(hipStreamBeginCapture(d0->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipEventRecord(j.fork, d0->stream);
+    for (size_t i = 1; i < n && e == hipSuccess; ++i)
+        e = hipStreamWaitEvent(plans[i].d->stream, j.fork, 0);
+    for (size_t i = 0; i < n && e == hipSuccess; ++i)
+        e = plans[i].d->chunk_launches(j.stride, j.rebuild);
+    for (size_t i = 1; i < n && e == hipSuccess; ++i) {
+        e = hipEventRecord(j.joins[i], plans[i].d->stream);
+        if (e == hipSuccess)
+            e = hipStreamWaitEvent(d0->stream, j.joins[i], 0);
+    }
+    hipError_t const e2 = hipStreamEndCapture(d0->stream, &graph);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        return hip_fail(e != hipSuccess ? e : e2, "joint step graph capture");
+    }
+    e = hipGraphInstantiate(&j.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        j.exec = nullptr;
+        return hip_fail(e, "joint step graph instantiation");
+    }
+    return MDL_OK;
+}
+
+// One launch of the joint chunk; afterwards every driver's stream waits for it.
+static mdl_status joint_step(std::vector<RunPlan> &plans) {
+    size_t const n = plans.size();
+    Driver *d0 = plans[0].d;
+    MDL_HIP(hipSetDevice(d0->device));
+    std::lock_guard<std::mutex> lock(g_joint_mutex);
+    JointGraph *j = nullptr;
+    for (JointGraph &c : g_joint) {
+        if (c.ids.size() != n || c.stride != plans[0].stride || c.rebuild != plans[0].rebuild ||
+            c.steps != d0->graph_steps)
+            continue;
+        bool same = true;
+        for (size_t i = 0; i < n && same; ++i)
+            same = c.ids[i] == plans[i].d->uid;
+        if (same) {
+            j = &c;
+            break;
+        }
+    }
+    if (!j) {
+        g_joint.emplace_back();
+        j = &g_joint.back();
+        if (mdl_status st = joint_capture(plans, *j)) {
+            j->release();
+            g_joint.pop_back();
+            return st;
+        }
+    }
+    MDL_HIP(hipGraphLaunch(j->exec, d0->stream));
+    MDL_HIP(hipEventRecord(j->after, d0->stream));
+    for (size_t i = 0; i < n; ++i) {
+        RunPlan &r = plans[i];
+        if (i > 0)
+            MDL_HIP(hipStreamWaitEvent(r.d->stream, j->after, 0));
+        if (r.drec && j->nrec[i] > 0)
+            MDL_HIP(hipMemcpyAsync(r.drec + MDL_NRECORD * r.irec, j->rec[i],
+                                   sizeof(double) * MDL_NRECORD * j->nrec[i],
+                                   hipMemcpyDeviceToDevice, r.d->stream));
+        r.irec += j->nrec[i];
+        r.k += j->steps;
+        r.d->cycle_host += j->steps;
+    }
+    return MDL_OK;
+}
+
 extern "C" {
 
 mdl_status mdl_driver_run(mdl_driver *p, int64_t nsteps, int64_t sample_stride,
@@ -1069,9 +1242,16 @@ mdl_status mdl_drivers_run(mdl_driver *const *drivers, int32_t n, int64_t nsteps
             return s;
     }
     // step (or captured chunk) k of every driver, then the next: the streams
-    // run side by side
+    // run side by side; where every driver can take the same chunk, one joint
+    // graph launch advances them all
     for (bool more = true; more;) {
         more = false;
+        if (n > 1 && joint_fits(plans)) {
+            if (mdl_status s = joint_step(plans))
+                return s;
+            more = true;
+            continue;
+        }
         for (RunPlan &r : plans) {
             if (r.k >= nsteps)
                 continue;
@@ -1096,6 +1276,7 @@ mdl_status mdl_driver_set_graph_steps(mdl_driver *p, int32_t steps) {
         return bad("mdl_driver_set_graph_steps: bad arguments");
     MDL_HIP(hipSetDevice(d->device));
     d->drop_graph();
+    joint_forget(d->uid);
     d->graph_steps = steps;
     return MDL_OK;
 }

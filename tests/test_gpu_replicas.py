@@ -61,3 +61,35 @@ def test_graph_replay_equals_step_launches(precision):
     for da, db in zip(a, b):
         assert np.array_equal(da.forces_array(), db.forces_array())
         assert np.array_equal(da.positions_array(), db.positions_array())
+
+
+def test_joint_graph_equals_step_launches():
+    """Replicas whose chunks are captured together as one graph (one launch
+    per chunk for all of them) give the step-by-step results bit for bit,
+    also after a driver of the set is replaced."""
+    elements, pots, eams = get_potential.get_EAM_alloy_file("ZnCu.eam.alloy")
+    masses = [e[2] / U.Navogadro for e in elements]
+    specs = [dict(edge=5, N=500, x_zn=0.06, T=300.0, place_seed=1, vel_seed=2),
+             dict(edge=6, N=864, x_zn=0.0, T=800.0, place_seed=3, vel_seed=4),
+             dict(edge=5, N=500, x_zn=0.10, T=150.0, place_seed=5, vel_seed=6)]
+    a = [R.make_replica_driver(s, pots, eams, masses, precision="mixed") for s in specs]
+    b = [R.make_replica_driver(s, pots, eams, masses, precision="mixed") for s in specs]
+    for d in a:
+        d.graph_steps = 20
+    for rep in range(2):
+        ra = R.run_replicas(a, 61, 1, 20)
+        rb = R.run_replicas(b, 61, 1, 20)
+        for x, y in zip(ra, rb):
+            assert np.array_equal(x, y)
+    # a new driver in place of one of the set: the cached graph must not be reused
+    a[1] = R.make_replica_driver(specs[1], pots, eams, masses, precision="mixed")
+    b[1] = R.make_replica_driver(specs[1], pots, eams, masses, precision="mixed")
+    a[1].graph_steps = 20
+    a[1].run(122, 1, 20)
+    b[1].run(122, 1, 20)
+    ra = R.run_replicas(a, 60, 5, 20)
+    rb = R.run_replicas(b, 60, 5, 20)
+    for x, y in zip(ra, rb):
+        assert np.array_equal(x, y)
+    for da, db in zip(a, b):
+        assert np.array_equal(da.forces_array(), db.forces_array())
