@@ -249,3 +249,27 @@ def test_stein_identity_and_fallback_on_md_frames():
     Cp, sep = cv.estimate(ev)
     Cc, sec = cv.estimate(ev, cvm.controls(ev))
     assert np.array_equal(Cp, Cc) and np.array_equal(sep, sec)
+
+
+def test_learned_fields_are_kept_only_where_they_help():
+    """The fit measures, per head and on held-out frames, the plain variance,
+    the variance with the global fields and the variance with the learned
+    fields added, and keeps the lowest: the variate never does worse on the
+    held-out frames than the global fields alone or than plain."""
+    print("brass 500 atoms: network gain", flush=True)
+    drv, sites, types = cv.brass_driver(edge=5, vel_seed=17)
+    drv.run(4000, sample_stride=4000)
+    fr = cv.collect_frames(drv, 600, 20, 300.0, sites=sites)
+    train, valid = cv.subset(fr, slice(0, 480)), cv.subset(fr, slice(480, None))
+    cvm = cv.ControlVariate(sites, fr.box, 300.0, types, device=DEV, seed=3)
+    info = cvm.fit(train, valid, epochs=4)
+    print({k: v for k, v in info.items() if k != "history"}, flush=True)
+    vp, vg, vf, vc = (np.asarray(info[k]) for k in ("var_plain_valid", "var_global_valid",
+                                                    "var_fields_valid", "var_cv_valid"))
+    assert np.all(vc <= vg) and np.all(vc == np.minimum(vg, vf))
+    assert np.array_equal(info["enabled"], vc < vp)
+    assert np.array_equal(info["uses_learned_fields"], (vf < vg) & (vc < vp))
+    # the kept controls reproduce the held-out variance the fit reported
+    s = cvm.controls(valid, enabled_only=False)
+    assert np.allclose((valid.y() - s).var(0), vc, rtol=1e-6)
+    assert len(info["history"]) == 4
