@@ -32,8 +32,9 @@ mdl_status mdl_cv_descriptors(const double *pos, const int32_t *types, int32_t N
                               float *D, float *J, float *L, void *stream);
 
 /* The same descriptors of the driver's current configuration, from its
- * neighbour list (lengths in A; the driver's species index is the type), and,
- * unless NULL, its positions and forces (device [N][3] FP64, the layout and
+ * neighbour list (lengths in A; the driver's species index is the type),
+ * unless NULL the Laplacian's axis terms La[a][i][q] = d^2 D[i][q] / d x_{i,a}^2
+ * ([3][N][64]), and, unless NULL, its positions and forces (device [N][3] FP64, the layout and
  * CGS units of mdl_driver_get_atoms; positions need a step taken). rc must not
  * exceed the list's guaranteed cut-off and the driver must have at most two
  * species. With a stream (hipStream_t) the kernels run there, ordered after
@@ -41,7 +42,7 @@ mdl_status mdl_cv_descriptors(const double *pos, const int32_t *types, int32_t N
  * host synchronisation); with NULL they run on the driver's stream and the
  * call returns when they are done. */
 mdl_status mdl_driver_cv_descriptors(mdl_driver *d, float rc, float mu0, float dmu, float width,
-                                     float *D, float *J, float *L, double *pos_out,
+                                     float *D, float *J, float *L, float *La, double *pos_out,
                                      double *force_out, void *stream);
 
 /* Device-to-device copy of a per-atom array of the driver (MDL_A_*, the

@@ -99,7 +99,7 @@ SIGNATURES = {
     "mdl_cv_descriptors": (C.c_int32, [_vp, _vp, C.c_int32, _dp, C.c_float, C.c_float,
                                        C.c_float, C.c_float, _vp, _vp, _vp, _vp]),
     "mdl_driver_cv_descriptors": (C.c_int32, [_vp, C.c_float, C.c_float, C.c_float, C.c_float,
-                                              _vp, _vp, _vp, _vp, _vp, _vp]),
+                                              _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "mdl_driver_copy_atoms": (C.c_int32, [_vp, C.c_int32, _vp]),
     "mdl_cv_gemm": (C.c_int32, [_vp, _vp, C.c_int32, _vp, _vp, C.c_int64, C.c_int32, C.c_int32,
                                 _vp]),

@@ -7,7 +7,8 @@
 // For one pair at distance r and separation vector d = x_i - x_j, atom i's
 // descriptor D_ik gains phi_k(r); its derivative along x_{i,a} gains
 // phi_k'(r) d_a / r; and the Laplacian sum_a d^2 D_ik / d x_{i,a}^2 gains
-// phi_k''(r) + 2 phi_k'(r) / r (the Laplacian of a radial function in 3-D).
+// phi_k''(r) + 2 phi_k'(r) / r (the Laplacian of a radial function in 3-D),
+// of which axis a contributes (phi'' - phi'/r)(d_a/r)^2 + phi'/r.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -30,9 +31,9 @@ __host__ __device__ inline CvBasis cv_basis(float rc, float mu0, float dmu, floa
     return b;
 }
 
-// phi_k, phi_k' / r and phi_k'' + 2 phi_k' / r of basis function k at r
+// phi_k, phi_k' / r and phi_k'' of basis function k at r
 __device__ __forceinline__ void cv_phi(const CvBasis &b, int k, float r, float &phi, float &w,
-                                       float &lap) {
+                                       float &d2) {
     float const u = r - (b.mu0 + b.dmu * (float)k);
     float const g = __expf(-0.5f * u * u * b.iw2);
     float sn, c;
@@ -43,30 +44,33 @@ __device__ __forceinline__ void cv_phi(const CvBasis &b, int k, float r, float &
     float const ui = u * b.iw2;
     phi = g * fc;
     float const d1 = g * (dfc - ui * fc);
-    float const d2 = g * ((ui * ui - b.iw2) * fc - 2.f * ui * dfc + d2fc);
-    float const ir = __builtin_amdgcn_rcpf(r);
-    w = d1 * ir;
-    lap = d2 + 2.f * d1 * ir;
+    d2 = g * ((ui * ui - b.iw2) * fc - 2.f * ui * dfc + d2fc);
+    w = d1 * __builtin_amdgcn_rcpf(r);
 }
 
 // One half-wave (32 lanes, lane = basis index) accumulates the pairs of one
-// atom; store its 64 descriptors, 3 x 64 derivatives and 64 Laplacians.
+// atom; store its 64 descriptors, 3 x 64 derivatives, 64 Laplacians and,
+// optionally, the Laplacian's three axis terms d^2 D / d x_a^2 =
+// (phi'' - phi'/r)(d_a/r)^2 + phi'/r (their sum is phi'' + 2 phi'/r).
 struct CvAcc {
-    float d[2], jx[2], jy[2], jz[2], l[2];
+    float d[2], jx[2], jy[2], jz[2], lx[2], ly[2], lz[2];
 
     __device__ __forceinline__ CvAcc() {
 #pragma unroll
         for (int s = 0; s < 2; ++s)
-            d[s] = jx[s] = jy[s] = jz[s] = l[s] = 0.f;
+            d[s] = jx[s] = jy[s] = jz[s] = lx[s] = ly[s] = lz[s] = 0.f;
     }
 
     __device__ __forceinline__ void add(const CvBasis &b, int k, int species, float x, float y,
                                         float z, float r) {
-        float phi, w, lap;
-        cv_phi(b, k, r, phi, w, lap);
+        float phi, w, d2;
+        cv_phi(b, k, r, phi, w, d2);
         int const s = species != 0;
         // two accumulators selected without dynamic register indexing
         float const m0 = s ? 0.f : 1.f, m1 = 1.f - m0;
+        float const ir2 = __builtin_amdgcn_rcpf(r * r);
+        float const q = (d2 - w) * ir2;
+        float const ax = fmaf(q, x * x, w), ay = fmaf(q, y * y, w), az = fmaf(q, z * z, w);
         d[0] += m0 * phi;
         d[1] += m1 * phi;
         jx[0] += m0 * w * x;
@@ -75,11 +79,16 @@ struct CvAcc {
         jy[1] += m1 * w * y;
         jz[0] += m0 * w * z;
         jz[1] += m1 * w * z;
-        l[0] += m0 * lap;
-        l[1] += m1 * lap;
+        lx[0] += m0 * ax;
+        lx[1] += m1 * ax;
+        ly[0] += m0 * ay;
+        ly[1] += m1 * ay;
+        lz[0] += m0 * az;
+        lz[1] += m1 * az;
     }
 
-    __device__ __forceinline__ void store(int i, int k, int N, float *D, float *J, float *L) const {
+    __device__ __forceinline__ void store(int i, int k, int N, float *D, float *J, float *L,
+                                          float *La = nullptr) const {
         int const F = 2 * CV_NB;
         size_t const plane = (size_t)N * F;
 #pragma unroll
@@ -90,7 +99,12 @@ struct CvAcc {
             J[plane + o] = jy[s];
             J[2 * plane + o] = jz[s];
             if (L)
-                L[o] = l[s];
+                L[o] = lx[s] + ly[s] + lz[s];
+            if (La) {
+                La[o] = lx[s];
+                La[plane + o] = ly[s];
+                La[2 * plane + o] = lz[s];
+            }
         }
     }
 };

@@ -807,7 +807,8 @@ __global__ __launch_bounds__(256) void k_cv_desc_list(AtomArrays a, ListArrays l
                                                       double to_A, CvBasis basis, int use_full,
                                                       float *__restrict__ D,
                                                       float *__restrict__ J,
-                                                      float *__restrict__ Lp) {
+                                                      float *__restrict__ Lp,
+                                                      float *__restrict__ La) {
     int const lane = threadIdx.x & 63, k = lane & 31;
     int const i = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 2 + (lane >> 5);
     bool const active = i < N;
@@ -846,7 +847,7 @@ __global__ __launch_bounds__(256) void k_cv_desc_list(AtomArrays a, ListArrays l
         }
     }
     if (active)
-        acc.store(i, k, N, D, J, Lp);
+        acc.store(i, k, N, D, J, Lp, La);
 }
 
 // ---------------------------------------------------------------------------
@@ -1346,7 +1347,7 @@ mdl_status mdl_driver_get_atoms(mdl_driver *p, int32_t what, double *out) {
 }
 
 mdl_status mdl_driver_cv_descriptors(mdl_driver *p, float rc, float mu0, float dmu, float width,
-                                     float *Dp, float *Jp, float *Lp, double *pos_out,
+                                     float *Dp, float *Jp, float *Lp, float *La, double *pos_out,
                                      double *force_out, void *stream) {
     Driver *d = D(p);
     if (!d || !Dp || !Jp || !(rc > 0.f) || !(width > 0.f))
@@ -1378,7 +1379,7 @@ mdl_status mdl_driver_cv_descriptors(mdl_driver *p, float rc, float mu0, float d
         (d->cycle_host == 0 || (double)rc * rc * 1e-16 > d->rc2 * (1.0 + 1e-6)) ? 1 : 0;
     int const waves = (d->N + 1) / 2;
     k_cv_desc_list<<<(waves * 64 + 255) / 256, 256, 0, st>>>(
-        d->a, d->l, d->N, d->gs, 1e8, cv_basis(rc, mu0, dmu, width), use_full, Dp, Jp, Lp);
+        d->a, d->l, d->N, d->gs, 1e8, cv_basis(rc, mu0, dmu, width), use_full, Dp, Jp, Lp, La);
     MDL_HIP(hipGetLastError());
     unsigned const g = (d->N + 255) / 256;
     if (pos_out) {

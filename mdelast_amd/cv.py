@@ -106,10 +106,11 @@ def descriptors(pos_A, types, box_A, basis: Basis = Basis()):
     return D, J, L
 
 
-def driver_descriptors(drv, basis: Basis = Basis(), device="cuda:0", out=None):
+def driver_descriptors(drv, basis: Basis = Basis(), device="cuda:0", out=None, La=None):
     """D, J, L of the driver's current configuration from its neighbour list,
     and its positions and forces (CGS, [N, 3]); ``out`` = (D, J, L, pos, F)
-    device buffers to fill, else new tensors."""
+    device buffers to fill, else new tensors. With ``La`` (a [3, N, 64] FP32
+    device tensor) also the Laplacian's axis terms."""
     torch = _torch()
     N = drv.N
     if out is None:
@@ -119,13 +120,13 @@ def driver_descriptors(drv, basis: Basis = Basis(), device="cuda:0", out=None):
                torch.empty((N, 3), device=device, dtype=torch.float64),
                torch.empty((N, 3), device=device, dtype=torch.float64))
     D, J, L, pos, F = out
-    for t in out:
+    for t in out + ((La,) if La is not None else ()):
         if not t.is_contiguous():
             raise ValueError("driver_descriptors: output buffers must be contiguous")
     # on torch's current stream, ordered against the driver's stream by events
     _check(nat.lib().mdl_driver_cv_descriptors(drv.handle, basis.rc, basis.mu0, basis.dmu,
-                                               basis.width, _p(D), _p(J), _p(L), _p(pos),
-                                               _p(F), _stream()))
+                                               basis.width, _p(D), _p(J), _p(L), _p(La),
+                                               _p(pos), _p(F), _stream()))
     return out
 
 

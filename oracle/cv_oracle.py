@@ -50,17 +50,20 @@ def min_image(d, box):
     return d - box * np.rint(d / box)
 
 
-def descriptors(pos, types, box, rc=6.8, mu0=2.0, dmu=0.15, width=0.15):
+def descriptors(pos, types, box, rc=6.8, mu0=2.0, dmu=0.15, width=0.15, axes=False):
     """D [N, 64], J [3, N, 64] = d D_i / d x_{i,a} and L [N, 64] =
-    sum_a d^2 D_i / d x_{i,a}^2, brute force over pairs.
+    sum_a d^2 D_i / d x_{i,a}^2, brute force over pairs; with axes also the
+    axis terms La [3, N, 64] = d^2 D_i / d x_{i,a}^2.
     phi_k(r) = exp(-(r - mu_k)^2 / (2 w^2)) (cos(pi r / rc) + 1) / 2; the
-    Laplacian of a radial function is phi'' + 2 phi' / r."""
+    Laplacian of a radial function is phi'' + 2 phi' / r, of which axis a
+    holds (phi'' - phi'/r)(d_a/r)^2 + phi'/r."""
     pos = np.asarray(pos, dtype=float)
     N = len(pos)
     mu = mu0 + dmu * np.arange(NB)
     D = np.zeros((N, 2 * NB))
     J = np.zeros((3, N, 2 * NB))
     L = np.zeros((N, 2 * NB))
+    La = np.zeros((3, N, 2 * NB))
     for i in range(N):
         d = min_image(pos[i] - pos, box)
         r = np.sqrt((d ** 2).sum(1))
@@ -85,7 +88,9 @@ def descriptors(pos, types, box, rc=6.8, mu0=2.0, dmu=0.15, width=0.15):
             L[i, s * NB:(s + 1) * NB] = lap[sel].sum(0)
             for a in range(3):
                 J[a, i, s * NB:(s + 1) * NB] = (w[sel] * d[sel, a:a + 1]).sum(0)
-    return D, J, L
+                cos2 = (d[sel, a] / r[sel]) ** 2
+                La[a, i, s * NB:(s + 1) * NB] = ((d2[sel] - w[sel]) * cos2[:, None] + w[sel]).sum(0)
+    return (D, J, L, La) if axes else (D, J, L)
 
 
 def dense_forward(X, T, W, b, act, emulate_bf16=False):

@@ -63,6 +63,13 @@ def test_descriptor_derivatives_are_finite_differences():
             assert np.allclose(J[a, i], fd, rtol=1e-6, atol=1e-8 * np.abs(J).max())
             lap += _fd(lambda p: O.descriptors(p, types, box)[1][a, i], pos, i, a)
         assert np.allclose(L[i], lap, rtol=1e-5, atol=1e-7 * np.abs(L).max())
+    # the axis terms: second differences along each own coordinate, summing to L
+    D2, J2, L2, La = O.descriptors(pos, types, box, axes=True)
+    assert np.allclose(La.sum(0), L, rtol=1e-12, atol=1e-12 * np.abs(L).max())
+    for i in (5, 60):
+        for a in range(3):
+            fd = _fd(lambda p: O.descriptors(p, types, box)[1][a, i], pos, i, a)
+            assert np.allclose(La[a, i], fd, rtol=1e-5, atol=1e-7 * np.abs(La).max())
 
 
 def test_network_tangents_and_field_divergence_are_finite_differences():

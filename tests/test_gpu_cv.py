@@ -55,10 +55,12 @@ def test_driver_descriptors_match_the_oracle():
     print("driver descriptors: brass 500 atoms", flush=True)
     drv, sites, types = cv.brass_driver(edge=5)
     drv.run(300, sample_stride=300)
-    D, J, L, P2, F2 = cv.driver_descriptors(drv)
+    La = torch.empty((3, drv.N, 64), device=DEV)
+    D, J, L, P2, F2 = cv.driver_descriptors(drv, La=La)
     pos = drv.positions_array() / cv.U.angstrom
     box = np.diag(drv.box.h) / cv.U.angstrom
-    Do, Jo, Lo = O.descriptors(pos, types, box)
+    Do, Jo, Lo, Lao = O.descriptors(pos, types, box, axes=True)
+    assert np.allclose(La.cpu().numpy(), Lao, rtol=2e-5, atol=2e-6 * np.abs(Lao).max())
     assert np.allclose(D.cpu().numpy(), Do, rtol=2e-5, atol=2e-6 * np.abs(Do).max())
     assert np.allclose(J.cpu().numpy(), Jo, rtol=2e-5, atol=2e-6 * np.abs(Jo).max())
     assert np.allclose(L.cpu().numpy(), Lo, rtol=2e-5, atol=2e-6 * np.abs(Lo).max())
