@@ -46,8 +46,15 @@ struct ListArrays {
 // so the final reduction reads them coalesced.
 constexpr int PART_STRIDE = 2048;
 
+// The block coordinates a kernel body works with: the hardware's, or one
+// system's share of a launch that covers several systems.
+struct Blk {
+    int id, n;
+};
+__device__ __forceinline__ Blk hw_blk() { return Blk{(int)blockIdx.x, (int)gridDim.x}; }
+
 template <int NV>
-__device__ __forceinline__ void block_sum_store(double (&acc)[NV], double *out) {
+__device__ __forceinline__ void block_sum_store(double (&acc)[NV], double *out, int bid) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         double v = acc[k];
@@ -69,8 +76,13 @@ __device__ __forceinline__ void block_sum_store(double (&acc)[NV], double *out) 
 #pragma unroll
         for (int w = 0; w < BLOCK / 64; ++w)
             s += sm[w][threadIdx.x];
-        out[(size_t)threadIdx.x * PART_STRIDE + blockIdx.x] = s;
+        out[(size_t)threadIdx.x * PART_STRIDE + bid] = s;
     }
+}
+
+template <int NV>
+__device__ __forceinline__ void block_sum_store(double (&acc)[NV], double *out) {
+    block_sum_store<NV>(acc, out, (int)blockIdx.x);
 }
 
 // XCD-aware slice of [0, nitems): blocks are dispatched round-robin over the
@@ -78,9 +90,10 @@ __device__ __forceinline__ void block_sum_store(double (&acc)[NV], double *out) 
 // its L2 holds one spatial region of the neighbour gathers.
 struct XcdRange {
     int first, end, step;
-    __device__ __forceinline__ XcdRange(int nitems) {
-        int const nper = gridDim.x >> 3;
-        int const xcd = blockIdx.x & 7, b = blockIdx.x >> 3;
+    __device__ __forceinline__ XcdRange(int nitems) : XcdRange(nitems, hw_blk()) {}
+    __device__ __forceinline__ XcdRange(int nitems, Blk bk) {
+        int const nper = bk.n >> 3;
+        int const xcd = bk.id & 7, b = bk.id >> 3;
         int const chunk = (nitems + 7) >> 3;
         first = xcd * chunk + b;
         end = min(nitems, (xcd + 1) * chunk);
@@ -548,16 +561,15 @@ __device__ __forceinline__ void pre_step(DevState &s, int sample) {
 // First U2 (md_driver.cpp:433-452) and U1 drift (md_driver.cpp:462-499).
 // Each block leaves the largest squared displacement of its atoms since the
 // last list build in dmax_part[blockIdx.x].
-__global__ __launch_bounds__(BLOCK) void k_kick_drift(AtomArrays a, int N,
-                                                     DevState *__restrict__ gs,
-                                                     double *__restrict__ dmax_part) {
+__device__ __forceinline__ void kick_drift_body(AtomArrays a, int N, DevState *__restrict__ gs,
+                                                double *__restrict__ dmax_part, Blk bk) {
     if (gs->err)
         return;
     double const s = gs->kick_s, e = gs->kick_e, ch = gs->kick_c, cf = gs->drift_cf;
     double const Ld[3] = {gs->drift_L[0], gs->drift_L[1], gs->drift_L[2]};
     double const Ln[3] = {gs->L[0], gs->L[1], gs->L[2]};
     double dmax = 0.;
-    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < N; i += gridDim.x * BLOCK) {
+    for (int i = bk.id * BLOCK + threadIdx.x; i < N; i += bk.n * BLOCK) {
         double const m = gs->mass[a.type[i]];
         double vx = a.vx[i] * s, vy = a.vy[i] * s, vz = a.vz[i] * s;
         vx = vx * e + ch * a.fx[i] / m;
@@ -585,8 +597,14 @@ __global__ __launch_bounds__(BLOCK) void k_kick_drift(AtomArrays a, int N,
         double m = sm[0];
         for (int w = 1; w < BLOCK / 64; ++w)
             m = fmax(m, sm[w]);
-        dmax_part[blockIdx.x] = m;
+        dmax_part[bk.id] = m;
     }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_kick_drift(AtomArrays a, int N,
+                                                     DevState *__restrict__ gs,
+                                                     double *__restrict__ dmax_part) {
+    kick_drift_body(a, N, gs, dmax_part, hw_blk());
 }
 
 // ---------------------------------------------------------------------------
@@ -964,7 +982,7 @@ __global__ __launch_bounds__(BLOCK) void k_rb_build(AtomArrays a, ListArrays l, 
 // blocks poll with relaxed loads (an acquiring poll would invalidate the
 // caches on every iteration) and fence once when released. The wait is
 // bounded: past BARRIER_TICKS the block reports DE_BARRIER and goes on.
-__device__ bool grid_barrier(unsigned *bar, DevState *gs) {
+__device__ bool grid_barrier(unsigned *bar, DevState *gs, int nblk) {
     __syncthreads();
     __shared__ int ok;
     if (threadIdx.x == 0) {
@@ -973,7 +991,7 @@ __device__ bool grid_barrier(unsigned *bar, DevState *gs) {
         unsigned const gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         unsigned const arrived =
             __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == gridDim.x - 1) {
+        if (arrived == (unsigned)nblk - 1) {
             __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
@@ -993,11 +1011,10 @@ __device__ bool grid_barrier(unsigned *bar, DevState *gs) {
     return ok != 0;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_list(AtomArrays a, ListArrays l, int N,
-                                               DevState *__restrict__ gs,
-                                               unsigned *__restrict__ bar,
-                                               const double *__restrict__ dmax_part, int nparts,
-                                               int use_cells) {
+__device__ __forceinline__ void list_body(AtomArrays a, ListArrays l, int N,
+                                          DevState *__restrict__ gs, unsigned *__restrict__ bar,
+                                          const double *__restrict__ dmax_part, int nparts,
+                                          int use_cells, Blk bk) {
     if (gs->err)
         return;
     // the decision, from the same inputs in every block (u_max^2 from
@@ -1031,35 +1048,43 @@ __global__ __launch_bounds__(BLOCK) void k_list(AtomArrays a, ListArrays l, int 
     __shared__ ListSmem sm;
     if (threadIdx.x == 0) {
         sm.entries = 0ull;
-        if (blockIdx.x == 0)
+        if (bk.id == 0)
             gs->nlist_entries = 0;   // not a decision input; summed only after barrier 1
     }
-    bool ok = grid_barrier(bar, gs);   // every block has read the decision inputs
-    if (blockIdx.x == 0 && threadIdx.x == 0)
+    bool ok = grid_barrier(bar, gs, bk.n);   // every block has read the decision inputs
+    if (bk.id == 0 && threadIdx.x == 0)
         rb_begin(gs, use_cells);
     double const L[3] = {gs->L[0], gs->L[1], gs->L[2]};
     double const r_list = gs->r_list;
-    int const nthreads = gridDim.x * BLOCK;
+    int const nthreads = bk.n * BLOCK;
     if (!use_cells) {
         if (ok)
-            rb_build_brute(a, l, N, gs, L, r_list, (blockIdx.x * BLOCK + threadIdx.x) / LIST_LANES,
+            rb_build_brute(a, l, N, gs, L, r_list, (bk.id * BLOCK + threadIdx.x) / LIST_LANES,
                            nthreads / LIST_LANES, &sm.entries);
     } else {
-        rb_bin(a, l, N, gs, blockIdx.x * BLOCK, nthreads);
-        ok = grid_barrier(bar, gs) && ok;
-        if (blockIdx.x == 0)
+        rb_bin(a, l, N, gs, bk.id * BLOCK, nthreads);
+        ok = grid_barrier(bar, gs, bk.n) && ok;
+        if (bk.id == 0)
             rb_scan(l, sm.sc);
-        ok = grid_barrier(bar, gs) && ok;
-        rb_scatter(l, N, blockIdx.x * BLOCK, nthreads);
-        ok = grid_barrier(bar, gs) && ok;
-        for (int c = blockIdx.x; c < l.ncells; c += gridDim.x)
+        ok = grid_barrier(bar, gs, bk.n) && ok;
+        rb_scatter(l, N, bk.id * BLOCK, nthreads);
+        ok = grid_barrier(bar, gs, bk.n) && ok;
+        for (int c = bk.id; c < l.ncells; c += bk.n)
             rb_sort_cell(l, c, reinterpret_cast<int *>(sm.stage), gs);
-        ok = grid_barrier(bar, gs) && ok;
+        ok = grid_barrier(bar, gs, bk.n) && ok;
         if (ok)
-            for (int c = blockIdx.x; c < l.ncells; c += gridDim.x)
+            for (int c = bk.id; c < l.ncells; c += bk.n)
                 rb_build_cell(a, l, c, gs, L, r_list, sm);
     }
     rb_flush_entries(gs, sm.entries);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_list(AtomArrays a, ListArrays l, int N,
+                                               DevState *__restrict__ gs,
+                                               unsigned *__restrict__ bar,
+                                               const double *__restrict__ dmax_part, int nparts,
+                                               int use_cells) {
+    list_body(a, l, N, gs, bar, dmax_part, nparts, use_cells, hw_blk());
 }
 
 // ---------------------------------------------------------------------------
@@ -1334,9 +1359,9 @@ __global__ __launch_bounds__(BLOCK) void k_force(AtomArrays a, ListArrays l, RTa
 // also the embedding part of the Born tensor and Laplacian, F''_i g_i g_i
 // and F''_i |h_i|^2 (md_driver.cpp:410-419), from pass A's per-atom g, |h|^2.
 template <bool SAMPLE>
-__global__ __launch_bounds__(BLOCK) void k_kick2(AtomArrays a, int N,
-                                                const DevState *__restrict__ gs, int init,
-                                                int has_eam, double *__restrict__ part) {
+__device__ __forceinline__ void kick2_body(AtomArrays a, int N, const DevState *__restrict__ gs,
+                                           int init, int has_eam, double *__restrict__ part,
+                                           Blk bk) {
     if (gs->err)
         return;
     double const e = init ? 1. : gs->kick_e, ch = init ? 0. : gs->kick_c;
@@ -1344,7 +1369,7 @@ __global__ __launch_bounds__(BLOCK) void k_kick2(AtomArrays a, int N,
 #pragma unroll
     for (int k = 0; k < PK_N; ++k)
         acc[k] = 0.;
-    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < N; i += gridDim.x * BLOCK) {
+    for (int i = bk.id * BLOCK + threadIdx.x; i < N; i += bk.n * BLOCK) {
         double const m = gs->mass[a.type[i]];
         double const fx = a.fx[i], fy = a.fy[i], fz = a.fz[i];
         double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
@@ -1383,7 +1408,14 @@ __global__ __launch_bounds__(BLOCK) void k_kick2(AtomArrays a, int N,
             acc[PK_LAPF] += d2F * gd.x;
         }
     }
-    block_sum_store<PK_N>(acc, part);
+    block_sum_store<PK_N>(acc, part, bk.id);
+}
+
+template <bool SAMPLE>
+__global__ __launch_bounds__(BLOCK) void k_kick2(AtomArrays a, int N,
+                                                const DevState *__restrict__ gs, int init,
+                                                int has_eam, double *__restrict__ part) {
+    kick2_body<SAMPLE>(a, N, gs, init, has_eam, part, hw_blk());
 }
 
 // Deterministic sums of the per-block partial records, one block per
@@ -1398,13 +1430,15 @@ constexpr int NTOTAL = PA_N + PB_N + PK_N;
 __device__ __forceinline__ void post_step(DevState &s, const double *tA, const double *tB, const double *tK,
                           int mode, double *__restrict__ rec_out);
 
-__global__ __launch_bounds__(BLOCK) void k_finish(DevState *__restrict__ gs,
-                                                 const double *__restrict__ pA, int nA,
-                                                 const double *__restrict__ pB, int nB,
-                                                 const double *__restrict__ pK, int nK,
-                                                 double *__restrict__ totals,
-                                                 unsigned *__restrict__ ticket, int mode,
-                                                 double *__restrict__ rec_out, int next_sample) {
+// Blocks are numbered by blockIdx.x within one system (a launch over several
+// systems puts them in blockIdx.y).
+__device__ __forceinline__ void finish_body(DevState *__restrict__ gs,
+                                            const double *__restrict__ pA, int nA,
+                                            const double *__restrict__ pB, int nB,
+                                            const double *__restrict__ pK, int nK,
+                                            double *__restrict__ totals,
+                                            unsigned *__restrict__ ticket, int mode,
+                                            double *__restrict__ rec_out, int next_sample) {
     if (gs->err)
         return;
     int k = blockIdx.x;
@@ -1470,6 +1504,16 @@ __global__ __launch_bounds__(BLOCK) void k_finish(DevState *__restrict__ gs,
             *ticket = 0u;
     }
     state_store(gs, s);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_finish(DevState *__restrict__ gs,
+                                                 const double *__restrict__ pA, int nA,
+                                                 const double *__restrict__ pB, int nB,
+                                                 const double *__restrict__ pK, int nK,
+                                                 double *__restrict__ totals,
+                                                 unsigned *__restrict__ ticket, int mode,
+                                                 double *__restrict__ rec_out, int next_sample) {
+    finish_body(gs, pA, nA, pB, nB, pK, nK, totals, ticket, mode, rec_out, next_sample);
 }
 
 __device__ __forceinline__ void post_step(DevState &s, const double *tA, const double *tB, const double *tK,

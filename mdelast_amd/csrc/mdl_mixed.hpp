@@ -89,10 +89,10 @@ __device__ __forceinline__ float sym_dedr(float dphi, float Fi, float drj, float
 // rho(r) and F(rho) polynomials and the g/h sums in FP32.
 // ---------------------------------------------------------------------------
 template <int KIND, int LPA, bool SAMPLE>
-__global__ __launch_bounds__(BLOCK) void k_density_mx(AtomArrays a, ListArrays l, RTablesF rt,
-                                                     FTablesF ft, int N, int npair, int ntypes,
-                                                     double rc2, DevState *__restrict__ gs,
-                                                     double *__restrict__ part) {
+__device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTablesF rt,
+                                                FTablesF ft, int N, int npair, int ntypes,
+                                                double rc2, DevState *__restrict__ gs,
+                                                double *__restrict__ part, Blk bk) {
     if (gs->err)
         return;
     using P = PolyF<KIND>;
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(BLOCK) void k_density_mx(AtomArrays a, ListArrays l
 #pragma unroll
     for (int k = 0; k < PA_N; ++k)
         acc[k] = 0.;
-    XcdRange const R((N + APB - 1) / APB);
+    XcdRange const R((N + APB - 1) / APB, bk);
     for (int grp = R.first; grp < R.end; grp += R.step) {
         int const i = grp * APB + slot;
         bool const active = i < N;
@@ -217,7 +217,15 @@ __global__ __launch_bounds__(BLOCK) void k_density_mx(AtomArrays a, ListArrays l
             }
         }
     }
-    block_sum_store<PA_N>(acc, part);
+    block_sum_store<PA_N>(acc, part, bk.id);
+}
+
+template <int KIND, int LPA, bool SAMPLE>
+__global__ __launch_bounds__(BLOCK) void k_density_mx(AtomArrays a, ListArrays l, RTablesF rt,
+                                                     FTablesF ft, int N, int npair, int ntypes,
+                                                     double rc2, DevState *__restrict__ gs,
+                                                     double *__restrict__ part) {
+    density_mx_body<KIND, LPA, SAMPLE>(a, l, rt, ft, N, npair, ntypes, rc2, gs, part, hw_blk());
 }
 
 // ---------------------------------------------------------------------------
@@ -227,10 +235,9 @@ __global__ __launch_bounds__(BLOCK) void k_density_mx(AtomArrays a, ListArrays l
 // Outputs per atom in CGS; block sums in eV and Angstrom (scaled in k_post).
 // ---------------------------------------------------------------------------
 template <int KIND, int LPA, bool SAMPLE>
-__global__ __launch_bounds__(BLOCK) void k_force_mx(AtomArrays a, ListArrays l, RTablesF rt,
-                                                   int N, int npair, float rmin,
-                                                   DevState *__restrict__ gs,
-                                                   double *__restrict__ part) {
+__device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTablesF rt, int N,
+                                              int npair, float rmin, DevState *__restrict__ gs,
+                                              double *__restrict__ part, Blk bk) {
     if (gs->err)
         return;
     using P = PolyF<KIND>;
@@ -249,7 +256,7 @@ __global__ __launch_bounds__(BLOCK) void k_force_mx(AtomArrays a, ListArrays l, 
     float lap = 0.f;
     double epot = 0.;
     int nrmin = 0;
-    XcdRange const R((N + APB - 1) / APB);
+    XcdRange const R((N + APB - 1) / APB, bk);
     for (int grp = R.first; grp < R.end; grp += R.step) {
         int const i = grp * APB + slot;
         bool const active = i < N;
@@ -365,7 +372,15 @@ __global__ __launch_bounds__(BLOCK) void k_force_mx(AtomArrays a, ListArrays l, 
         acc[PB_CB + k] = cb[k];
     acc[PB_LAP] = lap;
     acc[PB_WRMIN] = (double)nrmin;
-    block_sum_store<PB_N>(acc, part);
+    block_sum_store<PB_N>(acc, part, bk.id);
+}
+
+template <int KIND, int LPA, bool SAMPLE>
+__global__ __launch_bounds__(BLOCK) void k_force_mx(AtomArrays a, ListArrays l, RTablesF rt,
+                                                   int N, int npair, float rmin,
+                                                   DevState *__restrict__ gs,
+                                                   double *__restrict__ part) {
+    force_mx_body<KIND, LPA, SAMPLE>(a, l, rt, N, npair, rmin, gs, part, hw_blk());
 }
 
 }  // namespace mdl
