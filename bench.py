@@ -118,14 +118,17 @@ def bench_replicas(args):
     elements, pots, eams = get_potential.get_EAM_alloy_file("ZnCu.eam.alloy")
     masses = [e[2] / U.Navogadro for e in elements]
     specs = R.batched_replica_specs(args.replicas)
-    drivers = [R.make_replica_driver(s, pots, eams, masses, precision=args.precision)
-               for s in specs]
+    # one launch per pass over all replicas needs one lane count (8 or 16)
+    lpa = args.lanes_per_atom or (16 if args.batched else 0)
+    drivers = [R.make_replica_driver(s, pots, eams, masses, precision=args.precision,
+                                     lanes_per_atom=lpa) for s in specs]
+    graph_steps = 0 if args.batched else args.graph_steps
     for d in drivers:
-        d.graph_steps = args.graph_steps
+        d.graph_steps = graph_steps
     natoms = sum(s["N"] for s in specs)
-    R.run_replicas(drivers, args.warmup, 1, 20, records=False)
+    R.run_replicas(drivers, args.warmup, 1, 20, records=False, batched=args.batched)
     t0 = time.perf_counter()
-    recs = R.run_replicas(drivers, args.steps, 1, 20, records=True)
+    recs = R.run_replicas(drivers, args.steps, 1, 20, records=True, batched=args.batched)
     wall = time.perf_counter() - t0
     Tk = [float(r[-1][11]) for r in recs]
     out = {
@@ -151,7 +154,10 @@ def bench_replicas(args):
             "atoms_max": max(s["N"] for s in specs),
             "precision": args.precision,
             "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "default"),
-            "graph_steps": args.graph_steps,
+            "graph_steps": graph_steps,
+            "launches": "one per pass over all replicas" if args.batched
+                        else "per replica, each on its own stream",
+            "lanes_per_atom": lpa or "autotuned",
             "T_kin_final_minmax": [min(Tk), max(Tk)],
         },
     }
@@ -277,6 +283,9 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=20,
                     help="replicas: replay chunks of this many steps as one captured HIP graph "
                          "per replica (0: launch step by step)")
+    ap.add_argument("--batched", action="store_true",
+                    help="replicas: each pass of a step as one launch over every replica "
+                         "(mdl_replicas_run) instead of per-replica launches")
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="hardware queues of the process (GPU_MAX_HW_QUEUES, at most 32)")
     ap.add_argument("--no-learned-cv", action="store_true",

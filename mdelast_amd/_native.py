@@ -87,6 +87,8 @@ SIGNATURES = {
                                    C.POINTER(C.c_int64)]),
     "mdl_drivers_run": (C.c_int32, [C.POINTER(_vp), C.c_int32, C.c_int64, C.c_int64, C.c_int64,
                                     C.POINTER(_dp), C.POINTER(C.c_int64)]),
+    "mdl_replicas_run": (C.c_int32, [C.POINTER(_vp), C.c_int32, C.c_int64, C.c_int64, C.c_int64,
+                                    C.POINTER(_dp), C.POINTER(C.c_int64)]),
     "mdl_driver_calculate_neighbors": (C.c_int32, [_vp]),
     "mdl_driver_set_sample_observables": (C.c_int32, [_vp, C.c_int32]),
     "mdl_driver_get_sample_observables": (C.c_int32, [_vp, C.POINTER(C.c_int32)]),

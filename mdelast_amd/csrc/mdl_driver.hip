@@ -1030,9 +1030,9 @@ struct RunPlan {
         return MDL_OK;
     }
 
-    // Launch step k (asynchronous), or a captured chunk of steps.
-    mdl_status step() {
-        if (graph_fits())
+    // Launch step k (asynchronous), or a captured chunk of steps when allowed.
+    mdl_status step(bool graphs = true) {
+        if (graphs && graph_fits())
             return graph_step();
         int64_t const cyc = d->cycle_host;
         bool const s = sampled(cyc);
@@ -1203,6 +1203,346 @@ static mdl_status joint_step(std::vector<RunPlan> &plans) {
     return MDL_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Several systems per launch. One table entry per replica holds what its
+// kernels take; a block map gives every block of a launch its replica and its
+// block index within that replica's share, which is the replica's own grid
+// (the same blocks and partial sums as its own launches, so the results are
+// bit for bit those of the replica run alone with the same lanes per atom).
+// The list check runs as one block per replica.
+// ---------------------------------------------------------------------------
+struct RepArgs {
+    AtomArrays a;
+    ListArrays l;
+    RTablesF rt;
+    FTablesF ft;
+    DevState *gs;
+    double *pA, *pB, *pK, *totals, *dmax, *rec;
+    unsigned *ticket, *bar;
+    int N, npair, ntypes, gridA, gridB, gridK, use_cells, has_eam;
+    double rc2A;
+    float rmin;
+};
+
+__global__ __launch_bounds__(BLOCK) void k_kick_drift_rep(const RepArgs *__restrict__ R,
+                                                         const int2 *__restrict__ map) {
+    int2 const m = map[blockIdx.x];
+    RepArgs const &r = R[m.x];
+    kick_drift_body(r.a, r.N, r.gs, r.dmax, Blk{m.y, r.gridK});
+}
+
+__global__ __launch_bounds__(BLOCK) void k_list_rep(const RepArgs *__restrict__ R) {
+    RepArgs const &r = R[blockIdx.x];
+    list_body(r.a, r.l, r.N, r.gs, r.bar, r.dmax, r.gridK, r.use_cells, Blk{0, 1});
+}
+
+template <int KIND, int LPA, bool SAMPLE>
+__global__ __launch_bounds__(BLOCK) void k_density_rep(const RepArgs *__restrict__ R,
+                                                      const int2 *__restrict__ map) {
+    int2 const m = map[blockIdx.x];
+    RepArgs const &r = R[m.x];
+    density_mx_body<KIND, LPA, SAMPLE>(r.a, r.l, r.rt, r.ft, r.N, r.npair, r.ntypes, r.rc2A,
+                                       r.gs, r.pA, Blk{m.y, r.gridA});
+}
+
+template <int KIND, int LPA, bool SAMPLE>
+__global__ __launch_bounds__(BLOCK) void k_force_rep(const RepArgs *__restrict__ R,
+                                                    const int2 *__restrict__ map) {
+    int2 const m = map[blockIdx.x];
+    RepArgs const &r = R[m.x];
+    force_mx_body<KIND, LPA, SAMPLE>(r.a, r.l, r.rt, r.N, r.npair, r.rmin, r.gs, r.pB,
+                                     Blk{m.y, r.gridB});
+}
+
+// `init` and `mode` stay kernel arguments as in k_kick2 and k_finish: a
+// literal would let the compiler fold the branches they select and contract
+// the remaining sums differently, and the kinetic sums would then differ from
+// the replica's own launches in the last bit.
+template <bool SAMPLE>
+__global__ __launch_bounds__(BLOCK) void k_kick2_rep(const RepArgs *__restrict__ R,
+                                                    const int2 *__restrict__ map, int init) {
+    int2 const m = map[blockIdx.x];
+    RepArgs const &r = R[m.x];
+    kick2_body<SAMPLE>(r.a, r.N, r.gs, init, r.has_eam, r.pK, Blk{m.y, r.gridK});
+}
+
+// The end of the step in two launches: the totals (one block per component
+// and replica, blockIdx.y the replica), then the scalar tail (one block per
+// replica), the same sums and tail as k_finish.
+__global__ __launch_bounds__(BLOCK) void k_finish_sum_rep(const RepArgs *__restrict__ R) {
+    RepArgs const &r = R[blockIdx.y];
+    if (r.gs->err)
+        return;
+    double const t =
+        finish_sum(r.gs, r.pA, r.gridA, r.pB, r.gridB, r.pK, r.gridK, (int)blockIdx.x);
+    if (threadIdx.x == 0)
+        r.totals[blockIdx.x] = t;
+}
+
+// irec < 0: not a sampled step
+__global__ __launch_bounds__(BLOCK) void k_finish_tail_rep(const RepArgs *__restrict__ R, int mode,
+                                                          int irec, int next_sample) {
+    RepArgs const &r = R[blockIdx.x];
+    if (r.gs->err)
+        return;
+    double *ro = (irec >= 0 && r.rec) ? r.rec + (size_t)MDL_NRECORD * irec : nullptr;
+    finish_tail(r.gs, r.totals, nullptr, mode, ro, next_sample);
+}
+
+// A forced list build of every replica, in five launches (those of
+// Driver::launch_build, each over the blocks of all replicas). Replicas with
+// a brute-force list take no part in the scan, scatter and sort launches; the
+// maps give each launch its blocks.
+__global__ __launch_bounds__(BLOCK) void k_rb_bin_rep(const RepArgs *__restrict__ R,
+                                                     const int2 *__restrict__ map,
+                                                     const int *__restrict__ nblk) {
+    int2 const m = map[blockIdx.x];
+    RepArgs const &r = R[m.x];
+    if (r.gs->err)
+        return;
+    if (m.y == 0 && threadIdx.x == 0) {
+        rb_begin(r.gs, r.use_cells);
+        r.gs->nlist_entries = 0;
+    }
+    if (r.use_cells) {
+        ListArrays l = r.l;
+        rb_bin(r.a, l, r.N, r.gs, m.y * BLOCK, nblk[m.x] * BLOCK);
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rb_scan_rep(const RepArgs *__restrict__ R,
+                                                      const int *__restrict__ which) {
+    RepArgs const &r = R[which[blockIdx.x]];
+    if (r.gs->err)
+        return;
+    __shared__ int sc[BLOCK];
+    ListArrays l = r.l;
+    rb_scan(l, sc);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rb_scatter_rep(const RepArgs *__restrict__ R,
+                                                         const int2 *__restrict__ map,
+                                                         const int *__restrict__ nblk) {
+    int2 const m = map[blockIdx.x];
+    RepArgs const &r = R[m.x];
+    if (r.gs->err)
+        return;
+    ListArrays l = r.l;
+    rb_scatter(l, r.N, m.y * BLOCK, nblk[m.x] * BLOCK);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rb_sort_rep(const RepArgs *__restrict__ R,
+                                                      const int2 *__restrict__ map) {
+    int2 const m = map[blockIdx.x];
+    RepArgs const &r = R[m.x];
+    if (r.gs->err)
+        return;
+    __shared__ int v[CELL_MAX];
+    ListArrays l = r.l;
+    rb_sort_cell(l, m.y, v, r.gs);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rb_build_rep(const RepArgs *__restrict__ R,
+                                                       const int2 *__restrict__ map,
+                                                       const int *__restrict__ nblk) {
+    int2 const m = map[blockIdx.x];
+    RepArgs const &r = R[m.x];
+    if (r.gs->err)
+        return;
+    __shared__ ListSmem sm;
+    if (threadIdx.x == 0)
+        sm.entries = 0ull;
+    double const L[3] = {r.gs->L[0], r.gs->L[1], r.gs->L[2]};
+    double const r_list = r.gs->r_list;
+    AtomArrays a = r.a;
+    ListArrays l = r.l;
+    if (r.use_cells)
+        rb_build_cell(a, l, m.y, r.gs, L, r_list, sm);
+    else
+        rb_build_brute(a, l, r.N, r.gs, L, r_list, (m.y * BLOCK + threadIdx.x) / LIST_LANES,
+                       nblk[m.x] * BLOCK / LIST_LANES, &sm.entries);
+    rb_flush_entries(r.gs, sm.entries);
+}
+
+using RepPassFn = void (*)(const RepArgs *, const int2 *);
+
+template <int KIND, bool S> static RepPassFn rep_density_fn(int lpa) {
+    return lpa == 8 ? k_density_rep<KIND, 8, S> : k_density_rep<KIND, 16, S>;
+}
+template <int KIND, bool S> static RepPassFn rep_force_fn(int lpa) {
+    return lpa == 8 ? k_force_rep<KIND, 8, S> : k_force_rep<KIND, 16, S>;
+}
+static RepPassFn pick_density_rep(int kind, int lpa, bool s) {
+    if (kind == MDL_QUINTIC)
+        return s ? rep_density_fn<MDL_QUINTIC, true>(lpa) : rep_density_fn<MDL_QUINTIC, false>(lpa);
+    return s ? rep_density_fn<MDL_BSPLINE, true>(lpa) : rep_density_fn<MDL_BSPLINE, false>(lpa);
+}
+static RepPassFn pick_force_rep(int kind, int lpa, bool s) {
+    if (kind == MDL_QUINTIC)
+        return s ? rep_force_fn<MDL_QUINTIC, true>(lpa) : rep_force_fn<MDL_QUINTIC, false>(lpa);
+    return s ? rep_force_fn<MDL_BSPLINE, true>(lpa) : rep_force_fn<MDL_BSPLINE, false>(lpa);
+}
+
+// The drivers can share launches (checked before any of them is touched).
+static mdl_status replicas_check(const std::vector<Driver *> &ds) {
+    Driver *d0 = ds[0];
+    for (Driver *d : ds) {
+        if (mdl_status s = guard(d))
+            return s;
+        if (d->precision != MDL_MIXED)
+            return bad("mdl_replicas_run: every driver must use mixed precision");
+        if (d->kind != d0->kind || d->lpa != d0->lpa || d->device != d0->device)
+            return bad("mdl_replicas_run: the drivers differ in interpolation, lanes per atom "
+                       "or device");
+        if (d->lpa != 8 && d->lpa != 16)
+            return bad("mdl_replicas_run: lanes per atom must be 8 or 16");
+        if (d->gridK > 4 * BLOCK)
+            return bad("mdl_replicas_run: a replica is too large for the one-block list check");
+        if (d->cycle_host != d0->cycle_host)
+            return bad("mdl_replicas_run: the drivers are not at the same cycle");
+    }
+    return MDL_OK;
+}
+
+static mdl_status replicas_run(std::vector<RunPlan> &plans, int64_t nsteps) {
+    size_t const n = plans.size();
+    Driver *d0 = plans[0].d;
+    // a requested build is taken by the replicas' own launches first
+    bool pending = false;
+    for (RunPlan &r : plans)
+        pending = pending || r.d->build_pending;
+    if (pending)
+        for (RunPlan &r : plans)
+            if (r.k < nsteps)
+                if (mdl_status st = r.step(false))
+                    return st;
+    for (RunPlan &r : plans)
+        MDL_HIP(hipStreamSynchronize(r.d->stream));
+    hipStream_t const st = d0->stream;
+    std::vector<RepArgs> tab(n);
+    std::vector<int2> mapA, mapB, mapK, mapBin, mapScat, mapSort, mapBuild;
+    std::vector<int> nblkN(n), nblkB(n), scanWhich;
+    for (size_t i = 0; i < n; ++i) {
+        Driver *d = plans[i].d;
+        RepArgs &e = tab[i];
+        e.a = d->a;
+        e.l = d->l;
+        e.rt = d->rtf;
+        e.ft = d->ftf;
+        e.gs = d->gs;
+        e.pA = d->pA;
+        e.pB = d->pB;
+        e.pK = d->pK;
+        e.totals = d->d_totals;
+        e.dmax = d->d_dmax;
+        e.rec = plans[i].drec;
+        e.ticket = d->d_ticket_fin;
+        e.bar = d->d_bar;
+        e.N = d->N;
+        e.npair = d->npair;
+        e.ntypes = d->ntypes;
+        e.gridA = d->gridA;
+        e.gridB = d->gridB;
+        e.gridK = d->gridK;
+        e.use_cells = d->use_cells ? 1 : 0;
+        e.has_eam = d->has_eam;
+        e.rc2A = d->rc2 * 1e16;
+        e.rmin = (float)(d->rmin * 1e8);
+        for (int b = 0; b < d->gridA; ++b)
+            mapA.push_back(make_int2((int)i, b));
+        for (int b = 0; b < d->gridB; ++b)
+            mapB.push_back(make_int2((int)i, b));
+        for (int b = 0; b < d->gridK; ++b)
+            mapK.push_back(make_int2((int)i, b));
+        // the forced build's launches (Driver::launch_build's grids)
+        int const gN = (d->N + BLOCK - 1) / BLOCK;
+        int const gB = d->use_cells ? d->l.ncells
+                                    : (int)(((int64_t)d->N * LIST_LANES + BLOCK - 1) / BLOCK);
+        nblkN[i] = gN;
+        nblkB[i] = gB;
+        for (int b = 0; b < (d->use_cells ? gN : 1); ++b)
+            mapBin.push_back(make_int2((int)i, b));
+        if (d->use_cells) {
+            scanWhich.push_back((int)i);
+            for (int b = 0; b < gN; ++b)
+                mapScat.push_back(make_int2((int)i, b));
+            for (int c = 0; c < d->l.ncells; ++c)
+                mapSort.push_back(make_int2((int)i, c));
+        }
+        for (int b = 0; b < gB; ++b)
+            mapBuild.push_back(make_int2((int)i, b));
+    }
+    // every map in one device buffer: the int2 maps, then the int arrays
+    std::vector<int2> all;
+    std::vector<size_t> at;
+    for (const std::vector<int2> *v : {&mapA, &mapB, &mapK, &mapBin, &mapScat, &mapSort,
+                                       &mapBuild}) {
+        at.push_back(all.size());
+        all.insert(all.end(), v->begin(), v->end());
+    }
+    std::vector<int> ints(nblkN);
+    ints.insert(ints.end(), nblkB.begin(), nblkB.end());
+    ints.insert(ints.end(), scanWhich.begin(), scanWhich.end());
+    RepArgs *dtab = nullptr;
+    int2 *dmap = nullptr;
+    size_t const bytes = sizeof(int2) * all.size() + sizeof(int) * ints.size();
+    MDL_HIP(hipMalloc(&dtab, sizeof(RepArgs) * n));
+    if (hipMalloc(&dmap, bytes) != hipSuccess) {
+        (void)hipFree(dtab);
+        return bad("mdl_replicas_run: out of device memory");
+    }
+    hipError_t e = hipMemcpy(dtab, tab.data(), sizeof(RepArgs) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(dmap, all.data(), sizeof(int2) * all.size(), hipMemcpyHostToDevice);
+    int *dint = reinterpret_cast<int *>(dmap + all.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(dint, ints.data(), sizeof(int) * ints.size(), hipMemcpyHostToDevice);
+    int2 const *mA = dmap + at[0], *mB = dmap + at[1], *mK = dmap + at[2];
+    int2 const *mBin = dmap + at[3], *mScat = dmap + at[4], *mSort = dmap + at[5],
+               *mBuild = dmap + at[6];
+    int const *dN = dint, *dB = dint + n, *dWhich = dint + 2 * n;
+    RunPlan &p0 = plans[0];
+    while (e == hipSuccess && p0.k < nsteps) {
+        int64_t const cyc = d0->cycle_host;
+        bool const smp = p0.sampled(cyc);
+        bool const build = p0.forced(cyc);
+        int const ns = p0.k + 1 < nsteps ? (p0.sampled(cyc + 1) ? 1 : 0) : -1;
+        k_kick_drift_rep<<<(unsigned)mapK.size(), BLOCK, 0, st>>>(dtab, mK);
+        if (build) {
+            k_rb_bin_rep<<<(unsigned)mapBin.size(), BLOCK, 0, st>>>(dtab, mBin, dN);
+            if (!scanWhich.empty()) {
+                k_rb_scan_rep<<<(unsigned)scanWhich.size(), BLOCK, 0, st>>>(dtab, dWhich);
+                k_rb_scatter_rep<<<(unsigned)mapScat.size(), BLOCK, 0, st>>>(dtab, mScat, dN);
+                k_rb_sort_rep<<<(unsigned)mapSort.size(), BLOCK, 0, st>>>(dtab, mSort);
+            }
+            k_rb_build_rep<<<(unsigned)mapBuild.size(), BLOCK, 0, st>>>(dtab, mBuild, dB);
+        } else
+            k_list_rep<<<(unsigned)n, BLOCK, 0, st>>>(dtab);
+        pick_density_rep(d0->kind, d0->lpa, smp)<<<(unsigned)mapA.size(), BLOCK, 0, st>>>(dtab,
+                                                                                         mA);
+        pick_force_rep(d0->kind, d0->lpa, smp)<<<(unsigned)mapB.size(), BLOCK, 0, st>>>(dtab, mB);
+        if (smp)
+            k_kick2_rep<true><<<(unsigned)mapK.size(), BLOCK, 0, st>>>(dtab, mK, 0);
+        else
+            k_kick2_rep<false><<<(unsigned)mapK.size(), BLOCK, 0, st>>>(dtab, mK, 0);
+        k_finish_sum_rep<<<dim3(NTOTAL, (unsigned)n), BLOCK, 0, st>>>(dtab);
+        k_finish_tail_rep<<<(unsigned)n, BLOCK, 0, st>>>(dtab, 1, smp ? (int)p0.irec : -1, ns);
+        e = hipGetLastError();
+        for (RunPlan &r : plans) {
+            ++r.d->cycle_host;
+            ++r.k;
+            if (smp)
+                ++r.irec;
+        }
+    }
+    hipError_t const e2 = hipStreamSynchronize(st);
+    (void)hipFree(dtab);
+    (void)hipFree(dmap);
+    if (e != hipSuccess || e2 != hipSuccess)
+        return hip_fail(e != hipSuccess ? e : e2, "mdl_replicas_run");
+    return MDL_OK;
+}
+
 extern "C" {
 
 mdl_status mdl_driver_run(mdl_driver *p, int64_t nsteps, int64_t sample_stride,
@@ -1232,6 +1572,13 @@ mdl_status mdl_drivers_run(mdl_driver *const *drivers, int32_t n, int64_t nsteps
         for (int32_t j = 0; j < i; ++j)
             if (drivers[j] == drivers[i])
                 return bad("a driver appears twice in mdl_drivers_run");
+        // every driver is checked before any run begins (begin() advances the state)
+        if (mdl_status s = guard(D(drivers[i])))
+            return s;
+    }
+    if (sample_stride < 1)
+        return bad("sample_stride must be >= 1");
+    for (int32_t i = 0; i < n; ++i) {
         RunPlan &r = plans[(size_t)i];
         r.d = D(drivers[i]);
         r.nsteps = nsteps;
@@ -1267,6 +1614,43 @@ mdl_status mdl_drivers_run(mdl_driver *const *drivers, int32_t n, int64_t nsteps
         mdl_status s = r.end();
         if (s != MDL_OK && first == MDL_OK)
             first = s;
+    }
+    return first;
+}
+
+mdl_status mdl_replicas_run(mdl_driver *const *drivers, int32_t n, int64_t nsteps,
+                            int64_t sample_stride, int64_t rebuild_every, double *const *records,
+                            int64_t *n_records) {
+    if (!drivers || n < 1)
+        return bad("no drivers");
+    std::vector<Driver *> ds((size_t)n);
+    for (int32_t i = 0; i < n; ++i) {
+        for (int32_t j = 0; j < i; ++j)
+            if (drivers[j] == drivers[i])
+                return bad("a driver appears twice in mdl_replicas_run");
+        ds[(size_t)i] = D(drivers[i]);
+    }
+    if (mdl_status s = replicas_check(ds))
+        return s;
+    std::vector<RunPlan> plans((size_t)n);
+    for (int32_t i = 0; i < n; ++i) {
+        RunPlan &r = plans[(size_t)i];
+        r.d = D(drivers[i]);
+        r.nsteps = nsteps;
+        r.stride = sample_stride;
+        r.rebuild = rebuild_every;
+        r.records = records ? records[i] : nullptr;
+        r.n_records = n_records ? n_records + i : nullptr;
+        if (mdl_status s = r.begin())
+            return s;
+    }
+    MDL_HIP(hipSetDevice(plans[0].d->device));
+    mdl_status const s = replicas_run(plans, nsteps);
+    mdl_status first = s;
+    for (RunPlan &r : plans) {
+        mdl_status const e = r.end();
+        if (e != MDL_OK && first == MDL_OK)
+            first = e;
     }
     return first;
 }

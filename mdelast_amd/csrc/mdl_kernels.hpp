@@ -1430,18 +1430,12 @@ constexpr int NTOTAL = PA_N + PB_N + PK_N;
 __device__ __forceinline__ void post_step(DevState &s, const double *tA, const double *tB, const double *tK,
                           int mode, double *__restrict__ rec_out);
 
-// Blocks are numbered by blockIdx.x within one system (a launch over several
-// systems puts them in blockIdx.y).
-__device__ __forceinline__ void finish_body(DevState *__restrict__ gs,
-                                            const double *__restrict__ pA, int nA,
-                                            const double *__restrict__ pB, int nB,
-                                            const double *__restrict__ pK, int nK,
-                                            double *__restrict__ totals,
-                                            unsigned *__restrict__ ticket, int mode,
-                                            double *__restrict__ rec_out, int next_sample) {
-    if (gs->err)
-        return;
-    int k = blockIdx.x;
+// Component k of the step's totals: the sum of its per-block partials in a
+// fixed order (the value is returned to thread 0).
+__device__ __forceinline__ double finish_sum(const DevState *__restrict__ gs,
+                                             const double *__restrict__ pA, int nA,
+                                             const double *__restrict__ pB, int nB,
+                                             const double *__restrict__ pK, int nK, int k) {
     const double *p;
     int nb;
     if (k < PA_N) {
@@ -1473,22 +1467,22 @@ __device__ __forceinline__ void finish_body(DevState *__restrict__ gs,
     for (int off = 32; off > 0; off >>= 1)
         sum += __shfl_xor(sum, off, 64);
     __shared__ double sm[BLOCK / 64];
-    __shared__ int last;
     if ((threadIdx.x & 63) == 0)
         sm[threadIdx.x >> 6] = sum;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.;
+    double t = 0.;
+    if (threadIdx.x == 0)
         for (int w = 0; w < BLOCK / 64; ++w)
             t += sm[w];
-        totals[blockIdx.x] = t;
-        __threadfence();
-        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last)
-        return;
-    __threadfence();
+    return t;
+}
+
+// The scalar end of the step from the totals (one block; the first wave
+// works); ticket, unless null, is reset for the next step.
+__device__ __forceinline__ void finish_tail(DevState *__restrict__ gs,
+                                            const double *__restrict__ totals,
+                                            unsigned *__restrict__ ticket, int mode,
+                                            double *__restrict__ rec_out, int next_sample) {
     __shared__ double tot[NTOTAL];
     for (int c = threadIdx.x; c < NTOTAL; c += BLOCK)
         tot[c] = __hip_atomic_load(&totals[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1500,10 +1494,35 @@ __device__ __forceinline__ void finish_body(DevState *__restrict__ gs,
         post_step(s, tot, tot + PA_N, tot + PA_N + PB_N, mode, rec_out);
         if (next_sample >= 0 && !s.err)
             pre_step(s, next_sample);
-        if (threadIdx.x == 0)
+        if (ticket && threadIdx.x == 0)
             *ticket = 0u;
     }
     state_store(gs, s);
+}
+
+// One block per component (blockIdx.x); the block that completes the set
+// runs the tail.
+__device__ __forceinline__ void finish_body(DevState *__restrict__ gs,
+                                            const double *__restrict__ pA, int nA,
+                                            const double *__restrict__ pB, int nB,
+                                            const double *__restrict__ pK, int nK,
+                                            double *__restrict__ totals,
+                                            unsigned *__restrict__ ticket, int mode,
+                                            double *__restrict__ rec_out, int next_sample) {
+    if (gs->err)
+        return;
+    double const t = finish_sum(gs, pA, nA, pB, nB, pK, nK, (int)blockIdx.x);
+    __shared__ int last;
+    if (threadIdx.x == 0) {
+        totals[blockIdx.x] = t;
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last)
+        return;
+    __threadfence();
+    finish_tail(gs, totals, ticket, mode, rec_out, next_sample);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_finish(DevState *__restrict__ gs,

@@ -187,10 +187,16 @@ __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTab
         if (active && lane == 0) {
             l.ccnt[i] = nin;
             int const ti = (int)ci.w;
-            if (rho > ft.rhomax[ti]) {
+            // selected, not indexed: a dynamic index into the table struct
+            // would put a copy of it in scratch memory
+            double rmax = ft.rhomax[0];
+#pragma unroll
+            for (int t = 1; t < MAX_TYPES; ++t)
+                rmax = ti == t ? ft.rhomax[t] : rmax;
+            if (rho > rmax) {
                 atomicOr(&gs->err, (int)DE_RHOMAX);
                 gs->err_val[0] = rho * 1e24;
-                gs->err_val[1] = ft.rhomax[ti] * 1e24;
+                gs->err_val[1] = rmax * 1e24;
             }
             double const u = rho * ft.inv_h;
             int it = 0;

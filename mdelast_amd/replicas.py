@@ -6,7 +6,9 @@ Here each replica is one driver with its own stream, and
 ``run_replicas`` advances all of them with one host call
 (``mdl_drivers_run``): step k of every replica is launched before step k + 1
 of any, so replicas of uneven size, composition and temperature overlap on the
-device instead of queueing behind one another.
+device instead of queueing behind one another. With ``batched=True``
+(``mdl_replicas_run``) each pass of a step is a single launch whose blocks
+cover every replica, so many small systems fill the device together.
 
 ``batched_replica_specs`` draws the batched-replica workload: cubic FCC
 supercells with edge 5 to 20 cells (500 to 32 000 atoms), each pure Cu or a
@@ -27,9 +29,15 @@ from . import units as U
 
 
 def run_replicas(drivers: Sequence["mdm.MDDriver"], nsteps: int, sample_stride: int = 1,
-                 rebuild_every: int = 0, records: bool = True) -> List[np.ndarray]:
+                 rebuild_every: int = 0, records: bool = True,
+                 batched: bool = False) -> List[np.ndarray]:
     """nsteps steps of every driver (each as MDDriver.run would do them); returns
-    each driver's per-sample records (MDL_S_* layout)."""
+    each driver's per-sample records (MDL_S_* layout).
+
+    batched: every pass of a step is one launch over all replicas
+    (``mdl_replicas_run``; mixed precision, one interpolation kind, 8 or 16
+    lanes per atom, all drivers at the same cycle) instead of one launch per
+    replica on its own stream (``mdl_drivers_run``)."""
     n = len(drivers)
     if n == 0:
         return []
@@ -47,8 +55,9 @@ def run_replicas(drivers: Sequence["mdm.MDDriver"], nsteps: int, sample_stride: 
     for d in drivers:
         d._cache = None
     try:
-        st = nat.lib().mdl_drivers_run(handles, n, int(nsteps), int(sample_stride),
-                                       int(rebuild_every), ptrs if records else None, counts)
+        fn = nat.lib().mdl_replicas_run if batched else nat.lib().mdl_drivers_run
+        st = fn(handles, n, int(nsteps), int(sample_stride), int(rebuild_every),
+                ptrs if records else None, counts)
     finally:
         for d in drivers:
             d._cache = None
@@ -83,7 +92,7 @@ def fcc_positions(edge: int, a: float) -> np.ndarray:
 
 def make_replica_driver(spec, pots, eams, masses, *, a=3.6345 * U.angstrom, zn_type=0,
                         cu_type=1, dt=1e-15, M=10, tau_T=5e-14, precision="mixed",
-                        lanes_per_atom=0):
+                        lanes_per_atom=0, skin=1.2):
     """One NVT replica, set up as MolecularDynamics does (velocities at T with the
     centre-of-mass momentum removed and an exact rescale, chain momenta drawn
     from the chain masses)."""
@@ -109,5 +118,5 @@ def make_replica_driver(spec, pots, eams, masses, *, a=3.6345 * U.angstrom, zn_t
     L = spec["edge"] * a
     return mdm.MDDriver(pos, v, mdm.PotentialVector(pots), mdm.EAMVector(eams), mdm.MDDriver.NVT,
                         mdm.MDDriver.THERMOSTAT_KINETIC, th, ba, mdm.Box(np.diag([L] * 3)), [1e-10],
-                        Nf, masses, types, dt, True, 1.2, 1e3, precision=precision,
+                        Nf, masses, types, dt, True, skin, 1e3, precision=precision,
                         lanes_per_atom=lanes_per_atom)

@@ -93,3 +93,61 @@ def test_joint_graph_equals_step_launches():
         assert np.array_equal(x, y)
     for da, db in zip(a, b):
         assert np.array_equal(da.forces_array(), db.forces_array())
+
+
+@pytest.mark.parametrize("skin", [1.2, 1.04])
+def test_batched_launches_equal_one_at_a_time(skin):
+    """Every pass of a step as one launch over all replicas (mdl_replicas_run)
+    gives each replica's records, forces and positions bit for bit as its own
+    launches do: with forced rebuilds (the build chains on the shared stream),
+    with a requested rebuild, and (thin skin) with rebuilds that the one-block
+    list check decides by itself, over brute-force and cell lists."""
+    elements, pots, eams = get_potential.get_EAM_alloy_file("ZnCu.eam.alloy")
+    masses = [e[2] / U.Navogadro for e in elements]
+    specs = [dict(edge=5, N=500, x_zn=0.06, T=300.0, place_seed=1, vel_seed=2),
+             dict(edge=6, N=864, x_zn=0.0, T=800.0, place_seed=3, vel_seed=4),
+             dict(edge=8, N=2048, x_zn=0.10, T=150.0, place_seed=5, vel_seed=6),
+             dict(edge=5, N=500, x_zn=0.0, T=900.0, place_seed=7, vel_seed=8)]
+
+    def make():
+        return [R.make_replica_driver(s, pots, eams, masses, precision="mixed",
+                                      lanes_per_atom=16, skin=skin) for s in specs]
+
+    a, b = make(), make()
+    builds0 = [d.neighbor_list_builds for d in a]
+    ra = R.run_replicas(a, 60, 1, 20, batched=True)
+    rb = [d.run(60, 1, 20) for d in b]
+    for x, y in zip(ra, rb):
+        assert x.shape == (60, 72)
+        assert np.array_equal(x, y)
+    for d in a + b:
+        d.calculate_neighbors()
+    ra = R.run_replicas(a, 45, 5, 0, batched=True)
+    rb = [d.run(45, 5, 0) for d in b]
+    for x, y in zip(ra, rb):
+        assert x.shape == (9, 72)
+        assert np.array_equal(x, y)
+    for da, db in zip(a, b):
+        assert np.array_equal(da.forces_array(), db.forces_array())
+        assert np.array_equal(da.positions_array(), db.positions_array())
+        assert da.neighbor_list_builds == db.neighbor_list_builds
+    if skin < 1.1:
+        # rebuilds beyond the 3 forced and 1 requested ones came from the check
+        assert all(d.neighbor_list_builds - b0 > 4 for d, b0 in zip(a, builds0))
+
+
+def test_batched_launches_refuse_mixed_settings():
+    elements, pots, eams = get_potential.get_EAM_alloy_file("ZnCu.eam.alloy")
+    masses = [e[2] / U.Navogadro for e in elements]
+    s = dict(edge=5, N=500, x_zn=0.06, T=300.0, place_seed=1, vel_seed=2)
+    d8 = R.make_replica_driver(s, pots, eams, masses, lanes_per_atom=8)
+    d16 = R.make_replica_driver(s, pots, eams, masses, lanes_per_atom=16)
+    with pytest.raises(Exception, match="lanes per atom"):
+        R.run_replicas([d8, d16], 5, 1, 0, batched=True)
+    dfp = R.make_replica_driver(s, pots, eams, masses, precision="fp64", lanes_per_atom=16)
+    with pytest.raises(Exception, match="mixed precision"):
+        R.run_replicas([d16, dfp], 5, 1, 0, batched=True)
+    d16.run(3, 1, 0)
+    d8b = R.make_replica_driver(s, pots, eams, masses, lanes_per_atom=16)
+    with pytest.raises(Exception, match="same cycle"):
+        R.run_replicas([d16, d8b], 5, 1, 0, batched=True)
