@@ -262,6 +262,16 @@ template <int M> struct RegChain {
     }
 };
 
+// Lane j's value of v, to every lane, through scalar registers (v_readlane:
+// no LDS round trip on the chain's critical path, unlike __shfl). The lane
+// must be active; j is a compile-time constant where this is used.
+__device__ __forceinline__ double lane_value(double v, int j) {
+    long long const b = __double_as_longlong(v);
+    unsigned const lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, j);
+    unsigned const hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), j);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // expx_operator for code that one lane runs, reporting an argument beyond
 // EXPX_SMALL in `general`. The branch is made uniform (readfirstlane), so the
 // path not taken is jumped over instead of issued with an empty exec mask.
@@ -353,8 +363,8 @@ __device__ __forceinline__ bool nhc_prop(double (&p)[M], double (&eta)[M], const
                 general = true;
 #pragma unroll
             for (int j = 0; j + 1 < M; ++j) {
-                E[j] = __shfl(el, j, 64);
-                O[j] = __shfl(ol, j, 64);
+                E[j] = lane_value(el, j);
+                O[j] = lane_value(ol, j);
             }
         }
         double const e = BARO ? 0.5 * (kin * kin) * inv_W : cw * kin + c0;
