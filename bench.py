@@ -118,7 +118,7 @@ def bench_replicas(args):
     elements, pots, eams = get_potential.get_EAM_alloy_file("ZnCu.eam.alloy")
     masses = [e[2] / U.Navogadro for e in elements]
     specs = R.batched_replica_specs(args.replicas)
-    # one launch per pass over all replicas needs one lane count (8 or 16)
+    # one lane count for the whole batch (16 measured best for this table)
     lpa = args.lanes_per_atom or (16 if args.batched else 0)
     drivers = [R.make_replica_driver(s, pots, eams, masses, precision=args.precision,
                                      lanes_per_atom=lpa) for s in specs]

@@ -149,13 +149,14 @@ mdl_status mdl_drivers_run(mdl_driver *const *drivers, int32_t n, int64_t nsteps
                            int64_t sample_stride, int64_t rebuild_every, double *const *records,
                            int64_t *n_records);
 /* mdl_drivers_run with every replica in the same launches: each pass of a
- * step is one launch over the blocks of all n replicas (each replica keeps
- * its own grid, partial sums and their order, so its results are bit for bit
- * those of mdl_driver_run with the same lanes per atom), the list check runs
- * as one block per replica, and the whole step is one chain on the first
- * driver's stream. The drivers must share device, interpolation kind, mixed
- * precision, lanes per atom (8 or 16) and cycle; at most 4 * 256 atom blocks
- * per replica. A requested rebuild is taken first by one ordinary step of
+ * step is one launch over the blocks of all n replicas (the pair passes one
+ * launch per group of equal interpolation kind and lanes per atom). Each
+ * replica keeps its own grid, partial sums and their order, so its results
+ * are bit for bit those of mdl_driver_run. The list check runs as one block
+ * per replica, and the whole step is one chain on the first driver's stream.
+ * The drivers must use mixed precision and share device and cycle; at most
+ * 4 * 256 atom blocks per replica (MDL_ERR_ARG before any driver is touched
+ * otherwise). A requested rebuild is taken first by one ordinary step of
  * every driver. No counterpart in the reference (batched small replicas). */
 mdl_status mdl_replicas_run(mdl_driver *const *drivers, int32_t n, int64_t nsteps,
                             int64_t sample_stride, int64_t rebuild_every, double *const *records,

@@ -1367,10 +1367,22 @@ __global__ __launch_bounds__(BLOCK) void k_rb_build_rep(const RepArgs *__restric
 using RepPassFn = void (*)(const RepArgs *, const int2 *);
 
 template <int KIND, bool S> static RepPassFn rep_density_fn(int lpa) {
-    return lpa == 8 ? k_density_rep<KIND, 8, S> : k_density_rep<KIND, 16, S>;
+    switch (lpa) {
+    case 4: return k_density_rep<KIND, 4, S>;
+    case 8: return k_density_rep<KIND, 8, S>;
+    case 16: return k_density_rep<KIND, 16, S>;
+    case 32: return k_density_rep<KIND, 32, S>;
+    default: return k_density_rep<KIND, 64, S>;
+    }
 }
 template <int KIND, bool S> static RepPassFn rep_force_fn(int lpa) {
-    return lpa == 8 ? k_force_rep<KIND, 8, S> : k_force_rep<KIND, 16, S>;
+    switch (lpa) {
+    case 4: return k_force_rep<KIND, 4, S>;
+    case 8: return k_force_rep<KIND, 8, S>;
+    case 16: return k_force_rep<KIND, 16, S>;
+    case 32: return k_force_rep<KIND, 32, S>;
+    default: return k_force_rep<KIND, 64, S>;
+    }
 }
 static RepPassFn pick_density_rep(int kind, int lpa, bool s) {
     if (kind == MDL_QUINTIC)
@@ -1391,11 +1403,8 @@ static mdl_status replicas_check(const std::vector<Driver *> &ds) {
             return s;
         if (d->precision != MDL_MIXED)
             return bad("mdl_replicas_run: every driver must use mixed precision");
-        if (d->kind != d0->kind || d->lpa != d0->lpa || d->device != d0->device)
-            return bad("mdl_replicas_run: the drivers differ in interpolation, lanes per atom "
-                       "or device");
-        if (d->lpa != 8 && d->lpa != 16)
-            return bad("mdl_replicas_run: lanes per atom must be 8 or 16");
+        if (d->device != d0->device)
+            return bad("mdl_replicas_run: the drivers are on different devices");
         if (d->gridK > 4 * BLOCK)
             return bad("mdl_replicas_run: a replica is too large for the one-block list check");
         if (d->cycle_host != d0->cycle_host)
@@ -1420,8 +1429,16 @@ static mdl_status replicas_run(std::vector<RunPlan> &plans, int64_t nsteps) {
         MDL_HIP(hipStreamSynchronize(r.d->stream));
     hipStream_t const st = d0->stream;
     std::vector<RepArgs> tab(n);
-    std::vector<int2> mapA, mapB, mapK, mapBin, mapScat, mapSort, mapBuild;
+    std::vector<int2> mapK, mapBin, mapScat, mapSort, mapBuild;
     std::vector<int> nblkN(n), nblkB(n), scanWhich;
+    // the pair passes: one launch per group of replicas with the same
+    // interpolation kind and lanes per atom
+    struct Group {
+        int kind, lpa;
+        std::vector<int2> mapA, mapB;
+        size_t atA = 0, atB = 0;
+    };
+    std::vector<Group> groups;
     for (size_t i = 0; i < n; ++i) {
         Driver *d = plans[i].d;
         RepArgs &e = tab[i];
@@ -1448,10 +1465,18 @@ static mdl_status replicas_run(std::vector<RunPlan> &plans, int64_t nsteps) {
         e.has_eam = d->has_eam;
         e.rc2A = d->rc2 * 1e16;
         e.rmin = (float)(d->rmin * 1e8);
+        Group *g = nullptr;
+        for (Group &x : groups)
+            if (x.kind == d->kind && x.lpa == d->lpa)
+                g = &x;
+        if (!g) {
+            groups.push_back(Group{d->kind, d->lpa, {}, {}});
+            g = &groups.back();
+        }
         for (int b = 0; b < d->gridA; ++b)
-            mapA.push_back(make_int2((int)i, b));
+            g->mapA.push_back(make_int2((int)i, b));
         for (int b = 0; b < d->gridB; ++b)
-            mapB.push_back(make_int2((int)i, b));
+            g->mapB.push_back(make_int2((int)i, b));
         for (int b = 0; b < d->gridK; ++b)
             mapK.push_back(make_int2((int)i, b));
         // the forced build's launches (Driver::launch_build's grids)
@@ -1475,10 +1500,15 @@ static mdl_status replicas_run(std::vector<RunPlan> &plans, int64_t nsteps) {
     // every map in one device buffer: the int2 maps, then the int arrays
     std::vector<int2> all;
     std::vector<size_t> at;
-    for (const std::vector<int2> *v : {&mapA, &mapB, &mapK, &mapBin, &mapScat, &mapSort,
-                                       &mapBuild}) {
+    for (const std::vector<int2> *v : {&mapK, &mapBin, &mapScat, &mapSort, &mapBuild}) {
         at.push_back(all.size());
         all.insert(all.end(), v->begin(), v->end());
+    }
+    for (Group &g : groups) {
+        g.atA = all.size();
+        all.insert(all.end(), g.mapA.begin(), g.mapA.end());
+        g.atB = all.size();
+        all.insert(all.end(), g.mapB.begin(), g.mapB.end());
     }
     std::vector<int> ints(nblkN);
     ints.insert(ints.end(), nblkB.begin(), nblkB.end());
@@ -1497,9 +1527,8 @@ static mdl_status replicas_run(std::vector<RunPlan> &plans, int64_t nsteps) {
     int *dint = reinterpret_cast<int *>(dmap + all.size());
     if (e == hipSuccess)
         e = hipMemcpy(dint, ints.data(), sizeof(int) * ints.size(), hipMemcpyHostToDevice);
-    int2 const *mA = dmap + at[0], *mB = dmap + at[1], *mK = dmap + at[2];
-    int2 const *mBin = dmap + at[3], *mScat = dmap + at[4], *mSort = dmap + at[5],
-               *mBuild = dmap + at[6];
+    int2 const *mK = dmap + at[0], *mBin = dmap + at[1], *mScat = dmap + at[2],
+               *mSort = dmap + at[3], *mBuild = dmap + at[4];
     int const *dN = dint, *dB = dint + n, *dWhich = dint + 2 * n;
     RunPlan &p0 = plans[0];
     while (e == hipSuccess && p0.k < nsteps) {
@@ -1518,9 +1547,12 @@ static mdl_status replicas_run(std::vector<RunPlan> &plans, int64_t nsteps) {
             k_rb_build_rep<<<(unsigned)mapBuild.size(), BLOCK, 0, st>>>(dtab, mBuild, dB);
         } else
             k_list_rep<<<(unsigned)n, BLOCK, 0, st>>>(dtab);
-        pick_density_rep(d0->kind, d0->lpa, smp)<<<(unsigned)mapA.size(), BLOCK, 0, st>>>(dtab,
-                                                                                         mA);
-        pick_force_rep(d0->kind, d0->lpa, smp)<<<(unsigned)mapB.size(), BLOCK, 0, st>>>(dtab, mB);
+        for (const Group &g : groups)
+            pick_density_rep(g.kind, g.lpa, smp)<<<(unsigned)g.mapA.size(), BLOCK, 0, st>>>(
+                dtab, dmap + g.atA);
+        for (const Group &g : groups)
+            pick_force_rep(g.kind, g.lpa, smp)<<<(unsigned)g.mapB.size(), BLOCK, 0, st>>>(
+                dtab, dmap + g.atB);
         if (smp)
             k_kick2_rep<true><<<(unsigned)mapK.size(), BLOCK, 0, st>>>(dtab, mK, 0);
         else

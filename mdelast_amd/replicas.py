@@ -35,8 +35,8 @@ def run_replicas(drivers: Sequence["mdm.MDDriver"], nsteps: int, sample_stride: 
     each driver's per-sample records (MDL_S_* layout).
 
     batched: every pass of a step is one launch over all replicas
-    (``mdl_replicas_run``; mixed precision, one interpolation kind, 8 or 16
-    lanes per atom, all drivers at the same cycle) instead of one launch per
+    (``mdl_replicas_run``; mixed precision, all drivers at the same cycle; the
+    pair passes one launch per lanes-per-atom group) instead of one launch per
     replica on its own stream (``mdl_drivers_run``)."""
     n = len(drivers)
     if n == 0:

@@ -95,13 +95,15 @@ def test_joint_graph_equals_step_launches():
         assert np.array_equal(da.forces_array(), db.forces_array())
 
 
+@pytest.mark.parametrize("lanes", [[16, 16, 16, 16], [8, 32, 0, 16]])
 @pytest.mark.parametrize("skin", [1.2, 1.04])
-def test_batched_launches_equal_one_at_a_time(skin):
-    """Every pass of a step as one launch over all replicas (mdl_replicas_run)
-    gives each replica's records, forces and positions bit for bit as its own
-    launches do: with forced rebuilds (the build chains on the shared stream),
-    with a requested rebuild, and (thin skin) with rebuilds that the one-block
-    list check decides by itself, over brute-force and cell lists."""
+def test_batched_launches_equal_one_at_a_time(skin, lanes):
+    """Every pass of a step as one launch over all replicas (mdl_replicas_run;
+    one launch per lanes-per-atom group) gives each replica's records, forces
+    and positions bit for bit as its own launches do: with forced rebuilds
+    (batched build launches), with a requested rebuild, and (thin skin) with
+    rebuilds that the one-block list check decides by itself, over brute-force
+    and cell lists; lanes per atom fixed or autotuned (0)."""
     elements, pots, eams = get_potential.get_EAM_alloy_file("ZnCu.eam.alloy")
     masses = [e[2] / U.Navogadro for e in elements]
     specs = [dict(edge=5, N=500, x_zn=0.06, T=300.0, place_seed=1, vel_seed=2),
@@ -111,7 +113,8 @@ def test_batched_launches_equal_one_at_a_time(skin):
 
     def make():
         return [R.make_replica_driver(s, pots, eams, masses, precision="mixed",
-                                      lanes_per_atom=16, skin=skin) for s in specs]
+                                      lanes_per_atom=lp, skin=skin)
+                for s, lp in zip(specs, lanes)]
 
     a, b = make(), make()
     builds0 = [d.neighbor_list_builds for d in a]
@@ -137,17 +140,18 @@ def test_batched_launches_equal_one_at_a_time(skin):
 
 
 def test_batched_launches_refuse_mixed_settings():
+    """A set that cannot share launches is refused before any driver moves."""
     elements, pots, eams = get_potential.get_EAM_alloy_file("ZnCu.eam.alloy")
     masses = [e[2] / U.Navogadro for e in elements]
     s = dict(edge=5, N=500, x_zn=0.06, T=300.0, place_seed=1, vel_seed=2)
-    d8 = R.make_replica_driver(s, pots, eams, masses, lanes_per_atom=8)
     d16 = R.make_replica_driver(s, pots, eams, masses, lanes_per_atom=16)
-    with pytest.raises(Exception, match="lanes per atom"):
-        R.run_replicas([d8, d16], 5, 1, 0, batched=True)
+    ref = R.make_replica_driver(s, pots, eams, masses, lanes_per_atom=16)
     dfp = R.make_replica_driver(s, pots, eams, masses, precision="fp64", lanes_per_atom=16)
     with pytest.raises(Exception, match="mixed precision"):
         R.run_replicas([d16, dfp], 5, 1, 0, batched=True)
-    d16.run(3, 1, 0)
-    d8b = R.make_replica_driver(s, pots, eams, masses, lanes_per_atom=16)
+    late = R.make_replica_driver(s, pots, eams, masses, lanes_per_atom=16)
+    late.run(3, 1, 0)
     with pytest.raises(Exception, match="same cycle"):
-        R.run_replicas([d16, d8b], 5, 1, 0, batched=True)
+        R.run_replicas([d16, late], 5, 1, 0, batched=True)
+    # the refused calls left d16 untouched
+    assert np.array_equal(d16.run(4, 1, 0), ref.run(4, 1, 0))
