@@ -226,12 +226,12 @@ def learned_cv(n_eval):
     learned Stein control variate: 4,000-atom alpha-brass (6 at.% Zn random
     fcc, 10^3 cells, a = 3.667 A), NVT 300 K, dt 1 fs, frames every 20 steps;
     trained on 2,000 frames of a plain run (400 more held out), evaluated on
-    two fresh trajectories (new velocity seeds). GPU-s = frames needed x the
+    three fresh trajectories (new velocity seeds). GPU-s = frames needed x the
     device time per frame (20 MD steps of the device loop, plus the marginal
     time of the controls for the controlled estimate)."""
     from mdelast_amd import cv
     try:
-        r = cv.brass_experiment(n_train=2000, n_valid=400, n_eval=n_eval, eval_runs=2,
+        r = cv.brass_experiment(n_train=2000, n_valid=400, n_eval=n_eval, eval_runs=3,
                                 stride=20, epochs=20)
     except Exception as e:        # reported, never hides the throughput line
         return {"error": f"{type(e).__name__}: {e}"}
@@ -260,6 +260,9 @@ def learned_cv(n_eval):
         "pressure_proxy_r2": round(float(r["fit"]["proxy_r2"]), 5),
         "heldout_var_ratio": f(np.asarray(r["fit"]["var_cv_valid"]) /
                                np.asarray(r["fit"]["var_plain_valid"])),
+        "eval_var_ratio": f(np.mean([x["var_ratio"] for x in runs], 0)),
+        "eval_se2_ratio": f((np.asarray(r["se_cv"]) / np.asarray(r["se_plain"])) ** 2),
+        "eval_runs": len(runs),
         "t_step_us": round(r["t_step_s"] * 1e6, 3),
         "t_controls_us_per_frame": round(r["t_controls_s"] * 1e6, 3),
         "frames_eval": r["n_eval"] * r["eval_runs"],
@@ -290,7 +293,7 @@ def main():
                     help="hardware queues of the process (GPU_MAX_HW_QUEUES, at most 32)")
     ap.add_argument("--no-learned-cv", action="store_true",
                     help="skip the learned control-variate comparison (4,000-atom alpha-brass)")
-    ap.add_argument("--cv-eval-frames", type=int, default=4000)
+    ap.add_argument("--cv-eval-frames", type=int, default=6000)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB three times "
                          "(a kernel of known HBM traffic, to scale rocprofv3 --pmc "

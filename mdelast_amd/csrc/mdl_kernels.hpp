@@ -1369,6 +1369,49 @@ __global__ __launch_bounds__(BLOCK) void k_force(AtomArrays a, ListArrays l, RTa
 // also the embedding part of the Born tensor and Laplacian, F''_i g_i g_i
 // and F''_i |h_i|^2 (md_driver.cpp:410-419), from pass A's per-atom g, |h|^2.
 template <bool SAMPLE>
+__device__ __forceinline__ void kick2_atom(const AtomArrays &a, int i,
+                                           const DevState *__restrict__ gs, double e, double ch,
+                                           bool init, int has_eam, double (&acc)[PK_N]) {
+    double const m = gs->mass[a.type[i]];
+    double const fx = a.fx[i], fy = a.fy[i], fz = a.fz[i];
+    double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
+    if (!init) {
+        vx = vx * e + ch * fx / m;
+        vy = vy * e + ch * fy / m;
+        vz = vz * e + ch * fz / m;
+        a.vx[i] = vx;
+        a.vy[i] = vy;
+        a.vz[i] = vz;
+    }
+    acc[PK_K] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+    acc[PK_MVV + 0] += m * vx * vx;
+    acc[PK_MVV + 1] += m * vy * vy;
+    acc[PK_MVV + 2] += m * vz * vz;
+    acc[PK_MVV + 3] += m * vy * vz;
+    acc[PK_MVV + 4] += m * vx * vz;
+    acc[PK_MVV + 5] += m * vx * vy;
+    acc[PK_MOM + 0] += m * vx;
+    acc[PK_MOM + 1] += m * vy;
+    acc[PK_MOM + 2] += m * vz;
+    acc[PK_F2] += fx * fx + fy * fy + fz * fz;
+    if (SAMPLE && has_eam) {
+        double const d2F = a.fprime[i].y;
+        const double2 *gp = a.geam + 4 * (size_t)i;
+        double2 const ga = gp[0], gb = gp[1], gc = gp[2], gd = gp[3];
+        double const g[6] = {ga.x, ga.y, gb.x, gb.y, gc.x, gc.y};
+        int q = 0;
+#pragma unroll
+        for (int v = 0; v < 6; ++v) {
+            double const fg = d2F * g[v];
+#pragma unroll
+            for (int w = v; w < 6; ++w)
+                acc[PK_CBF + q++] += fg * g[w];
+        }
+        acc[PK_LAPF] += d2F * gd.x;
+    }
+}
+
+template <bool SAMPLE>
 __device__ __forceinline__ void kick2_body(AtomArrays a, int N, const DevState *__restrict__ gs,
                                            int init, int has_eam, double *__restrict__ part,
                                            Blk bk) {
@@ -1379,45 +1422,8 @@ __device__ __forceinline__ void kick2_body(AtomArrays a, int N, const DevState *
 #pragma unroll
     for (int k = 0; k < PK_N; ++k)
         acc[k] = 0.;
-    for (int i = bk.id * BLOCK + threadIdx.x; i < N; i += bk.n * BLOCK) {
-        double const m = gs->mass[a.type[i]];
-        double const fx = a.fx[i], fy = a.fy[i], fz = a.fz[i];
-        double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
-        if (!init) {
-            vx = vx * e + ch * fx / m;
-            vy = vy * e + ch * fy / m;
-            vz = vz * e + ch * fz / m;
-            a.vx[i] = vx;
-            a.vy[i] = vy;
-            a.vz[i] = vz;
-        }
-        acc[PK_K] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
-        acc[PK_MVV + 0] += m * vx * vx;
-        acc[PK_MVV + 1] += m * vy * vy;
-        acc[PK_MVV + 2] += m * vz * vz;
-        acc[PK_MVV + 3] += m * vy * vz;
-        acc[PK_MVV + 4] += m * vx * vz;
-        acc[PK_MVV + 5] += m * vx * vy;
-        acc[PK_MOM + 0] += m * vx;
-        acc[PK_MOM + 1] += m * vy;
-        acc[PK_MOM + 2] += m * vz;
-        acc[PK_F2] += fx * fx + fy * fy + fz * fz;
-        if (SAMPLE && has_eam) {
-            double const d2F = a.fprime[i].y;
-            const double2 *gp = a.geam + 4 * (size_t)i;
-            double2 const ga = gp[0], gb = gp[1], gc = gp[2], gd = gp[3];
-            double const g[6] = {ga.x, ga.y, gb.x, gb.y, gc.x, gc.y};
-            int q = 0;
-#pragma unroll
-            for (int v = 0; v < 6; ++v) {
-                double const fg = d2F * g[v];
-#pragma unroll
-                for (int w = v; w < 6; ++w)
-                    acc[PK_CBF + q++] += fg * g[w];
-            }
-            acc[PK_LAPF] += d2F * gd.x;
-        }
-    }
+    for (int i = bk.id * BLOCK + threadIdx.x; i < N; i += bk.n * BLOCK)
+        kick2_atom<SAMPLE>(a, i, gs, e, ch, init != 0, has_eam, acc);
     block_sum_store<PK_N>(acc, part, bk.id);
 }
 

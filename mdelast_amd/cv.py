@@ -61,7 +61,8 @@ NFIELD = 6         # per head: one field as is, one weighted by h - h0
 NOUT = NFIELD * NF
 HEADS = ("C11", "C12", "C44")
 NGLOBAL = 27       # global_features: S, T, Q
-NGLOBAL_USED = 27  # the controls use S, T and the h-weighted displacement sums
+NGLOBAL_USED = 27
+FIELD_GAIN = 0.05     # least held-out variance cut that keeps the learned fields of a head  # the controls use S, T and the h-weighted displacement sums
 
 
 def _torch():
@@ -723,15 +724,19 @@ class ControlVariate:
             self.coef = self._fit_linear(X, yc)
             var_net = (yv - self.controls(valid, enabled_only=False)).var(0)
             say(f"learned fields: held-out variance / plain {np.round(var_net / var_plain, 4)}")
-        use_fields = var_net < var_glob
+        # the learned fields cost a network pass per frame: kept for a head
+        # only where they cut its held-out variance by FIELD_GAIN or more
+        use_fields = var_net < (1.0 - FIELD_GAIN) * var_glob
         coef = self.coef.copy()
         for h in range(3):
             if not use_fields[h]:
                 coef[h] = glob[h]
         self.coef = coef          # a new array: the device copy is refreshed
-        var_cv = np.minimum(var_net, var_glob)
+        var_cv = np.where(use_fields, var_net, var_glob)
         self.enabled = var_cv < var_plain
         self.uses_fields = use_fields & self.enabled
+        # no head takes the fields: the controls skip the network
+        self.use_net = bool(self.uses_fields.any())
         return dict(proxy_r2=r2, var_plain_valid=var_plain, var_global_valid=var_glob,
                     var_fields_valid=var_net, var_cv_valid=var_cv,
                     enabled=self.enabled.copy(), uses_learned_fields=self.uses_fields.copy(),
@@ -900,6 +905,7 @@ def brass_experiment(n_train=2000, n_valid=400, n_eval=4000, eval_runs=2, stride
         y = ev.y()
         runs.append(dict(C_plain=Cp, se_plain=sep, C_cv=Cc, se_cv=sec,
                          var_ratio_all_heads=(y - s).var(0) / y.var(0),
+                         var_ratio=(y - sc).var(0) / y.var(0),
                          mean_s=s.mean(0),
                          se_mean_s=np.array([block_bootstrap_se(s[:, h]) for h in range(3)])))
         say(f"eval run {r}: plain {np.round(Cp, 2)} +- {np.round(sep, 3)}; "

@@ -95,6 +95,28 @@ def test_joint_graph_equals_step_launches():
         assert np.array_equal(da.forces_array(), db.forces_array())
 
 
+def test_joint_graph_waits_for_every_stream():
+    """A joint chunk launched on the first driver's stream starts after the
+    steps still queued on the other drivers' streams: a small first replica
+    and a large second one, stepped one by one up to the chunk boundary, then
+    advanced by joint chunks, agree bit for bit with step launches."""
+    elements, pots, eams = get_potential.get_EAM_alloy_file("ZnCu.eam.alloy")
+    masses = [e[2] / U.Navogadro for e in elements]
+    specs = [dict(edge=5, N=500, x_zn=0.06, T=300.0, place_seed=1, vel_seed=2),
+             dict(edge=20, N=32000, x_zn=0.06, T=600.0, place_seed=3, vel_seed=4)]
+    a = [R.make_replica_driver(s, pots, eams, masses, precision="mixed") for s in specs]
+    b = [R.make_replica_driver(s, pots, eams, masses, precision="mixed") for s in specs]
+    for d in a:
+        d.graph_steps = 20
+    for n in (7, 53, 60):
+        ra = R.run_replicas(a, n, 1, 20)
+        rb = R.run_replicas(b, n, 1, 20)
+        for x, y in zip(ra, rb):
+            assert np.array_equal(x, y)
+    for da, db in zip(a, b):
+        assert np.array_equal(da.positions_array(), db.positions_array())
+
+
 @pytest.mark.parametrize("lanes", [[16, 16, 16, 16], [8, 32, 0, 16]])
 @pytest.mark.parametrize("skin", [1.2, 1.04])
 def test_batched_launches_equal_one_at_a_time(skin, lanes):
