@@ -45,6 +45,21 @@ mdl_status mdl_driver_cv_descriptors(mdl_driver *d, float rc, float mu0, float d
                                      float *D, float *J, float *L, float *La, double *pos_out,
                                      double *force_out, void *stream);
 
+/* The same inputs, taken inside mdl_driver_run / mdl_drivers_run at every
+ * sampled step (on the driver's stream, right after the step) into `slots`
+ * frame slots used in turn from slot 0: D [slots][N][64], J [slots][3][N][64],
+ * L [slots][N][64] (FP32), pos_out and force_out [slots][N][3] (FP64, CGS; all
+ * device memory that must outlive the runs). A call restarts at slot 0;
+ * slots = 0 stops the capture. While it is on, steps launch one by one (no
+ * captured graphs) and mdl_replicas_run refuses the driver. Work queued on the
+ * driver's stream (mdl_driver_stream) after a run reads the slots in order. */
+mdl_status mdl_driver_set_cv_capture(mdl_driver *d, float rc, float mu0, float dmu, float width,
+                                     float *D, float *J, float *L, double *pos_out,
+                                     double *force_out, int32_t slots);
+
+/* The driver's stream (a hipStream_t) into *out. */
+mdl_status mdl_driver_stream(mdl_driver *d, void **out);
+
 /* Device-to-device copy of a per-atom array of the driver (MDL_A_*, the
  * layout and units of mdl_driver_get_atoms) into device memory `out`;
  * returns when done. */

@@ -102,6 +102,9 @@ SIGNATURES = {
                                        C.c_float, C.c_float, _vp, _vp, _vp, _vp]),
     "mdl_driver_cv_descriptors": (C.c_int32, [_vp, C.c_float, C.c_float, C.c_float, C.c_float,
                                               _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mdl_driver_set_cv_capture": (C.c_int32, [_vp, C.c_float, C.c_float, C.c_float, C.c_float,
+                                              _vp, _vp, _vp, _vp, _vp, C.c_int32]),
+    "mdl_driver_stream": (C.c_int32, [_vp, C.POINTER(_vp)]),
     "mdl_driver_copy_atoms": (C.c_int32, [_vp, C.c_int32, _vp]),
     "mdl_cv_gemm": (C.c_int32, [_vp, _vp, C.c_int32, _vp, _vp, C.c_int64, C.c_int32, C.c_int32,
                                 _vp]),

@@ -130,6 +130,14 @@ struct Driver {
     // timing of the pair passes
     std::vector<hipEvent_t> ev;
     hipEvent_t ev_cv[2] = {nullptr, nullptr};   // ordering with a caller's stream
+    // control-variate inputs of the sampled steps of a run, written into frame
+    // slots in turn (mdl_driver_set_cv_capture)
+    struct CvCapture {
+        float *D = nullptr, *J = nullptr, *L = nullptr;
+        double *pos = nullptr, *force = nullptr;
+        int slots = 0, next = 0, use_full = 0;
+        CvBasis basis{};
+    } cvc;
     // run() replays graph_steps consecutive steps as one captured graph (0: off)
     int graph_steps = 0;
     hipGraphExec_t gexec = nullptr;
@@ -862,6 +870,25 @@ __global__ __launch_bounds__(256) void k_cv_desc_list(AtomArrays a, ListArrays l
         acc.store(i, k, N, D, J, Lp, La);
 }
 
+// The control-variate inputs of the step just launched, into the next frame
+// slot (descriptors from the compacted list, positions and forces), on the
+// driver's stream.
+static void cv_capture(Driver &d) {
+    Driver::CvCapture &c = d.cvc;
+    int const j = c.next % c.slots;
+    ++c.next;
+    size_t const nf = (size_t)d.N * 2 * CV_NB;
+    int const waves = (d.N + 1) / 2;
+    k_cv_desc_list<<<(waves * 64 + 255) / 256, 256, 0, d.stream>>>(
+        d.a, d.l, d.N, d.gs, 1e8, c.basis, c.use_full, c.D + j * nf, c.J + j * 3 * nf,
+        c.L + j * nf, nullptr);
+    unsigned const g = (d.N + 255) / 256;
+    k_materialize<<<g, 256, 0, d.stream>>>(d.a, d.N, d.gs, MDL_A_POSITIONS,
+                                           c.pos + (size_t)j * 3 * d.N);
+    k_materialize<<<g, 256, 0, d.stream>>>(d.a, d.N, d.gs, MDL_A_FORCES,
+                                           c.force + (size_t)j * 3 * d.N);
+}
+
 // ---------------------------------------------------------------------------
 // One captured graph for a chunk of steps of several drivers: each driver's
 // launches form a branch of its own (its stream joins the capture through a
@@ -1029,7 +1056,7 @@ struct RunPlan {
     bool graph_fits() const {
         int64_t const G = d->graph_steps;
         return G > 0 && k + G < nsteps && d->cycle_host % G == 0 && !d->build_pending &&
-               G % stride == 0 && (rebuild <= 0 || G % rebuild == 0);
+               G % stride == 0 && (rebuild <= 0 || G % rebuild == 0) && d->cvc.slots == 0;
     }
 
     mdl_status graph_step() {
@@ -1067,6 +1094,10 @@ struct RunPlan {
         d->build_pending = false;
         ++k;
         MDL_HIP(d->launch_step(s, ro, tev, build, ns));
+        if (s && d->cvc.slots > 0) {
+            cv_capture(*d);
+            MDL_HIP(hipGetLastError());
+        }
         return MDL_OK;
     }
 
@@ -1423,6 +1454,8 @@ static mdl_status replicas_check(const std::vector<Driver *> &ds) {
             return bad("mdl_replicas_run: every driver must use mixed precision");
         if (d->device != d0->device)
             return bad("mdl_replicas_run: the drivers are on different devices");
+        if (d->cvc.slots > 0)
+            return bad("mdl_replicas_run: a driver captures control-variate inputs");
         if (d->gridK > 4 * BLOCK)
             return bad("mdl_replicas_run: a replica is too large for the one-block list check");
         if (d->cycle_host != d0->cycle_host)
@@ -1826,6 +1859,44 @@ mdl_status mdl_driver_cv_descriptors(mdl_driver *p, float rc, float mu0, float d
     } else {
         MDL_HIP(hipStreamSynchronize(d->stream));
     }
+    return MDL_OK;
+}
+
+mdl_status mdl_driver_set_cv_capture(mdl_driver *p, float rc, float mu0, float dmu, float width,
+                                     float *Dp, float *Jp, float *Lp, double *pos_out,
+                                     double *force_out, int32_t slots) {
+    Driver *d = D(p);
+    if (!d || slots < 0)
+        return bad("mdl_driver_set_cv_capture: bad arguments");
+    if (slots == 0) {
+        d->cvc = Driver::CvCapture{};
+        return MDL_OK;
+    }
+    if (!Dp || !Jp || !Lp || !pos_out || !force_out || !(rc > 0.f) || !(width > 0.f))
+        return bad("mdl_driver_set_cv_capture: bad arguments");
+    if (d->ntypes > 2)
+        return bad("mdl_driver_set_cv_capture: the descriptors have two species channels");
+    if (!((double)rc * 1e-8 <= d->hs.rc * (1.0 + 1e-6)))
+        return bad("mdl_driver_set_cv_capture: rc exceeds the neighbour list's cut-off");
+    Driver::CvCapture c;
+    c.D = Dp;
+    c.J = Jp;
+    c.L = Lp;
+    c.pos = pos_out;
+    c.force = force_out;
+    c.slots = slots;
+    c.next = 0;
+    c.use_full = (double)rc * rc * 1e-16 > d->rc2 * (1.0 + 1e-6) ? 1 : 0;
+    c.basis = cv_basis(rc, mu0, dmu, width);
+    d->cvc = c;
+    return MDL_OK;
+}
+
+mdl_status mdl_driver_stream(mdl_driver *p, void **out) {
+    Driver *d = D(p);
+    if (!d || !out)
+        return bad("null argument");
+    *out = (void *)d->stream;
     return MDL_OK;
 }
 

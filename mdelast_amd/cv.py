@@ -762,11 +762,19 @@ class ControlBatch:
                     torch.empty((B, N, 3), device=dev, dtype=torch.float64))
         self.U = np.zeros(B)
         self.out = []
+        # the controls run on the stream current now; the per-frame call takes
+        # prepared pointers (slot k of each contiguous buffer) instead of views
+        self.stream = torch.cuda.current_stream(dev)
+        self._st = C.c_void_p(self.stream.cuda_stream)
+        self._slots = [tuple(C.c_void_p(b[k].data_ptr()) for b in self.buf) for k in range(B)]
+        bs = cvm.basis
+        self._basis = (bs.rc, bs.mu0, bs.dmu, bs.width)
+        self._fn = nat.lib().mdl_driver_cv_descriptors
 
     def push(self, drv, epot):
         """Take the driver's current step (epot: configurational energy, eV)."""
-        frame_inputs(drv, self.cvm.basis, self.cvm.device,
-                     out=tuple(b[self.k] for b in self.buf))
+        D, J, L, pos, F = self._slots[self.k]
+        _check(self._fn(drv.handle, *self._basis, D, J, L, None, pos, F, self._st))
         self.U[self.k] = epot
         self.k += 1
         if self.k == self.B:
@@ -778,19 +786,55 @@ class ControlBatch:
         if self.k == 0:
             return
         k = self.k
-        # a fresh pinned copy per batch: the asynchronous upload reads it later
-        Ut = torch.tensor(self.U[:k], dtype=torch.float64).pin_memory().to(
-            self.cvm.device, non_blocking=True)
-        X = self.cvm._x(*[b[:k] for b in self.buf], Ut)
-        co = self.cvm._device_consts()[2]
-        self.out.append((X * co[None]).sum(2))
+        with torch.cuda.stream(self.stream):
+            # a fresh pinned copy per batch: the asynchronous upload reads it later
+            Ut = torch.tensor(self.U[:k], dtype=torch.float64).pin_memory().to(
+                self.cvm.device, non_blocking=True)
+            X = self.cvm._x(*[b[:k] for b in self.buf], Ut)
+            co = self.cvm._device_consts()[2]
+            self.out.append((X * co[None]).sum(2))
         self.k = 0
 
+    def run(self, drv, nframes, stride):
+        """Advance ``drv`` by nframes x stride steps, a frame every ``stride``
+        steps, and take every frame's controls: the driver captures each
+        frame's inputs inside its run (one run per B frames,
+        mdl_driver_set_cv_capture) and the controls of those frames are queued
+        on the driver's stream right after the run, before the next one
+        overwrites the slots. Returns the frames' records (MDL_S_* rows)."""
+        torch = _torch()
+        lib = nat.lib()
+        ptr = C.c_void_p()
+        _check(lib.mdl_driver_stream(drv.handle, C.byref(ptr)))
+        self._ext = torch.cuda.ExternalStream(ptr.value, device=self.cvm.device)
+        slots = [C.c_void_p(b.data_ptr()) for b in self.buf]
+        recs = []
+        try:
+            for f0 in range(0, nframes, self.B):
+                nb = min(self.B, nframes - f0)
+                _check(lib.mdl_driver_set_cv_capture(drv.handle, *self._basis, *slots, self.B))
+                rec = drv.run(nb * stride, sample_stride=stride)
+                recs.append(rec)
+                U = np.array([_record_scalars(r, drv.N)[0] for r in rec])
+                with torch.cuda.stream(self._ext):
+                    Ut = torch.tensor(U, dtype=torch.float64).pin_memory().to(
+                        self.cvm.device, non_blocking=True)
+                    X = self.cvm._x(*[b[:nb] for b in self.buf], Ut)
+                    co = self.cvm._device_consts()[2]
+                    self.out.append((X * co[None]).sum(2))
+        finally:
+            lib.mdl_driver_set_cv_capture(drv.handle, 0.0, 0.0, 0.0, 0.0, None, None, None,
+                                          None, None, 0)
+        return np.concatenate(recs) if recs else np.zeros((0, nat.NRECORD))
+
     def result(self):
-        """s [n, 3] of every frame pushed so far (all heads)."""
+        """s [n, 3] of every frame taken so far (all heads)."""
         torch = _torch()
         self.flush()
-        return torch.cat(self.out).cpu().numpy() if self.out else np.zeros((0, 3))
+        if getattr(self, "_ext", None) is not None:
+            self.stream.wait_stream(self._ext)
+        with torch.cuda.stream(self.stream):
+            return torch.cat(self.out).cpu().numpy() if self.out else np.zeros((0, 3))
 
 
 def estimate(fr: Frames, s=None, nblocks=64):
@@ -838,16 +882,17 @@ def time_steps(drv, nsteps):
 
 
 def frame_loop_time(drv, nframes, stride, cbat=None):
-    """Seconds per frame of a loop of run(stride) calls reading each frame's
-    record, and with ``cbat`` taking the controls of every frame."""
+    """Seconds per frame of the frame loop: runs of 16 frames (stride steps
+    each, records of every frame), and with ``cbat`` the controls of every
+    frame taken as the evaluation takes them (ControlBatch.run)."""
     torch = _torch()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(nframes):
-        rec = drv.run(stride, sample_stride=stride)[-1]
-        if cbat is not None:
-            cbat.push(drv, _record_scalars(rec, drv.N)[0])
-    if cbat is not None:
+    if cbat is None:
+        for f0 in range(0, nframes, 16):
+            drv.run(min(16, nframes - f0) * stride, sample_stride=stride)
+    else:
+        cbat.run(drv, nframes, stride)
         cbat.result()
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) / nframes
@@ -892,10 +937,9 @@ def brass_experiment(n_train=2000, n_valid=400, n_eval=4000, eval_runs=2, stride
         sb = np.zeros((n_eval, 6))
         cb = np.zeros((n_eval, 6, 6))
         cbat = ControlBatch(cvm, d.N)
+        recs = cbat.run(d, n_eval, stride)
         for k in range(n_eval):
-            rec = d.run(stride, sample_stride=stride)[-1]
-            ep[k], sb[k], cb[k] = _record_scalars(rec, d.N)
-            cbat.push(d, ep[k])
+            ep[k], sb[k], cb[k] = _record_scalars(recs[k], d.N)
         s = cbat.result()
         box = np.diag(d.box.h) / U.angstrom
         ev = Frames(ep, sb, cb, box, T, d.N / np.prod(box))

@@ -247,6 +247,29 @@ def test_stein_identity_and_fallback_on_md_frames():
     # another order for another batch shape
     print("in-loop", s_loop.ravel(), "stored", s_ref.ravel(), flush=True)
     assert _rel(s_loop, s_ref) < 1e-5
+    # inputs captured inside the runs (ControlBatch.run) == stored frames of an
+    # identical trajectory; the capture leaves the trajectory untouched
+    d1, _, _ = cv.brass_driver(edge=5, vel_seed=777)
+    d2, _, _ = cv.brass_driver(edge=5, vel_seed=777)
+    d1.run(200, sample_stride=200)
+    d2.run(200, sample_stride=200)
+    cap = cv.ControlBatch(cvm, d1.N, B=4)
+    recs = cap.run(d1, 10, 20)
+    s_cap = cap.result()
+    st = {q: [] for q in ("D", "J", "L", "pos", "F")}
+    recs2 = []
+    for k in range(10):
+        recs2.append(d2.run(20, sample_stride=20)[-1])
+        for q, v in zip(("D", "J", "L", "pos", "F"), cv.frame_inputs(d2)):
+            st[q].append(v)
+    assert np.array_equal(recs, np.array(recs2))
+    ref2 = cv.collect_frames(d2, 0, 20, 300.0)
+    ref2.store = {q: torch.stack(v) for q, v in st.items()}
+    ref2.epot = np.array([cv._record_scalars(r, d2.N)[0] for r in recs2])
+    s_ref2 = cvm.controls(ref2, enabled_only=False)
+    print("captured", s_cap.ravel(), "stored", s_ref2.ravel(), flush=True)
+    assert s_cap.shape == (10, 3)
+    assert _rel(s_cap, s_ref2) < 1e-5
     cvm.enabled[:] = False
     Cp, sep = cv.estimate(ev)
     Cc, sec = cv.estimate(ev, cvm.controls(ev))
