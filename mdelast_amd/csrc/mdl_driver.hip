@@ -47,8 +47,7 @@ template <int KIND, bool S, bool E> static ForceFn force_fn(int lpa) {
 }
 using DensMxFn = void (*)(AtomArrays, ListArrays, RTablesF, FTablesF, int, int, int, double,
                           DevState *, double *);
-using ForceMxFn = void (*)(AtomArrays, ListArrays, RTablesF, int, int, float, DevState *, double *,
-                           double *);
+using ForceMxFn = void (*)(AtomArrays, ListArrays, RTablesF, int, int, float, DevState *, double *);
 template <int KIND, bool S> static DensMxFn dens_mx_fn(int lpa) {
     switch (lpa) {
     case 4: return k_density_mx<KIND, 4, S>;
@@ -58,13 +57,13 @@ template <int KIND, bool S> static DensMxFn dens_mx_fn(int lpa) {
     default: return k_density_mx<KIND, 64, S>;
     }
 }
-template <int KIND, bool S, bool K> static ForceMxFn force_mx_fn(int lpa) {
+template <int KIND, bool S> static ForceMxFn force_mx_fn(int lpa) {
     switch (lpa) {
-    case 4: return k_force_mx<KIND, 4, S, K>;
-    case 8: return k_force_mx<KIND, 8, S, K>;
-    case 16: return k_force_mx<KIND, 16, S, K>;
-    case 32: return k_force_mx<KIND, 32, S, K>;
-    default: return k_force_mx<KIND, 64, S, K>;
+    case 4: return k_force_mx<KIND, 4, S>;
+    case 8: return k_force_mx<KIND, 8, S>;
+    case 16: return k_force_mx<KIND, 16, S>;
+    case 32: return k_force_mx<KIND, 32, S>;
+    default: return k_force_mx<KIND, 64, S>;
     }
 }
 static DensMxFn pick_dens_mx(int kind, int lpa, bool s) {
@@ -72,14 +71,10 @@ static DensMxFn pick_dens_mx(int kind, int lpa, bool s) {
         return s ? dens_mx_fn<MDL_QUINTIC, true>(lpa) : dens_mx_fn<MDL_QUINTIC, false>(lpa);
     return s ? dens_mx_fn<MDL_BSPLINE, true>(lpa) : dens_mx_fn<MDL_BSPLINE, false>(lpa);
 }
-template <int KIND> static ForceMxFn force_mx_kind(int lpa, bool s, bool kick) {
-    if (kick)
-        return s ? force_mx_fn<KIND, true, true>(lpa) : force_mx_fn<KIND, false, true>(lpa);
-    return s ? force_mx_fn<KIND, true, false>(lpa) : force_mx_fn<KIND, false, false>(lpa);
-}
-static ForceMxFn pick_force_mx(int kind, int lpa, bool s, bool kick) {
-    return kind == MDL_QUINTIC ? force_mx_kind<MDL_QUINTIC>(lpa, s, kick)
-                               : force_mx_kind<MDL_BSPLINE>(lpa, s, kick);
+static ForceMxFn pick_force_mx(int kind, int lpa, bool s) {
+    if (kind == MDL_QUINTIC)
+        return s ? force_mx_fn<MDL_QUINTIC, true>(lpa) : force_mx_fn<MDL_QUINTIC, false>(lpa);
+    return s ? force_mx_fn<MDL_BSPLINE, true>(lpa) : force_mx_fn<MDL_BSPLINE, false>(lpa);
 }
 static DensFn pick_dens(int kind, int lpa, bool s) {
     if (kind == MDL_QUINTIC)
@@ -201,16 +196,13 @@ struct Driver {
         }
     }
 
-    // nK: blocks whose kinetic partials pK holds (gridK after k_kick2, gridB
-    // after a mixed force pass that gave the kick)
-    void launch_finish(int mode, double *rec_out, int next_sample, int nK) {
-        k_finish<<<NTOTAL, BLOCK, 0, stream>>>(gs, pA, gridA, pB, gridB, pK, nK, d_totals,
+    void launch_finish(int mode, double *rec_out, int next_sample) {
+        k_finish<<<NTOTAL, BLOCK, 0, stream>>>(gs, pA, gridA, pB, gridB, pK, gridK, d_totals,
                                                d_ticket_fin, mode, rec_out, next_sample);
     }
 
-    // kick: the mixed force pass also gives the second half kick (a step)
     void launch_forces(bool s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
-                       hipEvent_t e2 = nullptr, bool kick = false) {
+                       hipEvent_t e2 = nullptr) {
         if (e0)
             (void)hipEventRecord(e0, stream);
         if (precision == MDL_MIXED) {
@@ -219,8 +211,8 @@ struct Driver {
             fa<<<gridA, BLOCK, 0, stream>>>(a, l, rtf, ftf, N, npair, ntypes, rc2A, gs, pA);
             if (e1)
                 (void)hipEventRecord(e1, stream);
-            ForceMxFn fb = pick_force_mx(kind, lpa, s, kick);
-            fb<<<gridB, BLOCK, 0, stream>>>(a, l, rtf, N, npair, (float)(rmin * 1e8), gs, pB, pK);
+            ForceMxFn fb = pick_force_mx(kind, lpa, s);
+            fb<<<gridB, BLOCK, 0, stream>>>(a, l, rtf, N, npair, (float)(rmin * 1e8), gs, pB);
             if (e2)
                 (void)hipEventRecord(e2, stream);
             return;
@@ -248,19 +240,15 @@ struct Driver {
             launch_build();
         else
             launch_list();
-        // the mixed force pass gives the second half kick itself
-        bool const fused = precision == MDL_MIXED;
         if (tev)
-            launch_forces(s, tev[0], tev[1], tev[2], fused);
+            launch_forces(s, tev[0], tev[1], tev[2]);
         else
-            launch_forces(s, nullptr, nullptr, nullptr, fused);
-        if (!fused) {
-            if (s)
-                k_kick2<true><<<gridK, BLOCK, 0, stream>>>(a, N, gs, 0, has_eam, pK);
-            else
-                k_kick2<false><<<gridK, BLOCK, 0, stream>>>(a, N, gs, 0, has_eam, pK);
-        }
-        launch_finish(1, rec_out, next_sample, fused ? gridB : gridK);
+            launch_forces(s);
+        if (s)
+            k_kick2<true><<<gridK, BLOCK, 0, stream>>>(a, N, gs, 0, has_eam, pK);
+        else
+            k_kick2<false><<<gridK, BLOCK, 0, stream>>>(a, N, gs, 0, has_eam, pK);
+        launch_finish(1, rec_out, next_sample);
         ++cycle_host;
         return hipGetLastError();
     }
@@ -807,7 +795,7 @@ static mdl_status create(const mdl_config *c, Driver &d) {
             return st;
     d.launch_forces(true);
     k_kick2<true><<<d.gridK, BLOCK, 0, d.stream>>>(d.a, N, d.gs, 1, d.has_eam, d.pK);
-    d.launch_finish(0, nullptr, -1, d.gridK);
+    d.launch_finish(0, nullptr, -1);
     MDL_HIP(hipGetLastError());
     if (mdl_status st = d.sync_state())
         return st;
@@ -1303,14 +1291,25 @@ __global__ __launch_bounds__(BLOCK) void k_density_rep(const RepArgs *__restrict
                                        r.gs, r.pA, Blk{m.y, r.gridA});
 }
 
-// the force pass with the second half kick, as a mixed driver's own step
 template <int KIND, int LPA, bool SAMPLE>
 __global__ __launch_bounds__(BLOCK) void k_force_rep(const RepArgs *__restrict__ R,
                                                     const int2 *__restrict__ map) {
     int2 const m = map[blockIdx.x];
     RepArgs const &r = R[m.x];
-    force_mx_body<KIND, LPA, SAMPLE, true>(r.a, r.l, r.rt, r.N, r.npair, r.rmin, r.gs, r.pB,
-                                           r.pK, Blk{m.y, r.gridB});
+    force_mx_body<KIND, LPA, SAMPLE>(r.a, r.l, r.rt, r.N, r.npair, r.rmin, r.gs, r.pB,
+                                     Blk{m.y, r.gridB});
+}
+
+// `init` and `mode` stay kernel arguments as in k_kick2 and k_finish: a
+// literal would let the compiler fold the branches they select and contract
+// the remaining sums differently, and the kinetic sums would then differ from
+// the replica's own launches in the last bit.
+template <bool SAMPLE>
+__global__ __launch_bounds__(BLOCK) void k_kick2_rep(const RepArgs *__restrict__ R,
+                                                    const int2 *__restrict__ map, int init) {
+    int2 const m = map[blockIdx.x];
+    RepArgs const &r = R[m.x];
+    kick2_body<SAMPLE>(r.a, r.N, r.gs, init, r.has_eam, r.pK, Blk{m.y, r.gridK});
 }
 
 // The end of the step in two launches: the totals (one block per component
@@ -1321,14 +1320,12 @@ __global__ __launch_bounds__(BLOCK) void k_finish_sum_rep(const RepArgs *__restr
     if (r.gs->err)
         return;
     double const t =
-        finish_sum(r.gs, r.pA, r.gridA, r.pB, r.gridB, r.pK, r.gridB, (int)blockIdx.x);
+        finish_sum(r.gs, r.pA, r.gridA, r.pB, r.gridB, r.pK, r.gridK, (int)blockIdx.x);
     if (threadIdx.x == 0)
         r.totals[blockIdx.x] = t;
 }
 
-// irec < 0: not a sampled step. `mode` stays a kernel argument as in
-// k_finish: a literal would let the compiler fold the branches it selects and
-// contract the remaining arithmetic differently from the replica's own launch.
+// irec < 0: not a sampled step
 __global__ __launch_bounds__(BLOCK) void k_finish_tail_rep(const RepArgs *__restrict__ R, int mode,
                                                           int irec, int next_sample) {
     RepArgs const &r = R[blockIdx.x];
@@ -1604,6 +1601,10 @@ static mdl_status replicas_run(std::vector<RunPlan> &plans, int64_t nsteps) {
         for (const Group &g : groups)
             pick_force_rep(g.kind, g.lpa, smp)<<<(unsigned)g.mapB.size(), BLOCK, 0, st>>>(
                 dtab, dmap + g.atB);
+        if (smp)
+            k_kick2_rep<true><<<(unsigned)mapK.size(), BLOCK, 0, st>>>(dtab, mK, 0);
+        else
+            k_kick2_rep<false><<<(unsigned)mapK.size(), BLOCK, 0, st>>>(dtab, mK, 0);
         k_finish_sum_rep<<<dim3(NTOTAL, (unsigned)n), BLOCK, 0, st>>>(dtab);
         k_finish_tail_rep<<<(unsigned)n, BLOCK, 0, st>>>(dtab, 1, smp ? (int)p0.irec : -1, ns);
         e = hipGetLastError();

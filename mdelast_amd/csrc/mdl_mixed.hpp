@@ -239,15 +239,11 @@ __global__ __launch_bounds__(BLOCK) void k_density_mx(AtomArrays a, ListArrays l
 // Born stress, and when sampled the Born monomials and Laplacian (FP32 per
 // lane). The virial is taken as the trace of the Born stress in k_post.
 // Outputs per atom in CGS; block sums in eV and Angstrom (scaled in k_post).
-// KICK: the block then gives the atoms whose forces it wrote their second
-// half kick and takes their kinetic sums into partK (k_kick2's work and
-// partials, one launch fewer per step).
 // ---------------------------------------------------------------------------
-template <int KIND, int LPA, bool SAMPLE, bool KICK>
+template <int KIND, int LPA, bool SAMPLE>
 __device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTablesF rt, int N,
                                               int npair, float rmin, DevState *__restrict__ gs,
-                                              double *__restrict__ part,
-                                              double *__restrict__ partK, Blk bk) {
+                                              double *__restrict__ part, Blk bk) {
     if (gs->err)
         return;
     using P = PolyF<KIND>;
@@ -383,34 +379,14 @@ __device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTable
     acc[PB_LAP] = lap;
     acc[PB_WRMIN] = (double)nrmin;
     block_sum_store<PB_N>(acc, part, bk.id);
-    if (KICK) {
-        // the block's groups again, one thread per atom: thread t takes atom
-        // t % APB of every LPA-th group from group t / APB on; the forces were
-        // written by this block's threads before the barrier
-        __syncthreads();
-        double const ke = gs->kick_e, kc = gs->kick_c;
-        int const has_eam = gs->has_eam;
-        double ak[PK_N];
-#pragma unroll
-        for (int k = 0; k < PK_N; ++k)
-            ak[k] = 0.;
-        for (int grp = R.first + (int)(threadIdx.x / APB) * R.step; grp < R.end;
-             grp += LPA * R.step) {
-            int const i = grp * APB + (int)(threadIdx.x % APB);
-            if (i < N)
-                kick2_atom<SAMPLE>(a, i, gs, ke, kc, false, has_eam, ak);
-        }
-        block_sum_store<PK_N>(ak, partK, bk.id);
-    }
 }
 
-template <int KIND, int LPA, bool SAMPLE, bool KICK>
+template <int KIND, int LPA, bool SAMPLE>
 __global__ __launch_bounds__(BLOCK) void k_force_mx(AtomArrays a, ListArrays l, RTablesF rt,
                                                    int N, int npair, float rmin,
                                                    DevState *__restrict__ gs,
-                                                   double *__restrict__ part,
-                                                   double *__restrict__ partK) {
-    force_mx_body<KIND, LPA, SAMPLE, KICK>(a, l, rt, N, npair, rmin, gs, part, partK, hw_blk());
+                                                   double *__restrict__ part) {
+    force_mx_body<KIND, LPA, SAMPLE>(a, l, rt, N, npair, rmin, gs, part, hw_blk());
 }
 
 }  // namespace mdl
