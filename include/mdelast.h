@@ -170,6 +170,17 @@ mdl_status mdl_replicas_run(mdl_driver *const *drivers, int32_t n, int64_t nstep
  * No counterpart in the reference (a launch-overhead measure for small
  * systems and many replicas). */
 mdl_status mdl_driver_set_graph_steps(mdl_driver *d, int32_t steps);
+/* Pair correlation of the current configuration: MDDriver::
+ * calculate_pair_correlation (mode 0: gr), calculate_pair_correlations (mode 1:
+ * gr and gr_partial) and calculate_partial_pair_correlation (mode 2: gr_partial)
+ * of md_driver.cpp:732-863. Every pair i < j whose minimum-image distance r is
+ * below rg[nbin-1] + drg/2 (drg = rg[1] - rg[0], rg cell-centred as the
+ * driver's rg) counts in bin floor(r / drg); the counts are normalised as
+ * there. gr: nbin doubles; gr_partial: S(S+1)/2 rows of nbin (species pair
+ * (a <= b) in row b(b+1)/2 + a), host memory; the array a mode does not
+ * compute is not touched. Synchronous. */
+mdl_status mdl_driver_pair_correlation(mdl_driver *d, const double *rg, int32_t nbin,
+                                       int32_t mode, double *gr, double *gr_partial);
 /* Force a rebuild before the next force evaluation (md_driver.cpp:279-298). */
 mdl_status mdl_driver_calculate_neighbors(mdl_driver *d);
 /* MDDriver::sample_observables (md_driver.hpp:163-167) */

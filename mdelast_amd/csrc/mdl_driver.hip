@@ -125,6 +125,7 @@ struct Driver {
     // timing of the pair passes
     std::vector<hipEvent_t> ev;
     hipEvent_t ev_cv[2] = {nullptr, nullptr};   // ordering with a caller's stream
+    std::vector<int64_t> type_count;             // atoms per species
     // control-variate inputs of the sampled steps of a run, written into frame
     // slots in turn (mdl_driver_set_cv_capture)
     struct CvCapture {
@@ -699,6 +700,9 @@ static mdl_status create(const mdl_config *c, Driver &d) {
         vz[i] = c->velocities[3 * i + 2];
     }
     d.init_pos.assign(c->positions, c->positions + 3 * (size_t)N);
+    d.type_count.assign((size_t)c->ntypes, 0);
+    for (int i = 0; i < N; ++i)
+        ++d.type_count[(size_t)typ[i]];
     MDL_HIP(hipMemcpy(a.pos, pos.data(), sizeof(double4) * N, hipMemcpyHostToDevice));
     MDL_HIP(hipMemcpy(a.cref, pos.data(), sizeof(double4) * N, hipMemcpyHostToDevice));
     MDL_HIP(hipMemcpy(a.vx, vx.data(), sizeof(double) * N, hipMemcpyHostToDevice));
@@ -875,6 +879,83 @@ static void cv_capture(Driver &d) {
                                            c.pos + (size_t)j * 3 * d.N);
     k_materialize<<<g, 256, 0, d.stream>>>(d.a, d.N, d.gs, MDL_A_FORCES,
                                            c.force + (size_t)j * 3 * d.N);
+}
+
+// ---------------------------------------------------------------------------
+// Pair correlation (md_driver.cpp:732-863): every pair i < j whose minimum-image
+// distance r is below rmaxg counts in bin floor(r / drg), in the total and in
+// its unordered species pair. The distance is formed as the reference forms it
+// (no contraction, correctly rounded sqrt and division), so the integer counts
+// are the reference's. Block (I, J), J >= I, takes atoms [I T, I T + T) against
+// [J T, J T + T); counts go to LDS when they fit, then to 64-bit global counters.
+// ---------------------------------------------------------------------------
+constexpr int GR_T = 256;
+
+__device__ __forceinline__ double pair_distance_exact(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+    return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+__global__ __launch_bounds__(GR_T) void k_pair_hist(const double4 *__restrict__ pos, int N,
+                                                    const DevState *__restrict__ gs, int nbin,
+                                                    double drg, double rmaxg, int want_total,
+                                                    int want_partial, int ntypes,
+                                                    unsigned long long *__restrict__ cnt,
+                                                    int use_lds) {
+    extern __shared__ unsigned hist[];   // [1 + npair][nbin] when use_lds
+    __shared__ double4 pj[GR_T];
+    int const I = blockIdx.x, J = blockIdx.y;
+    if (J < I)
+        return;
+    int const npair = ntypes * (ntypes + 1) / 2;
+    int const nh = (1 + npair) * nbin;
+    if (use_lds)
+        for (int q = threadIdx.x; q < nh; q += GR_T)
+            hist[q] = 0u;
+    int const j0 = J * GR_T;
+    if (j0 + (int)threadIdx.x < N)
+        pj[threadIdx.x] = pos[j0 + threadIdx.x];
+    __syncthreads();
+    double const L[3] = {gs->L[0], gs->L[1], gs->L[2]};
+    int const i = I * GR_T + threadIdx.x;
+    if (i < N) {
+        double4 const ci = pos[i];
+        int const ti = (int)ci.w;
+        int const jn = min(GR_T, N - j0);
+        for (int u = 0; u < jn; ++u) {
+            if (j0 + u <= i)
+                continue;
+            double4 const cj = pj[u];
+            double dx, dy, dz;
+            min_image(ci, cj, L, dx, dy, dz);
+            double const r = pair_distance_exact(dx, dy, dz);
+            if (!(r < rmaxg))
+                continue;
+            int const k = (int)floor(r / drg);
+            if (k >= nbin)
+                continue;
+            int const tj = (int)cj.w;
+            int const tmax = max(ti, tj), tmin = min(ti, tj);
+            int const slot = 1 + ((tmax * (tmax + 1)) >> 1) + tmin;
+            if (use_lds) {
+                if (want_total)
+                    atomicAdd(&hist[k], 1u);
+                if (want_partial)
+                    atomicAdd(&hist[slot * nbin + k], 1u);
+            } else {
+                if (want_total)
+                    atomicAdd(&cnt[k], 1ull);
+                if (want_partial)
+                    atomicAdd(&cnt[(size_t)slot * nbin + k], 1ull);
+            }
+        }
+    }
+    if (use_lds) {
+        __syncthreads();
+        for (int q = threadIdx.x; q < nh; q += GR_T)
+            if (hist[q])
+                atomicAdd(&cnt[q], (unsigned long long)hist[q]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1859,6 +1940,81 @@ mdl_status mdl_driver_cv_descriptors(mdl_driver *p, float rc, float mu0, float d
         MDL_HIP(hipStreamWaitEvent(d->stream, d->ev_cv[1], 0));
     } else {
         MDL_HIP(hipStreamSynchronize(d->stream));
+    }
+    return MDL_OK;
+}
+
+mdl_status mdl_driver_pair_correlation(mdl_driver *p, const double *rg, int32_t nbin,
+                                       int32_t mode, double *gr, double *gr_partial) {
+    Driver *d = D(p);
+    if (!d || !rg || nbin < 2 || mode < 0 || mode > 2)
+        return bad("mdl_driver_pair_correlation: bad arguments");
+    bool const want_total = mode != 2, want_partial = mode != 0;
+    if ((want_total && !gr) || (want_partial && !gr_partial))
+        return bad("mdl_driver_pair_correlation: missing output array");
+    double const drg = rg[1] - rg[0];
+    if (!(drg > 0.))
+        return bad("mdl_driver_pair_correlation: rg must increase");
+    double const rmaxg = rg[nbin - 1] + 0.5 * drg;   // md_driver.cpp:41
+    MDL_HIP(hipSetDevice(d->device));
+    if (d->failed)
+        return bad("the driver stopped on an earlier error");
+    int const npair = d->ntypes * (d->ntypes + 1) / 2;
+    size_t const nh = (size_t)(1 + npair) * nbin;
+    unsigned long long *cnt = nullptr;
+    MDL_HIP(hipMalloc(&cnt, nh * sizeof(unsigned long long)));
+    std::vector<unsigned long long> h(nh);
+    size_t const lds = nh * sizeof(unsigned);
+    int const use_lds = lds <= 48 * 1024 ? 1 : 0;
+    int const T = (d->N + GR_T - 1) / GR_T;
+    hipError_t e = hipMemsetAsync(cnt, 0, nh * sizeof(unsigned long long), d->stream);
+    if (e == hipSuccess) {
+        k_pair_hist<<<dim3(T, T), GR_T, use_lds ? lds : 0, d->stream>>>(
+            d->a.pos, d->N, d->gs, nbin, drg, rmaxg, want_total ? 1 : 0, want_partial ? 1 : 0,
+            d->ntypes, cnt, use_lds);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h.data(), cnt, nh * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, d->stream);
+    double L[3] = {0., 0., 0.};
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(L, d->gs->L, sizeof L, hipMemcpyDeviceToHost, d->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(d->stream);
+    (void)hipFree(cnt);
+    if (e != hipSuccess)
+        return hip_fail(e, "mdl_driver_pair_correlation");
+    // normalisation of md_driver.cpp:749-751 and :786-809, in the same order
+    double const V = L[0] * (L[1] * L[2]);
+    double const Np = (double)d->N;
+    if (want_total)
+        for (int k = 0; k < nbin; ++k)
+            gr[k] = (double)h[k] * (V / (2. * M_PI * Np * Np * rg[k] * rg[k] * drg));
+    if (want_partial) {
+        std::vector<double> cnt_t(d->ntypes, 0.);
+        for (int t = 0; t < d->ntypes; ++t)
+            cnt_t[t] = (double)d->type_count[t];
+        int q = 0;
+        for (int b = 0; b < d->ntypes; ++b)
+            for (int a = 0; a <= b; ++a, ++q) {
+                double const Na = cnt_t[a], Nb = cnt_t[b];
+                double pc = 0.;
+                if (a == b)
+                    pc = Na > 1 ? Na * (Na - 1.) / 2. : 0.;
+                else
+                    pc = Na * Nb;
+                double *g = gr_partial + (size_t)q * nbin;
+                const unsigned long long *c = h.data() + (size_t)(1 + q) * nbin;
+                if (pc > 0.) {
+                    double const pref = V / (4. * M_PI * pc * drg);
+                    for (int k = 0; k < nbin; ++k)
+                        g[k] = (double)c[k] * (pref / (rg[k] * rg[k]));
+                } else {
+                    for (int k = 0; k < nbin; ++k)
+                        g[k] = 0.;
+                }
+            }
     }
     return MDL_OK;
 }

@@ -11,7 +11,8 @@ Differences a caller can see:
   ``exit(1)``); the message is the reference's FATAL text.
 * ``Morse`` and ``LennardJones`` exist and evaluate on the host, but an
   ``MDDriver`` accepts only tabulated potentials.
-* ``calculate_pair_correlation*`` raise ``NotImplementedError``.
+* ``calculate_pair_correlation*`` count the pairs on the device (the same
+  counts and normalisation as the reference's all-pairs loop).
 * extra, optional: ``MDDriver(..., precision="fp64", lanes_per_atom=0)``,
   ``MDDriver.run(...)`` (many steps in one call, records returned as an
   array), ``*_array()`` getters returning numpy arrays, ``kernel_times()``.
@@ -518,11 +519,28 @@ class MDDriver:
         self._call(nat.lib().mdl_driver_kernel_times(self.handle, nat.dptr(out)))
         return {"density_ms": out[0], "force_ms": out[1], "step_ms": out[2]}
 
-    def calculate_pair_correlation(self):
-        raise NotImplementedError("g(r) is not on the GPU path yet")
+    def _pair_correlation(self, mode):
+        rg = np.asarray(self._rg, dtype=np.float64)
+        gr = np.zeros(len(rg))
+        gp = np.zeros((len(self._gr_partial), len(rg)))
+        self._call(nat.lib().mdl_driver_pair_correlation(self.handle, nat.dptr(rg), len(rg),
+                                                          mode, nat.dptr(gr), nat.dptr(gp)))
+        if mode != 2:
+            self._gr = gr.tolist()
+        if mode != 0:
+            self._gr_partial = gp.tolist()
 
-    calculate_pair_correlations = calculate_pair_correlation
-    calculate_partial_pair_correlation = calculate_pair_correlation
+    def calculate_pair_correlation(self):
+        """g(r) of the current configuration into ``gr`` (md_driver.cpp:732-752)."""
+        self._pair_correlation(0)
+
+    def calculate_pair_correlations(self):
+        """``gr`` and the per-species-pair ``gr_partial`` (md_driver.cpp:754-810)."""
+        self._pair_correlation(1)
+
+    def calculate_partial_pair_correlation(self):
+        """``gr_partial`` only (md_driver.cpp:812-863)."""
+        self._pair_correlation(2)
 
     # -- state -------------------------------------------------------------
     def _scalars(self) -> np.ndarray:

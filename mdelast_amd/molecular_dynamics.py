@@ -10,8 +10,9 @@ Mirror of ``src/python/molecular_dynamics.py`` of mdelast v1.2 (its
   masses Q_0 = N_f tau^2 kT, Q_j = tau^2 kT, barostat mass W = N_f tau_P^2 kT, and
   the random-number order v_x, v_y, v_z -> p_eta -> p_eta(barostat) -> p_eps;
 - the step loop (``:1240-1285``): list rebuilds every ``cycles_neigbors`` steps,
-  sampling every ``sample_stride`` steps, trajectory dumps, thermal averages every
-  ``plot_freq`` steps after thermalisation. The loop runs on the device in
+  sampling every ``sample_stride`` steps, trajectory dumps, g(r) (and the partial
+  g(r) of each species pair when there are several species) every ``gr_stride``
+  steps and thermal averages every ``plot_freq`` steps after thermalisation. The loop runs on the device in
   stretches between those events (``MDDriver.run``), one Python call per stretch
   instead of one per step;
 - ``perform_thermal_average`` (``:664-1170``): the stress-fluctuation estimator of
@@ -20,8 +21,8 @@ Mirror of ``src/python/molecular_dynamics.py`` of mdelast v1.2 (its
   s = sigma^B - (N kT / V) delta in NPT; compressibility, bulk modulus and
   thermal expansion in NPT; heat capacities in NVE/NVT.
 
-Not here: g(r) (the driver does not compute it; ``gr`` stays zero), plots,
-pickle and HDF5 files (``save_json`` writes the thermal results instead).
+Not here: plots, pickle and HDF5 files (``save_json`` writes the thermal
+results instead).
 """
 
 from __future__ import annotations
@@ -292,6 +293,14 @@ class MolecularDynamics:
         self.series = Series()
         self.gr_sum = np.zeros(len(self.rg))
         self.gr_samples = 0
+        # partial g(r) only for more than one species (molecular_dynamics.py:566-576)
+        self.Ntypes_unique = len(set(int(t) for t in self.Ntypes))
+        if self.Ntypes_unique > 1:
+            g = self.driver.gr_partial
+            self.gr_partial_sum = np.zeros((len(g), len(g[0])))
+        else:
+            self.gr_partial_sum = None
+        self.gr_partial_samples = 0
 
     # -- set-up (molecular_dynamics.py:582-642) ------------------------------
     def build_fcc_lattice(self):
@@ -378,15 +387,28 @@ class MolecularDynamics:
             # run up to the next dump, thermal average or the end
             stop = min(self.cycles_end + 1, (c // self.dump_cycles + 1) * self.dump_cycles)
             t0 = self.cycles_thermalization
-            if c + 1 > t0:
-                k = (c + 1 - t0 + self.plot_freq - 1) // self.plot_freq
-                stop = min(stop, t0 + k * self.plot_freq)
-            else:
-                stop = min(stop, t0 + self.plot_freq)
+            if self.gr_stride < 1:
+                raise ValueError(f"gr_stride must be >= 1, got {self.gr_stride}")
+            for every in (self.plot_freq, self.gr_stride):
+                # the first cycle after c (and after t0) where t0 + k * every acts
+                k = max(1, (c + 1 - t0 + every - 1) // every)
+                stop = min(stop, t0 + k * every)
             rec = d.run(stop - c, self.sample_stride, rebuild, records=True)
             self._append_records(rec)
             self.thermalized = d.cycle > self.cycles_thermalization
-            if self.thermalized and (d.cycle - self.cycles_thermalization) % self.plot_freq == 0:
+            if self.thermalized and (d.cycle - t0) % self.gr_stride == 0:
+                # molecular_dynamics.py:1258-1272
+                if self.gr_partial_sum is not None:
+                    d.calculate_pair_correlations()
+                    self.gr_sum += np.array(d.gr)
+                    self.gr_samples += 1
+                    self.gr_partial_sum += np.array(d.gr_partial)
+                    self.gr_partial_samples += 1
+                else:
+                    d.calculate_pair_correlation()
+                    self.gr_sum += np.array(d.gr)
+                    self.gr_samples += 1
+            if self.thermalized and (d.cycle - t0) % self.plot_freq == 0:
                 self.perform_thermal_average()
         self.runtime = time.time() - self.runtime_begin
         return self
@@ -411,6 +433,10 @@ class MolecularDynamics:
         s.particle_density.data = list(self.Nparticles / (np.array(s.V_V0.data) * V0))
         s.particle_density.set_averages(cut, cut)
         tr.gr = self.gr_sum / self.gr_samples if self.gr_samples > 0 else np.zeros_like(self.gr_sum)
+        if self.gr_partial_sum is not None:
+            tr.gr_partial = (self.gr_partial_sum / self.gr_partial_samples
+                             if self.gr_partial_samples > 0
+                             else np.zeros_like(self.gr_partial_sum))
         if self.ensamble in (MDDriver.NVT, MDDriver.NPT):
             Tth = d.thermostat.temperature
             tr.tk_tnh_err = abs(s.temperature_kin.mean / Tth - 1.0)

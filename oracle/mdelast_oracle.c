@@ -1048,3 +1048,77 @@ long orc_md_nlist_size(const orc_md *md) {
 long orc_md_warnings(const orc_md *md, int which) {
     return which == 0 ? md->warn_virial : md->warn_rmin;
 }
+
+/* M_PI of <math.h> (not defined under -std=c99) */
+static const double PI_D = 3.14159265358979323846;
+
+/* md_driver.cpp:732-863: pair correlations over all pairs i < j whose
+ * minimum-image distance is below rmaxg = rg[nbin-1] + drg/2 (md_driver.cpp:41),
+ * counted in bin floor(r / drg); mode 0 total, 1 total and per species pair,
+ * 2 per species pair only; the array a mode does not compute is not touched. */
+int orc_md_pair_correlation(orc_md *md, const double *rg, int nbin, int mode, double *gr,
+                            double *gr_partial) {
+    if (nbin < 2 || mode < 0 || mode > 2)
+        return fail(md, 1, "pair correlation: bad arguments");
+    double const drg = rg[1] - rg[0], rmaxg = rg[nbin - 1] + 0.5 * drg;
+    int const S = md->ntypes, npair = S * (S + 1) / 2;
+    int const tot = mode != 2, part = mode != 0;
+    if (tot)
+        for (int k = 0; k < nbin; ++k)
+            gr[k] = 0.;
+    if (part)
+        for (int k = 0; k < npair * nbin; ++k)
+            gr_partial[k] = 0.;
+    for (int i = 0; i < md->N; ++i) {
+        const double *ci = md->coord + 3 * (size_t)i;
+        for (int j = i + 1; j < md->N; ++j) {
+            const double *cj = md->coord + 3 * (size_t)j;
+            double const s[3] = {ci[0] - cj[0], ci[1] - cj[1], ci[2] - cj[2]};
+            double d[3];
+            get_position(md, s, d);
+            double const dr = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            if (dr < rmaxg) {
+                int const k = (int)floor(dr / drg);
+                if (k >= nbin)
+                    continue;
+                if (tot)
+                    gr[k] += 1.;
+                if (part) {
+                    unsigned const a = md->types[i], b = md->types[j];
+                    unsigned const tmax = a > b ? a : b, tmin = a > b ? b : a;
+                    gr_partial[(size_t)(tmax * (tmax + 1) / 2 + tmin) * nbin + k] += 1.;
+                }
+            }
+        }
+    }
+    double const V = box_volume(md->box), Np = (double)md->N;
+    if (tot)
+        for (int k = 0; k < nbin; ++k)
+            gr[k] *= V / (2. * PI_D * Np * Np * rg[k] * rg[k] * drg);
+    if (part) {
+        double *cnt = calloc((size_t)S, sizeof(double));
+        for (int i = 0; i < md->N; ++i)
+            cnt[md->types[i]] += 1.;
+        int q = 0;
+        for (int b = 0; b < S; ++b)
+            for (int a = 0; a <= b; ++a, ++q) {
+                double const Na = cnt[a], Nb = cnt[b];
+                double pc = 0.;
+                if (a == b)
+                    pc = Na > 1 ? Na * (Na - 1.) / 2. : 0.;
+                else
+                    pc = Na * Nb;
+                double *g = gr_partial + (size_t)q * nbin;
+                if (pc > 0.) {
+                    double const pref = V / (4. * PI_D * pc * drg);
+                    for (int k = 0; k < nbin; ++k)
+                        g[k] *= pref / (rg[k] * rg[k]);
+                } else {
+                    for (int k = 0; k < nbin; ++k)
+                        g[k] = 0.;
+                }
+            }
+        free(cnt);
+    }
+    return 0;
+}

@@ -124,6 +124,9 @@ int orc_md_atoms(const orc_md *md, int what, double *out);
 int orc_md_chain(const orc_md *md, int which, double *p_eta, double *eta);
 /* Neighbour-list size (point[N]) and soft-warning counters. */
 long orc_md_nlist_size(const orc_md *md);
+/* md_driver.cpp:732-863 (mode 0 gr, 1 gr and gr_partial, 2 gr_partial) */
+int orc_md_pair_correlation(orc_md *md, const double *rg, int nbin, int mode, double *gr,
+                            double *gr_partial);
 long orc_md_warnings(const orc_md *md, int which);  /* 0 virial self-check, 1 r < r_min */
 
 #ifdef __cplusplus

@@ -93,6 +93,8 @@ def lib():
         L.orc_md_nlist_size.argtypes = [C.c_void_p]
         L.orc_md_warnings.restype = C.c_long
         L.orc_md_warnings.argtypes = [C.c_void_p, C.c_int]
+        L.orc_md_pair_correlation.restype = C.c_int
+        L.orc_md_pair_correlation.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, _dp, _dp]
         _lib = L
     return _lib
 
@@ -274,3 +276,13 @@ class MD:
 
     def warnings(self, which):
         return lib().orc_md_warnings(self.handle, which)
+
+    def pair_correlation(self, rg, mode=1):
+        """(gr, gr_partial [S(S+1)/2, nbin]) of md_driver.cpp:732-863; mode 0 fills
+        gr only, 2 gr_partial only (the other array is returned as zeros)."""
+        rg = np.ascontiguousarray(rg, dtype=float)
+        gr = np.zeros(len(rg))
+        gp = np.zeros((self.ntypes * (self.ntypes + 1) // 2, len(rg)))
+        self._check(lib().orc_md_pair_correlation(self.handle, _ptr(rg), len(rg), int(mode),
+                                                   _ptr(gr), _ptr(gp)))
+        return gr, gp

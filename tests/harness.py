@@ -68,7 +68,7 @@ def gpu_tables(data, interpolation):
 
 def gpu_md(cell, setup, *, interpolation="quintic", ensemble="NVE", dt=1e-15,
            neighbors_list=False, r_skin=1.0, fatal=1e3, coupling=0, lanes_per_atom=0,
-           precision="fp64"):
+           precision="fp64", rg=None):
     from mdelast_amd import md_driver_module as mdm
     pots, eams = gpu_tables(cell["data"], interpolation)
     thT, thq, thp, thNf = setup["thermostat"]
@@ -77,7 +77,8 @@ def gpu_md(cell, setup, *, interpolation="quintic", ensemble="NVE", dt=1e-15,
     th = mdm.Thermostat(thT, thq, thp, thNf)
     ba = mdm.Barostat(bP, bW, bpe, mdm.Thermostat(bT, bq, bp, bNf))
     box = mdm.Box(np.diag(cell["box"]))
-    rg = [(k + 0.5) * 0.01 * U.angstrom for k in range(10)]
+    if rg is None:
+        rg = [(k + 0.5) * 0.01 * U.angstrom for k in range(10)]
     return mdm.MDDriver(cell["pos"], setup["vel"], mdm.PotentialVector(pots),
                         mdm.EAMVector(eams), ENS[ensemble], coupling, th, ba, box, rg,
                         setup["Nf"], cells.masses(cell["data"]),

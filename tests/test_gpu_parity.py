@@ -208,3 +208,57 @@ def test_deterministic_bits():
     b = H.gpu_md(cell, setup, ensemble="NVT")
     assert np.array_equal(a.run(20, 1, 10), b.run(20, 1, 10))
     assert np.array_equal(a.forces_array(), b.forces_array())
+
+
+def _rg(nside, a):
+    """The reference caller's g(r) grid (molecular_dynamics.py:510-514), CGS."""
+    drg = a / 500.0
+    ng = int(nside * a / 2.0 / drg)
+    return np.array([(k + 0.5) * drg for k in range(ng)]) * U.angstrom
+
+
+@pytest.mark.parametrize("nside", [5, 15])
+def test_pair_correlation_vs_oracle(nside):
+    """calculate_pair_correlation(s) and calculate_partial_pair_correlation on
+    the device give the oracle's g(r) and partial g(r): the same integer pair
+    counts (the distance is formed without contraction) and normalisation.
+    5^3 cells keep the counts in LDS, 15^3 (13 500 atoms, 3 750 bins) take the
+    global-counter path."""
+    cell = cells.alloy_cell(nside, 3.6345, "ZnCu.eam.alloy", [0.0, 0.94], seed=42, disp=0.08,
+                            disp_seed=7)
+    rg = _rg(nside, 3.6345)
+    setup = H.md_setup(cell, static=True)
+    o = H.oracle_md(cell, setup)
+    og, op = o.pair_correlation(rg, mode=1)
+    g = H.gpu_md(cell, setup, rg=list(rg))
+    g.calculate_pair_correlation()
+    assert np.array_equal(np.array(g.gr), og)
+    assert not np.any(np.array(g.gr_partial))
+    g.calculate_partial_pair_correlation()
+    assert np.array_equal(np.array(g.gr_partial), op)
+    g2 = H.gpu_md(cell, setup, rg=list(rg))
+    g2.calculate_pair_correlations()
+    assert np.array_equal(np.array(g2.gr), og) and np.array_equal(np.array(g2.gr_partial), op)
+    assert og.max() > 1.0
+
+
+def test_pair_correlation_after_steps_uses_the_minimum_image():
+    """After 200 NVT steps the driver's coordinates are unwrapped; its g(r)
+    agrees with the oracle's for the wrapped positions the driver reports (a
+    pair may change bin only where the two roundings of its distance straddle
+    a bin edge)."""
+    cell = cells.zn06_cell(5)
+    setup = H.md_setup(cell, T=600.0, M=3, seed=1)
+    rg = _rg(5, 3.6345)
+    g = H.gpu_md(cell, setup, ensemble="NVT", dt=2e-15, neighbors_list=True, r_skin=1.2,
+                 rg=list(rg))
+    g.run(200, 100, 20)
+    g.calculate_pair_correlation()
+    now = dict(cell, pos=g.positions_array())
+    o = H.oracle_md(now, H.md_setup(now, static=True))
+    og, _ = o.pair_correlation(rg, mode=0)
+    N, V, drg = len(cell["pos"]), float(np.prod(cell["box"])), rg[1] - rg[0]
+    to_count = 2 * np.pi * N * N * rg ** 2 * drg / V
+    cg, co = np.rint(np.array(g.gr) * to_count), np.rint(og * to_count)
+    assert abs(cg.sum() - co.sum()) <= 2
+    assert np.abs(cg - co).max() <= 2

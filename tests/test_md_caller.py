@@ -37,7 +37,14 @@ class FakeDriver:
         self.thermostat = types.SimpleNamespace(temperature=T)
         self.barostat = types.SimpleNamespace(pressure=P)
         self.record_fn = record_fn
-        self.rebuilds, self.samples = [], []
+        self.rebuilds, self.samples, self.gr_calls = [], [], []
+        self.gr, self.gr_partial = [1.0] * 4, [[2.0] * 4 for _ in range(3)]
+
+    def calculate_pair_correlation(self):
+        self.gr_calls.append((self.cycle, "total"))
+
+    def calculate_pair_correlations(self):
+        self.gr_calls.append((self.cycle, "both"))
 
     def run(self, nsteps, sample_stride, rebuild_every, records=True):
         out = []
@@ -53,7 +60,7 @@ class FakeDriver:
 
 
 def _md(driver, *, ensamble, N, T, therm, plot_freq, end, stride=1, dump=10 ** 9, nbr=20,
-        P=1e-10 * Units.GPa):
+        P=1e-10 * Units.GPa, gr_stride=10 ** 9, partial=False):
     md = MDmod.MolecularDynamics.__new__(MDmod.MolecularDynamics)
     md.driver, md.ensamble, md.Nparticles = driver, ensamble, N
     md.temperature_thermostat, md.pressure_barostat = T, P
@@ -64,6 +71,8 @@ def _md(driver, *, ensamble, N, T, therm, plot_freq, end, stride=1, dump=10 ** 9
     md.ndensity = N / driver.box_initial.volume()
     md.series = MDmod.Series()
     md.gr_sum, md.gr_samples, md.rg = np.zeros(4), 0, np.zeros(4)
+    md.gr_stride = gr_stride
+    md.gr_partial_sum, md.gr_partial_samples = (np.zeros((3, 4)) if partial else None), 0
     md.thermalized = False
     md.runtime_begin = 0.0
     md.dumps, md.averages = [], []
@@ -87,9 +96,9 @@ def _record(L, sb, cb, T=300.0, etot=0.0):
     return r
 
 
-def _reference_schedule(therm, plot_freq, end, stride, dump, nbr):
+def _reference_schedule(therm, plot_freq, end, stride, dump, nbr, gr_stride):
     """molecular_dynamics.py:1240-1285, one step at a time."""
-    cycle, dumps, rebuilds, samples, averages = 0, [], [], [], []
+    cycle, dumps, rebuilds, samples, averages, grs = 0, [], [], [], [], []
     while cycle <= end:
         if cycle % dump == 0:
             dumps.append(cycle)
@@ -98,25 +107,36 @@ def _reference_schedule(therm, plot_freq, end, stride, dump, nbr):
         cycle += 1
         if cycle % stride == 0:
             samples.append(cycle)
+        if cycle > therm and (cycle - therm) % gr_stride == 0:
+            grs.append(cycle)
         if cycle > therm and (cycle - therm) % plot_freq == 0:
             averages.append(cycle)
-    return dumps, rebuilds, samples, averages
+    return dumps, rebuilds, samples, averages, grs
 
 
-@pytest.mark.parametrize("therm,plot_freq,end,stride,dump,nbr",
-                         [(50, 30, 140, 1, 25, 20), (40, 40, 120, 2, 50, 7), (0, 10, 35, 5, 100, 15)])
-def test_device_loop_schedule_matches_step_loop(therm, plot_freq, end, stride, dump, nbr):
+@pytest.mark.parametrize("therm,plot_freq,end,stride,dump,nbr,gr_stride",
+                         [(50, 30, 140, 1, 25, 20, 7), (40, 40, 120, 2, 50, 7, 1),
+                          (0, 10, 35, 5, 100, 15, 4)])
+@pytest.mark.parametrize("partial", [False, True])
+def test_device_loop_schedule_matches_step_loop(therm, plot_freq, end, stride, dump, nbr,
+                                                gr_stride, partial):
     L0 = 1e-7
     cb = np.eye(6)
     drv = FakeDriver(L0, MDDriver.NVT, 300.0, record_fn=lambda c: _record(L0, np.ones(6), cb))
     md = _md(drv, ensamble=MDDriver.NVT, N=100, T=300.0, therm=therm, plot_freq=plot_freq,
-             end=end, stride=stride, dump=dump, nbr=nbr)
+             end=end, stride=stride, dump=dump, nbr=nbr, gr_stride=gr_stride, partial=partial)
     md.run()
-    ref = _reference_schedule(therm, plot_freq, end, stride, dump, nbr)
+    ref = _reference_schedule(therm, plot_freq, end, stride, dump, nbr, gr_stride)
     assert md.dumps == ref[0]
     assert drv.rebuilds == ref[1]
     assert drv.samples == ref[2]
     assert md.averages == ref[3]
+    assert drv.gr_calls == [(c, "both" if partial else "total") for c in ref[4]]
+    assert md.gr_samples == len(ref[4])
+    assert np.all(md.series.thermal_results.gr == 1.0)
+    if partial:
+        assert md.gr_partial_samples == len(ref[4])
+        assert np.all(md.series.thermal_results.gr_partial == 2.0)
     assert drv.cycle == end + 1
 
 

@@ -305,3 +305,74 @@ def test_fatal_npt_volume():
         for _ in range(10):
             o.integrate()
     assert e.value.code == 6
+
+
+def _rg(nside, a):
+    """The reference caller's g(r) grid (molecular_dynamics.py:510-514), CGS."""
+    drg = a / 500.0
+    ng = int(nside * a / 2.0 / drg)
+    return np.array([(k + 0.5) * drg for k in range(ng)]) * U.angstrom
+
+
+def _np_counts(cell, rg, ntypes):
+    """Independent pair counts: numpy over all pairs i < j, the same distance
+    formula (minimum image of scaled differences, sqrt of x^2 + y^2 + z^2)."""
+    L = np.asarray(cell["box"], dtype=float)
+    s = np.asarray(cell["pos"], dtype=float) / L
+    s = s - np.where(s < 0, np.trunc(s - 0.5), np.trunc(s + 0.5))
+    N = len(s)
+    drg = rg[1] - rg[0]
+    rmax = rg[-1] + 0.5 * drg
+    tot = np.zeros(len(rg))
+    part = np.zeros((ntypes * (ntypes + 1) // 2, len(rg)))
+    t = np.asarray(cell["types"])
+    for i in range(N - 1):
+        d = s[i] - s[i + 1:]
+        n = np.where(d < 0, np.trunc(d - 0.5), np.trunc(d + 0.5))
+        x = (d - n) * L
+        r = np.sqrt(x[:, 0] * x[:, 0] + x[:, 1] * x[:, 1] + x[:, 2] * x[:, 2])
+        m = r < rmax
+        k = np.floor(r[m] / drg).astype(int)
+        ok = k < len(rg)
+        np.add.at(tot, k[ok], 1.0)
+        tj = t[i + 1:][m][ok]
+        hi, lo = np.maximum(t[i], tj), np.minimum(t[i], tj)
+        np.add.at(part, (hi * (hi + 1) // 2 + lo, k[ok]), 1.0)
+    return tot, part
+
+
+def test_pair_correlation_counts_and_normalisation():
+    """md_driver.cpp:732-863 restated: the counts equal an independent numpy
+    count of all pairs; the normalisations are the reference's closed forms;
+    the partial rows sum to the total counts; a perfect FCC lattice puts its
+    6 N nearest-neighbour pairs in the bin of a / sqrt(2)."""
+    cell = cells.alloy_cell(4, 3.6345, "ZnCu.eam.alloy", [0.0, 0.9], seed=3, disp=0.07,
+                            disp_seed=5)
+    rg = _rg(4, 3.6345)
+    o = O.MD(positions=cell["pos"], velocities=np.zeros_like(cell["pos"]), types=cell["types"],
+             mass=cells.masses(cell["data"]), box=cell["box"], data=cell["data"])
+    gr, gp = o.pair_correlation(rg, mode=1)
+    tot, part = _np_counts(cell, rg, 2)
+    N = len(cell["pos"])
+    V = float(np.prod(cell["box"]))
+    drg = rg[1] - rg[0]
+    assert np.allclose(gr * (2 * np.pi * N * N * rg ** 2 * drg) / V, tot, rtol=1e-12, atol=1e-9)
+    assert np.allclose(part.sum(0), tot)
+    Na = np.bincount(cell["types"], minlength=2).astype(float)
+    pcs = [Na[0] * (Na[0] - 1) / 2, Na[0] * Na[1], Na[1] * (Na[1] - 1) / 2]
+    for q, pc in enumerate(pcs):
+        assert np.allclose(gp[q] * (4 * np.pi * pc * rg ** 2 * drg) / V, part[q],
+                           rtol=1e-12, atol=1e-9)
+    # modes: 0 leaves gr_partial alone, 2 leaves gr alone
+    g0, p0 = o.pair_correlation(rg, mode=0)
+    g2, p2 = o.pair_correlation(rg, mode=2)
+    assert np.array_equal(g0, gr) and not p0.any()
+    assert np.array_equal(p2, gp) and not g2.any()
+    # the lattice: 6 N pairs at a / sqrt(2)
+    lat = cells.alloy_cell(4, 3.6345, "ZnCu.eam.alloy", [0.0, 1.0], seed=3)
+    ol = O.MD(positions=lat["pos"], velocities=np.zeros_like(lat["pos"]), types=lat["types"],
+              mass=cells.masses(lat["data"]), box=lat["box"], data=lat["data"])
+    gl, _ = ol.pair_correlation(rg, mode=0)
+    cl = gl * (2 * np.pi * N * N * rg ** 2 * drg) / V
+    k1 = int(3.6345 / np.sqrt(2) * U.angstrom // drg)
+    assert round(cl[k1]) == 6 * N and round(cl[:k1].sum()) == 0
