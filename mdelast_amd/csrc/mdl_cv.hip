@@ -53,7 +53,7 @@ __global__ __launch_bounds__(256) void k_cv_descriptors(const double *__restrict
     bool const active = i < N;
     int const ia = active ? i : 0;
     double const xi = pos[3 * ia], yi = pos[3 * ia + 1], zi = pos[3 * ia + 2];
-    mdl::CvAcc acc;
+    mdl::CvAcc<> acc;
     for (int j = 0; j < N; ++j) {
         double sx = xi - pos[3 * j], sy = yi - pos[3 * j + 1], sz = zi - pos[3 * j + 2];
         sx -= Lx * rint(sx / Lx);

@@ -49,10 +49,11 @@ __device__ __forceinline__ void cv_phi(const CvBasis &b, int k, float r, float &
 }
 
 // One half-wave (32 lanes, lane = basis index) accumulates the pairs of one
-// atom; store its 64 descriptors, 3 x 64 derivatives, 64 Laplacians and,
-// optionally, the Laplacian's three axis terms d^2 D / d x_a^2 =
-// (phi'' - phi'/r)(d_a/r)^2 + phi'/r (their sum is phi'' + 2 phi'/r).
-struct CvAcc {
+// atom; store its 64 descriptors, 3 x 64 derivatives, 64 Laplacians and, with
+// AXES, the Laplacian's three axis terms d^2 D / d x_a^2 =
+// (phi'' - phi'/r)(d_a/r)^2 + phi'/r (their sum is phi'' + 2 phi'/r, which is
+// all that is accumulated without AXES).
+template <bool AXES = false> struct CvAcc {
     float d[2], jx[2], jy[2], jz[2], lx[2], ly[2], lz[2];
 
     __device__ __forceinline__ CvAcc() {
@@ -68,9 +69,6 @@ struct CvAcc {
         int const s = species != 0;
         // two accumulators selected without dynamic register indexing
         float const m0 = s ? 0.f : 1.f, m1 = 1.f - m0;
-        float const ir2 = __builtin_amdgcn_rcpf(r * r);
-        float const q = (d2 - w) * ir2;
-        float const ax = fmaf(q, x * x, w), ay = fmaf(q, y * y, w), az = fmaf(q, z * z, w);
         d[0] += m0 * phi;
         d[1] += m1 * phi;
         jx[0] += m0 * w * x;
@@ -79,12 +77,21 @@ struct CvAcc {
         jy[1] += m1 * w * y;
         jz[0] += m0 * w * z;
         jz[1] += m1 * w * z;
-        lx[0] += m0 * ax;
-        lx[1] += m1 * ax;
-        ly[0] += m0 * ay;
-        ly[1] += m1 * ay;
-        lz[0] += m0 * az;
-        lz[1] += m1 * az;
+        if (AXES) {
+            float const ir2 = __builtin_amdgcn_rcpf(r * r);
+            float const q = (d2 - w) * ir2;
+            float const ax = fmaf(q, x * x, w), ay = fmaf(q, y * y, w), az = fmaf(q, z * z, w);
+            lx[0] += m0 * ax;
+            lx[1] += m1 * ax;
+            ly[0] += m0 * ay;
+            ly[1] += m1 * ay;
+            lz[0] += m0 * az;
+            lz[1] += m1 * az;
+        } else {
+            float const l = fmaf(2.f, w, d2);   // phi'' + 2 phi'/r
+            lx[0] += m0 * l;
+            lx[1] += m1 * l;
+        }
     }
 
     __device__ __forceinline__ void store(int i, int k, int N, float *D, float *J, float *L,
@@ -99,8 +106,8 @@ struct CvAcc {
             J[plane + o] = jy[s];
             J[2 * plane + o] = jz[s];
             if (L)
-                L[o] = lx[s] + ly[s] + lz[s];
-            if (La) {
+                L[o] = AXES ? lx[s] + ly[s] + lz[s] : lx[s];
+            if (AXES && La) {
                 La[o] = lx[s];
                 La[plane + o] = ly[s];
                 La[2 * plane + o] = lz[s];

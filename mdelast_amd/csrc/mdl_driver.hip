@@ -814,6 +814,7 @@ using namespace mdl;
 // that pass A of the last force evaluation kept within the potential's
 // cut-off (cnbr; use_full: the full list, before any force evaluation).
 // One wave per two atoms, lane k%32 = radial function k; lengths in A.
+template <bool AXES>
 __global__ __launch_bounds__(256) void k_cv_desc_list(AtomArrays a, ListArrays l, int N,
                                                       const DevState *__restrict__ gs,
                                                       double to_A, CvBasis basis, int use_full,
@@ -829,7 +830,7 @@ __global__ __launch_bounds__(256) void k_cv_desc_list(AtomArrays a, ListArrays l
     double4 const ci = a.pos[ia];
     int const cnt = active ? (use_full ? l.nnbr[ia] : l.ccnt[ia]) : 0;
     const int *nb = (use_full ? l.nbr : l.cnbr) + (size_t)ia * l.stride;
-    CvAcc acc;
+    CvAcc<AXES> acc;
     // the 32 lanes of an atom gather 32 pairs at once (one load round trip per
     // 32 pairs), then take them one by one from the lane that holds each, in
     // list order
@@ -871,7 +872,7 @@ static void cv_capture(Driver &d) {
     ++c.next;
     size_t const nf = (size_t)d.N * 2 * CV_NB;
     int const waves = (d.N + 1) / 2;
-    k_cv_desc_list<<<(waves * 64 + 255) / 256, 256, 0, d.stream>>>(
+    k_cv_desc_list<false><<<(waves * 64 + 255) / 256, 256, 0, d.stream>>>(
         d.a, d.l, d.N, d.gs, 1e8, c.basis, c.use_full, c.D + j * nf, c.J + j * 3 * nf,
         c.L + j * nf, nullptr);
     unsigned const g = (d.N + 255) / 256;
@@ -1923,8 +1924,14 @@ mdl_status mdl_driver_cv_descriptors(mdl_driver *p, float rc, float mu0, float d
     int const use_full =
         (d->cycle_host == 0 || (double)rc * rc * 1e-16 > d->rc2 * (1.0 + 1e-6)) ? 1 : 0;
     int const waves = (d->N + 1) / 2;
-    k_cv_desc_list<<<(waves * 64 + 255) / 256, 256, 0, st>>>(
-        d->a, d->l, d->N, d->gs, 1e8, cv_basis(rc, mu0, dmu, width), use_full, Dp, Jp, Lp, La);
+    if (La)
+        k_cv_desc_list<true><<<(waves * 64 + 255) / 256, 256, 0, st>>>(
+            d->a, d->l, d->N, d->gs, 1e8, cv_basis(rc, mu0, dmu, width), use_full, Dp, Jp, Lp,
+            La);
+    else
+        k_cv_desc_list<false><<<(waves * 64 + 255) / 256, 256, 0, st>>>(
+            d->a, d->l, d->N, d->gs, 1e8, cv_basis(rc, mu0, dmu, width), use_full, Dp, Jp, Lp,
+            nullptr);
     MDL_HIP(hipGetLastError());
     unsigned const g = (d->N + 255) / 256;
     if (pos_out) {

@@ -724,9 +724,22 @@ class ControlVariate:
             self.coef = self._fit_linear(X, yc)
             var_net = (yv - self.controls(valid, enabled_only=False)).var(0)
             say(f"learned fields: held-out variance / plain {np.round(var_net / var_plain, 4)}")
-        # the learned fields cost a network pass per frame: kept for a head
-        # only where they cut its held-out variance by FIELD_GAIN or more
+        # The learned fields cost a network pass per frame, shared by all heads,
+        # and the head with the largest controlled/plain variance ratio sets the
+        # cost of reaching the target on all three constants. A head keeps its
+        # fields where they cut its held-out variance by FIELD_GAIN or more, and
+        # the network stays only if that cuts the worst ratio by FIELD_GAIN too.
         use_fields = var_net < (1.0 - FIELD_GAIN) * var_glob
+        on_glob = var_glob < var_plain
+        on_net = np.where(use_fields, var_net, var_glob) < var_plain
+
+        def worst(v, on):
+            r = np.where(on, v / np.maximum(var_plain, 1e-300), 1.0)
+            return float(r.max())
+
+        if worst(np.where(use_fields, var_net, var_glob), on_net) >= \
+                (1.0 - FIELD_GAIN) * worst(var_glob, on_glob):
+            use_fields[:] = False
         coef = self.coef.copy()
         for h in range(3):
             if not use_fields[h]:

@@ -279,7 +279,9 @@ def test_stein_identity_and_fallback_on_md_frames():
 def test_learned_fields_are_kept_only_where_they_help():
     """The fit measures, per head and on held-out frames, the plain variance,
     the variance with the global fields and the variance with the learned
-    fields added, and keeps the lowest: the variate never does worse on the
+    fields added. A head keeps its learned fields only where they cut its
+    variance by 5 %, and the network only if that cuts the worst head's
+    controlled/plain ratio by 5 %: the variate never does worse on the
     held-out frames than the global fields alone or than plain."""
     print("brass 500 atoms: network gain", flush=True)
     drv, sites, types = cv.brass_driver(edge=5, vel_seed=17)
@@ -291,9 +293,17 @@ def test_learned_fields_are_kept_only_where_they_help():
     print({k: v for k, v in info.items() if k != "history"}, flush=True)
     vp, vg, vf, vc = (np.asarray(info[k]) for k in ("var_plain_valid", "var_global_valid",
                                                     "var_fields_valid", "var_cv_valid"))
-    assert np.all(vc <= vg) and np.all(vc == np.minimum(vg, vf))
-    assert np.array_equal(info["enabled"], vc < vp)
-    assert np.array_equal(info["uses_learned_fields"], (vf < vg) & (vc < vp))
+    used = np.asarray(info["uses_learned_fields"])
+    on = np.asarray(info["enabled"])
+    assert np.all(vc <= vg)
+    assert np.array_equal(on, vc < vp)
+    assert np.array_equal(vc[on], np.where(used, vf, vg)[on])
+    assert np.all(vf[used] < 0.95 * vg[used])
+    if used.any():
+        def worst(v):
+            return np.max(np.where(v < vp, v / vp, 1.0))
+        assert worst(vc) < 0.95 * worst(vg)
+    assert cvm.use_net == bool(used.any())
     # the kept controls reproduce the held-out variance the fit reported
     s = cvm.controls(valid, enabled_only=False)
     assert np.allclose((valid.y() - s).var(0), vc, rtol=1e-6)
