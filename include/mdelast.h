@@ -43,8 +43,9 @@ enum {
     MDL_ERR_TABLE = 14        /* invalid table (poly_table.hpp:39-44) */
 };
 
-/* Table interpolation, poly_table.hpp:140-152 */
-enum { MDL_QUINTIC = 0, MDL_BSPLINE = 1 };
+/* Table interpolation, poly_table.hpp:140-152; analytic pair forms,
+ * potential.hpp:29-75 (Morse, LennardJones) */
+enum { MDL_QUINTIC = 0, MDL_BSPLINE = 1, MDL_MORSE = 2, MDL_LJ = 3 };
 /* MDDriver::Ensamble and ThermostatCoupling, md_driver.hpp:20-21 */
 enum { MDL_NVE = 0, MDL_NVT = 1, MDL_NPT = 2, MDL_NPT_FLEX = 3 };
 enum { MDL_KINETIC = 0, MDL_CONFIGURATIONAL = 1, MDL_AVERAGE = 2 };
@@ -70,6 +71,17 @@ typedef struct mdl_table mdl_table;
  * Host-only: no device memory is touched until a driver or eval uses it. */
 mdl_status mdl_table_create(int32_t kind, double h, int64_t n_nodes, const double *f,
                             const double *f1, const double *f2, mdl_table **out);
+/* An analytic pair potential as a table handle. Replaces potential::Morse and
+ * potential::LennardJones (potential.hpp:29-75, bound at md_driver_module.cpp:45-56).
+ * MDL_MORSE: param = {D, alpha, r0}, V = D (e^2 - 2 e), e = exp(-alpha (r - r0)).
+ * MDL_LJ: param = {epsilon, sigma, rcut}, V = 4 epsilon ((s/r)^12 - (s/r)^6) - vcut
+ * with vcut the unshifted value at rcut. Neither cuts at rcut here: the driver
+ * takes the cut-off from mdl_config.pair_rcut, as for tables. A driver accepts
+ * these handles as its pair potentials (all of one form) when has_eam = 0, in
+ * MDL_FP64; mdl_table_eval evaluates them; mdl_table_coefficients returns one
+ * record of stride 4, the device parameters (Morse {D, alpha, r0, 0}, LJ
+ * {epsilon, sigma, vcut, 0}). */
+mdl_status mdl_table_create_analytic(int32_t kind, const double *param, mdl_table **out);
 mdl_status mdl_table_destroy(mdl_table *t);
 /* Host copy of the per-interval polynomial coefficients: quintic [n][6]
  * (value polynomial in t); B-spline [n][12] (cubics of f, f', f'' in t, 4 each).

@@ -13,7 +13,7 @@ import numpy as np
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmdelast_hip.so")
 
-QUINTIC, BSPLINE = 0, 1
+QUINTIC, BSPLINE, MORSE, LJ = 0, 1, 2, 3
 NVE, NVT, NPT, NPT_FLEX = 0, 1, 2, 3
 KINETIC, CONFIGURATIONAL, AVERAGE = 0, 1, 2
 FP64, MIXED = 0, 1
@@ -76,6 +76,7 @@ SIGNATURES = {
     "mdl_device_info": (C.c_int32, [C.c_char_p, C.c_int32, C.POINTER(C.c_int32)]),
     "mdl_table_create": (C.c_int32, [C.c_int32, C.c_double, C.c_int64, _dp, _dp, _dp,
                                      C.POINTER(_vp)]),
+    "mdl_table_create_analytic": (C.c_int32, [C.c_int32, _dp, C.POINTER(_vp)]),
     "mdl_table_destroy": (C.c_int32, [_vp]),
     "mdl_table_coefficients": (C.c_int32, [_vp, _dp, C.POINTER(C.c_int64),
                                            C.POINTER(C.c_int32)]),

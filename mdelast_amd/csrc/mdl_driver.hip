@@ -82,6 +82,11 @@ static DensFn pick_dens(int kind, int lpa, bool s) {
     return s ? dens_fn<MDL_BSPLINE, true>(lpa) : dens_fn<MDL_BSPLINE, false>(lpa);
 }
 static ForceFn pick_force(int kind, int lpa, bool s, bool e) {
+    // analytic pair forms run without EAM (checked at creation)
+    if (kind == MDL_MORSE)
+        return s ? force_fn<MDL_MORSE, true, false>(lpa) : force_fn<MDL_MORSE, false, false>(lpa);
+    if (kind == MDL_LJ)
+        return s ? force_fn<MDL_LJ, true, false>(lpa) : force_fn<MDL_LJ, false, false>(lpa);
     if (kind == MDL_QUINTIC) {
         if (e)
             return s ? force_fn<MDL_QUINTIC, true, true>(lpa) : force_fn<MDL_QUINTIC, false, true>(lpa);
@@ -480,6 +485,11 @@ static mdl_status create(const mdl_config *c, Driver &d) {
         if (!t)
             return bad("mdl_driver_create: null table");
     d.kind = rtabs[0]->kind;
+    if (analytic_kind(d.kind) && d.has_eam) {
+        set_error("analytic pair potentials (Morse, Lennard-Jones) run without EAM tables "
+                  "on the GPU path");
+        return MDL_ERR_UNSUPPORTED;
+    }
     for (auto *t : rtabs)
         if (t->kind != d.kind || t->h != rtabs[0]->h || t->n != rtabs[0]->n) {
             set_error("the GPU path needs all r-tables on one grid with one interpolation");
@@ -510,10 +520,11 @@ static mdl_status create(const mdl_config *c, Driver &d) {
             rhomax[t] = c->eam_rhomax[t];
         }
     }
-    // interleaved r records: [interval][function][S]
+    // interleaved r records: [interval][function][S]; analytic forms hold one
+    // record of PAIR_PARAMS parameters per pair slot
     int const S = rtabs[0]->stride;
     int const nf = (int)rtabs.size();
-    int64_t const nr = rtabs[0]->n;
+    int64_t const nr = analytic_kind(d.kind) ? 1 : rtabs[0]->n;
     std::vector<double> rrec((size_t)nr * nf * S);
     for (int64_t i = 0; i < nr; ++i)
         for (int f = 0; f < nf; ++f)

@@ -188,6 +188,40 @@ template <> struct Poly<MDL_BSPLINE> {
     }
 };
 
+// Analytic pair forms. The device record of one pair slot is 4 doubles:
+// Morse {D, alpha, r0, 0}, Lennard-Jones {epsilon, sigma, vcut, 0}.
+constexpr int PAIR_PARAMS = 4;
+constexpr bool analytic_kind(int kind) { return kind == MDL_MORSE || kind == MDL_LJ; }
+
+// (s)^6 by binary powering, as boost::math::pow<6> forms it: ((s s) s)^2
+__host__ __device__ __forceinline__ double pow6_binary(double s) {
+    double const s3 = s * s * s;
+    return s3 * s3;
+}
+
+// V, dV/dr, d2V/dr2 of the analytic forms inside the cut-off: Morse
+// potential.hpp:40-49, Lennard-Jones potential.hpp:65-73 (same products, in
+// the same order).
+template <int KIND>
+__host__ __device__ __forceinline__ void pair_analytic(const double *p, double r, double &V,
+                                                       double &dV, double &d2V) {
+    if constexpr (KIND == MDL_MORSE) {
+        double const D = p[0], alpha = p[1], r0 = p[2];
+        double const e1 = exp(-alpha * (r - r0));
+        double const e2 = e1 * e1;
+        V = D * (e2 - 2. * e1);
+        dV = -2. * D * alpha * (e2 - e1);
+        d2V = 2. * alpha * alpha * D * (2. * e2 - e1);
+    } else {
+        double const eps = p[0], sigma = p[1], vcut = p[2];
+        double const v6 = pow6_binary(sigma / r);
+        double const v12 = v6 * v6;
+        V = 4. * eps * (v12 - v6) - vcut;
+        dV = 24. * eps * (v6 - 2. * v12) / r;
+        d2V = 24. * eps * (26. * v12 - 7. * v6) / (r * r);
+    }
+}
+
 // Round half away from zero (md_driver.hpp:48-50): trunc(x -+ 0.5)
 __device__ __forceinline__ double nint_away(double x) {
     return x < 0. ? trunc(x - 0.5) : trunc(x + 0.5);

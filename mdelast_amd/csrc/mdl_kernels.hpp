@@ -1242,7 +1242,11 @@ __global__ __launch_bounds__(BLOCK) void k_force(AtomArrays a, ListArrays l, RTa
                                                 double *__restrict__ part) {
     if (gs->err)
         return;
-    using P = Poly<KIND>;
+    // analytic pair forms (Morse, Lennard-Jones) evaluate in place of the
+    // pair table; they run without EAM
+    constexpr bool ANALYTIC = analytic_kind(KIND);
+    static_assert(!(ANALYTIC && EAM), "analytic pair forms run without EAM tables");
+    using P = Poly<ANALYTIC ? MDL_QUINTIC : KIND>;
     constexpr int APB = BLOCK / LPA;
     int const lane = threadIdx.x % LPA, slot = threadIdx.x / LPA;
     double const L[3] = {gs->L[0], gs->L[1], gs->L[2]};
@@ -1275,17 +1279,24 @@ __global__ __launch_bounds__(BLOCK) void k_force(AtomArrays a, ListArrays l, RTa
                 int const pslot = tmax * (tmax + 1) / 2 + tmin;
                 double const r = sqrt(r2);
                 nrmin += r < rmin;
-                double t;
-                int const it = tab_locate(r, rt.inv_h, rt.n, t);
-                const double *base = rt.rec + (size_t)it * rt.nf * P::S;
+                double t = 0.;
+                const double *base = rt.rec;
+                if constexpr (!ANALYTIC) {
+                    int const it = tab_locate(r, rt.inv_h, rt.n, t);
+                    base += (size_t)it * rt.nf * P::S;
+                }
                 double V = 0., dV = 0., d2V = 0.;
-                if (!(r > rc_pair)) {   // potential.hpp:97-103
-                    P pp;
-                    pp.load(base + pslot * P::S);
-                    V = pp.v(t);
-                    dV = pp.d1(t, rt.inv_h);
-                    if (SAMPLE)
-                        d2V = pp.d2(t, rt.inv_h2);
+                if (!(r > rc_pair)) {   // potential.hpp:36-37, 62-63, 97-103
+                    if constexpr (ANALYTIC) {
+                        pair_analytic<KIND>(rt.rec + pslot * PAIR_PARAMS, r, V, dV, d2V);
+                    } else {
+                        P pp;
+                        pp.load(base + pslot * P::S);
+                        V = pp.v(t);
+                        dV = pp.d1(t, rt.inv_h);
+                        if (SAMPLE)
+                            d2V = pp.d2(t, rt.inv_h2);
+                    }
                 }
                 double const ir = 1. / r;
                 double dedr = dV, X = 0.;

@@ -107,6 +107,17 @@ __global__ void k_table_eval(int kind, const double *rec, int n, double inv_h, d
     int64_t const k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= m)
         return;
+    if (kind == MDL_MORSE || kind == MDL_LJ) {
+        double V, dV, d2V;
+        if (kind == MDL_MORSE)
+            pair_analytic<MDL_MORSE>(rec, x[k], V, dV, d2V);
+        else
+            pair_analytic<MDL_LJ>(rec, x[k], V, dV, d2V);
+        v[k] = V;
+        d1[k] = dV;
+        d2[k] = d2V;
+        return;
+    }
     double t;
     int const i = tab_locate(x[k], inv_h, n, t);
     if (kind == MDL_QUINTIC) {
@@ -188,6 +199,43 @@ mdl_status mdl_table_create(int32_t kind, double h, int64_t n_nodes, const doubl
             for (int64_t i = 0; i < t->n; ++i)
                 bspline_cubic(beta, avg, i, t->rec.data() + 12 * i + 4 * s);
         }
+    }
+    *out = reinterpret_cast<mdl_table *>(t.release());
+    return MDL_OK;
+}
+
+mdl_status mdl_table_create_analytic(int32_t kind, const double *param, mdl_table **out) {
+    if (!out || !param) {
+        set_error("mdl_table_create_analytic: null argument");
+        return MDL_ERR_ARG;
+    }
+    *out = nullptr;
+    if (kind != MDL_MORSE && kind != MDL_LJ) {
+        set_error("mdl_table_create_analytic: unknown analytic form %d (use Morse or LJ)", kind);
+        return MDL_ERR_ARG;
+    }
+    for (int k = 0; k < 3; ++k)
+        if (!std::isfinite(param[k])) {
+            set_error("mdl_table_create_analytic: parameter %d is not finite", k);
+            return MDL_ERR_ARG;
+        }
+    if (kind == MDL_LJ && !(param[2] > 0.)) {
+        set_error("mdl_table_create_analytic: Lennard-Jones rcut must be > 0 (got %g)", param[2]);
+        return MDL_ERR_ARG;
+    }
+    auto t = std::make_unique<HostTable>();
+    t->kind = kind;
+    t->n = 1;   // one record: the parameters
+    t->stride = PAIR_PARAMS;
+    t->rec.assign(PAIR_PARAMS, 0.);
+    t->rec[0] = param[0];
+    t->rec[1] = param[1];
+    if (kind == MDL_MORSE) {
+        t->rec[2] = param[2];
+    } else {
+        // vcut, potential.hpp:57-59: 4 epsilon ((s/rc)^12 - (s/rc)^6)
+        double const v6 = pow6_binary(param[1] / param[2]);
+        t->rec[2] = 4. * param[0] * (v6 * v6 - v6);
     }
     *out = reinterpret_cast<mdl_table *>(t.release());
     return MDL_OK;

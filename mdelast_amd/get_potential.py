@@ -40,3 +40,9 @@ def get_EAM_alloy_file(file_name, interpolation=None):
                 for e in data.elements]
     potentials, eams = tables_from_setfl(data, interpolation)
     return elements, potentials, eams
+
+
+def get_Ar_LJ(*, epsilon=None, sigma=None, rcut_fac=3.5):
+    """Argon Lennard-Jones ``Material`` (get_potential.py:260-273)."""
+    from .molecular_dynamics import get_Ar_LJ as _get
+    return _get(epsilon=epsilon, sigma=sigma, rcut_fac=rcut_fac)

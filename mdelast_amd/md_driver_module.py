@@ -9,8 +9,8 @@ Differences a caller can see:
 
 * FATAL conditions raise :class:`MDDriverError` (the reference prints and calls
   ``exit(1)``); the message is the reference's FATAL text.
-* ``Morse`` and ``LennardJones`` exist and evaluate on the host, but an
-  ``MDDriver`` accepts only tabulated potentials.
+* ``Morse`` and ``LennardJones`` run in an ``MDDriver`` as pair potentials
+  without EAM, in FP64 (the reference also allows them beside EAM tables).
 * ``calculate_pair_correlation*`` count the pairs on the device (the same
   counts and normalisation as the reference's all-pairs loop).
 * extra, optional: ``MDDriver(..., precision="fp64", lanes_per_atom=0)``,
@@ -153,13 +153,41 @@ class Potential:
         raise NotImplementedError
 
 
+def _pow6(s: float) -> float:
+    """s^6 by binary powering, ((s s) s)^2, as boost::math::pow<6> forms it."""
+    s3 = s * s * s
+    return s3 * s3
+
+
+class _AnalyticTable(_Table):
+    """An analytic pair form as an mdl_table handle (mdl_table_create_analytic):
+    the driver evaluates the formula on the GPU in place of a table lookup."""
+
+    def __init__(self, kind: int, params):
+        self.name = {nat.MORSE: "morse", nat.LJ: "lennard_jones"}[kind]
+        self.h, self.n_nodes = 0.0, 0
+        p = np.ascontiguousarray(params, dtype=float)
+        h_ = C.c_void_p()
+        nat.check(nat.lib().mdl_table_create_analytic(kind, nat.dptr(p), C.byref(h_)))
+        self.handle = h_
+
+
 class Morse(Potential):
-    """Analytic Morse pair potential (potential.hpp:29-51). Host evaluation only;
-    the GPU driver takes tabulated potentials."""
+    """Analytic Morse pair potential (potential.hpp:29-51). ``calculate`` evaluates
+    on the host as the reference does; an ``MDDriver`` evaluates the same formula
+    on the GPU (pair potentials without EAM, FP64)."""
 
     def __init__(self, rcut, D, alpha, r0):
         super().__init__(rcut)
         self.D, self.alpha, self.r0 = float(D), float(alpha), float(r0)
+        self.rmin_fatal = 0.0
+        self._table = None
+
+    @property
+    def table(self) -> _Table:
+        if self._table is None:
+            self._table = _AnalyticTable(nat.MORSE, [self.D, self.alpha, self.r0])
+        return self._table
 
     def calculate(self, r):
         if r > self.rcut:
@@ -172,17 +200,27 @@ class Morse(Potential):
 
 class LennardJones(Potential):
     """Analytic Lennard-Jones pair potential, shifted to 0 at rcut (potential.hpp:53-75).
-    Host evaluation only; the GPU driver takes tabulated potentials."""
+    ``calculate`` evaluates on the host as the reference does; an ``MDDriver``
+    evaluates the same formula on the GPU (pair potentials without EAM, FP64)."""
 
     def __init__(self, rcut, epsilon, sigma):
         super().__init__(rcut)
         self.epsilon, self.sigma = float(epsilon), float(sigma)
-        self.vcut = 4.0 * self.epsilon * ((self.sigma / self.rcut) ** 12 - (self.sigma / self.rcut) ** 6)
+        v6 = _pow6(self.sigma / self.rcut)
+        self.vcut = 4.0 * self.epsilon * (v6 * v6 - v6)
+        self.rmin_fatal = 0.0
+        self._table = None
+
+    @property
+    def table(self) -> _Table:
+        if self._table is None:
+            self._table = _AnalyticTable(nat.LJ, [self.epsilon, self.sigma, self.rcut])
+        return self._table
 
     def calculate(self, r):
         if r > self.rcut:
             return PotentialValue(0.0, 0.0, 0.0)
-        v6 = (self.sigma / r) ** 6
+        v6 = _pow6(self.sigma / r)
         v12 = v6 * v6
         return PotentialValue(4.0 * self.epsilon * (v12 - v6) - self.vcut,
                               24.0 * self.epsilon * (v6 - 2.0 * v12) / r,
@@ -380,9 +418,8 @@ class MDDriver:
         if eams and len(eams) != ntypes:
             raise ValueError(f"{len(eams)} EAM objects for {ntypes} species")
         for p in pots:
-            if not isinstance(p, TablePotential):
-                raise MDDriverError(12, f"{type(p).__name__}: the GPU driver takes "
-                                        "TablePotential pair potentials only")
+            if not isinstance(p, (TablePotential, Morse, LennardJones)):
+                raise TypeError(f"{type(p).__name__} is not a pair potential")
         hb = box.h if isinstance(box, Box) else np.asarray(box, dtype=float)
         if np.any(hb != np.diag(np.diag(hb))):
             raise MDDriverError(12, "the box must be diagonal (md_driver.cpp:60)")

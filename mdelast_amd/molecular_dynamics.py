@@ -161,6 +161,37 @@ class Material:
             self.box_length = box_length
 
 
+def get_Cu_Morse(a_factor=1.0):
+    """Copper as the analytic Morse pair potential (molecular_dynamics.py:259-275):
+    D = 0.3429 eV, alpha = 1.3588 / A, r0 = 2.866 A, cut at 10/alpha + r0."""
+    from .md_driver_module import Morse
+    D = 0.3429 * Units.ev
+    alpha = 1.3588 / Units.angstrom
+    r0 = 2.866 * Units.angstrom
+    return Material(potential=Morse(10.0 / alpha + r0, D, alpha, r0), atomic_weight=63.546,
+                    atom_name="Cu", a_lattice=a_factor * 3.615 * Units.angstrom)
+
+
+def get_Cu_Table_Morse():
+    """Unavailable, as in mdelast v1.2: the reference reads ``rcut`` from
+    ``Material.potential``, which is a list, and raises AttributeError before it
+    builds the table (molecular_dynamics.py:237-241, docs/API_REFERENCE.md:39-40).
+    The same error is raised here. A tabulated Morse potential is built with
+    ``TablePotential`` from ``Morse.calculate`` at the nodes."""
+    potential = get_Cu_Morse().potential
+    raise AttributeError(f"'{type(potential).__name__}' object has no attribute 'rcut'")
+
+
+def get_Ar_LJ(*, epsilon=None, sigma=None, rcut_fac=3.5):
+    """Argon as the Lennard-Jones pair potential, epsilon = 0.0104 eV and
+    sigma = 3.405 A by default, cut at rcut_fac sigma (molecular_dynamics.py:278-291)."""
+    from .md_driver_module import LennardJones
+    epsilon = 0.0104 * Units.ev if epsilon is None else epsilon
+    sigma = 3.405 * Units.angstrom if sigma is None else sigma
+    return Material(potential=LennardJones(rcut_fac * sigma, epsilon, sigma),
+                    atomic_weight=39.948, atom_name="Ar")
+
+
 # MDL_S_* columns of the per-sample records returned by MDDriver.run
 _R = nat.S_INDEX
 _R_SB, _R_ST, _R_CB = nat.S_BORN_STRESS, nat.S_STRESS, nat.S_BORN_ELAST

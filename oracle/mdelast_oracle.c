@@ -37,6 +37,7 @@ struct orc_table {
     int n;          /* intervals (quintic) */
     double *coef;   /* [n][6] */
     bspline s[3];   /* f, f', f'' (B-spline kind only) */
+    double prm[3];  /* analytic kinds: Morse {D, alpha, r0}; LJ {epsilon, sigma, vcut} */
 };
 
 /* Quintic Hermite coefficients per interval in t = (x - x_i)/h, formed in long
@@ -210,14 +211,70 @@ void orc_table_free(orc_table *t) {
     free(t);
 }
 
-/* Table dispatch (poly_table.hpp:164-175) */
+/* boost::math::pow<6> (Boost.Math special_functions/pow.hpp, a header-only
+ * dependency of the reference, version unpinned): binary powering, x^6 = (x^3)^2
+ * with x^3 = (x x) x. */
+static double pow6_binary(double x) {
+    double const x3 = x * x * x;
+    return x3 * x3;
+}
+
+orc_table *orc_table_analytic(int kind, const double *param) {
+    if ((kind != ORC_MORSE && kind != ORC_LJ) || !param)
+        return NULL;
+    orc_table *t = calloc(1, sizeof *t);
+    if (!t)
+        return NULL;
+    t->kind = kind;
+    t->prm[0] = param[0];
+    t->prm[1] = param[1];
+    if (kind == ORC_MORSE) {
+        t->prm[2] = param[2];
+    } else {
+        /* LennardJones ctor, potential.hpp:55-59 */
+        double const p6 = pow6_binary(param[1] / param[2]), p12 = p6 * p6;
+        t->prm[2] = 4. * param[0] * (p12 - p6);
+    }
+    return t;
+}
+
+/* Morse::calculate (potential.hpp:40-49) and LennardJones::calculate (:65-73),
+ * inside the cut-off (the callers apply it); which = 0, 1, 2 for V, V', V''. */
+static double analytic_eval(const orc_table *t, double r, int which) {
+    if (t->kind == ORC_MORSE) {
+        double const D = t->prm[0], alpha = t->prm[1], r0 = t->prm[2];
+        double const exp1 = exp(-alpha * (r - r0));
+        double const exp2 = exp1 * exp1;
+        if (which == 0)
+            return D * (exp2 - 2. * exp1);
+        if (which == 1)
+            return -2. * D * alpha * (exp2 - exp1);
+        return 2. * alpha * alpha * D * (2 * exp2 - exp1);
+    }
+    double const eps = t->prm[0], sigma = t->prm[1], vcut = t->prm[2];
+    double const v6 = pow6_binary(sigma / r);
+    double const v12 = v6 * v6;
+    if (which == 0)
+        return 4. * eps * (v12 - v6) - vcut;
+    if (which == 1)
+        return 24. * eps * (v6 - 2. * v12) / r;
+    return 24. * eps * (26. * v12 - 7. * v6) / (r * r);
+}
+
+/* Table dispatch (poly_table.hpp:164-175), analytic forms (potential.hpp:29-75) */
 double orc_table_value(const orc_table *t, double x) {
+    if (t->kind == ORC_MORSE || t->kind == ORC_LJ)
+        return analytic_eval(t, x, 0);
     return t->kind == ORC_QUINTIC ? quintic_value(t, x) : bspline_eval(&t->s[0], x);
 }
 double orc_table_d1(const orc_table *t, double x) {
+    if (t->kind == ORC_MORSE || t->kind == ORC_LJ)
+        return analytic_eval(t, x, 1);
     return t->kind == ORC_QUINTIC ? quintic_d1(t, x) : bspline_eval(&t->s[1], x);
 }
 double orc_table_d2(const orc_table *t, double x) {
+    if (t->kind == ORC_MORSE || t->kind == ORC_LJ)
+        return analytic_eval(t, x, 2);
     return t->kind == ORC_QUINTIC ? quintic_d2(t, x) : bspline_eval(&t->s[2], x);
 }
 const double *orc_table_coef(const orc_table *t, int *n_intervals) {

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-enum { ORC_QUINTIC = 0, ORC_BSPLINE = 1 };
+enum { ORC_QUINTIC = 0, ORC_BSPLINE = 1, ORC_MORSE = 2, ORC_LJ = 3 };
 enum { ORC_NVE = 0, ORC_NVT = 1, ORC_NPT = 2, ORC_NPT_FLEX = 3 };
 enum { ORC_KINETIC = 0, ORC_CONFIGURATIONAL = 1, ORC_AVERAGE = 2 };
 
@@ -44,6 +44,10 @@ typedef struct orc_table orc_table;
  * cubic B-splines through f, f1, f2 (poly_table.hpp:120-135). */
 orc_table *orc_table_new(int kind, double h, int n_nodes, const double *f, const double *f1,
                          const double *f2);
+/* Analytic pair forms behind the same evaluation calls: potential::Morse
+ * (potential.hpp:29-51), param {D, alpha, r0}; potential::LennardJones
+ * (:53-75), param {epsilon, sigma, rcut}. No cut-off inside: pair_calc applies it. */
+orc_table *orc_table_analytic(int kind, const double *param);
 void orc_table_free(orc_table *t);
 double orc_table_value(const orc_table *t, double x);
 double orc_table_d1(const orc_table *t, double x);

@@ -16,7 +16,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "build", "liborc.so")
 
-QUINTIC, BSPLINE = 0, 1
+QUINTIC, BSPLINE, MORSE, LJ = 0, 1, 2, 3
 NVE, NVT, NPT = 0, 1, 2
 KINETIC, CONFIGURATIONAL, AVERAGE = 0, 1, 2
 
@@ -68,6 +68,8 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.orc_table_new.restype = C.c_void_p
         L.orc_table_new.argtypes = [C.c_int, C.c_double, C.c_int, _dp, _dp, _dp]
+        L.orc_table_analytic.restype = C.c_void_p
+        L.orc_table_analytic.argtypes = [C.c_int, _dp]
         L.orc_table_free.argtypes = [C.c_void_p]
         for f in ("orc_table_value", "orc_table_d1", "orc_table_d2"):
             getattr(L, f).restype = C.c_double
@@ -141,6 +143,18 @@ class Table:
         if not p:
             return None
         return np.ctypeslib.as_array(p, shape=(n.value, 6)).copy()
+
+
+class AnalyticPair(Table):
+    """Analytic Morse {D, alpha, r0} or Lennard-Jones {epsilon, sigma, rcut} pair
+    form (potential.hpp:29-75) behind the table evaluation calls."""
+
+    def __init__(self, kind, param, rcut):
+        self._keep = [np.ascontiguousarray(param, dtype=float)]
+        self.h, self.kind, self.rcut = 0.0, kind, float(rcut)
+        self.handle = lib().orc_table_analytic(kind, _ptr(self._keep[0]))
+        if not self.handle:
+            raise OracleError(1, "invalid analytic pair form")
 
 
 def tables_from_setfl(data, interpolation="quintic"):

@@ -117,3 +117,24 @@ def maxwell_velocities(cell, T, seed):
     ekin = (3.0 * N - 3.0) * U.k_boltz * T / 2.0
     v *= np.sqrt(ekin / (0.5 * np.sum(m[:, None] * v ** 2)))
     return v
+
+
+def pair_cell(nside, a, spec, atomic_weights, type_fractions=None, seed=0, disp=0.0,
+              disp_seed=None):
+    """fcc cell (lattice constant a in A) for analytic pair potentials.
+    spec = dict(form="lj"|"morse", rcut=<cm>, params=[per slot: (epsilon, sigma) or
+    (D, alpha, r0)]); type_fractions: fraction of each species (one species if None)."""
+    from types import SimpleNamespace
+    pos = fcc_positions(nside, a)
+    rs = np.random.RandomState(seed)
+    types = np.zeros(len(pos), dtype=int)
+    if type_fractions is not None:
+        types = rs.choice(len(type_fractions), size=len(pos), p=type_fractions)
+    if disp:
+        pos = pos + np.random.RandomState(disp_seed).normal(scale=disp, size=pos.shape)
+    L = nside * a
+    pos = np.mod(pos + L / 2, L) - L / 2
+    data = SimpleNamespace(path=None,
+                           elements=[SimpleNamespace(atomic_weight=w) for w in atomic_weights])
+    return dict(data=data, pair=spec, pos=pos * U.angstrom, types=types,
+                box=np.full(3, L * U.angstrom))
