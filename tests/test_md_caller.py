@@ -247,3 +247,27 @@ def test_batched_replica_table():
     assert R.batched_replica_specs(64) == specs        # seeded
     pos = R.fcc_positions(5, 3.6345e-8)
     assert pos.shape == (500, 3) and np.all(np.abs(pos) < 5 * 3.6345e-8 / 2)
+
+
+def test_nve_run_completes_the_final_averages():
+    """In mdelast v1.2 an NVE ``run()`` raises at its final averaging step
+    (np.polyfit on TimeSeries objects, molecular_dynamics.py:1114-1116; noted in
+    docs/USAGE.md:46-47). Here the fit takes the sample values: the run ends
+    with the NVE heat-capacity results. Records with E_pot = c - E_kin give a
+    slope of -1 and equal standard deviations."""
+    n, T = 400, 300.0
+    rs = np.random.RandomState(5)
+    ek = 0.039 * Units.ev * (1.0 + 0.01 * rs.normal(size=n))
+    recs = []
+    for k in range(n):
+        r = _record(1.8e-7, np.full(6, 1e9), np.eye(6) * 1e12, T=T)
+        r[nat.S_INDEX["ekin_atom"]] = ek[k]
+        r[nat.S_INDEX["epot_atom"]] = -3.5 * Units.ev - ek[k]
+        recs.append(r)
+    drv = FakeDriver(1.8e-7, MDDriver.NVE, T, record_fn=lambda c: recs[c - 1])
+    md = _md(drv, ensamble=MDDriver.NVE, N=500, T=T, therm=100, plot_freq=300, end=n - 1)
+    md.run()
+    tr = md.series.thermal_results
+    assert tr.epot_ekin_slope_fit == pytest.approx(-1.0, rel=1e-9)
+    assert tr.epot_ekin_std_err < 1e-9
+    assert np.isfinite(tr.cv_ekin) and tr.cv_epot_ekin_err < 1e-9
