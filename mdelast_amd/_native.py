@@ -78,6 +78,14 @@ SIGNATURES = {
                                      C.POINTER(_vp)]),
     "mdl_table_create_analytic": (C.c_int32, [C.c_int32, _dp, C.POINTER(_vp)]),
     "mdl_table_destroy": (C.c_int32, [_vp]),
+    "mdl_setfl_read": (C.c_int32, [C.c_char_p, C.c_int32, C.POINTER(_vp)]),
+    "mdl_setfl_destroy": (C.c_int32, [_vp]),
+    "mdl_setfl_grid": (C.c_int32, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64), _dp,
+                                   C.POINTER(C.c_int64), _dp, _dp, _dp, _dp]),
+    "mdl_setfl_element": (C.c_int32, [_vp, C.c_int32, C.c_char_p, C.c_int32, _dp, _dp, _dp,
+                                      C.c_char_p, C.c_int32]),
+    "mdl_setfl_nodes": (C.c_int32, [_vp, C.c_int32, C.c_int32, _dp, _dp, _dp]),
+    "mdl_spline5_node_derivatives": (C.c_int32, [C.c_int64, _dp, _dp, _dp, _dp]),
     "mdl_table_coefficients": (C.c_int32, [_vp, _dp, C.POINTER(C.c_int64),
                                            C.POINTER(C.c_int32)]),
     "mdl_table_eval": (C.c_int32, [_vp, C.c_int64, _dp, _dp, _dp, _dp]),

@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmdelast_hip.so")
-SOURCES = ["mdl_tables.hip", "mdl_driver.hip", "mdl_cv.hip"]
+SOURCES = ["mdl_tables.hip", "mdl_driver.hip", "mdl_cv.hip", "mdl_setfl.hip"]
 HEADERS = ["mdl_internal.hpp", "mdl_kernels.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

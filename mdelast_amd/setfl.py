@@ -19,6 +19,12 @@ Semantics follow src/python/get_potential.py:58-147 of the reference:
 * node derivatives are those of the k=5 interpolating spline through the
   nodes, SciPy ``UnivariateSpline(k=5, s=0)`` (get_potential.py:49-55).
 
+``read_setfl`` parses and differentiates in the library (``mdl_setfl_read``,
+include/mdelast_setfl.h): no SciPy on the production path. The spline is the
+same one SciPy builds (degree 5, interior knots at x[3..n-4]), solved from its
+collocation equations in long double; ``read_setfl_python`` (numpy parsing,
+SciPy derivatives) is kept as its cross-check.
+
 The tokenizer reads whitespace-separated values rather than one per line, so
 files with several values per line also load; for one-value-per-line files
 (all three shipped potentials) the numbers are identical to ``numpy.loadtxt``.
@@ -39,10 +45,21 @@ POTENTIAL_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__f
 
 
 def node_derivatives(x: np.ndarray, y: np.ndarray):
-    """First and second derivatives at the nodes of the quintic interpolating spline.
+    """First and second derivatives at the nodes of the quintic interpolating
+    spline (get_potential.py:49-55), computed in the library."""
+    import ctypes as C
+    from . import _native as nat
+    x = np.ascontiguousarray(x, dtype=float)
+    y = np.ascontiguousarray(y, dtype=float)
+    d1, d2 = np.zeros_like(x), np.zeros_like(x)
+    nat.check(nat.lib().mdl_spline5_node_derivatives(C.c_int64(len(x)), nat.dptr(x), nat.dptr(y),
+                                                     nat.dptr(d1), nat.dptr(d2)))
+    return d1, d2
 
-    Same construction as get_potential.py:49-55 (FITPACK k=5, s=0).
-    """
+
+def node_derivatives_scipy(x: np.ndarray, y: np.ndarray):
+    """The same derivatives from SciPy, as get_potential.py:49-55 computes them
+    (FITPACK k=5, s=0); the cross-check of ``node_derivatives``."""
     from scipy.interpolate import UnivariateSpline
 
     spl = UnivariateSpline(x=x, y=y, k=5, s=0)
@@ -102,6 +119,55 @@ def resolve_path(file_name: str) -> str:
 
 
 def read_setfl(file_name: str, with_derivatives: bool = True) -> SetflData:
+    """Parse, convert and differentiate in the library (mdl_setfl_read)."""
+    import ctypes as C
+    from . import _native as nat
+    path = resolve_path(file_name)
+    L = nat.lib()
+    h = C.c_void_p()
+    nat.check(L.mdl_setfl_read(path.encode(), int(bool(with_derivatives)), C.byref(h)))
+    try:
+        nel, nrho, nr = C.c_int32(), C.c_int64(), C.c_int64()
+        drho, dr, rcut, rhomax, rch = (C.c_double() for _ in range(5))
+        nat.check(L.mdl_setfl_grid(h, C.byref(nel), C.byref(nrho), C.byref(drho), C.byref(nr),
+                                   C.byref(dr), C.byref(rcut), C.byref(rhomax), C.byref(rch)))
+
+        def nodes(which, index, n):
+            f, d1, d2 = (np.zeros(n) for _ in range(3))
+            nat.check(L.mdl_setfl_nodes(h, which, index, nat.dptr(f), nat.dptr(d1),
+                                        nat.dptr(d2)))
+            return (f, d1, d2) if with_derivatives else (f, None, None)
+
+        elements = []
+        for e in range(nel.value):
+            name, lat = C.create_string_buffer(64), C.create_string_buffer(64)
+            Z, mass, a = C.c_double(), C.c_double(), C.c_double()
+            nat.check(L.mdl_setfl_element(h, e, name, 64, C.byref(Z), C.byref(mass), C.byref(a),
+                                          lat, 64))
+            F, dF, d2F = nodes(0, e, nrho.value)
+            rho, drh, d2rh = nodes(1, e, nr.value)
+            elements.append(SetflElement(name.value.decode(), Z.value, mass.value, a.value, F, dF,
+                                         d2F, rho, drh, d2rh))
+        n = nel.value
+        pairs = []
+        for j in range(n):
+            for i in range(j + 1):
+                phi, d1, d2 = nodes(2, j * (j + 1) // 2 + i, nr.value)
+                pairs.append(SetflPair(i, j, phi, d1, d2))
+    finally:
+        L.mdl_setfl_destroy(h)
+    r_grid = np.linspace(0.0, dr.value * (nr.value - 1.0), nr.value)
+    rho_grid = np.linspace(0.0, drho.value * (nrho.value - 1.0), nrho.value)
+    return SetflData(path=path, elements=elements, pairs=pairs, Nrho=nrho.value, drho=drho.value,
+                     Nr=nr.value, dr=dr.value, rcut=rcut.value, rhomax=rhomax.value,
+                     r_grid=r_grid, rho_grid=rho_grid)
+
+
+def read_setfl_python(file_name: str, with_derivatives: bool = True,
+                      derivatives=node_derivatives_scipy) -> SetflData:
+    """numpy parsing and SciPy derivatives, as the reference reads the file: the
+    cross-check of ``read_setfl``."""
+    node_derivatives = derivatives
     path = resolve_path(file_name)
     with open(path, "r") as fh:
         lines = fh.read().splitlines()

@@ -21,8 +21,12 @@ _cache = {}
 
 
 def setfl_data(name):
+    """Node data through the reference's own derivative path (numpy parsing, SciPy
+    k=5 spline), so that the oracle reproduces the reference's golden values to
+    ~1e-14; the library's reader (setfl.read_setfl) is pinned against this one
+    in test_setfl_native.py."""
     if name not in _cache:
-        _cache[name] = setfl.read_setfl(name)
+        _cache[name] = setfl.read_setfl_python(name)
     return _cache[name]
 
 

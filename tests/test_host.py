@@ -24,7 +24,7 @@ def header_symbols():
     inc = os.path.join(ROOT, "include")
     text = "".join(open(os.path.join(inc, f)).read() for f in sorted(os.listdir(inc))
                    if f.endswith(".h"))
-    return sorted(set(re.findall(r"\b(mdl_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(mdl_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
