@@ -21,8 +21,10 @@ Mirror of ``src/python/molecular_dynamics.py`` of mdelast v1.2 (its
   s = sigma^B - (N kT / V) delta in NPT; compressibility, bulk modulus and
   thermal expansion in NPT; heat capacities in NVE/NVT.
 
-Not here: plots, pickle and HDF5 files (``save_json`` writes the thermal
-results instead).
+Not here: plots and pickle files. The series and thermal results go to
+``series.npz`` with the keys of the reference's ``series.h5`` (``save_series``,
+``series_io``), after every thermal average as the reference writes its file;
+``save_json`` writes the thermal results alone.
 """
 
 from __future__ import annotations
@@ -441,8 +443,19 @@ class MolecularDynamics:
                     self.gr_samples += 1
             if self.thermalized and (d.cycle - t0) % self.plot_freq == 0:
                 self.perform_thermal_average()
+                self.save_series()      # where the reference writes series.h5 (:1276-1279)
         self.runtime = time.time() - self.runtime_begin
         return self
+
+    def save_series(self, filename=None):
+        """The series and thermal results with the reference's series.h5 schema
+        (molecular_dynamics.py:1182-1195), as ``series.npz`` in the work directory;
+        ``series_io.summarize_run`` reads it as bench_lib.summarize_run reads
+        series.h5."""
+        from . import series_io
+        self.series_path = series_io.save_series(
+            self.series, filename or path.join(self.work_dir, "series.npz"))
+        return self.series_path
 
     # -- averages (molecular_dynamics.py:664-1170) ---------------------------------
     def perform_thermal_average(self):

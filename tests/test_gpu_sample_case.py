@@ -126,3 +126,13 @@ def test_zn06_npt_to_nvt_statistics(tmp_path):
     bad = _check(got_npt, "npt")
     bad.update({f"nvt {k}": v for k, v in _check(got_nvt, "nvt").items()})
     assert not bad, bad
+
+    # the written series files through the reference's benchmark summary and
+    # checks (bench_lib.summarize_run, verify.py), every quantity of both stages
+    from mdelast_amd import series_io
+    expected = json.load(open(os.path.join(HERE, "golden", "zn06_300K.json")))
+    rows, summ = series_io.verify_md(expected, {"npt": npt.series_path, "nvt": nvt.series_path})
+    for status, msg in rows:
+        print(f"{status:4}  {msg}")
+    assert summ["nvt"]["C11"] == got_nvt["C11"] and summ["npt"]["CB44"] == got_npt["CB44"]
+    assert all(status != "FAIL" for status, _ in rows), [r for r in rows if r[0] == "FAIL"]

@@ -14,6 +14,8 @@ import json
 import os
 import types
 
+import tempfile
+
 import numpy as np
 import pytest
 
@@ -74,6 +76,7 @@ def _md(driver, *, ensamble, N, T, therm, plot_freq, end, stride=1, dump=10 ** 9
     md.gr_stride = gr_stride
     md.gr_partial_sum, md.gr_partial_samples = (np.zeros((3, 4)) if partial else None), 0
     md.thermalized = False
+    md.work_dir = tempfile.mkdtemp(prefix="mdl_caller_")
     md.runtime_begin = 0.0
     md.dumps, md.averages = [], []
     md.dump = lambda: md.dumps.append(driver.cycle)
@@ -89,6 +92,7 @@ def _record(L, sb, cb, T=300.0, etot=0.0):
     r[nat.S_INDEX["temperature_conf_inverse"]] = 1.0 / T
     r[nat.S_INDEX["pressure"]] = 1.0
     r[nat.S_INDEX["etot_atom"]] = etot
+    r[nat.S_INDEX["etot_cons"]] = -3.5e-12 + etot
     r[nat.S_INDEX["ekin_atom"]] = 1.0
     r[nat.S_INDEX["epot_atom"]] = 1.0
     r[nat.S_BORN_STRESS:nat.S_BORN_STRESS + 6] = sb
@@ -152,6 +156,8 @@ def _synthetic_run(ensamble, n=4000, seed=3):
     L = L0 * (1.0 + (rs.normal(size=n) * 1e-3 if ensamble == MDDriver.NPT else np.zeros(n)))
     etot = rs.normal(size=n) * 1e-13
     recs = [_record(L[k], sb[k], cb, T=T, etot=etot[k]) for k in range(n)]
+    for k, r in enumerate(recs):
+        r[nat.S_INDEX["time"]] = (k + 1) * 1e-15
     drv = FakeDriver(L0, ensamble, T, record_fn=lambda c: recs[c - 1])
     therm = n // 4
     md = _md(drv, ensamble=ensamble, N=500, T=T, therm=therm, plot_freq=n - therm, end=n - 1)
@@ -271,3 +277,46 @@ def test_nve_run_completes_the_final_averages():
     assert tr.epot_ekin_slope_fit == pytest.approx(-1.0, rel=1e-9)
     assert tr.epot_ekin_std_err < 1e-9
     assert np.isfinite(tr.cv_ekin) and tr.cv_epot_ekin_err < 1e-9
+
+
+def test_series_file_schema_summary_and_checks():
+    """series.npz keeps the reference's series.h5 paths; summarize_run reads the
+    quantities bench_lib.summarize_run reads (bench_lib.py:47-107), equal to the
+    in-memory series; verify_md flags values and run lengths as verify.py does."""
+    from mdelast_amd import series_io as S
+    md, x = _synthetic_run(MDDriver.NPT, n=800)
+    s, tr = md.series, md.series.thermal_results
+    f = S.SeriesFile(md.series_path)
+    for key in ("series/time", "series/temperature_kin/cycles_thermalization",
+                "series/stress_fluc_1_1/mean", "series/elasticity_ideal_gas_4_4/mean",
+                "series/born_elasticity_tensor_11/mean", "series/born_stress_tensor_6/data",
+                "series/born_elasticity_tensor_6_6/std_cum", "series/L_L0/mean",
+                "thermal_results/elasticity_tensor", "thermal_results/bulk_modulus",
+                "thermal_results/sample_stride"):
+        assert key in f, key
+    f.close()
+    got = S.summarize_run(md.series_path)
+    G = S.GPA
+    assert got["n_steps"] == len(s.time) and got["n_thermalization"] == md.samples_thermalization
+    assert got["C11"] == tr.elasticity_tensor_11 / G and got["C44"] == tr.elasticity_tensor_44 / G
+    assert got["CB12"] == s.born_elasticity_tensor_12.mean / G
+    sf = s.stress_fluc.data
+    assert np.isclose(got["fluc11"], (sf[0][0].mean + sf[1][1].mean + sf[2][2].mean) / 3 / G,
+                      rtol=1e-15)
+    assert got["kin11"] == np.mean([s.elasticity_ideal_gas.data[k][k].mean for k in range(3)]) / G
+    assert got["K_npt"] == tr.bulk_modulus / G
+    assert got["T_kin"] == s.temperature_kin.mean and got["L_L0"] == s.L_L0.mean
+    e = np.array(s.etot_cons.data)
+    assert got["econs_err_max"] == np.max(np.abs(e / e[0] - 1.0))
+    exp = {"stages": {"npt": {"n_steps": got["n_steps"], "n_thermalization": got["n_thermalization"],
+                              "quantities": {"C11": {"expected": got["C11"], "tol": 1e-9},
+                                             "C44": {"expected": got["C44"] + 1.0, "tol": 0.5},
+                                             "econs_err_max": {"max": 1.0},
+                                             "kin11": {"info": "report only"}}}}}
+    rows, _ = S.verify_md(exp, {"npt": md.series_path})
+    assert [r[0] for r in rows] == ["PASS", "FAIL", "PASS", "INFO"], rows
+    exp["stages"]["npt"]["n_steps"] += 1
+    rows, _ = S.verify_md(exp, {"npt": md.series_path})
+    assert rows[0][0] == "FAIL" and "run length" in rows[0][1]
+    rows, _ = S.verify_md(exp, {})
+    assert rows == [("FAIL", "npt: no series file")]
