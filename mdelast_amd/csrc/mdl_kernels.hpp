@@ -85,6 +85,51 @@ __device__ __forceinline__ void block_sum_store(double (&acc)[NV], double *out) 
     block_sum_store<NV>(acc, out, (int)blockIdx.x);
 }
 
+// block_sum_store without the components [S0, S1): a step that does not sample
+// leaves the sampled-only partials (Born and Laplacian terms) unreduced and
+// their slots untouched; post_step reads them only on sampled steps. The other
+// components are reduced exactly as by block_sum_store.
+template <int NV, int S0, int S1>
+__device__ __forceinline__ void block_sum_store_skip(double (&acc)[NV], double *out, int bid) {
+    static_assert(0 <= S0 && S0 <= S1 && S1 <= NV, "skip range");
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        if (k >= S0 && k < S1)
+            continue;
+        double v = acc[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            v += __shfl_xor(v, off, 64);
+        acc[k] = v;
+    }
+    __shared__ double sm[BLOCK / 64][NV];
+    int const wave = threadIdx.x >> 6, l64 = threadIdx.x & 63;
+    if (l64 == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            if (k < S0 || k >= S1)
+                sm[wave][k] = acc[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < NV && (threadIdx.x < S0 || threadIdx.x >= S1)) {
+        double s = 0.;
+#pragma unroll
+        for (int w = 0; w < BLOCK / 64; ++w)
+            s += sm[w][threadIdx.x];
+        out[(size_t)threadIdx.x * PART_STRIDE + bid] = s;
+    }
+}
+
+// The partial sums of a pass: all components on sampled steps, the others
+// without [S0, S1).
+template <bool SAMPLE, int NV, int S0, int S1>
+__device__ __forceinline__ void pass_sum_store(double (&acc)[NV], double *out, int bid) {
+    if constexpr (SAMPLE)
+        block_sum_store<NV>(acc, out, bid);
+    else
+        block_sum_store_skip<NV, S0, S1>(acc, out, bid);
+}
+
 // XCD-aware slice of [0, nitems): blocks are dispatched round-robin over the
 // 8 XCDs, so block b runs on XCD b % 8; give each XCD one contiguous chunk so
 // its L2 holds one spatial region of the neighbour gathers.
@@ -1372,7 +1417,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(AtomArrays a, ListArrays l, RTa
         }
     }
     acc[PB_WRMIN] = (double)nrmin;
-    block_sum_store<PB_N>(acc, part);
+    pass_sum_store<SAMPLE, PB_N, PB_CB, PB_LAP + 1>(acc, part, (int)blockIdx.x);
 }
 
 // Second U2 kick (md_driver.cpp:433-452) and the per-atom sums the
@@ -1435,7 +1480,7 @@ __device__ __forceinline__ void kick2_body(AtomArrays a, int N, const DevState *
         acc[k] = 0.;
     for (int i = bk.id * BLOCK + threadIdx.x; i < N; i += bk.n * BLOCK)
         kick2_atom<SAMPLE>(a, i, gs, e, ch, init != 0, has_eam, acc);
-    block_sum_store<PK_N>(acc, part, bk.id);
+    pass_sum_store<SAMPLE, PK_N, PK_CBF, PK_LAPF + 1>(acc, part, bk.id);
 }
 
 template <bool SAMPLE>

@@ -378,7 +378,7 @@ __device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTable
         acc[PB_CB + k] = cb[k];
     acc[PB_LAP] = lap;
     acc[PB_WRMIN] = (double)nrmin;
-    block_sum_store<PB_N>(acc, part, bk.id);
+    pass_sum_store<SAMPLE, PB_N, PB_CB, PB_LAP + 1>(acc, part, bk.id);
 }
 
 template <int KIND, int LPA, bool SAMPLE>
