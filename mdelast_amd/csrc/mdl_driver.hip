@@ -353,6 +353,12 @@ struct Driver {
                       "list holds %d", hs.nbr_needed, l.stride);
             return MDL_ERR_NEIGHBORS;
         }
+        if (e & DE_CELL_OVERFLOW) {
+            set_error("neighbour cells: %.0f atoms fall into one cell (at most %d): the "
+                      "positions are non-finite or have collapsed, so the integration has "
+                      "diverged", hs.err_val[1], CELL_MAX);
+            return MDL_ERR_NEIGHBORS;
+        }
         if (e & DE_CELL_GRID) {
             set_error("the box shrank below the cell grid of the neighbour list (L = %g)",
                       hs.err_val[0]);

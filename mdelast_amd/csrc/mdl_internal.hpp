@@ -48,6 +48,7 @@ enum DevErr : int {
     DE_NBR_OVERFLOW = 1 << 5,
     DE_CELL_GRID = 1 << 6,
     DE_BARRIER = 1 << 7,
+    DE_CELL_OVERFLOW = 1 << 8,   // one cell holds more atoms than the build stages
 };
 
 // ----------------------------------------------------------------------------

@@ -804,8 +804,8 @@ __device__ __forceinline__ void rb_sort_cell(ListArrays &l, int c, int *v, DevSt
     int const b = l.cell_start[c], n = l.cell_start[c + 1] - b;
     if (n > CELL_MAX) {
         if (threadIdx.x == 0) {
-            atomicOr(&gs->err, (int)DE_CELL_GRID);
-            gs->err_val[0] = (double)n;
+            atomicOr(&gs->err, (int)DE_CELL_OVERFLOW);
+            gs->err_val[1] = (double)n;
         }
         return;   // uniform in the block
     }
