@@ -32,7 +32,7 @@ gradients are known: s[H g] = H s[g] + sum_i grad_i H . g_i. The fields are
 The coefficients of each head are fitted by least squares on frames of a
 plain run after the network has been trained on them (Adam on the residual
 variance); a head whose held-out variance the variate does not cut is switched
-off, and the plain estimator stays the judge.
+off, and that head keeps the plain estimator.
 
 Device work is hand-written HIP (include/mdelast_cv.h): descriptors with
 their derivatives and Laplacians from the driver's neighbour list, and the
