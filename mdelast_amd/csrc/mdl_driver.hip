@@ -389,7 +389,37 @@ static mdl_status bad(const char *msg) {
     return MDL_ERR_ARG;
 }
 
-// Time the two passes for each candidate lanes-per-atom and keep the fastest.
+// Displace every atom by up to `amp` (scaled units, per axis) with a fixed
+// hash of its index: a thermally disordered stand-in for the autotune, whose
+// lattice-perfect start would favour the table-gather pattern of equal
+// distances. The caller restores the positions afterwards.
+__global__ __launch_bounds__(BLOCK) void k_autotune_jitter(double4 *__restrict__ pos, int N,
+                                                           double ax, double ay, double az) {
+    int const i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= N)
+        return;
+    unsigned h = (unsigned)i * 2654435761u + 0x9e3779b9u;
+    double u[3];
+    for (int c = 0; c < 3; ++c) {
+        h ^= h >> 16;
+        h *= 0x7feb352du;
+        h ^= h >> 15;
+        h *= 0x846ca68bu;
+        h ^= h >> 16;
+        u[c] = (double)h * (2.0 / 4294967296.0) - 1.0;   // [-1, 1)
+    }
+    double4 p = pos[i];
+    p.x += ax * u[0];
+    p.y += ay * u[1];
+    p.z += az * u[2];
+    pos[i] = p;
+}
+
+// Time the two passes for each candidate lanes-per-atom and keep the fastest,
+// on the positions displaced by up to 0.1 A per axis (about the thermal
+// amplitude of the alloys here at a few hundred K; the list radius exceeds r_c
+// by far more, and the list stays the one built on the undisplaced positions).
+// The positions are restored exactly and the error flags cleared afterwards.
 static mdl_status autotune(Driver &d) {
     auto key = std::make_tuple((int64_t)d.N, d.ntypes, d.kind, d.l.stride / 32);
     auto it = g_lpa_cache.find(key);
@@ -401,6 +431,16 @@ static mdl_status autotune(Driver &d) {
     hipEvent_t e0, e1;
     MDL_HIP(hipEventCreate(&e0));
     MDL_HIP(hipEventCreate(&e1));
+    double4 *saved_pos = nullptr;
+    MDL_HIP(hipMalloc(&saved_pos, sizeof(double4) * (size_t)d.N));
+    MDL_HIP(hipMemcpyAsync(saved_pos, d.a.pos, sizeof(double4) * (size_t)d.N,
+                           hipMemcpyDeviceToDevice, d.stream));
+    {
+        double const amp = 0.1e-8;   // 0.1 A in cm
+        k_autotune_jitter<<<(unsigned)((d.N + BLOCK - 1) / BLOCK), BLOCK, 0, d.stream>>>(
+            d.a.pos, d.N, amp / d.hs.L[0], amp / d.hs.L[1], amp / d.hs.L[2]);
+        MDL_HIP(hipGetLastError());
+    }
     double best = 1e300;
     int best_lpa = 16;
     int const cand[] = {4, 8, 16, 32, 64};
@@ -422,6 +462,14 @@ static mdl_status autotune(Driver &d) {
         }
     }
     (void)saved;
+    MDL_HIP(hipMemcpyAsync(d.a.pos, saved_pos, sizeof(double4) * (size_t)d.N,
+                           hipMemcpyDeviceToDevice, d.stream));
+    {
+        int const zero = 0;   // a displaced density above rho_max is no error of the run
+        MDL_HIP(hipMemcpyAsync(&d.gs->err, &zero, sizeof(int), hipMemcpyHostToDevice, d.stream));
+        MDL_HIP(hipStreamSynchronize(d.stream));
+    }
+    MDL_HIP(hipFree(saved_pos));
     d.lpa = best_lpa;
     d.gridA = d.gridB = d.grid_pairs(best_lpa);
     g_lpa_cache[key] = best_lpa;
