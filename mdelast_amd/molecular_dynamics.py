@@ -163,6 +163,14 @@ class Material:
             self.box_length = box_length
 
 
+def get_EAM_file(*, file_name, a_factor=1.0):
+    """One-element EAM file as a ``Material`` with quintic tables
+    (molecular_dynamics.py:123-234); ``get_potential.get_EAM_file`` takes the
+    interpolation as well."""
+    from .get_potential import get_EAM_file as _get
+    return _get(file_name=file_name, a_factor=a_factor)
+
+
 def get_Cu_Morse(a_factor=1.0):
     """Copper as the analytic Morse pair potential (molecular_dynamics.py:259-275):
     D = 0.3429 eV, alpha = 1.3588 / A, r0 = 2.866 A, cut at 10/alpha + r0."""

@@ -105,6 +105,9 @@ class SetflData:
     r_grid: np.ndarray = field(repr=False)
     rho_grid: np.ndarray = field(repr=False)
     rmin_fatal: float = 1e-2 * U.angstrom
+    # "alloy": get_EAM_alloy_file's conversion; "single": get_EAM_file's (the
+    # grids are formed in A and scaled afterwards, so the last bits differ)
+    reader: str = "alloy"
 
     @property
     def names(self):
@@ -237,3 +240,83 @@ def read_setfl_python(file_name: str, with_derivatives: bool = True,
     return SetflData(path=path, elements=elements, pairs=pairs, Nrho=Nrho, drho=drho, Nr=Nr,
                      dr=dr, rcut=float(r_grid[-1]), rhomax=float(rho_grid[-1]), r_grid=r_grid,
                      rho_grid=rho_grid)
+
+
+# Leading entries of rho(r) and r*phi(r) in the Mendelev Cu FS file that hold the
+# fill value 1e12 (get_potential.py:213-226); they take the first tabulated value.
+_FS_FILL, _FS_NFILL = 1e12, 167
+
+
+def read_eam_file(file_name: str, derivatives=node_derivatives) -> SetflData:
+    """One-element EAM file as get_EAM_file reads it (get_potential.py:150-257,
+    molecular_dynamics.py:123-234 of the reference).
+
+    * line 6: ``Z mass a lattice``; line 5: ``Nrho drho Nr dr rcut``; every
+      value after line 6 is table data, read as whitespace-separated tokens
+      (``numpy.loadtxt(skiprows=6)``);
+    * ``*.eam.alloy.txt`` (SETFL): Nrho values of F(rho), Nr of rho(r), Nr of
+      r*phi(r), nothing else; ``*.eam.fs.txt`` (Finnis-Sinclair, five values a
+      line): the same three blocks, Nrho and Nr multiples of 5, and the first
+      167 entries of rho and of r*phi hold 1e12 and take entry 167; any other
+      name is refused (the reference logs FATAL and exits);
+    * unlike the alloy reader, the grids are formed in A and A^-3 and then
+      scaled, and phi = (r phi) / r[A] * eV; phi[0] = phi[1];
+    * node derivatives of the quintic interpolating spline on the CGS grids.
+
+    Returns the node data as a one-element :class:`SetflData` (``reader`` =
+    "single"); the file's element symbol is kept when line 4 names one."""
+    path = resolve_path(file_name)
+    if not (file_name.endswith("eam.alloy.txt") or file_name.endswith("eam.fs.txt")):
+        raise ValueError(f"undefined potential format '{file_name}'")
+    with open(path, "r") as fh:
+        lines = fh.read().splitlines()
+    if len(lines) < 7:
+        raise ValueError(f"{path}: too short for an EAM file")
+    Z, mass, a_lat = (float(s) for s in lines[5].split()[:-1][:3])
+    p = [float(s) for s in lines[4].split()]
+    if len(p) < 5:
+        raise ValueError(f"{path}: line 5 must hold Nrho drho Nr dr rcut")
+    Nrho, drho_a, Nr, dr_a = int(p[0]), p[1], int(p[2]), p[3]
+    head = lines[3].split()
+    name = head[1] if len(head) > 1 else "Cu"
+    r_grid_a = np.linspace(0.0, dr_a * (Nr - 1.0), Nr)
+    rho_grid_a = np.linspace(0.0, drho_a * (Nrho - 1.0), Nrho)
+    values = np.array([float(t) for ln in lines[6:] for t in ln.split()], dtype=float)
+    if file_name.endswith("eam.alloy.txt"):
+        if any(len(ln.split()) > 1 for ln in lines[6:]) or len(values) != Nrho + 2 * Nr:
+            raise ValueError(f"{path}: SETFL single-element file must hold Nrho + 2 Nr values "
+                             "one per line")
+        F = values[:Nrho].copy()
+        rho = values[Nrho:Nrho + Nr].copy()
+        phir = values[Nrho + Nr:].copy()
+    else:
+        rows = [ln.split() for ln in lines[6:] if ln.strip()]
+        if any(len(r) != 5 for r in rows) or 5 * len(rows) != Nrho + 2 * Nr \
+                or Nrho % 5 or Nr % 5:
+            raise ValueError(f"{path}: FS file must hold Nrho + 2 Nr values, five a line, "
+                             "with Nrho and Nr multiples of 5")
+        F = values[:Nrho].copy()
+        rho = values[Nrho:Nrho + Nr].copy()
+        phir = values[Nrho + Nr:].copy()
+        for arr, what in ((rho, "rho"), (phir, "r*phi")):
+            if set(arr[:_FS_NFILL]) != {_FS_FILL}:
+                raise ValueError(f"{path}: the first {_FS_NFILL} {what} values must be the "
+                                 f"fill value {_FS_FILL:g}")
+            arr[:_FS_NFILL] = arr[_FS_NFILL]
+    F *= U.ev
+    with np.errstate(divide="ignore", invalid="ignore"):
+        phi = phir / r_grid_a * U.ev
+    phi[0] = phi[1]
+    dr = dr_a * U.angstrom
+    r_grid = r_grid_a * U.angstrom
+    drho = drho_a / U.angstrom ** 3
+    rho /= U.angstrom ** 3
+    rho_grid = rho_grid_a / U.angstrom ** 3
+    dphi, d2phi = derivatives(r_grid, phi)
+    drh, d2rh = derivatives(r_grid, rho)
+    dF, d2F = derivatives(rho_grid, F)
+    el = SetflElement(name, Z, mass, a_lat, F, dF, d2F, rho, drh, d2rh)
+    return SetflData(path=path, elements=[el], pairs=[SetflPair(0, 0, phi, dphi, d2phi)],
+                     Nrho=Nrho, drho=drho, Nr=Nr, dr=dr, rcut=float(r_grid[-1]),
+                     rhomax=float(rho_grid[-1]), r_grid=r_grid, rho_grid=rho_grid,
+                     reader="single")

@@ -71,10 +71,12 @@ _table_cache = {}
 
 def gpu_tables(data, interpolation):
     from mdelast_amd import get_potential
-    key = (data.path, interpolation)
+    # keyed by the node-data object (kept alive in the entry, so its id is not
+    # reused): one file read by two readers or derivative paths gives two entries
+    key = (id(data), interpolation)
     if key not in _table_cache:
-        _table_cache[key] = get_potential.tables_from_setfl(data, interpolation)
-    return _table_cache[key]
+        _table_cache[key] = (data, get_potential.tables_from_setfl(data, interpolation))
+    return _table_cache[key][1]
 
 
 def gpu_md(cell, setup, *, interpolation="quintic", ensemble="NVE", dt=1e-15,
@@ -150,3 +152,48 @@ def compare_static(o, g, tol):
     bad = {k: v for k, v in errs.items() if not v <= tol}
     assert not bad, f"above {tol}: {bad}"
     return errs
+
+
+def traj_compare(cell, setup, nsteps, rebuild_every=20, **kw):
+    """Step the oracle and the GPU driver side by side (integrate() one step at a
+    time, a requested list rebuild every rebuild_every steps when the list is on)
+    and return the worst relative differences of the per-step observables, the
+    coordinates, velocities and chain momenta, with both drivers."""
+    o = oracle_md(cell, setup, **kw)
+    g = gpu_md(cell, setup, **kw)
+    neighbors = kw.get("neighbors_list", False)
+    ensemble = kw.get("ensemble", "NVE")
+    worst = {}
+
+    def upd(k, v):
+        worst[k] = max(worst.get(k, 0.0), float(v))
+
+    for step in range(nsteps):
+        if neighbors and rebuild_every and step % rebuild_every == 0:
+            o.calculate_neighbors()
+            g.calculate_neighbors()
+        o.integrate()
+        g.integrate()
+        so, sg = o.scalars(), gpu_scalars(g)
+        for k in ("etot_cons", "temperature_kin", "epot_atom", "ekin_atom"):
+            upd(k, abs(sg[k] / so[k] - 1))
+        if so["temperature_conf_inverse"]:
+            upd("T_conf_inv", abs(sg["temperature_conf_inverse"] / so["temperature_conf_inverse"]
+                                  - 1))
+        scale_p = abs(so["virial"]) + 2 * so["ekin_tot"] / 3
+        V = so["box0"] * so["box1"] * so["box2"]
+        upd("pressure", abs(sg["pressure"] - so["pressure"]) * V / scale_p)
+        upd("coords", np.abs(o.atoms("coordinates") - g.coordinates_array()).max())
+        vo, vg = o.atoms("velocities"), g.velocities_array()
+        upd("vel", np.abs(vo - vg).max() / np.abs(vo).max())
+        if ensemble == "NPT":
+            upd("box", abs(sg["box0"] / so["box0"] - 1))
+        assert sg["cycle"] == so["cycle"]
+    if ensemble != "NVE":
+        po, _ = o.chain(0)
+        upd("p_eta", np.max(np.abs(np.array(g.thermostat.p_eta) - po) / np.abs(po).max()))
+    if ensemble == "NPT":
+        pb, _ = o.chain(1)
+        upd("p_eta_baro", np.max(np.abs(np.array(g.barostat.thermostat.p_eta) - pb)
+                                 / np.abs(pb).max()))
+    return worst, o, g

@@ -1,0 +1,165 @@
+"""The reference's FATAL conditions on the device path, reached through
+ordinary inputs, each decided as the oracle decides it on the same input.
+
+The reference prints and exits (md_driver.cpp:91-106, :493-497, :620-627,
+:693-698, :700-725; eam.hpp:48-70). Here the condition is a device flag raised
+inside the step kernels (or, for the initial state, inside the constructor's
+thermodynamics), returned as an MDL_ERR_* code and raised as MDDriverError with
+the matching ``kind``; a driver that failed refuses every later step with
+STATE. Inputs:
+
+* CONSERVATION: fatal_conservation = 1e-30 on a 500-atom NVT run: the relative
+  change of E_cons is far above it at once, and the check, which skips cycle 0
+  (md_driver.cpp:700-725), fires in step 2;
+* RHOMAX: a 1 keV knock-on atom aimed at its nearest neighbour in alpha-brass
+  (a radiation-damage start; momentum removed as MolecularDynamics does): the
+  collision drives rho above rho_max = 100 A^-3 at step 19;
+* VOLUME: NPT from a cell compressed to 77 % of its lattice constant with a
+  50 fs barostat: the first step changes the volume by more than 5 %;
+* MOMENTUM: the knock-on velocity without the momentum removal, at construction;
+* PRESSURE: the perfect Cu01 lattice at rest at its zero-pressure lattice
+  constant (3.6150000694094566 A): P and tr(sigma)/3 are both round-off there,
+  so their relative difference exceeds 1e-5 (the reference's own check fires
+  on this input too);
+* POSITION, TYPE: an atom outside the centred box, a type beyond the species.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import cells
+import harness as H
+from mdelast_amd import md_driver_module as mdm
+from mdelast_amd import units as U
+
+pytestmark = pytest.mark.gpu
+
+KINDS = {5: "RHOMAX", 6: "VOLUME", 7: "MOMENTUM", 8: "PRESSURE", 9: "CONSERVATION",
+         2: "POSITION", 3: "TYPE"}
+
+
+def _oracle_fail_step(cell, setup, nsteps, rebuild_every=0, **kw):
+    """(kind, step) of the oracle's FATAL: step 0 = in the constructor."""
+    import oracle as O
+    try:
+        o = H.oracle_md(cell, setup, **kw)
+    except O.OracleError as e:
+        return KINDS[e.code], 0
+    for k in range(nsteps):
+        if rebuild_every and k % rebuild_every == 0:
+            o.calculate_neighbors()
+        try:
+            o.integrate()
+        except O.OracleError as e:
+            return KINDS[e.code], k + 1
+    return None, None
+
+
+def _gpu_fail_step(cell, setup, nsteps, rebuild_every=0, **kw):
+    try:
+        g = H.gpu_md(cell, setup, **kw)
+    except mdm.MDDriverError as e:
+        return e.kind, 0, None
+    for k in range(nsteps):
+        if rebuild_every and k % rebuild_every == 0:
+            g.calculate_neighbors()
+        try:
+            g.integrate()
+        except mdm.MDDriverError as e:
+            return e.kind, k + 1, g
+    return None, None, g
+
+
+def _assert_refuses_further_steps(g):
+    with pytest.raises(mdm.MDDriverError) as ei:
+        g.integrate()
+    assert ei.value.kind == "STATE"
+    with pytest.raises(mdm.MDDriverError) as ei:
+        g.run(3, 1, 0)
+    assert ei.value.kind == "STATE"
+
+
+def _knock_on_cell(E_eV, remove_momentum=True):
+    cell = cells.alloy_cell(5, 3.6345, "ZnCu.eam.alloy", [0.06, 0.94], seed=1)
+    setup = H.md_setup(cell, T=300.0, seed=3)
+    m = np.array(cells.masses(cell["data"]))[cell["types"]]
+    v = setup["vel"].copy()
+    # atom 0 towards its nearest neighbour at a (1/2, 1/2, 0)
+    v[0] = np.array([1.0, 1.0, 0.0]) / np.sqrt(2.0) * np.sqrt(2.0 * E_eV * U.ev / m[0])
+    if remove_momentum:
+        v -= (v * m[:, None]).sum(0) / m.sum()
+    return cell, dict(setup, vel=v)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_conservation(precision):
+    cell = cells.zn06_cell(5)
+    setup = H.md_setup(cell, T=300.0, M=3, seed=0)
+    kw = dict(ensemble="NVT", dt=2e-15, neighbors_list=True, r_skin=1.2, fatal=1e-30)
+    assert _oracle_fail_step(cell, setup, 3, **kw) == ("CONSERVATION", 2)
+    kind, step, g = _gpu_fail_step(cell, setup, 3, precision=precision, **kw)
+    assert (kind, step) == ("CONSERVATION", 2)
+    _assert_refuses_further_steps(g)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_rhomax_knock_on(precision):
+    cell, setup = _knock_on_cell(1000.0)
+    kw = dict(ensemble="NVE", dt=2e-16, neighbors_list=True, r_skin=1.2)
+    assert _oracle_fail_step(cell, setup, 40, rebuild_every=10, **kw) == ("RHOMAX", 19)
+    kind, step, g = _gpu_fail_step(cell, setup, 40, rebuild_every=10, precision=precision, **kw)
+    assert (kind, step) == ("RHOMAX", 19)
+    _assert_refuses_further_steps(g)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_volume_change_npt(precision):
+    cell = cells.alloy_cell(5, 3.6345 * 0.77, "ZnCu.eam.alloy", [0.06, 0.94], seed=1)
+    setup = H.md_setup(cell, T=300.0, seed=3, P=300.0 * U.GPa, tau_P=5e-14)
+    kw = dict(ensemble="NPT", dt=1e-15, neighbors_list=True, r_skin=1.2)
+    assert _oracle_fail_step(cell, setup, 5, **kw) == ("VOLUME", 1)
+    kind, step, g = _gpu_fail_step(cell, setup, 5, precision=precision, **kw)
+    assert (kind, step) == ("VOLUME", 1)
+    _assert_refuses_further_steps(g)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_momentum_at_construction(precision):
+    cell, setup = _knock_on_cell(1000.0, remove_momentum=False)
+    kw = dict(ensemble="NVE", dt=2e-16, neighbors_list=True, r_skin=1.2)
+    assert _oracle_fail_step(cell, setup, 1, **kw) == ("MOMENTUM", 0)
+    kind, step, _ = _gpu_fail_step(cell, setup, 1, precision=precision, **kw)
+    assert (kind, step) == ("MOMENTUM", 0)
+    # the same velocities with the momentum removed construct and step
+    cell, setup = _knock_on_cell(1000.0)
+    g = H.gpu_md(cell, setup, precision=precision, **kw)
+    g.integrate()
+
+
+def test_pressure_trace_at_zero_pressure():
+    from test_eam_file import cu01_cell
+    cell = cu01_cell(a=3.6150000694094566)
+    setup = H.md_setup(cell, static=True)
+    assert _oracle_fail_step(cell, setup, 1) == ("PRESSURE", 0)
+    kind, step, _ = _gpu_fail_step(cell, setup, 1)
+    assert (kind, step) == ("PRESSURE", 0)
+    # 1e-4 A away the pressure is well resolved and both construct
+    cell = cu01_cell(a=3.6151)
+    g = H.gpu_md(cell, setup)
+    s = H.gpu_scalars(g)
+    assert abs(s["pressure"] / (s["stress_tensor"][:3].sum() / 3) - 1) < 1e-8
+
+
+def test_bad_position_and_type():
+    cell = cells.zn06_cell(5)
+    setup = H.md_setup(cell, T=300.0, seed=0)
+    bad = dict(cell, pos=cell["pos"].copy())
+    bad["pos"][7, 1] = 0.5 * cell["box"][1]          # |y| >= L/2
+    assert _oracle_fail_step(bad, setup, 1) == ("POSITION", 0)
+    assert _gpu_fail_step(bad, setup, 1)[:2] == ("POSITION", 0)
+    bad = dict(cell, types=cell["types"].copy())
+    bad["types"][11] = 2                               # two species
+    assert _oracle_fail_step(bad, setup, 1) == ("TYPE", 0)
+    assert _gpu_fail_step(bad, setup, 1)[:2] == ("TYPE", 0)
