@@ -193,6 +193,18 @@ mdl_status mdl_driver_set_graph_steps(mdl_driver *d, int32_t steps);
  * compute is not touched. Synchronous. */
 mdl_status mdl_driver_pair_correlation(mdl_driver *d, const double *rg, int32_t nbin,
                                        int32_t mode, double *gr, double *gr_partial);
+/* mdl_driver_pair_correlation without any allocation: the same results, bit for
+ * bit, with the bin counters in a per-device scratch array of the library
+ * (MDL_GR_SCRATCH_COUNTS 64-bit counters, 4 MiB, held on each GPU the call
+ * runs on until the process ends) and the counts read back through a fixed
+ * host buffer. No counterpart in the reference: it is for callers that cannot
+ * allocate (loops under a no-allocation rule, code between captured work).
+ * (1 + S(S+1)/2) * nbin above MDL_GR_SCRATCH_COUNTS gives MDL_ERR_ARG. Calls on
+ * one GPU share the scratch; the library serialises its own calls, and other
+ * code overlapping them on that GPU would race. Synchronous. */
+#define MDL_GR_SCRATCH_COUNTS 524288
+mdl_status mdl_driver_pair_correlation_noalloc(mdl_driver *d, const double *rg, int32_t nbin,
+                                               int32_t mode, double *gr, double *gr_partial);
 /* Force a rebuild before the next force evaluation (md_driver.cpp:279-298). */
 mdl_status mdl_driver_calculate_neighbors(mdl_driver *d);
 /* MDDriver::sample_observables (md_driver.hpp:163-167) */

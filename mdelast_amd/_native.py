@@ -100,6 +100,8 @@ SIGNATURES = {
                                     C.POINTER(_dp), C.POINTER(C.c_int64)]),
     "mdl_driver_calculate_neighbors": (C.c_int32, [_vp]),
     "mdl_driver_pair_correlation": (C.c_int32, [_vp, _dp, C.c_int32, C.c_int32, _dp, _dp]),
+    "mdl_driver_pair_correlation_noalloc": (C.c_int32, [_vp, _dp, C.c_int32, C.c_int32, _dp,
+                                                        _dp]),
     "mdl_driver_set_sample_observables": (C.c_int32, [_vp, C.c_int32]),
     "mdl_driver_get_sample_observables": (C.c_int32, [_vp, C.POINTER(C.c_int32)]),
     "mdl_driver_get_scalars": (C.c_int32, [_vp, _dp]),

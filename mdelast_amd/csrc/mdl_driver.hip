@@ -1024,6 +1024,131 @@ __global__ __launch_bounds__(GR_T) void k_pair_hist(const double4 *__restrict__ 
     }
 }
 
+__global__ __launch_bounds__(256) void k_fill_u64(unsigned long long *__restrict__ p, size_t n) {
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256)
+        p[k] = 0ull;
+}
+
+// One g(r) call: its bins, rows (row 0 the total, row 1 + q species pair q)
+// and the normalisation constants of md_driver.cpp:749-751 and :786-809.
+struct GrCall {
+    int nbin = 0, nrow = 0, ntypes = 0;
+    bool want_total = false, want_partial = false;
+    double drg = 0., rmaxg = 0., V = 0., Np = 0.;
+    double pref[MAX_TYPES * (MAX_TYPES + 1) / 2] = {};   // V / (4 pi n_pairs drg), 0 if none
+    size_t nh = 0;
+};
+
+static mdl_status gr_prepare(Driver *d, const double *rg, int32_t nbin, int32_t mode,
+                             double *gr, double *gr_partial, GrCall &c, size_t max_nh = 0) {
+    if (!d || !rg || nbin < 2 || mode < 0 || mode > 2)
+        return bad("mdl_driver_pair_correlation: bad arguments");
+    c.want_total = mode != 2;
+    c.want_partial = mode != 0;
+    if ((c.want_total && !gr) || (c.want_partial && !gr_partial))
+        return bad("mdl_driver_pair_correlation: missing output array");
+    c.drg = rg[1] - rg[0];
+    if (!(c.drg > 0.))
+        return bad("mdl_driver_pair_correlation: rg must increase");
+    c.rmaxg = rg[nbin - 1] + 0.5 * c.drg;   // md_driver.cpp:41
+    MDL_HIP(hipSetDevice(d->device));
+    if (d->failed)
+        return bad("the driver stopped on an earlier error");
+    c.nbin = nbin;
+    c.ntypes = d->ntypes;
+    int const npair = d->ntypes * (d->ntypes + 1) / 2;
+    c.nrow = 1 + npair;
+    c.nh = (size_t)c.nrow * nbin;
+    if (max_nh && c.nh > max_nh) {
+        set_error("mdl_driver_pair_correlation_noalloc: %zu bin counters ((1 + species "
+                  "pairs) x nbin) exceed the scratch of %zu; use mdl_driver_pair_correlation",
+                  c.nh, max_nh);
+        return MDL_ERR_ARG;
+    }
+    double L[3] = {0., 0., 0.};
+    MDL_HIP(hipMemcpyAsync(L, d->gs->L, sizeof L, hipMemcpyDeviceToHost, d->stream));
+    MDL_HIP(hipStreamSynchronize(d->stream));
+    c.V = L[0] * (L[1] * L[2]);
+    c.Np = (double)d->N;
+    int q = 0;
+    for (int b = 0; b < d->ntypes; ++b)
+        for (int a = 0; a <= b; ++a, ++q) {
+            double const Na = (double)d->type_count[a], Nb = (double)d->type_count[b];
+            double pc = 0.;
+            if (a == b)
+                pc = Na > 1 ? Na * (Na - 1.) / 2. : 0.;
+            else
+                pc = Na * Nb;
+            c.pref[q] = pc > 0. ? c.V / (4. * M_PI * pc * c.drg) : 0.;
+        }
+    return MDL_OK;
+}
+
+// Zero the counters (a fill kernel: the call may follow captured work) and
+// count every pair i < j into them.
+static hipError_t gr_count(Driver *d, const GrCall &c, unsigned long long *cnt) {
+    unsigned const fill = (unsigned)std::min<size_t>((c.nh + 255) / 256, 1024);
+    k_fill_u64<<<fill, 256, 0, d->stream>>>(cnt, c.nh);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    size_t const lds = c.nh * sizeof(unsigned);
+    int const use_lds = lds <= 48 * 1024 ? 1 : 0;
+    int const T = (d->N + GR_T - 1) / GR_T;
+    k_pair_hist<<<dim3(T, T), GR_T, use_lds ? lds : 0, d->stream>>>(
+        d->a.pos, d->N, d->gs, c.nbin, c.drg, c.rmaxg, c.want_total ? 1 : 0,
+        c.want_partial ? 1 : 0, d->ntypes, cnt, use_lds);
+    return hipGetLastError();
+}
+
+// Where row `row` goes (nullptr: the mode does not compute it).
+static double *gr_row_out(const GrCall &c, int row, double *gr, double *gr_partial) {
+    if (row == 0)
+        return c.want_total ? gr : nullptr;
+    return c.want_partial ? gr_partial + (size_t)(row - 1) * c.nbin : nullptr;
+}
+
+// Bins k0 .. k0 + n - 1 of one row from their counts cnt[0 .. n-1], in the
+// reference's order of operations.
+static void gr_normalize(const GrCall &c, int row, int k0, int n,
+                         const unsigned long long *cnt, const double *rg, double *out) {
+    for (int u = 0; u < n; ++u) {
+        int const k = k0 + u;
+        if (row == 0)
+            out[k] = (double)cnt[u] * (c.V / (2. * M_PI * c.Np * c.Np * rg[k] * rg[k] * c.drg));
+        else if (c.pref[row - 1] > 0.)
+            out[k] = (double)cnt[u] * (c.pref[row - 1] / (rg[k] * rg[k]));
+        else
+            out[k] = 0.;
+    }
+}
+
+// Per-device scratch of mdl_driver_pair_correlation_noalloc: the bin counters
+// live in this file-scope array (one copy per GPU, held until the process
+// ends), so the call needs no device allocation. Its device address is taken
+// once per device with hipGetSymbolAddress; host code uses that address, never
+// the array's name. All calls on one GPU share it: they are serialised by
+// g_gr_scratch_mutex, and calls from other code that overlap would race.
+static __device__ __align__(256) unsigned long long scratch[MDL_GR_SCRATCH_COUNTS];
+static std::mutex g_gr_scratch_mutex;
+static std::map<int, unsigned long long *> g_gr_scratch_ptr;
+
+static mdl_status gr_scratch(int device, unsigned long long **out) {
+    auto it = g_gr_scratch_ptr.find(device);
+    if (it != g_gr_scratch_ptr.end()) {
+        *out = it->second;
+        return MDL_OK;
+    }
+    void *ptr = nullptr;
+    hipError_t const e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                        "hipGetSymbolAddress(scratch)");
+    g_gr_scratch_ptr[device] = static_cast<unsigned long long *>(ptr);
+    *out = static_cast<unsigned long long *>(ptr);
+    return MDL_OK;
+}
+
 // ---------------------------------------------------------------------------
 // One captured graph for a chunk of steps of several drivers: each driver's
 // launches form a branch of its own (its stream joins the capture through a
@@ -2019,75 +2144,61 @@ mdl_status mdl_driver_cv_descriptors(mdl_driver *p, float rc, float mu0, float d
 mdl_status mdl_driver_pair_correlation(mdl_driver *p, const double *rg, int32_t nbin,
                                        int32_t mode, double *gr, double *gr_partial) {
     Driver *d = D(p);
-    if (!d || !rg || nbin < 2 || mode < 0 || mode > 2)
-        return bad("mdl_driver_pair_correlation: bad arguments");
-    bool const want_total = mode != 2, want_partial = mode != 0;
-    if ((want_total && !gr) || (want_partial && !gr_partial))
-        return bad("mdl_driver_pair_correlation: missing output array");
-    double const drg = rg[1] - rg[0];
-    if (!(drg > 0.))
-        return bad("mdl_driver_pair_correlation: rg must increase");
-    double const rmaxg = rg[nbin - 1] + 0.5 * drg;   // md_driver.cpp:41
-    MDL_HIP(hipSetDevice(d->device));
-    if (d->failed)
-        return bad("the driver stopped on an earlier error");
-    int const npair = d->ntypes * (d->ntypes + 1) / 2;
-    size_t const nh = (size_t)(1 + npair) * nbin;
+    GrCall c;
+    if (mdl_status s = gr_prepare(d, rg, nbin, mode, gr, gr_partial, c))
+        return s;
     unsigned long long *cnt = nullptr;
-    MDL_HIP(hipMalloc(&cnt, nh * sizeof(unsigned long long)));
-    std::vector<unsigned long long> h(nh);
-    size_t const lds = nh * sizeof(unsigned);
-    int const use_lds = lds <= 48 * 1024 ? 1 : 0;
-    int const T = (d->N + GR_T - 1) / GR_T;
-    hipError_t e = hipMemsetAsync(cnt, 0, nh * sizeof(unsigned long long), d->stream);
-    if (e == hipSuccess) {
-        k_pair_hist<<<dim3(T, T), GR_T, use_lds ? lds : 0, d->stream>>>(
-            d->a.pos, d->N, d->gs, nbin, drg, rmaxg, want_total ? 1 : 0, want_partial ? 1 : 0,
-            d->ntypes, cnt, use_lds);
-        e = hipGetLastError();
-    }
+    MDL_HIP(hipMalloc(&cnt, c.nh * sizeof(unsigned long long)));
+    std::vector<unsigned long long> h(c.nh);
+    hipError_t e = gr_count(d, c, cnt);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(h.data(), cnt, nh * sizeof(unsigned long long),
+        e = hipMemcpyAsync(h.data(), cnt, c.nh * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, d->stream);
-    double L[3] = {0., 0., 0.};
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(L, d->gs->L, sizeof L, hipMemcpyDeviceToHost, d->stream);
     if (e == hipSuccess)
         e = hipStreamSynchronize(d->stream);
     (void)hipFree(cnt);
     if (e != hipSuccess)
         return hip_fail(e, "mdl_driver_pair_correlation");
-    // normalisation of md_driver.cpp:749-751 and :786-809, in the same order
-    double const V = L[0] * (L[1] * L[2]);
-    double const Np = (double)d->N;
-    if (want_total)
-        for (int k = 0; k < nbin; ++k)
-            gr[k] = (double)h[k] * (V / (2. * M_PI * Np * Np * rg[k] * rg[k] * drg));
-    if (want_partial) {
-        std::vector<double> cnt_t(d->ntypes, 0.);
-        for (int t = 0; t < d->ntypes; ++t)
-            cnt_t[t] = (double)d->type_count[t];
-        int q = 0;
-        for (int b = 0; b < d->ntypes; ++b)
-            for (int a = 0; a <= b; ++a, ++q) {
-                double const Na = cnt_t[a], Nb = cnt_t[b];
-                double pc = 0.;
-                if (a == b)
-                    pc = Na > 1 ? Na * (Na - 1.) / 2. : 0.;
-                else
-                    pc = Na * Nb;
-                double *g = gr_partial + (size_t)q * nbin;
-                const unsigned long long *c = h.data() + (size_t)(1 + q) * nbin;
-                if (pc > 0.) {
-                    double const pref = V / (4. * M_PI * pc * drg);
-                    for (int k = 0; k < nbin; ++k)
-                        g[k] = (double)c[k] * (pref / (rg[k] * rg[k]));
-                } else {
-                    for (int k = 0; k < nbin; ++k)
-                        g[k] = 0.;
-                }
-            }
+    for (int row = 0; row < c.nrow; ++row)
+        if (double *out = gr_row_out(c, row, gr, gr_partial))
+            gr_normalize(c, row, 0, nbin, h.data() + (size_t)row * nbin, rg, out + 0);
+    return MDL_OK;
+}
+
+mdl_status mdl_driver_pair_correlation_noalloc(mdl_driver *p, const double *rg, int32_t nbin,
+                                               int32_t mode, double *gr, double *gr_partial) {
+    Driver *d = D(p);
+    GrCall c;
+    if (mdl_status s = gr_prepare(d, rg, nbin, mode, gr, gr_partial, c,
+                                  (size_t)MDL_GR_SCRATCH_COUNTS))
+        return s;
+    std::lock_guard<std::mutex> lock(g_gr_scratch_mutex);
+    unsigned long long *cnt = nullptr;   // device address of `scratch` on this device
+    if (mdl_status s = gr_scratch(d->device, &cnt))
+        return s;
+    hipError_t e = gr_count(d, c, cnt);
+    // counts to the host through a fixed buffer, one chunk of a row at a time
+    constexpr int CHUNK = 512;
+    unsigned long long buf[CHUNK];
+    for (int row = 0; row < c.nrow && e == hipSuccess; ++row) {
+        double *out = gr_row_out(c, row, gr, gr_partial);
+        if (!out)
+            continue;
+        for (int k0 = 0; k0 < nbin && e == hipSuccess; k0 += CHUNK) {
+            int const n = std::min(CHUNK, nbin - k0);
+            e = hipMemcpyAsync(buf, cnt + (size_t)row * nbin + k0,
+                               (size_t)n * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                               d->stream);
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(d->stream);
+            if (e == hipSuccess)
+                gr_normalize(c, row, k0, n, buf, rg, out);
+        }
     }
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(d->stream);
+    if (e != hipSuccess)
+        return hip_fail(e, "mdl_driver_pair_correlation_noalloc");
     return MDL_OK;
 }
 
