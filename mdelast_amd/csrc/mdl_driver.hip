@@ -116,7 +116,7 @@ struct Driver {
     FTables ft{};
     RTablesF rtf{};
     FTablesF ftf{};
-    float *d_rrec_f = nullptr, *d_frec_f = nullptr;
+    float *d_rrec_f = nullptr, *d_frec_f = nullptr, *d_prec_f = nullptr;
     double rc2 = 0, rc_pair = 0, rc_rho = 0, rmin = 0;
     double *d_rrec = nullptr, *d_frec = nullptr;
     double *pA = nullptr, *pB = nullptr, *pK = nullptr;
@@ -636,7 +636,43 @@ static mdl_status create(const mdl_config *c, Driver &d) {
         if (mdl_status st = d.alloc(&d.d_rrec_f, rf.size()))
             return st;
         MDL_HIP(hipMemcpy(d.d_rrec_f, rf.data(), sizeof(float) * rf.size(), hipMemcpyHostToDevice));
+        // the force pass's fused records [interval][pair slot][PairRecF::NF]:
+        // phi of the slot, then the derivative parts of rho of its two species,
+        // the same floats as the records above
+        {
+            int const NFP = d.kind == MDL_QUINTIC ? 16 : 32;
+            std::vector<float> pf((size_t)nr * d.npair * NFP, 0.f);
+            for (int64_t i = 0; i < nr; ++i)
+                for (int b = 0; b < d.ntypes; ++b)
+                    for (int a = 0; a <= b; ++a) {
+                        int const slot = b * (b + 1) / 2 + a;
+                        const float *phi = &rf[((size_t)i * nf + slot) * SF];
+                        const float *ra = &rf[((size_t)i * nf + d.npair + a) * SF];
+                        const float *rb = &rf[((size_t)i * nf + d.npair + b) * SF];
+                        float *dst = &pf[((size_t)i * d.npair + slot) * NFP];
+                        if (d.kind == MDL_QUINTIC) {
+                            for (int k = 0; k < 6; ++k)
+                                dst[k] = phi[k];
+                            for (int k = 0; k < 5; ++k) {
+                                dst[6 + k] = ra[1 + k];
+                                dst[11 + k] = rb[1 + k];
+                            }
+                        } else {
+                            for (int k = 0; k < 12; ++k)
+                                dst[k] = phi[k];
+                            for (int k = 0; k < 8; ++k) {
+                                dst[12 + k] = ra[4 + k];
+                                dst[20 + k] = rb[4 + k];
+                            }
+                        }
+                    }
+            if (mdl_status st = d.alloc(&d.d_prec_f, pf.size()))
+                return st;
+            MDL_HIP(hipMemcpy(d.d_prec_f, pf.data(), sizeof(float) * pf.size(),
+                              hipMemcpyHostToDevice));
+        }
         d.rtf.rec = d.d_rrec_f;
+        d.rtf.prec = d.d_prec_f;
         d.rtf.nf = nf;
         d.rtf.n = (int)nr;
         d.rtf.inv_h = (float)(rtabs[0]->inv_h * 1e-8);

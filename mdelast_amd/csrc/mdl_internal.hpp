@@ -223,9 +223,12 @@ __host__ __device__ __forceinline__ void pair_analytic(const double *p, double r
     }
 }
 
-// Round half away from zero (md_driver.hpp:48-50): trunc(x -+ 0.5)
+// Round half away from zero (md_driver.hpp:48-50): trunc(x -+ 0.5). The half
+// is selected first, so one add and one trunc run instead of two of each
+// (x + (-0.5) is x - 0.5 exactly).
 __device__ __forceinline__ double nint_away(double x) {
-    return x < 0. ? trunc(x - 0.5) : trunc(x + 0.5);
+    double const h = x < 0. ? -0.5 : 0.5;
+    return trunc(x + h);
 }
 
 }  // namespace mdl

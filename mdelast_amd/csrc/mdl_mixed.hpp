@@ -16,10 +16,25 @@
 
 namespace mdl {
 
+// s_waitcnt vmcnt(0) (every vector memory load of the wave has landed, also
+// those written to LDS) and s_waitcnt lgkmcnt(0) (every LDS access has
+// returned), in the gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4,
+// lgkmcnt 11:8; a field at its maximum does not wait.
+__device__ __forceinline__ void mdl_wait_vmem() {
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __builtin_amdgcn_wave_barrier();
+}
+__device__ __forceinline__ void mdl_wait_lds() {
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __builtin_amdgcn_wave_barrier();
+}
+
 // Table records in FP32, Angstrom units. Quintic records are padded to 8 floats
-// (32 B) so each loads as one float4 + one float2.
+// (32 B) so each loads as one float4 + one float2. `prec` holds the force
+// pass's fused records, one per interval and species pair (PairRecF).
 struct RTablesF {
     const float *rec;
+    const float *prec;
     int nf, n;
     float inv_h, inv_h2;   // 1/A, 1/A^2
 };
@@ -63,6 +78,85 @@ template <> struct PolyF<MDL_BSPLINE> {
     __device__ __forceinline__ float v(float t) const { return c[0] + t * (c[1] + t * (c[2] + t * c[3])); }
     __device__ __forceinline__ float d1(float t, float) const { return c[4] + t * (c[5] + t * (c[6] + t * c[7])); }
     __device__ __forceinline__ float d2(float t, float) const { return c[8] + t * (c[9] + t * (c[10] + t * c[11])); }
+};
+
+// The force pass's table record of one interval and one species pair (a <= b),
+// aligned to its size: phi_ab, then the derivative parts of rho_a and rho_b.
+// One record is one cache line or half of one, so a pair's three lookups touch
+// one line instead of three.
+//   quintic (16 floats, 64 B): phi c0..c5 | rho_a c1..c5 | rho_b c1..c5
+//   B-spline (32 floats, 128 B): phi value, d1, d2 cubics (12) | rho_a d1, d2
+//     cubics (8) | rho_b d1, d2 cubics (8) | 4 pad
+// The derivative polynomials are the PolyF ones, term for term.
+template <int KIND> struct PairRecF;
+template <> struct PairRecF<MDL_QUINTIC> {
+    static constexpr int NF = 16;
+    float c[16];
+    __device__ __forceinline__ void load(const float *p) {
+        const float4 *q = reinterpret_cast<const float4 *>(p);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float4 const v = q[k];
+            c[4 * k] = v.x; c[4 * k + 1] = v.y; c[4 * k + 2] = v.z; c[4 * k + 3] = v.w;
+        }
+    }
+    // from an LDS record whose chunk k sits in slot (k + rot) mod 4
+    __device__ __forceinline__ void load_rot(const float4 *p, int rot) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float4 const v = p[(k + rot) & 3];
+            c[4 * k] = v.x; c[4 * k + 1] = v.y; c[4 * k + 2] = v.z; c[4 * k + 3] = v.w;
+        }
+    }
+    __device__ __forceinline__ float v(float t) const {
+        return c[0] + t * (c[1] + t * (c[2] + t * (c[3] + t * (c[4] + t * c[5]))));
+    }
+    __device__ __forceinline__ float d1(float t, float ih) const {
+        return (c[1] + t * (2.f * c[2] + t * (3.f * c[3] + t * (4.f * c[4] + t * 5.f * c[5])))) * ih;
+    }
+    __device__ __forceinline__ float d2(float t, float ih2) const {
+        return (2.f * c[2] + t * (6.f * c[3] + t * (12.f * c[4] + t * 20.f * c[5]))) * ih2;
+    }
+    // rho of the lower (O = 6) or higher (O = 11) species: c1..c5 at O
+    template <int o> __device__ __forceinline__ float rd1(float t, float ih) const {
+        return (c[o] + t * (2.f * c[o + 1] + t * (3.f * c[o + 2] + t * (4.f * c[o + 3] +
+                                                                        t * 5.f * c[o + 4])))) * ih;
+    }
+    template <int o> __device__ __forceinline__ float rd2(float t, float ih2) const {
+        return (2.f * c[o + 1] + t * (6.f * c[o + 2] + t * (12.f * c[o + 3] +
+                                                             t * 20.f * c[o + 4]))) * ih2;
+    }
+    static constexpr int RHO_LO = 6, RHO_HI = 11;
+};
+template <> struct PairRecF<MDL_BSPLINE> {
+    static constexpr int NF = 32;
+    float c[28];
+    __device__ __forceinline__ void load(const float *p) {
+        const float4 *q = reinterpret_cast<const float4 *>(p);
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            float4 const v = q[k];
+            c[4 * k] = v.x; c[4 * k + 1] = v.y; c[4 * k + 2] = v.z; c[4 * k + 3] = v.w;
+        }
+    }
+    // from an LDS record whose chunk k sits in slot (k + rot) mod 8 (chunk 7 is padding)
+    __device__ __forceinline__ void load_rot(const float4 *p, int rot) {
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            float4 const v = p[(k + rot) & 7];
+            c[4 * k] = v.x; c[4 * k + 1] = v.y; c[4 * k + 2] = v.z; c[4 * k + 3] = v.w;
+        }
+    }
+    __device__ __forceinline__ float v(float t) const { return c[0] + t * (c[1] + t * (c[2] + t * c[3])); }
+    __device__ __forceinline__ float d1(float t, float) const { return c[4] + t * (c[5] + t * (c[6] + t * c[7])); }
+    __device__ __forceinline__ float d2(float t, float) const { return c[8] + t * (c[9] + t * (c[10] + t * c[11])); }
+    template <int o> __device__ __forceinline__ float rd1(float t, float) const {
+        return c[o] + t * (c[o + 1] + t * (c[o + 2] + t * c[o + 3]));
+    }
+    template <int o> __device__ __forceinline__ float rd2(float t, float) const {
+        return c[o + 4] + t * (c[o + 5] + t * (c[o + 6] + t * c[o + 7]));
+    }
+    static constexpr int RHO_LO = 12, RHO_HI = 20;
 };
 
 // Interval and local coordinate as tab_locate (range test before the
@@ -145,8 +239,11 @@ __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTab
             int const tj = (int)cj.w;
             float t;
             int const it = tab_locate_f(r, rt.inv_h, nm1, t);
+            // about half of the candidates lie beyond r_c: their lanes request
+            // no table record
             P p;
-            p.load(rt.rec + (it * rowS + (npair + tj) * P::S));
+            if (in)
+                p.load(rt.rec + (it * rowS + (npair + tj) * P::S));
             c0 = a.pos[j1];
             j0 = j1;
             j1 = k + 2 * LPA < cnt ? nb[k + 2 * LPA] : ia;
@@ -246,13 +343,12 @@ __device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTable
                                               double *__restrict__ part, Blk bk) {
     if (gs->err)
         return;
-    using P = PolyF<KIND>;
+    using Rec = PairRecF<KIND>;
     constexpr int APB = BLOCK / LPA;
     constexpr double F_CGS = 1.6021773e-12 / 1e-8;   // eV/A -> dyn
     constexpr double E_CGS = 1.6021773e-12;          // eV -> erg
     int const lane = threadIdx.x % LPA, slot = threadIdx.x / LPA;
     double const L[3] = {gs->L[0] * 1e8, gs->L[1] * 1e8, gs->L[2] * 1e8};
-    int const rowS = rt.nf * P::S;           // floats per interval (fits int)
     float const nm1 = (float)(rt.n - 1);
     float sb0 = 0.f, sb1 = 0.f, sb2 = 0.f, sb3 = 0.f, sb4 = 0.f, sb5 = 0.f;
     float cb[15];
@@ -262,47 +358,87 @@ __device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTable
     float lap = 0.f;
     double epot = 0.;
     int nrmin = 0;
+    // Cooperative record loads straight into LDS: in round q of CH, lane l
+    // fetches one 16-byte chunk of the record of lane p = q * (64 / CH) + l / CH,
+    // so the CH lanes that share a record share its cache line in one load
+    // instruction; the load writes lane l's chunk to stage + 16 l (the
+    // global-to-LDS form of gfx950), which is record p's 16 CH bytes. Within a
+    // record the chunk c sits in slot (c + rot(p)) mod CH, rot(p) = p CH / 16, so
+    // when each lane then reads its own record chunk by chunk, the 16 lanes of
+    // a 16-byte read fall into distinct banks.
+    // One stage per wave, filled by one round of cooperative loads per pair.
+    // The waits are explicit: the compiler does not order every LDS read after
+    // the global-to-LDS loads that fill it.
+    constexpr int CH = Rec::NF / 4;
+    __shared__ float4 stage_all[BLOCK / 64][64 * CH];
+    float4 *const stage = stage_all[threadIdx.x / 64];
+    int const l64 = threadIdx.x & 63;
+    int const rot = l64 * CH / 16;
+    const float4 *const prec4 = reinterpret_cast<const float4 *>(rt.prec);
     XcdRange const R((N + APB - 1) / APB, bk);
     for (int grp = R.first; grp < R.end; grp += R.step) {
         int const i = grp * APB + slot;
         bool const active = i < N;
         double fx = 0., fy = 0., fz = 0., e = 0.;
-        if (active) {
-            double4 const ci = a.pos[i];
+        // every lane of the wave takes part in every load round, so the trip
+        // count is the wave's longest list; lanes past their own list (and
+        // lanes of no atom) carry atom ia itself as a harmless dummy
+        int const ia = active ? i : 0;
+        int const cnt = active ? l.ccnt[i] : 0;
+        int wmax = cnt;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            wmax = max(wmax, __shfl_xor(wmax, off, 64));
+        {
+            double4 const ci = a.pos[ia];
             int const ti = (int)ci.w;
-            float2 const Fi = a.fprime_f[i];
-            int const cnt = l.ccnt[i];
-            const int *nb = l.cnbr + (size_t)i * l.stride;
-            // Software pipeline as in pass A: the next pair's position and the
-            // index after it are requested after this pair's table records.
-            int j0 = lane < cnt ? nb[lane] : i;
-            int j1 = LPA + lane < cnt ? nb[LPA + lane] : i;
-            double4 c0 = a.pos[j0];
-            for (int k = lane; k < cnt; k += LPA) {
-                int const j = j0;
-                double4 const cj = c0;
+            float2 const Fi = a.fprime_f[ia];
+            const int *nb = l.cnbr + (size_t)ia * l.stride;
+            struct Geo {
                 double dx, dy, dz;
-                min_image(ci, cj, L, dx, dy, dz);
-                float const x = (float)dx, y = (float)dy, z = (float)dz;
-                float const xx = x * x, yy = y * y, zz = z * z;
-                float const r = __builtin_amdgcn_sqrtf(xx + yy + zz);
-                int const tj = (int)cj.w;
-                int const tmax = max(ti, tj), tmin = min(ti, tj);
+                float x, y, z, r, t;
+                int tj;
+                bool act;
+            };
+            // pair vector, interval and record of one pair
+            auto geometry = [&](const double4 &cj, bool act, Geo &g) -> int {
+                min_image(ci, cj, L, g.dx, g.dy, g.dz);
+                g.x = (float)g.dx;
+                g.y = (float)g.dy;
+                g.z = (float)g.dz;
+                float const xx = g.x * g.x, yy = g.y * g.y, zz = g.z * g.z;
+                g.r = __builtin_amdgcn_sqrtf(xx + yy + zz);
+                g.tj = (int)cj.w;
+                g.act = act;
+                int const tmax = max(ti, g.tj), tmin = min(ti, g.tj);
                 int const pslot = ((tmax * (tmax + 1)) >> 1) + tmin;
+                int const it = tab_locate_f(g.r, rt.inv_h, nm1, g.t);
+                return act ? it * npair + pslot : 0;
+            };
+            // the wave's records into stage st, round q from the lanes q * (64 / CH) ..
+            auto request = [&](float4 *st, int rec) {
+#pragma unroll
+                for (int q = 0; q < CH; ++q) {
+                    int const src = q * (64 / CH) + l64 / CH;
+                    int const rsrc = __shfl(rec, src, 64);
+                    int const chunk = (l64 % CH - src * CH / 16) & (CH - 1);
+                    __builtin_amdgcn_global_load_lds(
+                        (const void __attribute__((address_space(1))) *)(prec4 + (size_t)rsrc * CH +
+                                                                          chunk),
+                        (void __attribute__((address_space(3))) *)(st + q * 64), 16, 0, 0);
+                }
+            };
+            auto physics = [&](const Geo &g, const Rec &pp, float2 Fj) {
+                float const x = g.x, y = g.y, z = g.z, r = g.r, t = g.t;
+                double const dx = g.dx, dy = g.dy, dz = g.dz;
+                float const xx = x * x, yy = y * y, zz = z * z;
                 nrmin += r < rmin;
-                float t;
-                int const it = tab_locate_f(r, rt.inv_h, nm1, t);
-                const float *base = rt.rec + it * rowS;
-                P pp, pi, pj;
-                pp.load(base + pslot * P::S);
-                pi.load(base + (npair + ti) * P::S);
-                pj.load(base + (npair + tj) * P::S);
-                float2 const Fj = a.fprime_f[j];
-                c0 = a.pos[j1];
-                j0 = j1;
-                j1 = k + 2 * LPA < cnt ? nb[k + 2 * LPA] : i;
+                // rho of species i and j: the record's lower or higher one
+                bool const ilo = ti <= g.tj;
                 float const V = pp.v(t), dV = pp.d1(t, rt.inv_h);
-                float const dri = pi.d1(t, rt.inv_h), drj = pj.d1(t, rt.inv_h);
+                float const dlo = pp.template rd1<Rec::RHO_LO>(t, rt.inv_h);
+                float const dhi = pp.template rd1<Rec::RHO_HI>(t, rt.inv_h);
+                float const dri = ilo ? dlo : dhi, drj = ilo ? dhi : dlo;
                 float const ir = __builtin_amdgcn_rcpf(r);
                 float const dedr = sym_dedr(dV, Fi.x, drj, Fj.x, dri);
                 float const s = -dedr * ir;
@@ -319,7 +455,9 @@ __device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTable
                 sb5 += w * (x * y);
                 if (SAMPLE) {
                     float const d2V = pp.d2(t, rt.inv_h2);
-                    float const d2ri = pi.d2(t, rt.inv_h2), d2rj = pj.d2(t, rt.inv_h2);
+                    float const d2lo = pp.template rd2<Rec::RHO_LO>(t, rt.inv_h2);
+                    float const d2hi = pp.template rd2<Rec::RHO_HI>(t, rt.inv_h2);
+                    float const d2ri = ilo ? d2lo : d2hi, d2rj = ilo ? d2hi : d2lo;
                     float X = d2V - dV * ir;
                     lap += 2.f * (d2V + 2.f * dV * ir);
                     lap += 2.f * (Fj.x * (d2ri + 2.f * dri * ir) + Fi.x * (d2rj + 2.f * drj * ir));
@@ -344,6 +482,27 @@ __device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTable
                     cb[M_Y2XZ] += Xyy * xz;
                     cb[M_Z2XY] += Xzz * xy;
                 }
+            };
+            // Software pipeline: the next pair's position and the index after it
+            // are requested after this pair's table record.
+            int j0 = lane < cnt ? nb[lane] : ia;
+            int j1 = LPA + lane < cnt ? nb[LPA + lane] : ia;
+            double4 c0 = a.pos[j0];
+            for (int k = lane; k - lane < wmax; k += LPA) {
+                Geo g;
+                int const rec = geometry(c0, k < cnt, g);
+                int const j = j0;
+                mdl_wait_lds();   // the stage was read out for the pair before
+                request(stage, rec);
+                float2 const Fj = a.fprime_f[j];
+                c0 = a.pos[j1];
+                j0 = j1;
+                j1 = k + 2 * LPA < cnt ? nb[k + 2 * LPA] : ia;
+                mdl_wait_vmem();
+                Rec pp;
+                pp.load_rot(stage + l64 * CH, rot);
+                if (g.act)
+                    physics(g, pp, Fj);
             }
         }
 #pragma unroll
