@@ -228,7 +228,8 @@ enum {
     MDL_S_WARN_RMIN = 77,     /* pair distances below r_min seen (potential.hpp:93-95) */
     MDL_S_NLIST_ENTRIES = 78, /* stored (full) list entries */
     MDL_S_NHC_GENERAL = 79,   /* chain propagations that left the short-argument path */
-    MDL_NSCALARS = 80
+    MDL_S_NMID_BUILDS = 80,   /* filterings of the mid list (pairs within r_c + 0.3 A) */
+    MDL_NSCALARS = 81
 };
 mdl_status mdl_driver_get_scalars(mdl_driver *d, double *out);
 

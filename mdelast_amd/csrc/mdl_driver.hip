@@ -743,6 +743,7 @@ static mdl_status create(const mdl_config *c, Driver &d) {
     ALLOC(a.fz, N);
     ALLOC(a.type, N);
     ALLOC(a.cref, N);
+    ALLOC(a.mref, N);
     ALLOC(a.fprime, N);
     ALLOC(a.geam, (size_t)4 * N);
     ALLOC(a.fprime_f, N);
@@ -753,6 +754,9 @@ static mdl_status create(const mdl_config *c, Driver &d) {
     ALLOC(d.l.nnbr, N);
     ALLOC(d.l.cnbr, (size_t)N * stride);
     ALLOC(d.l.ccnt, N);
+    ALLOC(d.l.mnbr, (size_t)N * stride);
+    ALLOC(d.l.mcnt, N);
+    MDL_HIP(hipMemset(d.l.mcnt, 0, sizeof(int) * N));
     if (cells) {
         ALLOC(d.l.cell_of, N);
         ALLOC(d.l.cell_count, d.l.ncells);
@@ -771,8 +775,9 @@ static mdl_status create(const mdl_config *c, Driver &d) {
     ALLOC(d.d_ticket_fin, 4);
     MDL_HIP(hipMemset(d.d_bar, 0, 4 * sizeof(unsigned)));
     MDL_HIP(hipMemset(d.d_ticket_fin, 0, 4 * sizeof(unsigned)));
-    ALLOC(d.d_dmax, GRID_ATOM_MAX);
-    MDL_HIP(hipMemset(d.d_dmax, 0, sizeof(double) * GRID_ATOM_MAX));
+    static_assert(GRID_ATOM_MAX <= DMAX_MID, "displacement maxima overlap");
+    ALLOC(d.d_dmax, 2 * DMAX_MID);
+    MDL_HIP(hipMemset(d.d_dmax, 0, sizeof(double) * 2 * DMAX_MID));
     ALLOC(d.gs, 1);
 #undef ALLOC
     MDL_HIP(hipMemset(a.fx, 0, sizeof(double) * N));
@@ -806,6 +811,7 @@ static mdl_status create(const mdl_config *c, Driver &d) {
         ++d.type_count[(size_t)typ[i]];
     MDL_HIP(hipMemcpy(a.pos, pos.data(), sizeof(double4) * N, hipMemcpyHostToDevice));
     MDL_HIP(hipMemcpy(a.cref, pos.data(), sizeof(double4) * N, hipMemcpyHostToDevice));
+    MDL_HIP(hipMemcpy(a.mref, pos.data(), sizeof(double4) * N, hipMemcpyHostToDevice));
     MDL_HIP(hipMemcpy(a.vx, vx.data(), sizeof(double) * N, hipMemcpyHostToDevice));
     MDL_HIP(hipMemcpy(a.vy, vy.data(), sizeof(double) * N, hipMemcpyHostToDevice));
     MDL_HIP(hipMemcpy(a.vz, vz.data(), sizeof(double) * N, hipMemcpyHostToDevice));
@@ -854,6 +860,12 @@ static mdl_status create(const mdl_config *c, Driver &d) {
     s.r_list = r_list;
     s.rc = rc;
     s.skin = r_list - rc;
+    // mid list: 0.3 A beyond r_c (half the skin when that is thinner); it is
+    // refiltered when an atom may have moved 0.15 A since the last filtering
+    s.r_mid = rc + std::min(0.3e-8, 0.5 * (r_list - rc));
+    s.mid_build = 1;
+    for (int k = 0; k < 3; ++k)
+        s.L_mid[k] = c->box[k];
     s.force_rebuild = 0;
     for (int k = 0; k < 3; ++k)
         s.ncell[k] = cells ? nc[k] : 1;
@@ -2092,6 +2104,7 @@ mdl_status mdl_driver_get_scalars(mdl_driver *p, double *out) {
     for (int k = 0; k < 3; ++k)
         out[MDL_S_BOX_INITIAL0 + k] = d->hs.L0[k];
     out[MDL_S_NLIST_ENTRIES] = (double)d->hs.nlist_entries;
+    out[MDL_S_NMID_BUILDS] = (double)d->hs.nmid_builds;
     return MDL_OK;
 }
 

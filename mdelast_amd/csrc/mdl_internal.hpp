@@ -97,7 +97,13 @@ struct DevState {
     double etot_cons_initial, etot_cons, etot_cons_err;
     double time;
     unsigned long long cycle;
-    // neighbour-list control
+    // neighbour-list control. The mid list holds the pairs of the full list
+    // within r_mid; pass A scans it instead of the full list while no pair
+    // outside it can have entered r_c, and refilters it from the full list
+    // (mid_build) after every full build and whenever that could have happened.
+    double r_mid, L_mid[3];
+    int mid_build, pad_mid;
+    long long nmid_builds;
     double r_list, skin, rc;
     double umax2;            // max squared displacement since the last build (k_kick_drift)
     int force_rebuild, do_rebuild;

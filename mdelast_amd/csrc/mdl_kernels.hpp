@@ -21,6 +21,7 @@ struct AtomArrays {
     double *fx, *fy, *fz;
     int *type;
     double4 *cref;     // coordinates at the last list build
+    double4 *mref;     // coordinates at the last build of the mid list
     double2 *fprime;   // F'(rho_i), F''(rho_i)
     double2 *geam;     // sampled steps: g_i (6, Voigt) and |h_i|^2, 4 double2 per atom
     float2 *fprime_f;  // mixed mode: F', F'' in eV A^3, eV A^6 for the pair gathers
@@ -34,6 +35,8 @@ struct ListArrays {
     int *nnbr;         // [N]
     int *cnbr;         // [N][stride] pairs inside r_c, compacted by pass A
     int *ccnt;         // [N]
+    int *mnbr;         // [N][stride] pairs inside r_mid (r_c < r_mid <= r_list), list order
+    int *mcnt;         // [N]
     int stride;
     int *cell_of, *cell_count, *cell_start, *cell_fill, *cell_atoms;
     int ncells;
@@ -45,6 +48,9 @@ struct ListArrays {
 // Partial records are stored component-major, part[k * PART_STRIDE + block],
 // so the final reduction reads them coalesced.
 constexpr int PART_STRIDE = 2048;
+// dmax_part[DMAX_MID + b]: block b's largest squared displacement since the
+// mid list was built (dmax_part[b]: since the full list was built)
+constexpr int DMAX_MID = 1024;
 
 // The block coordinates a kernel body works with: the hardware's, or one
 // system's share of a launch that covers several systems.
@@ -623,7 +629,7 @@ __device__ __forceinline__ void kick_drift_body(AtomArrays a, int N, DevState *_
     double const s = gs->kick_s, e = gs->kick_e, ch = gs->kick_c, cf = gs->drift_cf;
     double const Ld[3] = {gs->drift_L[0], gs->drift_L[1], gs->drift_L[2]};
     double const Ln[3] = {gs->L[0], gs->L[1], gs->L[2]};
-    double dmax = 0.;
+    double dmax = 0., dmid = 0.;
     for (int i = bk.id * BLOCK + threadIdx.x; i < N; i += bk.n * BLOCK) {
         double const m = gs->mass[a.type[i]];
         double vx = a.vx[i] * s, vy = a.vy[i] * s, vz = a.vz[i] * s;
@@ -641,18 +647,28 @@ __device__ __forceinline__ void kick_drift_body(AtomArrays a, int N, DevState *_
         double4 const r = a.cref[i];
         double const ux = (c.x - r.x) * Ln[0], uy = (c.y - r.y) * Ln[1], uz = (c.z - r.z) * Ln[2];
         dmax = fmax(dmax, ux * ux + uy * uy + uz * uz);
+        double4 const q = a.mref[i];
+        double const vx2 = (c.x - q.x) * Ln[0], vy2 = (c.y - q.y) * Ln[1], vz2 = (c.z - q.z) * Ln[2];
+        dmid = fmax(dmid, vx2 * vx2 + vy2 * vy2 + vz2 * vz2);
     }
-    for (int off = 32; off > 0; off >>= 1)
+    for (int off = 32; off > 0; off >>= 1) {
         dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
-    __shared__ double sm[BLOCK / 64];
-    if ((threadIdx.x & 63) == 0)
-        sm[threadIdx.x >> 6] = dmax;
+        dmid = fmax(dmid, __shfl_xor(dmid, off, 64));
+    }
+    __shared__ double sm[2][BLOCK / 64];
+    if ((threadIdx.x & 63) == 0) {
+        sm[0][threadIdx.x >> 6] = dmax;
+        sm[1][threadIdx.x >> 6] = dmid;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double m = sm[0];
-        for (int w = 1; w < BLOCK / 64; ++w)
-            m = fmax(m, sm[w]);
+        double m = sm[0][0], mm = sm[1][0];
+        for (int w = 1; w < BLOCK / 64; ++w) {
+            m = fmax(m, sm[0][w]);
+            mm = fmax(mm, sm[1][w]);
+        }
         dmax_part[bk.id] = m;
+        dmax_part[DMAX_MID + bk.id] = mm;
     }
 }
 
@@ -711,8 +727,12 @@ __device__ __forceinline__ void rb_begin(DevState *gs, int use_cells) {
     gs->force_rebuild = 0;
     gs->do_rebuild = 1;
     gs->nbuilds++;
-    for (int c = 0; c < 3; ++c)
+    gs->mid_build = 1;   // the mid list is filtered from the new list in pass A
+    gs->nmid_builds++;
+    for (int c = 0; c < 3; ++c) {
         gs->L_build[c] = gs->L[c];
+        gs->L_mid[c] = gs->L[c];
+    }
     if (use_cells)
         for (int c = 0; c < 3; ++c)
             if (gs->L[c] / gs->ncell[c] < gs->r_list) {
@@ -1074,31 +1094,55 @@ __device__ __forceinline__ void list_body(AtomArrays a, ListArrays l, int N,
         return;
     // the decision, from the same inputs in every block (u_max^2 from
     // k_kick_drift's per-block maxima, nparts <= 4 BLOCK)
-    __shared__ double red[BLOCK / 64];
-    double m = 0.;
+    __shared__ double red[2][BLOCK / 64];
+    double m = 0., mm = 0.;
     {
-        double v[4];
+        double v[4], w[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             int const b = threadIdx.x + u * BLOCK;
             v[u] = b < nparts ? dmax_part[b] : 0.;
+            w[u] = b < nparts ? dmax_part[DMAX_MID + b] : 0.;
         }
         m = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
+        mm = fmax(fmax(w[0], w[1]), fmax(w[2], w[3]));
     }
-    for (int off = 32; off > 0; off >>= 1)
+    for (int off = 32; off > 0; off >>= 1) {
         m = fmax(m, __shfl_xor(m, off, 64));
-    if ((threadIdx.x & 63) == 0)
-        red[threadIdx.x >> 6] = m;
+        mm = fmax(mm, __shfl_xor(mm, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = m;
+        red[1][threadIdx.x >> 6] = mm;
+    }
     __syncthreads();
-    m = red[0];
-    for (int w = 1; w < BLOCK / 64; ++w)
-        m = fmax(m, red[w]);
-    double lam = 1e300;
-    for (int c = 0; c < 3; ++c)
+    m = red[0][0];
+    mm = red[1][0];
+    for (int w = 1; w < BLOCK / 64; ++w) {
+        m = fmax(m, red[0][w]);
+        mm = fmax(mm, red[1][w]);
+    }
+    double lam = 1e300, lam_mid = 1e300;
+    for (int c = 0; c < 3; ++c) {
         lam = fmin(lam, gs->L[c] / gs->L_build[c]);
+        lam_mid = fmin(lam_mid, gs->L[c] / gs->L_mid[c]);
+    }
     bool const need = gs->force_rebuild || !(lam * gs->r_list - 2. * sqrt(m) > gs->rc);
-    if (!need)
+    if (!need) {
+        // the mid list: a pair outside it is at least lambda r_mid - 2 u apart,
+        // u the largest displacement since its build; block 0 records the
+        // decision (every block reads the same inputs, and only pass A uses it)
+        if (bk.id == 0 && threadIdx.x == 0) {
+            bool const mid = !(lam_mid * gs->r_mid - 2. * sqrt(mm) > gs->rc);
+            gs->mid_build = mid ? 1 : 0;
+            if (mid) {
+                gs->nmid_builds++;
+                for (int c = 0; c < 3; ++c)
+                    gs->L_mid[c] = gs->L[c];
+            }
+        }
         return;
+    }
 
     __shared__ ListSmem sm;
     if (threadIdx.x == 0) {
@@ -1166,6 +1210,9 @@ __global__ __launch_bounds__(BLOCK) void k_density(AtomArrays a, ListArrays l, R
     unsigned long long const gmask = group_mask<LPA>();
     unsigned long long const below = (l64 == 0) ? 0ull : (~0ull >> (64 - l64));
     double const L[3] = {gs->L[0], gs->L[1], gs->L[2]};
+    // scan the mid list, or the full list while the mid list is refiltered
+    bool const mid_build = gs->mid_build != 0;
+    double const rm2 = gs->r_mid * gs->r_mid;
     double acc[PA_N];
 #pragma unroll
     for (int k = 0; k < PA_N; ++k)
@@ -1180,14 +1227,16 @@ __global__ __launch_bounds__(BLOCK) void k_density(AtomArrays a, ListArrays l, R
         int cnt = 0;
         if (active) {
             ci = a.pos[i];
-            cnt = l.nnbr[i];
+            cnt = mid_build ? l.nnbr[i] : l.mcnt[i];
         }
-        const int *nb = l.nbr + (size_t)(active ? i : 0) * l.stride;
-        int *cb = l.cnbr + (size_t)(active ? i : 0) * l.stride;
-        int nin = 0;   // compacted entries so far (uniform in the group)
+        size_t const row = (size_t)(active ? i : 0) * l.stride;
+        const int *nb = (mid_build ? l.nbr : l.mnbr) + row;
+        int *cb = l.cnbr + row;
+        int *mb = l.mnbr + row;
+        int nin = 0, nmid = 0;   // compacted entries so far (uniform in the group)
         for (int k0 = 0; k0 < cnt; k0 += LPA) {
             int const k = k0 + lane;
-            bool in = false;
+            bool in = false, inm = false;
             int j = 0;
             double dx = 0., dy = 0., dz = 0., r2 = 0.;
             double4 cj;
@@ -1197,6 +1246,13 @@ __global__ __launch_bounds__(BLOCK) void k_density(AtomArrays a, ListArrays l, R
                 min_image(ci, cj, L, dx, dy, dz);
                 r2 = dx * dx + dy * dy + dz * dz;
                 in = !(r2 > rc2);
+                inm = !(r2 > rm2);
+            }
+            if (mid_build) {
+                unsigned long long const balm = __ballot(inm) & gmask;
+                if (inm)
+                    mb[nmid + __popcll(balm & below)] = j;
+                nmid += __popcll(balm);
             }
             unsigned long long const bal = __ballot(in) & gmask;
             if (in) {
@@ -1242,6 +1298,10 @@ __global__ __launch_bounds__(BLOCK) void k_density(AtomArrays a, ListArrays l, R
         }
         if (active && lane == 0) {
             l.ccnt[i] = nin;
+            if (mid_build) {
+                l.mcnt[i] = nmid;
+                a.mref[i] = ci;
+            }
             int const ti = (int)ci.w;
             if (rho > ft.rhomax[ti]) {
                 atomicOr(&gs->err, (int)DE_RHOMAX);

@@ -196,6 +196,9 @@ __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTab
     unsigned long long const gmask = group_mask<LPA>();
     unsigned long long const below = (l64 == 0) ? 0ull : (~0ull >> (64 - l64));
     double const L[3] = {gs->L[0] * 1e8, gs->L[1] * 1e8, gs->L[2] * 1e8};   // A
+    // scan the mid list, or the full list while the mid list is refiltered
+    bool const mid_build = gs->mid_build != 0;
+    double const rm2 = (gs->r_mid * 1e8) * (gs->r_mid * 1e8);   // A^2
     int const rowS = rt.nf * P::S;           // floats per interval (fits int)
     float const nm1 = (float)(rt.n - 1);
     double acc[PA_N];
@@ -213,12 +216,13 @@ __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTab
         int cnt = 0;
         if (active) {
             ci = a.pos[i];
-            cnt = l.nnbr[i];
+            cnt = mid_build ? l.nnbr[i] : l.mcnt[i];
         }
         int const ia = active ? i : 0;
-        const int *nb = l.nbr + (size_t)ia * l.stride;
+        const int *nb = (mid_build ? l.nbr : l.mnbr) + (size_t)ia * l.stride;
         int *cb = l.cnbr + (size_t)ia * l.stride;
-        int nin = 0;
+        int *mb = l.mnbr + (size_t)ia * l.stride;
+        int nin = 0, nmid = 0;
         // Software pipeline: the position of the next candidate and the index
         // of the one after are loaded while this candidate is processed (the
         // table record of this candidate is requested before them, so waiting
@@ -233,7 +237,15 @@ __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTab
             double4 const cj = c0;
             double dx, dy, dz;
             min_image(ci, cj, L, dx, dy, dz);
-            bool const in = k < cnt && !(dx * dx + dy * dy + dz * dz > rc2);
+            double const r2 = dx * dx + dy * dy + dz * dz;
+            bool const in = k < cnt && !(r2 > rc2);
+            if (mid_build) {
+                bool const inm = k < cnt && !(r2 > rm2);
+                unsigned long long const balm = __ballot(inm) & gmask;
+                if (inm)
+                    mb[nmid + __popcll(balm & below)] = j;
+                nmid += __popcll(balm);
+            }
             float const fx = (float)dx, fy = (float)dy, fz = (float)dz;
             float const r = __builtin_amdgcn_sqrtf(fx * fx + fy * fy + fz * fz);
             int const tj = (int)cj.w;
@@ -283,6 +295,10 @@ __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTab
         }
         if (active && lane == 0) {
             l.ccnt[i] = nin;
+            if (mid_build) {
+                l.mcnt[i] = nmid;
+                a.mref[i] = ci;
+            }
             int const ti = (int)ci.w;
             // selected, not indexed: a dynamic index into the table struct
             // would put a copy of it in scratch memory

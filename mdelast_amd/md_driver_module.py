@@ -756,6 +756,12 @@ class MDDriver:
         return int(self._scalars()[nat.S_NLIST_ENTRIES])
 
     @property
+    def mid_list_builds(self) -> int:
+        """Filterings of the mid list that pass A scans (pairs within r_c + 0.3 A),
+        a diagnostic without counterpart in the reference."""
+        return int(self._scalars()[nat.S_NMID_BUILDS])
+
+    @property
     def nhc_general_path_count(self):
         """Chain propagations that took the general (range-reduced exp) path: a chain
         argument dt p/q beyond 1/8 (diagnostic; no counterpart in the reference)."""

@@ -86,6 +86,11 @@ def test_nialco108k_mixed_nvt_properties():
         assert np.max(np.abs(ra[:, k] / rb[:, k] - 1)) < tol, k
     assert np.max(np.abs(ra[:, 7] - rb[:, 7])) < 2e-7
     assert a.neighbor_list_builds >= 10
+    # the density pass scans the mid list (pairs within r_c + 0.3 A), refiltered
+    # at every full build and when an atom may have moved 0.15 A since
+    nb, nm = a.neighbor_list_builds, a.mid_list_builds
+    print("full builds", nb, "mid filterings", nm)
+    assert nb < nm < 200
 
 
 def _oracle_from_driver(d, data, T, q, p_eta):
