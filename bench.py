@@ -86,29 +86,68 @@ def make_driver(w, precision="fp64", lanes_per_atom=0):
                         precision=precision, lanes_per_atom=lanes_per_atom), (pots, eams)
 
 
-def force_pass_bytes(N, entries):
-    """Algorithmic HBM bytes of one force-pass launch (k_force): the pass-B share of
-    the per-atom-step byte budget of the path, 41 B per atom (position 24 B, type
-    1 B, F'(rho) 4 B read; force 12 B written) plus 4 B per stored list entry.
-    Neighbour positions, F' of neighbours and the tables are gathers of data
-    other atoms also read: re-use, not compulsory traffic."""
-    return 41.0 * N + 4.0 * entries
+def pass_bytes(N, pairs_rc, mid_entries, list_entries, mid_frac):
+    """Algorithmic HBM bytes of one launch of each pair pass in this layout: each
+    atom's own data read once and its results written once; neighbour positions,
+    their F' and the tables are gathers of data other atoms read too (re-use).
+
+    force pass (k_force_mx): position 32 B, F'/F'' 8 B, F 8 B, list length 4 B
+      read, forces 3 x 8 B and energy 8 B written = 84 B per atom, plus 4 B per
+      entry of its list of pairs within r_c;
+    density pass (k_density_mx): position 32 B and list length 4 B read; list
+      length 4 B, F 8 B, rho 8 B, F'/F'' 16 + 8 B and the sampled g, |h|^2 64 B
+      written = 144 B per atom, plus 4 B per written r_c entry and 4 B per mid-list
+      entry scanned; on the fraction mid_frac of steps that refilter the mid
+      list it reads the full list (4 B per entry) as well."""
+    force = 84.0 * N + 4.0 * pairs_rc
+    density = 144.0 * N + 4.0 * pairs_rc + 4.0 * mid_entries + mid_frac * 4.0 * list_entries
+    return {"k_density_mx": density, "k_force_mx": force}
+
+
+def step_bytes(N, list_entries, rebuild_every=20):
+    """Algorithmic bytes of one whole step by the per-atom-step budget of the
+    path's analysis (mixed precision): 238 B + 8 B per stored list entry (two
+    passes) + (4 B per entry + 32 B) per rebuild, amortised."""
+    K = list_entries / N
+    return N * (238.0 + 8.0 * K + (4.0 * K + 32.0) / rebuild_every)
 
 
 def read_pmc_profile():
-    """HBM bytes per k_force dispatch from the committed rocprofv3 --pmc summary."""
-    path = os.path.join(ROOT, "profiles", "k_force_nialco108k_pmc.txt")
+    """Per-kernel counter figures from the committed rocprofv3 --pmc summary:
+    {kernel: {hbm_bytes_per_dispatch, valu_busy, ta_busy, tcp_accesses_per_dispatch}}."""
+    path = os.path.join(ROOT, "profiles", "pair_passes_nialco108k_pmc.txt")
     if not os.path.exists(path):
-        return None
-    vals = {}
+        return {}
+    out = {}
     for line in open(path):
-        if ":" in line and not line.startswith("#"):
-            k, v = line.split(":", 1)
-            vals[k.strip()] = v.strip()
+        line = line.split("#", 1)[0].strip()
+        if ":" not in line:
+            continue
+        k, v = line.split(":", 1)
+        if "." not in k:
+            continue
+        kern, field = k.strip().split(".", 1)
+        try:
+            out.setdefault(kern, {})[field] = float(v)
+        except ValueError:
+            pass
+    return out
+
+
+def cpu_info():
+    model = None
     try:
-        return float(vals["hbm_bytes_per_dispatch"])
-    except (KeyError, ValueError):
-        return None
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "cores_available": avail}
 
 
 def bench_replicas(args):
@@ -186,17 +225,70 @@ def copy_calibration(mib):
             "bytes_written": nbytes, "launches": 3}
 
 
-def cpu_baseline(w, nsteps=2, rebuild_every=20):
-    """The oracle (single-threaded FP64 C restatement of the reference) on one core:
-    one construction (O(N^2) list build + forces) and nsteps steps of the same
-    workload; per-step cost = step time + build time / rebuild_every."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
+def _pin_last_core():
     try:
         cpus = sorted(os.sched_getaffinity(0))
         os.sched_setaffinity(0, {cpus[-1]})
+        return cpus
     except (AttributeError, OSError):
-        cpus = None
+        return None
+
+
+def cpu_calibration(nsteps=300, rebuild_every=20):
+    """The oracle at the settings of the reference's published step time
+    (README.md:277-283: 500 atoms of Cu with 30 Zn, quintic tables, list r_skin 1.2
+    rebuilt every 20 steps, observables every step; 3.8 ms/step on one core of a
+    Xeon Gold 6534, GCC 14.2 -O2): NPT at 300 K, dt 2 fs, chains of 10, from the
+    FCC lattice a = 3.6345 A with Zn placed by RandomState(42)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    data = setfl.read_setfl("ZnCu.eam.alloy")
+    a, side, T = 3.6345, 5, 300.0
+    basis = np.array([[0, 0, 0], [.5, .5, 0], [.5, 0, .5], [0, .5, .5]])
+    idx = np.indices((side, side, side)).reshape(3, -1).T
+    pos = ((idx[:, None, :] + 0.25 + basis[None, :, :]) * a).reshape(-1, 3) - side * a / 2.0
+    N = len(pos)
+    types = np.full(N, data.names.index("Cu"), dtype=np.uint32)
+    types[np.random.RandomState(42).choice(N, 30, replace=False)] = data.names.index("Zn")
+    mass_t = np.array([e.atomic_weight / U.Navogadro for e in data.elements])
+    m = mass_t[types]
+    rs = np.random.RandomState(0)
+    sig = np.sqrt(U.k_boltz * T / m)
+    v = np.stack([rs.normal(0.0, sig) for _ in range(3)], axis=1)
+    v -= (v * m[:, None]).sum(0) / m.sum()
+    v *= np.sqrt((3.0 * N - 3.0) * U.k_boltz * T / 2.0 / (0.5 * np.sum(m[:, None] * v ** 2)))
+    Nf, kT = 3.0 * (N - 1.0), U.k_boltz * T
+    q = np.ones(10) * (5e-14) ** 2 * kT
+    q[0] *= Nf
+    qb = np.ones(10) * (5e-13) ** 2 * kT
+    W = Nf * (5e-13) ** 2 * kT
+    o = O.MD(positions=pos * U.angstrom, velocities=v, types=types, mass=mass_t,
+             box=[side * a * U.angstrom] * 3, data=data, ensemble=O.NPT,
+             thermostat=(T, q, rs.normal(0.0, np.sqrt(q * kT)), Nf),
+             barostat=(1e-10 * U.GPa, W, rs.normal(0.0, np.sqrt(W * kT))),
+             baro_thermostat=(T, qb, rs.normal(0.0, np.sqrt(qb * kT)), 1.0), Nf=Nf, dt=2e-15,
+             neighbors_list=True, r_skin=1.2)
+    t0 = time.perf_counter()
+    for k in range(nsteps):
+        if k % rebuild_every == 0:
+            o.calculate_neighbors()
+        o.integrate()
+    dt_step = (time.perf_counter() - t0) / nsteps
+    return {"ms_per_step": 1e3 * dt_step, "published_ms_per_step": 3.8,
+            "ratio_to_published": 1e3 * dt_step / 3.8,
+            "sample": f"{nsteps} NPT steps of the 500-atom 6 at.% Zn cell, list 1.2 rebuilt "
+                      f"every {rebuild_every}, observables every step"}
+
+
+def cpu_baseline(w, nsteps=10, rebuild_every=20):
+    """The oracle (single-threaded FP64 C restatement of the reference, gcc -O2) on one
+    pinned core of this host: one construction of the same workload (its O(N^2) list
+    build and forces), then nsteps NVT steps with the list rebuilt every
+    rebuild_every steps (nsteps >= 100 gives >= 5 rebuilds). Per step = mean step
+    time + construction time / rebuild_every when no rebuild fell in the window."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    cpus = _pin_last_core()
     try:
         th = (w["T"], w["q"], w["p_eta"], w["Nf"])
         t0 = time.perf_counter()
@@ -205,20 +297,33 @@ def cpu_baseline(w, nsteps=2, rebuild_every=20):
                  barostat=(0.0, 1.0, 0.0), baro_thermostat=(w["T"], [1.0], [0.0], 1.0),
                  Nf=w["Nf"], dt=1e-15, neighbors_list=True, r_skin=1.2)
         t1 = time.perf_counter()
-        o.integrate(nsteps)
+        nreb = 0
+        for k in range(nsteps):
+            if k > 0 and k % rebuild_every == 0:
+                o.calculate_neighbors()
+                nreb += 1
+            o.integrate()
         t2 = time.perf_counter()
+        calib = cpu_calibration()
     finally:
         if cpus:
             os.sched_setaffinity(0, set(cpus))
-    t_step = (t2 - t1) / nsteps
-    t_build = max(0.0, (t1 - t0) - t_step)
-    per_step = t_step + t_build / rebuild_every
-    return {"value": w["N"] / per_step, "unit": "atom-steps/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle (FP64 C restatement, gcc -O2) on the same {w['N']}-atom "
-                       f"workload: 1 construction with the O(N^2) list build "
-                       f"({t1 - t0:.1f} s) + {nsteps} NVT steps ({t2 - t1:.1f} s); "
-                       f"per step = step + build/{rebuild_every}"),
-            "ms_per_step": 1e3 * per_step}
+    if nreb > 0:
+        per_step = (t2 - t1) / nsteps
+        how = f"mean over {nsteps} steps with {nreb} rebuilds in the window"
+    else:
+        t_step = (t2 - t1) / nsteps
+        per_step = t_step + (t1 - t0) / rebuild_every
+        how = (f"step mean + construction/{rebuild_every} (no rebuild in the "
+               f"{nsteps}-step window; the construction stands in for one)")
+    out = {"value": w["N"] / per_step, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+           "sample": (f"oracle (FP64 C restatement, gcc -O2), 1 pinned core, same "
+                      f"{w['N']}-atom workload: construction with O(N^2) list build "
+                      f"{t1 - t0:.1f} s, {nsteps} NVT steps {t2 - t1:.1f} s; per step = {how}; "
+                      f"50,000 steps would take {50000 * per_step / 3600:.1f} h (extrapolated)"),
+           "ms_per_step": 1e3 * per_step, "calibration": calib}
+    out.update(cpu_info())
+    return out
 
 
 def learned_cv(n_eval):
@@ -278,6 +383,9 @@ def main():
     ap.add_argument("--nside", type=int, default=30)
     ap.add_argument("--lanes-per-atom", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-baseline-steps", type=int, default=10,
+                    help="NVT steps of the oracle after its construction (>= 100 for a "
+                         "window with >= 5 list rebuilds; about 0.5 s each on one core)")
     ap.add_argument("--workload", default="nialco108k", choices=["nialco108k", "replicas", "cv"],
                     help="nialco108k: the single 108,000-atom instance (default); replicas: "
                          "independent NVT Cu / Cu-Zn replicas of 500-32,000 atoms advanced side "
@@ -318,6 +426,7 @@ def main():
     w = build_workload(nside=args.nside)
     d, keep = make_driver(w, args.precision, args.lanes_per_atom)
     d.run(args.warmup, 1, 20, records=False)
+    mid0 = d.mid_list_builds
     t0 = time.perf_counter()
     rec = d.run(args.steps, 1, 20, records=True)   # synchronises before returning
     t1 = time.perf_counter()
@@ -326,10 +435,32 @@ def main():
     N = w["N"]
     value = N * args.steps / wall
     entries = d.neighbor_list_entries
-    fbytes = force_pass_bytes(N, entries)
-    achieved = fbytes / (kt["force_ms"] * 1e-3) / 1e9
-    # the committed --pmc summary measures the mixed-precision force kernel
-    traffic = read_pmc_profile() if args.precision == "mixed" else None
+    pairs_rc = float(d.pairs_within_rc().sum())
+    mid_entries = float(d.mid_list_lengths().sum())
+    mid_frac = (d.mid_list_builds - mid0) / args.steps
+    alg = pass_bytes(N, pairs_rc, mid_entries, entries, mid_frac)
+    times = {"k_density_mx": kt["density_ms"], "k_force_mx": kt["force_ms"]}
+    # counter figures are committed for the mixed-precision kernels only
+    pmc = read_pmc_profile() if args.precision == "mixed" else {}
+
+    def kernel_roof(k):
+        t = times[k] * 1e-3
+        r = {"algorithmic_bytes_per_launch": alg[k], "ms_per_launch": times[k],
+             "achieved": alg[k] / t / 1e9}
+        r["frac"] = r["achieved"] / HBM_PEAK_GBS
+        c = pmc.get(k, {})
+        traffic = c.get("hbm_bytes_per_dispatch")
+        r["traffic"] = traffic
+        r["achieved_counter"] = traffic / t / 1e9 if traffic else None
+        r["frac_counter"] = r["achieved_counter"] / HBM_PEAK_GBS if traffic else None
+        for f in ("valu_busy", "ta_busy", "tcp_accesses_per_pair"):
+            r[f] = c.get(f)
+        return r
+
+    dom = max(times, key=times.get)
+    other = "k_force_mx" if dom == "k_density_mx" else "k_density_mx"
+    rd, ro = kernel_roof(dom), kernel_roof(other)
+    sb = step_bytes(N, entries)
     out = {
         "metric": "atom-steps/s",
         "value": value,
@@ -352,26 +483,39 @@ def main():
             "ns_per_day": 1e-6 * (args.steps / wall) * 86400.0,
             "precision": args.precision,
             "neighbor_entries_per_atom": entries / N,
+            "pairs_within_rc_per_atom": pairs_rc / N,
+            "mid_list_entries_per_atom": mid_entries / N,
+            "mid_list_filterings_per_step": mid_frac,
             "list_builds": d.neighbor_list_builds,
+            "lanes_per_atom": d.lanes_per_atom_used,
             "nhc_general_path": d.nhc_general_path_count,
             "kernel_ms": {k: round(v, 5) for k, v in kt.items()},
             "final_T_kin": float(rec[-1][11]),
             "final_etot_cons_err": float(rec[-1][7]),
         },
         "roofline": {
-            "kernel": "k_force (force/virial/Born pass)",
+            "kernel": dom,
             "bound": "hbm",
-            "achieved": achieved,
+            "achieved": rd["achieved"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "algorithmic_bytes_per_launch": fbytes,
+            "frac": rd["frac"],
+            "traffic": rd["traffic"],
+            "algorithmic_bytes_per_launch": rd["algorithmic_bytes_per_launch"],
+            "achieved_counter": rd["achieved_counter"],
+            "frac_counter": rd["frac_counter"],
+            "valu_busy": rd["valu_busy"],
+            "ta_busy": rd["ta_busy"],
+            "tcp_accesses_per_pair": rd["tcp_accesses_per_pair"],
+            "ms_per_launch": rd["ms_per_launch"],
+            other: ro,
+            "step": {"bytes_per_atom_step": sb / N, "achieved": sb * args.steps / wall / 1e9,
+                     "frac": sb * args.steps / wall / 1e9 / HBM_PEAK_GBS},
         },
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
-        out["cpu_baseline"] = cpu_baseline(w)
+        out["cpu_baseline"] = cpu_baseline(w, nsteps=args.cpu_baseline_steps)
     if args.copy_calibration_mib > 0:
         out["copy_calibration"] = copy_calibration(args.copy_calibration_mib)
     if not args.no_learned_cv:

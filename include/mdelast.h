@@ -229,7 +229,8 @@ enum {
     MDL_S_NLIST_ENTRIES = 78, /* stored (full) list entries */
     MDL_S_NHC_GENERAL = 79,   /* chain propagations that left the short-argument path */
     MDL_S_NMID_BUILDS = 80,   /* filterings of the mid list (pairs within r_c + 0.3 A) */
-    MDL_NSCALARS = 81
+    MDL_S_LANES_PER_ATOM = 81,/* lanes per atom of the pair passes (given or autotuned) */
+    MDL_NSCALARS = 82
 };
 mdl_status mdl_driver_get_scalars(mdl_driver *d, double *out);
 
@@ -238,7 +239,11 @@ mdl_status mdl_driver_get_scalars(mdl_driver *d, double *out);
  * scaled ones. */
 enum {
     MDL_A_POSITIONS = 0, MDL_A_COORDINATES, MDL_A_VELOCITIES, MDL_A_FORCES, MDL_A_RHO,
-    MDL_A_EPOT, MDL_A_EKIN
+    MDL_A_EPOT, MDL_A_EKIN,
+    MDL_A_PAIRS_RC = 7,       /* N: pairs within r_c of each atom at the last density pass
+                                 (the force pass's list length; EAM drivers only) */
+    MDL_A_PAIRS_MID = 8       /* N: entries of each atom's mid list (pairs within r_c + 0.3 A
+                                 at its last filtering, the density pass's scan length) */
 };
 mdl_status mdl_driver_get_atoms(mdl_driver *d, int32_t what, double *out);
 /* p_eta and eta of the thermostat (which = 0) or barostat chain (which = 1). */

@@ -25,12 +25,14 @@ S_INDEX = dict(epot_eam_atom=0, epot_atom=1, ekin_atom=2, etot_atom=3, ekin_tot=
                time=14, cycle=15, box0=16, box1=17, box2=18, bar_eps=19, bar_peps=20,
                mom0=21, mom1=22, mom2=23)
 S_BORN_STRESS, S_STRESS, S_BORN_ELAST = 24, 30, 36
-NRECORD, NSCALARS = 72, 81
+NRECORD, NSCALARS = 72, 82
 S_BOX_INITIAL, S_NLIST_BUILDS, S_WARN_VIRIAL, S_WARN_RMIN, S_NLIST_ENTRIES = 72, 75, 76, 77, 78
 S_NHC_GENERAL = 79
 S_NMID_BUILDS = 80
+S_LANES_PER_ATOM = 81
 
-A_INDEX = dict(positions=0, coordinates=1, velocities=2, forces=3, rho=4, epot=5, ekin=6)
+A_INDEX = dict(positions=0, coordinates=1, velocities=2, forces=3, rho=4, epot=5, ekin=6,
+               pairs_rc=7, pairs_mid=8)
 
 ERRORS = {1: "ARG", 2: "POSITION", 3: "TYPE", 4: "ENSEMBLE", 5: "RHOMAX", 6: "VOLUME",
           7: "MOMENTUM", 8: "PRESSURE", 9: "CONSERVATION", 10: "STATE", 11: "DEVICE",

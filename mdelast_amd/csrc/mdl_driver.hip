@@ -1072,6 +1072,13 @@ __global__ __launch_bounds__(GR_T) void k_pair_hist(const double4 *__restrict__ 
     }
 }
 
+__global__ __launch_bounds__(256) void k_int_to_double(const int *__restrict__ in, int n,
+                                                       double *__restrict__ out) {
+    int const i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
+        out[i] = (double)in[i];
+}
+
 __global__ __launch_bounds__(256) void k_fill_u64(unsigned long long *__restrict__ p, size_t n) {
     for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256)
         p[k] = 0ull;
@@ -2105,6 +2112,7 @@ mdl_status mdl_driver_get_scalars(mdl_driver *p, double *out) {
         out[MDL_S_BOX_INITIAL0 + k] = d->hs.L0[k];
     out[MDL_S_NLIST_ENTRIES] = (double)d->hs.nlist_entries;
     out[MDL_S_NMID_BUILDS] = (double)d->hs.nmid_builds;
+    out[MDL_S_LANES_PER_ATOM] = (double)d->lpa;
     return MDL_OK;
 }
 
@@ -2112,9 +2120,20 @@ mdl_status mdl_driver_get_atoms(mdl_driver *p, int32_t what, double *out) {
     Driver *d = D(p);
     if (!d || !out)
         return bad("null argument");
-    if (what < MDL_A_POSITIONS || what > MDL_A_EKIN)
+    if (what < MDL_A_POSITIONS || what > MDL_A_PAIRS_MID)
         return bad("unknown per-atom array");
     MDL_HIP(hipSetDevice(d->device));
+    if (what == MDL_A_PAIRS_RC || what == MDL_A_PAIRS_MID) {
+        if (!d->has_eam)
+            return bad("MDL_A_PAIRS_RC / MDL_A_PAIRS_MID: the driver has no density pass");
+        k_int_to_double<<<(d->N + 255) / 256, 256, 0, d->stream>>>(
+            what == MDL_A_PAIRS_RC ? d->l.ccnt : d->l.mcnt, d->N, d->d_stage);
+        MDL_HIP(hipGetLastError());
+        MDL_HIP(hipMemcpyAsync(out, d->d_stage, sizeof(double) * d->N, hipMemcpyDeviceToHost,
+                               d->stream));
+        MDL_HIP(hipStreamSynchronize(d->stream));
+        return MDL_OK;
+    }
     if (what == MDL_A_POSITIONS) {
         if (mdl_status s = d->sync_state())
             return s;

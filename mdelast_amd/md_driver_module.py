@@ -755,6 +755,22 @@ class MDDriver:
     def neighbor_list_entries(self) -> int:
         return int(self._scalars()[nat.S_NLIST_ENTRIES])
 
+    def pairs_within_rc(self) -> np.ndarray:
+        """Per atom, the pairs within r_c at the last density pass (both directions;
+        the length of the list the force pass walks), a diagnostic without
+        counterpart in the reference."""
+        return self._atoms("pairs_rc").astype(np.int64)
+
+    def mid_list_lengths(self) -> np.ndarray:
+        """Per atom, the entries of the mid list the density pass scans (pairs within
+        r_c + 0.3 A at its last filtering), a diagnostic."""
+        return self._atoms("pairs_mid").astype(np.int64)
+
+    @property
+    def lanes_per_atom_used(self) -> int:
+        """Lanes per atom of the pair passes, as given or as the autotune chose."""
+        return int(self._scalars()[nat.S_LANES_PER_ATOM])
+
     @property
     def mid_list_builds(self) -> int:
         """Filterings of the mid list that pass A scans (pairs within r_c + 0.3 A),
