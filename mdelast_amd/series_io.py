@@ -11,10 +11,11 @@ same keys go into one ``series.npz`` (``numpy.savez``; ``numpy.load`` reads them
 back with the same paths). The Gaussian plotting aids ``data_gauss`` and
 ``gauss`` are not written.
 
-``summarize_run`` and ``check_value`` follow ``benchmarks/bench_lib.py:47-151``,
-``verify_md`` the MD part of ``benchmarks/verify.py:80-110``, so the reference's
-expected-value files (``benchmarks/expected/*.json``) apply to GPU-produced
-output unchanged.
+``summarize_run`` reduces a series file to the benchmark quantities through a
+table (quantity, unit, reducer), and ``check_value`` / ``verify_md`` hold them
+to the entries of the reference's expected-value files
+(``benchmarks/expected/*.json``: band, bounds or report only), so those files
+apply to GPU-produced output unchanged.
 """
 
 from __future__ import annotations
@@ -82,107 +83,165 @@ class SeriesFile:
         self._z.close()
 
 
-def _mean(f, name):
-    return float(f[f"series/{name}/mean"])
+# ---------------------------------------------------------------------------
+# The benchmark quantities of one run, as a table: each quantity names its
+# unit and how it reduces the series file. The quantities, their definitions
+# and the acceptance rules are those of the reference's expected-value files
+# (benchmarks/expected/*.json, documented with benchmarks/bench_lib.py:47-157
+# and benchmarks/verify.py:80-110); the code is this package's own.
+# ---------------------------------------------------------------------------
+class RunView:
+    """A series file seen as one run: production starts at the stored sample
+    index ``n0`` (series/temperature_kin/cycles_thermalization); one sample is
+    stored every ``stride`` steps (thermal_results/sample_stride, else 1)."""
+
+    def __init__(self, f):
+        self.f = f
+        self.n0 = int(f["series/temperature_kin/cycles_thermalization"])
+        self.stride = int(f["thermal_results/sample_stride"]) \
+            if "thermal_results/sample_stride" in f else 1
+        self.samples = int(len(f["series/time"]))
+
+    def result(self, key, default=None):
+        k = f"thermal_results/{key}"
+        return np.asarray(self.f[k]) if k in self.f else default
+
+    def mean(self, name):
+        """The run's stored production mean of a scalar series."""
+        return float(self.f[f"series/{name}/mean"])
+
+    def samples_of(self, name):
+        return np.asarray(self.f[f"series/{name}/data"], dtype=float)
+
+    def voigt_mean(self, prefix, group):
+        """Mean over the Voigt entries of a cubic group (11: the three normal
+        diagonals, 12: the three normal off-diagonals, 44: the three shears) of a
+        matrix series' production means."""
+        return float(np.mean([self.mean(f"{prefix}_{a}_{b}") for a, b in CUBIC[group]]))
+
+    def econs_relative(self):
+        e = self.samples_of("etot_cons")
+        return e / e[0] - 1.0
 
 
-def _data(f, name):
-    return np.asarray(f[f"series/{name}/data"], dtype=float)
+def _econs_drift_per_ns(v):
+    """Least-squares slope of E_cons / E_cons(0) - 1 against time over production."""
+    t = np.asarray(v.f["series/time"], dtype=float)[v.n0:] * 1e9
+    return float(np.polyfit(t, v.econs_relative()[v.n0:], 1)[0])
 
 
-def _cubic_mean(f, prefix):
-    return {key: float(np.mean([_mean(f, f"{prefix}_{i}_{j}") for i, j in comps]))
-            for key, comps in CUBIC.items()}
+def _quantity_table():
+    """[(name, unit, reducer(RunView) -> float)]"""
+    table = [(f"C{g}", "GPa", lambda v, g=g: float(v.result(f"elasticity_tensor_{g}")) / GPA)
+             for g in CUBIC]
+    table.append(("cubic_spread_C11", "GPa",
+                  lambda v: float(np.ptp(np.diag(v.result("elasticity_tensor"))[:3])) / GPA))
+    table += [(f"CB{g}", "GPa", lambda v, g=g: v.mean(f"born_elasticity_tensor_{g}") / GPA)
+              for g in CUBIC]
+    table += [(f"fluc{g}", "GPa", lambda v, g=g: v.voigt_mean("stress_fluc", g) / GPA)
+              for g in CUBIC]
+    table += [(f"kin{g}", "GPa", lambda v, g=g: v.voigt_mean("elasticity_ideal_gas", g) / GPA)
+              for g in ("11", "44")]
+    table += [
+        ("K_npt", "GPa", lambda v: float(v.result("bulk_modulus", 0.0)) / GPA),
+        ("T_kin", "K", lambda v: v.mean("temperature_kin")),
+        # 1 / <1/T_conf> (the estimator's form) and <T_conf> per sample
+        ("T_conf_recip", "K", lambda v: 1.0 / v.mean("temperature_conf_inverse")),
+        ("T_conf_sample", "K",
+         lambda v: float(np.mean(1.0 / v.samples_of("temperature_conf_inverse")[v.n0:]))),
+        ("pressure", "GPa", lambda v: v.mean("pressure") / GPA),
+        ("L_L0", "", lambda v: v.mean("L_L0")),
+        ("V_V0", "", lambda v: v.mean("V_V0")),
+        # over the whole run, thermalisation included
+        ("econs_err_max", "", lambda v: float(np.max(np.abs(v.econs_relative())))),
+        ("econs_drift_per_ns", "1/ns", _econs_drift_per_ns),
+    ]
+    return table
+
+
+QUANTITIES = _quantity_table()
 
 
 def summarize_run(path) -> dict:
-    """The benchmark quantities of one run (bench_lib.py:47-107), in GPa, K, 1/ns
-    and dimensionless ratios: C11, C12, C44, CB*, fluc*, kin11, kin44, K_npt, T_kin,
-    T_conf_recip, T_conf_sample, pressure, L_L0, V_V0, econs_err_max,
-    econs_drift_per_ns, cubic_spread_C11, and the run length."""
+    """Every quantity of ``QUANTITIES`` for the run in series file ``path`` (GPa,
+    K, 1/ns, ratios), plus its length: n_steps and n_thermalization in MD steps
+    and the sample stride."""
     f = SeriesFile(path)
     try:
-        n0 = int(f["series/temperature_kin/cycles_thermalization"])
-        stride = int(f["thermal_results/sample_stride"]) \
-            if "thermal_results/sample_stride" in f else 1
-        out = {"n_steps": int(len(f["series/time"])) * stride,
-               "n_thermalization": n0 * stride, "sample_stride": stride,
-               "C11": float(f["thermal_results/elasticity_tensor_11"]) / GPA,
-               "C12": float(f["thermal_results/elasticity_tensor_12"]) / GPA,
-               "C44": float(f["thermal_results/elasticity_tensor_44"]) / GPA}
-        full = np.asarray(f["thermal_results/elasticity_tensor"]) / GPA
-        out["cubic_spread_C11"] = float(np.ptp(np.diag(full)[:3]))
-        for key in CUBIC:
-            out[f"CB{key}"] = _mean(f, f"born_elasticity_tensor_{key}") / GPA
-        fluc = _cubic_mean(f, "stress_fluc")
-        kin = _cubic_mean(f, "elasticity_ideal_gas")
-        for key in CUBIC:
-            out[f"fluc{key}"] = fluc[key] / GPA
-        out["kin11"] = kin["11"] / GPA
-        out["kin44"] = kin["44"] / GPA
-        out["K_npt"] = float(f["thermal_results/bulk_modulus"]) / GPA \
-            if "thermal_results/bulk_modulus" in f else 0.0
-        out["T_kin"] = _mean(f, "temperature_kin")
-        inv = _data(f, "temperature_conf_inverse")
-        out["T_conf_recip"] = 1.0 / _mean(f, "temperature_conf_inverse")
-        out["T_conf_sample"] = float(np.mean(1.0 / inv[n0:]))
-        out["pressure"] = _mean(f, "pressure") / GPA
-        out["L_L0"] = _mean(f, "L_L0")
-        out["V_V0"] = _mean(f, "V_V0")
-        econs = _data(f, "etot_cons")
-        rel = econs / econs[0] - 1.0
-        out["econs_err_max"] = float(np.max(np.abs(rel)))
-        t_ns = np.asarray(f["series/time"], dtype=float)[n0:] * 1e9
-        out["econs_drift_per_ns"] = float(np.polyfit(t_ns, rel[n0:], 1)[0])
+        v = RunView(f)
+        out = {"n_steps": v.samples * v.stride, "n_thermalization": v.n0 * v.stride,
+               "sample_stride": v.stride}
+        for name, _unit, reduce in QUANTITIES:
+            out[name] = reduce(v)
     finally:
         f.close()
     return out
 
 
-def check_value(name, actual, spec):
-    """(status, message) of one value against its spec (bench_lib.py:117-151):
-    ``expected`` and ``tol``, or ``min``/``max`` bounds, or ``info`` (report only)."""
-    if actual is None or (isinstance(actual, float) and not math.isfinite(actual)):
-        return "FAIL", f"{name}: missing or non-finite value"
-    unit = spec.get("unit", "")
+# ---------------------------------------------------------------------------
+# Acceptance rules of an expected-value entry: a band (expected +/- tol),
+# bounds (min and/or max), or a value only reported (info).
+# ---------------------------------------------------------------------------
+def _rule(spec):
     if spec.get("info"):
-        return "INFO", f"{name} = {actual:.6g} {unit} (not checked: {spec['info']})"
+        return "info", None
     if "expected" in spec:
-        exp, tol = float(spec["expected"]), float(spec["tol"])
-        d = actual - exp
-        return ("PASS" if abs(d) <= tol else "FAIL"), \
-            f"{name} = {actual:.6g} {unit}, expected {exp:.6g} +/- {tol:.3g} (diff {d:+.3g})"
-    lo, hi = spec.get("min", -math.inf), spec.get("max", math.inf)
-    return ("PASS" if lo <= actual <= hi else "FAIL"), \
-        f"{name} = {actual:.6g} {unit}, required [{lo:.3g}, {hi:.3g}]"
+        return "band", (float(spec["expected"]), float(spec["tol"]))
+    return "bounds", (float(spec.get("min", -math.inf)), float(spec.get("max", math.inf)))
+
+
+def check_value(name, actual, spec):
+    """(status, message) of one value under its expected-value entry: PASS or FAIL
+    for a band or bounds, INFO for a report-only entry; a missing or non-finite
+    value fails."""
+    unit = spec.get("unit", "")
+    if actual is None or not math.isfinite(float(actual)):
+        return "FAIL", f"{name}: no finite value"
+    kind, arg = _rule(spec)
+    shown = f"{name} = {float(actual):.6g}{(' ' + unit) if unit else ''}"
+    if kind == "info":
+        return "INFO", f"{shown}  [reported only: {spec['info']}]"
+    if kind == "band":
+        centre, tol = arg
+        off = float(actual) - centre
+        ok = abs(off) <= tol
+        return ("PASS" if ok else "FAIL"), f"{shown}  [{centre:.6g} +/- {tol:.3g}, off by {off:+.3g}]"
+    lo, hi = arg
+    ok = lo <= float(actual) <= hi
+    return ("PASS" if ok else "FAIL"), f"{shown}  [within {lo:.3g} .. {hi:.3g}]"
 
 
 def verify_md(expected: dict, stage_paths: dict, require_full_length: bool = True):
-    """Rows (status, message) of an expected-value file's MD stages
-    (benchmarks/verify.py:80-110) against series files, ``stage_paths`` =
-    {"npt": path, "nvt": path}. A run length different from the reference's is a
-    FAIL when ``require_full_length`` (the tolerances hold for full-length runs)."""
+    """Check the MD stages of an expected-value file against series files
+    (``stage_paths`` = {"npt": path, "nvt": path}). Returns (rows, summaries):
+    one (status, message) row per checked entry, in the file's order, and each
+    stage's summarize_run. A stage without a file fails; so does a run whose
+    length differs from the reference's when ``require_full_length`` (the
+    tolerances are those of full-length runs). The E_cons drift is checked by
+    magnitude (econs_drift_abs_per_ns). With a zero_kelvin_reference block, the
+    NVT constants are also held to its relative deviation."""
     rows, summaries = [], {}
     for stage, st in expected["stages"].items():
-        if stage not in stage_paths:
+        path = stage_paths.get(stage)
+        if path is None:
             rows.append(("FAIL", f"{stage}: no series file"))
             continue
-        s = summarize_run(stage_paths[stage])
-        summaries[stage] = s
-        if require_full_length and \
-                (s["n_steps"], s["n_thermalization"]) != (st["n_steps"], st["n_thermalization"]):
-            rows.append(("FAIL", f"{stage}: run length {s['n_thermalization']}+"
-                                 f"{s['n_steps'] - s['n_thermalization']} differs from the "
-                                 f"reference's {st['n_thermalization']}+"
-                                 f"{st['n_steps'] - st['n_thermalization']}"))
-        s = dict(s, econs_drift_abs_per_ns=abs(s["econs_drift_per_ns"]))
-        for q, spec in st["quantities"].items():
-            rows.append(check_value(f"{stage}.{q}", s.get(q), spec))
+        got = summarize_run(path)
+        summaries[stage] = got
+        want = (st["n_steps"], st["n_thermalization"])
+        have = (got["n_steps"], got["n_thermalization"])
+        if require_full_length and have != want:
+            rows.append(("FAIL", f"{stage}: run length {have[1]} + {have[0] - have[1]} steps "
+                                 f"(thermalisation + production), the reference's is "
+                                 f"{want[1]} + {want[0] - want[1]}"))
+        values = dict(got, econs_drift_abs_per_ns=abs(got["econs_drift_per_ns"]))
+        rows += [check_value(f"{stage}.{q}", values.get(q), spec)
+                 for q, spec in st["quantities"].items()]
     zk = expected.get("zero_kelvin_reference")
     if zk and "nvt" in summaries:
+        dev = zk["max_rel_deviation"]
         for q, ref in zk["values_GPa"].items():
-            dev = (summaries["nvt"][q] - ref) / ref
-            rows.append(check_value(f"nvt.{q} vs 0 K reference {ref} GPa", dev,
-                                    {"min": -zk["max_rel_deviation"],
-                                     "max": zk["max_rel_deviation"]}))
+            rel = (summaries["nvt"][q] - ref) / ref
+            rows.append(check_value(f"nvt.{q} relative to the 0 K value {ref} GPa", rel,
+                                    {"min": -dev, "max": dev}))
     return rows, summaries
