@@ -156,12 +156,12 @@ def bench_replicas(args):
     from mdelast_amd import get_potential, replicas as R
     elements, pots, eams = get_potential.get_EAM_alloy_file("ZnCu.eam.alloy")
     masses = [e[2] / U.Navogadro for e in elements]
-    specs = R.batched_replica_specs(args.replicas)
+    specs = R.batched_replica_specs(args.replicas, edge=tuple(args.replica_edge))
     # one lane count for the whole batch (16 measured best for this table)
     lpa = args.lanes_per_atom or (16 if args.batched else 0)
     drivers = [R.make_replica_driver(s, pots, eams, masses, precision=args.precision,
                                      lanes_per_atom=lpa) for s in specs]
-    graph_steps = 0 if args.batched else args.graph_steps
+    graph_steps = args.graph_steps
     for d in drivers:
         d.graph_steps = graph_steps
     natoms = sum(s["N"] for s in specs)
@@ -184,8 +184,10 @@ def bench_replicas(args):
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{len(specs)} independent NVT replicas, ZnCu EAM, FCC edge 5-20 cells "
-                        "(500-32,000 atoms), pure Cu or Cu-Zn up to 10 at.% Zn, T 100-900 K, "
+            "workload": f"{len(specs)} independent NVT replicas, ZnCu EAM, FCC edge "
+                        f"{args.replica_edge[0]}-{args.replica_edge[1]} cells "
+                        f"({4 * args.replica_edge[0] ** 3:,}-{4 * args.replica_edge[1] ** 3:,} "
+                        "atoms), pure Cu or Cu-Zn up to 10 at.% Zn, T 100-900 K, "
                         "dt 1 fs, NHC M=10, Born every step, list r_skin 1.2 rebuilt every 20",
             "replicas": len(specs),
             "atoms_total": natoms,
@@ -327,6 +329,20 @@ def cpu_baseline(w, nsteps=10, rebuild_every=20):
 
 
 def learned_cv(n_eval):
+    """The learned control-variate comparison in a child process (`--workload cv`):
+    the control variate uses torch, which must bind the HIP runtime before the
+    library does, while this process's plain MD path runs on the system runtime."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--workload", "cv",
+           "--cv-eval-frames", str(n_eval)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    lines = [ln for ln in res.stdout.splitlines() if ln.startswith("{")]
+    if res.returncode != 0 or not lines:
+        return {"error": f"child exited {res.returncode}: {res.stderr.strip()[-400:]}"}
+    return json.loads(lines[-1]).get("learned_cv")
+
+
+def _learned_cv_here(n_eval):
     """GPU-seconds to C11, C12 and C44 at 2 SE <= 1 GPa, plain and with the
     learned Stein control variate: 4,000-atom alpha-brass (6 at.% Zn random
     fcc, 10^3 cells, a = 3.667 A), NVT 300 K, dt 1 fs, frames every 20 steps;
@@ -391,9 +407,12 @@ def main():
                          "independent NVT Cu / Cu-Zn replicas of 500-32,000 atoms advanced side "
                          "by side; cv: only the learned control-variate comparison")
     ap.add_argument("--replicas", type=int, default=64)
+    ap.add_argument("--replica-edge", type=int, nargs=2, default=[5, 20], metavar=("LO", "HI"),
+                    help="replicas: FCC edges (cells) drawn from LO..HI (4 edge^3 atoms)")
     ap.add_argument("--graph-steps", type=int, default=20,
                     help="replicas: replay chunks of this many steps as one captured HIP graph "
-                         "per replica (0: launch step by step)")
+                         "(per replica, or one graph of batched steps with --batched; 0: launch "
+                         "step by step)")
     ap.add_argument("--batched", action="store_true",
                     help="replicas: each pass of a step as one launch over every replica "
                          "(mdl_replicas_run) instead of per-replica launches")
@@ -418,7 +437,7 @@ def main():
     if args.workload == "replicas":
         return bench_replicas(args)
     if args.workload == "cv":
-        r = learned_cv(args.cv_eval_frames)
+        r = _learned_cv_here(args.cv_eval_frames)
         print(json.dumps({"metric": "GPU-s to C11/C12/C44 at +/-1 GPa (worst head)",
                           "value": r.get("gpu_s_cv_worst"), "unit": "s",
                           "higher_is_better": False, "n_gpus": 1, "learned_cv": r}), flush=True)

@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -136,20 +137,8 @@ SIGNATURES = {
 }
 
 _lib = None
-
-
-def _one_hip_runtime():
-    """Load torch's HIP runtime before this library when torch is installed.
-
-    PyTorch ships its own libamdhip64. If this library is loaded first, the
-    process maps the system runtime, torch later maps its own, and the second
-    runtime finds no device ("No HIP GPUs are available"). Loading torch first
-    makes this library bind to the runtime torch already mapped, so the
-    control variate's torch tensors and the driver share one runtime."""
-    try:
-        import torch  # noqa: F401
-    except ImportError:
-        pass
+# whether torch's HIP runtime was in the process when the library was loaded
+BOUND_WITH_TORCH = None
 
 
 def lib():
@@ -158,7 +147,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise MdlError(11, f"{LIB_PATH} is not built (run `python -m mdelast_amd.build`)")
-        _one_hip_runtime()
+        # The library binds the HIP runtime already in the process (torch's,
+        # when torch was imported first) or else the system one. torch asks for
+        # its bundled runtime by another file name, so importing torch after
+        # this point would map a second runtime that finds no device: a
+        # process that uses the control variate imports torch (mdelast_amd.cv
+        # does) before the library's first use.
+        global BOUND_WITH_TORCH
+        BOUND_WITH_TORCH = "torch" in sys.modules
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -70,6 +70,19 @@ def _torch():
     return torch
 
 
+# torch and the library must share one HIP runtime: torch first (see
+# _native.lib). The plain MD path never imports torch.
+if nat._lib is not None and not nat.BOUND_WITH_TORCH:
+    raise ImportError(
+        "mdelast_amd.cv needs torch, and the library was loaded without it (bound to the "
+        "system HIP runtime; torch would map a second runtime with no device). Import "
+        "mdelast_amd.cv (or torch) before the first driver or library call of the process.")
+try:
+    import torch  # noqa: F401,E402  (the runtime the library will bind)
+except ImportError:
+    pass
+
+
 def _p(t):
     return C.c_void_p(t.data_ptr()) if t is not None else None
 
@@ -984,3 +997,36 @@ def brass_experiment(n_train=2000, n_valid=400, n_eval=4000, eval_runs=2, stride
                training_gpu_s=float(train_s + (n_train + n_valid) * stride * t_step))
     out["speedup"] = out["gpu_s_plain_worst"] / out["gpu_s_cv_worst"]
     return out
+
+
+def profile_case(nframes=32, epochs=1, stride=20, equil=400, log=print):
+    """A short pass over every control-variate kernel, for profilers (counter
+    runs serialise each dispatch, so the full experiment is too long there):
+    the 4,000-atom brass cell, nframes stored training frames, the fit with
+    ``epochs`` epochs of the network (forced on), and the in-loop controls of
+    nframes more frames."""
+    torch = _torch()
+    drv, sites, types = brass_driver()
+    drv.run(equil, sample_stride=equil)
+    fr = collect_frames(drv, nframes + nframes // 4, stride, 300.0, sites=sites)
+    train, valid = subset(fr, slice(0, nframes)), subset(fr, slice(nframes, None))
+    cvm = ControlVariate(sites, fr.box, 300.0, types)
+    fit = cvm.fit(train, valid, epochs=epochs)
+    cvm.use_net = True            # the controls pass through the network too
+    cbat = ControlBatch(cvm, drv.N)
+    cbat.run(drv, nframes, stride)
+    s = cbat.result()
+    torch.cuda.synchronize()
+    log(f"profile case: {nframes} frames, {epochs} epoch(s), held-out variance / plain "
+        f"{np.round(np.asarray(fit['var_cv_valid']) / np.asarray(fit['var_plain_valid']), 3)}, "
+        f"controls of {len(s)} frames")
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="control-variate kernels on the 4,000-atom brass "
+                                             "cell (a short case for profilers)")
+    ap.add_argument("--frames", type=int, default=32)
+    ap.add_argument("--epochs", type=int, default=1)
+    a = ap.parse_args()
+    profile_case(nframes=a.frames, epochs=a.epochs)

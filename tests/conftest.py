@@ -11,3 +11,9 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+    # the session runs the plain driver and the control variate (torch) in one
+    # process: torch is imported before the library binds a HIP runtime
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass

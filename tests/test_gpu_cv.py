@@ -4,9 +4,9 @@ on molecular-dynamics frames of the device loop.
 
 Tolerances: the products round their operands to bfloat16 and accumulate in
 FP32; the oracle rounds the same operands the same way, so forward values and
-tangents agree to FP32 accumulation (norm-wise 1e-4; an operand that lands on
-the other side of a bfloat16 rounding boundary moves one element by 2^-9
-relative). Gradients are compared with an FP64 reference that does not round,
+tangents agree to FP32 accumulation (norm-wise 1e-4 and 2e-4; an operand that
+lands on the other side of a bfloat16 rounding boundary moves one element by
+2^-9 relative). Gradients are compared with an FP64 reference that does not round,
 so they agree to the bfloat16 operand precision (norm-wise 2e-2).
 """
 
@@ -116,11 +116,14 @@ def test_colsum(R, N):
 
 
 def _net(seed=3):
+    """A network with weights of order one; every draw seeded here (not from
+    the device generator, whose state depends on what ran before)."""
     net = cv.SteinNet(seed=seed, device=DEV)
+    rs = np.random.RandomState(100 + seed)
     with torch.no_grad():
-        net.W3.mul_(30.0)            # weights of order one
-        net.b1.normal_(0, 0.1)
-        net.b3.normal_(0, 0.1)
+        net.W3.mul_(30.0)
+        net.b1.copy_(torch.tensor(rs.normal(0, 0.1, net.b1.shape), dtype=torch.float32))
+        net.b3.copy_(torch.tensor(rs.normal(0, 0.1, net.b3.shape), dtype=torch.float32))
     return net
 
 
@@ -134,7 +137,10 @@ def test_network_and_field_sums_match_the_oracle():
     wo, wto = O.net_forward(net.numpy_params(), D.astype(np.float32), J.astype(np.float32),
                             emulate_bf16=True)
     assert _rel(w.detach().cpu().numpy(), wo) < 1e-4
-    assert _rel(wt.detach().cpu().numpy(), wto) < 1e-4
+    # the tangents pass their own operand (T W at each layer) through the
+    # bfloat16 rounding as well, so twice as many operands can land on the
+    # other side of a rounding boundary
+    assert _rel(wt.detach().cpu().numpy(), wto) < 2e-4
     F = np.random.RandomState(2).normal(size=(len(pos), 3))
     G = np.random.RandomState(3).normal(size=(len(pos), 3))
     A, Cs, E = net.field_sums(t[0], t[1], t[2],
