@@ -97,8 +97,38 @@ static ForceFn pick_force(int kind, int lpa, bool s, bool e) {
     return s ? force_fn<MDL_BSPLINE, true, false>(lpa) : force_fn<MDL_BSPLINE, false, false>(lpa);
 }
 
-// Autotuner cache: (N, ntypes, kind, list stride bucket) -> lanes per atom
-static std::map<std::tuple<int64_t, int, int, int>, int> g_lpa_cache;
+// Autotuner cache: (device, N, ntypes, kind, precision, EAM or not, list stride
+// bucket) -> lanes per atom; drivers may be created from several threads
+using LpaKey = std::tuple<int, int64_t, int, int, int, int, int>;
+static std::map<LpaKey, int> g_lpa_cache;
+static std::mutex g_lpa_mutex;
+
+// The batched replica runs a driver leads (comes first in the list of): the
+// launch tables in one device buffer, uploaded once and sent again only when
+// they change, and the captured graph of one chunk of batched steps.
+struct RepSet {
+    void *buf = nullptr;
+    size_t cap = 0;
+    std::vector<char> image;       // what buf holds
+    std::vector<int64_t> shape;    // launch sizes and pass groups of that image
+    hipGraphExec_t exec = nullptr;
+    int64_t stride = 0, rebuild = 0, steps = 0, nrec = 0;
+
+    void drop_graph() {
+        if (exec)
+            (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+    }
+    void release() {
+        drop_graph();
+        if (buf)
+            (void)hipFree(buf);
+        buf = nullptr;
+        cap = 0;
+        image.clear();
+        shape.clear();
+    }
+};
 
 struct Driver {
     int device = 0;
@@ -148,6 +178,7 @@ struct Driver {
     // device staging of run()'s records, kept between calls (grown on demand)
     double *run_rec = nullptr;
     int64_t run_rec_cap = 0;
+    RepSet rep;
     double t_dens = 0, t_force = 0, t_step = 0;
 
     template <class T> mdl_status alloc(T **p, size_t n) {
@@ -165,6 +196,7 @@ struct Driver {
                 (void)hipEventDestroy(e);
         if (gexec)
             (void)hipGraphExecDestroy(gexec);
+        rep.release();
         for (void *p : allocs)
             (void)hipFree(p);
         if (stream)
@@ -421,12 +453,16 @@ __global__ __launch_bounds__(BLOCK) void k_autotune_jitter(double4 *__restrict__
 // by far more, and the list stays the one built on the undisplaced positions).
 // The positions are restored exactly and the error flags cleared afterwards.
 static mdl_status autotune(Driver &d) {
-    auto key = std::make_tuple((int64_t)d.N, d.ntypes, d.kind, d.l.stride / 32);
-    auto it = g_lpa_cache.find(key);
-    if (it != g_lpa_cache.end()) {
-        d.lpa = it->second;
-        d.gridA = d.gridB = d.grid_pairs(d.lpa);
-        return MDL_OK;
+    LpaKey const key(d.device, (int64_t)d.N, d.ntypes, d.kind, d.precision, d.has_eam,
+                     d.l.stride / 32);
+    {
+        std::lock_guard<std::mutex> lock(g_lpa_mutex);
+        auto it = g_lpa_cache.find(key);
+        if (it != g_lpa_cache.end()) {
+            d.lpa = it->second;
+            d.gridA = d.gridB = d.grid_pairs(d.lpa);
+            return MDL_OK;
+        }
     }
     hipEvent_t e0, e1;
     MDL_HIP(hipEventCreate(&e0));
@@ -472,7 +508,10 @@ static mdl_status autotune(Driver &d) {
     MDL_HIP(hipFree(saved_pos));
     d.lpa = best_lpa;
     d.gridA = d.gridB = d.grid_pairs(best_lpa);
-    g_lpa_cache[key] = best_lpa;
+    {
+        std::lock_guard<std::mutex> lock(g_lpa_mutex);
+        g_lpa_cache[key] = best_lpa;
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return MDL_OK;
@@ -1652,14 +1691,22 @@ __global__ __launch_bounds__(BLOCK) void k_finish_sum_rep(const RepArgs *__restr
         r.totals[blockIdx.x] = t;
 }
 
-// irec < 0: not a sampled step
+// irec < 0: not a sampled step; else the record index, counted from *base
+// when base is given (the records of a replayed chunk of steps)
 __global__ __launch_bounds__(BLOCK) void k_finish_tail_rep(const RepArgs *__restrict__ R, int mode,
-                                                          int irec, int next_sample) {
+                                                          int irec, const int *__restrict__ base,
+                                                          int next_sample) {
     RepArgs const &r = R[blockIdx.x];
     if (r.gs->err)
         return;
-    double *ro = (irec >= 0 && r.rec) ? r.rec + (size_t)MDL_NRECORD * irec : nullptr;
+    int64_t const ix = irec < 0 ? -1 : (int64_t)irec + (base ? *base : 0);
+    double *ro = (ix >= 0 && r.rec) ? r.rec + (size_t)MDL_NRECORD * ix : nullptr;
     finish_tail(r.gs, r.totals, nullptr, mode, ro, next_sample);
+}
+
+__global__ void k_set_int(int *p, int v) {
+    if (threadIdx.x == 0)
+        *p = v;
 }
 
 // A forced list build of every replica, in five launches (those of
@@ -1872,7 +1919,8 @@ static mdl_status replicas_run(std::vector<RunPlan> &plans, int64_t nsteps) {
         for (int b = 0; b < gB; ++b)
             mapBuild.push_back(make_int2((int)i, b));
     }
-    // every map in one device buffer: the int2 maps, then the int arrays
+    // the tables in the lead driver's buffer: the entries, the int2 maps, the
+    // int arrays, then the record base of a replayed chunk
     std::vector<int2> all;
     std::vector<size_t> at;
     for (const std::vector<int2> *v : {&mapK, &mapBin, &mapScat, &mapSort, &mapBuild}) {
@@ -1888,29 +1936,47 @@ static mdl_status replicas_run(std::vector<RunPlan> &plans, int64_t nsteps) {
     std::vector<int> ints(nblkN);
     ints.insert(ints.end(), nblkB.begin(), nblkB.end());
     ints.insert(ints.end(), scanWhich.begin(), scanWhich.end());
-    RepArgs *dtab = nullptr;
-    int2 *dmap = nullptr;
-    size_t const bytes = sizeof(int2) * all.size() + sizeof(int) * ints.size();
-    MDL_HIP(hipMalloc(&dtab, sizeof(RepArgs) * n));
-    if (hipMalloc(&dmap, bytes) != hipSuccess) {
-        (void)hipFree(dtab);
-        return bad("mdl_replicas_run: out of device memory");
+    size_t const oMap = (sizeof(RepArgs) * n + 255) & ~(size_t)255;
+    size_t const oInt = oMap + sizeof(int2) * all.size();
+    size_t const oBase = (oInt + sizeof(int) * ints.size() + 7) & ~(size_t)7;
+    size_t const bytes = oBase + sizeof(int64_t);
+    std::vector<char> image(bytes, 0);
+    std::memcpy(image.data(), tab.data(), sizeof(RepArgs) * n);
+    std::memcpy(image.data() + oMap, all.data(), sizeof(int2) * all.size());
+    std::memcpy(image.data() + oInt, ints.data(), sizeof(int) * ints.size());
+    std::vector<int64_t> shape = {(int64_t)n,           (int64_t)mapK.size(),
+                                  (int64_t)mapBin.size(), (int64_t)mapScat.size(),
+                                  (int64_t)mapSort.size(), (int64_t)mapBuild.size(),
+                                  (int64_t)scanWhich.size()};
+    for (const Group &g : groups)
+        shape.insert(shape.end(), {g.kind, g.lpa, (int64_t)g.atA, (int64_t)g.mapA.size(),
+                                   (int64_t)g.atB, (int64_t)g.mapB.size()});
+    RepSet &rs = d0->rep;
+    if (bytes > rs.cap) {
+        rs.release();
+        if (hipMalloc(&rs.buf, bytes) != hipSuccess) {
+            rs.buf = nullptr;
+            return bad("mdl_replicas_run: out of device memory");
+        }
+        rs.cap = bytes;
     }
-    hipError_t e = hipMemcpy(dtab, tab.data(), sizeof(RepArgs) * n, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(dmap, all.data(), sizeof(int2) * all.size(), hipMemcpyHostToDevice);
-    int *dint = reinterpret_cast<int *>(dmap + all.size());
-    if (e == hipSuccess)
-        e = hipMemcpy(dint, ints.data(), sizeof(int) * ints.size(), hipMemcpyHostToDevice);
+    if (rs.image != image || rs.shape != shape) {
+        rs.drop_graph();   // its launches were sized for the old tables
+        MDL_HIP(hipMemcpy(rs.buf, image.data(), bytes, hipMemcpyHostToDevice));
+        rs.image.swap(image);
+        rs.shape.swap(shape);
+    }
+    char *const buf = static_cast<char *>(rs.buf);
+    RepArgs const *dtab = reinterpret_cast<RepArgs *>(buf);
+    int2 const *dmap = reinterpret_cast<int2 *>(buf + oMap);
+    int const *dint = reinterpret_cast<int *>(buf + oInt);
+    int *dbase = reinterpret_cast<int *>(buf + oBase);
     int2 const *mK = dmap + at[0], *mBin = dmap + at[1], *mScat = dmap + at[2],
                *mSort = dmap + at[3], *mBuild = dmap + at[4];
     int const *dN = dint, *dB = dint + n, *dWhich = dint + 2 * n;
-    RunPlan &p0 = plans[0];
-    while (e == hipSuccess && p0.k < nsteps) {
-        int64_t const cyc = d0->cycle_host;
-        bool const smp = p0.sampled(cyc);
-        bool const build = p0.forced(cyc);
-        int const ns = p0.k + 1 < nsteps ? (p0.sampled(cyc + 1) ? 1 : 0) : -1;
+
+    // one batched step; irec as k_finish_tail_rep takes it
+    auto launch = [&](bool smp, bool build, int irec, const int *base, int ns) {
         k_kick_drift_rep<<<(unsigned)mapK.size(), BLOCK, 0, st>>>(dtab, mK);
         if (build) {
             k_rb_bin_rep<<<(unsigned)mapBin.size(), BLOCK, 0, st>>>(dtab, mBin, dN);
@@ -1933,8 +1999,71 @@ static mdl_status replicas_run(std::vector<RunPlan> &plans, int64_t nsteps) {
         else
             k_kick2_rep<false><<<(unsigned)mapK.size(), BLOCK, 0, st>>>(dtab, mK, 0);
         k_finish_sum_rep<<<dim3(NTOTAL, (unsigned)n), BLOCK, 0, st>>>(dtab);
-        k_finish_tail_rep<<<(unsigned)n, BLOCK, 0, st>>>(dtab, 1, smp ? (int)p0.irec : -1, ns);
-        e = hipGetLastError();
+        k_finish_tail_rep<<<(unsigned)n, BLOCK, 0, st>>>(dtab, 1, smp ? irec : -1, base, ns);
+        return hipGetLastError();
+    };
+
+    // graph_steps (the lead driver's) batched steps as one replay: from an
+    // aligned cycle whose chunk the stride and build interval divide, never
+    // the run's last step; the chunk's records go from the record base on
+    RunPlan &p0 = plans[0];
+    int64_t const G = d0->graph_steps;
+    auto chunk_fits = [&]() {
+        return G > 0 && p0.k + G < nsteps && d0->cycle_host % G == 0 && G % p0.stride == 0 &&
+               (p0.rebuild <= 0 || G % p0.rebuild == 0);
+    };
+    auto capture = [&]() -> mdl_status {
+        rs.drop_graph();
+        int64_t const c0 = d0->cycle_host;
+        int nrec = 0;
+        hipGraph_t graph = nullptr;
+        MDL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        for (int64_t j = 0; j < G && e == hipSuccess; ++j) {
+            int64_t const cyc = c0 + j;
+            bool const smp = p0.sampled(cyc);
+            e = launch(smp, p0.forced(cyc), smp ? nrec++ : -1, dbase, p0.sampled(cyc + 1) ? 1 : 0);
+        }
+        hipError_t const e2 = hipStreamEndCapture(st, &graph);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (graph)
+                (void)hipGraphDestroy(graph);
+            return hip_fail(e != hipSuccess ? e : e2, "batched step graph capture");
+        }
+        e = hipGraphInstantiate(&rs.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            rs.exec = nullptr;
+            return hip_fail(e, "batched step graph instantiation");
+        }
+        rs.stride = p0.stride;
+        rs.rebuild = p0.rebuild;
+        rs.steps = G;
+        rs.nrec = nrec;
+        return MDL_OK;
+    };
+
+    hipError_t e = hipSuccess;
+    while (e == hipSuccess && p0.k < nsteps) {
+        int64_t const cyc = d0->cycle_host;
+        if (chunk_fits()) {
+            if (!rs.exec || rs.stride != p0.stride || rs.rebuild != p0.rebuild || rs.steps != G)
+                if (mdl_status s = capture())
+                    return s;
+            k_set_int<<<1, 64, 0, st>>>(dbase, (int)p0.irec);
+            e = hipGetLastError();
+            if (e == hipSuccess)
+                e = hipGraphLaunch(rs.exec, st);
+            for (RunPlan &r : plans) {
+                r.d->cycle_host += G;
+                r.k += G;
+                r.irec += rs.nrec;
+            }
+            continue;
+        }
+        bool const smp = p0.sampled(cyc);
+        int const ns = p0.k + 1 < nsteps ? (p0.sampled(cyc + 1) ? 1 : 0) : -1;
+        e = launch(smp, p0.forced(cyc), (int)p0.irec, nullptr, ns);
         for (RunPlan &r : plans) {
             ++r.d->cycle_host;
             ++r.k;
@@ -1943,8 +2072,6 @@ static mdl_status replicas_run(std::vector<RunPlan> &plans, int64_t nsteps) {
         }
     }
     hipError_t const e2 = hipStreamSynchronize(st);
-    (void)hipFree(dtab);
-    (void)hipFree(dmap);
     if (e != hipSuccess || e2 != hipSuccess)
         return hip_fail(e != hipSuccess ? e : e2, "mdl_replicas_run");
     return MDL_OK;
@@ -2068,6 +2195,7 @@ mdl_status mdl_driver_set_graph_steps(mdl_driver *p, int32_t steps) {
         return bad("mdl_driver_set_graph_steps: bad arguments");
     MDL_HIP(hipSetDevice(d->device));
     d->drop_graph();
+    d->rep.drop_graph();
     joint_forget(d->uid);
     d->graph_steps = steps;
     return MDL_OK;

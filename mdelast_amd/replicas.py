@@ -37,7 +37,9 @@ def run_replicas(drivers: Sequence["mdm.MDDriver"], nsteps: int, sample_stride: 
     batched: every pass of a step is one launch over all replicas
     (``mdl_replicas_run``; mixed precision, all drivers at the same cycle; the
     pair passes one launch per lanes-per-atom group) instead of one launch per
-    replica on its own stream (``mdl_drivers_run``)."""
+    replica on its own stream (``mdl_drivers_run``). In a batched run the first
+    driver's ``graph_steps`` (when set) replays chunks of that many batched steps
+    as one captured graph, with the same results."""
     n = len(drivers)
     if n == 0:
         return []

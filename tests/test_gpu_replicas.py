@@ -161,6 +161,53 @@ def test_batched_launches_equal_one_at_a_time(skin, lanes):
         assert all(d.neighbor_list_builds - b0 > 4 for d, b0 in zip(a, builds0))
 
 
+@pytest.mark.parametrize("skin", [1.2, 1.04])
+def test_batched_graph_replay_equals_batched_launches(skin):
+    """Chunks of 20 batched steps replayed as one captured graph (the lead
+    driver's graph_steps) give the batched launches' records, forces and
+    positions bit for bit: sampled strides 5 and 4, forced builds every 10 and
+    20 steps and none, a requested rebuild, rebuilds that the list check decides
+    inside a replay (thin skin), runs that end inside a chunk, and a second set
+    led by the same driver (tables sent again, graph captured again)."""
+    elements, pots, eams = get_potential.get_EAM_alloy_file("ZnCu.eam.alloy")
+    masses = [e[2] / U.Navogadro for e in elements]
+    specs = [dict(edge=5, N=500, x_zn=0.06, T=300.0, place_seed=1, vel_seed=2),
+             dict(edge=6, N=864, x_zn=0.0, T=800.0, place_seed=3, vel_seed=4),
+             dict(edge=5, N=500, x_zn=0.10, T=900.0, place_seed=7, vel_seed=8)]
+
+    def make():
+        return [R.make_replica_driver(s, pots, eams, masses, precision="mixed",
+                                      lanes_per_atom=lp, skin=skin)
+                for s, lp in zip(specs, (16, 8, 16))]
+
+    a, b = make(), make()
+    a[0].graph_steps = 20
+    builds0 = [d.neighbor_list_builds for d in a]
+    for k, (n, stride, rebuild) in enumerate([(73, 5, 10), (47, 4, 20), (60, 5, 0)]):
+        c0 = a[0].cycle
+        ra = R.run_replicas(a, n, stride, rebuild, batched=True)
+        rb = R.run_replicas(b, n, stride, rebuild, batched=True)
+        for x, y in zip(ra, rb):
+            assert x.shape[0] == sum((c0 + k) % stride == 0 for k in range(1, n + 1))
+            assert np.array_equal(x, y)
+        if k == 0:
+            a[1].calculate_neighbors()
+            b[1].calculate_neighbors()
+    ra = R.run_replicas(a[:2], 40, 5, 10, batched=True)
+    rb = R.run_replicas(b[:2], 40, 5, 10, batched=True)
+    for x, y in zip(ra, rb):
+        assert np.array_equal(x, y)
+    for da, db in zip(a, b):
+        assert np.array_equal(da.forces_array(), db.forces_array())
+        assert np.array_equal(da.positions_array(), db.positions_array())
+        assert da.neighbor_list_builds == db.neighbor_list_builds
+    if skin < 1.1:
+        # rebuilds beyond the forced (cycles 0-70 every 10, 80 and 100, 180-210
+        # every 10 for the first two) and requested ones came from the check
+        forced = (8 + 2 + 4, 8 + 2 + 4 + 1, 8 + 2)
+        assert all(d.neighbor_list_builds - b0 > f for d, b0, f in zip(a, builds0, forced))
+
+
 def test_batched_launches_refuse_mixed_settings():
     """A set that cannot share launches is refused before any driver moves."""
     elements, pots, eams = get_potential.get_EAM_alloy_file("ZnCu.eam.alloy")
