@@ -202,6 +202,41 @@ def get_Ar_LJ(*, epsilon=None, sigma=None, rcut_fac=3.5):
                     atomic_weight=39.948, atom_name="Ar")
 
 
+def npt_to_nvt_material(npt) -> "Material":
+    """The NVT stage's material after an NPT stage, handed over as the
+    reference's two-stage calculation does (sample_case/run.py:251-283): the
+    positions of the NPT run's last trajectory dump, scaled by the production
+    mean (V/V0)^(1/3), in the production-mean box (the NPT stage's initial box
+    times the same factor). ``npt`` is the finished NPT ``MolecularDynamics``;
+    the result keeps its potentials, EAM tables, atomic weights and species
+    names, and takes each atom's species from the dump."""
+    import glob
+    dumps = sorted(glob.glob(path.join(npt.dump_dir, "*.xyz")),
+                   key=lambda p: int(path.splitext(path.basename(p))[0]))
+    if not dumps:
+        raise ValueError(f"no trajectory dump in {npt.dump_dir}")
+    mat = npt.material
+    species = {name: k for k, name in enumerate(mat.atom_name.split("-"))}
+    scale = npt.series.V_V0.mean ** (1.0 / 3.0)
+    positions, types = [], []
+    with open(dumps[-1]) as f:
+        lines = f.read().splitlines()
+    for line in lines[2:]:
+        fields = line.split()
+        if len(fields) < 7:
+            continue
+        xyz = [float(v) * Units.angstrom * scale for v in fields[1:4]]
+        positions.append(Vector3d(*xyz))
+        types.append(species[fields[0]])
+    if len(positions) != npt.Nparticles:
+        raise ValueError(f"{dumps[-1]} holds {len(positions)} atoms, the run had "
+                         f"{npt.Nparticles}")
+    box = (npt.Lx * scale, npt.Ly * scale, npt.Lz * scale)
+    return Material(potential=mat.potential, atomic_weight=mat.atomic_weight,
+                    atom_name=mat.atom_name, eam=mat.eam, lattice=positions, atom_type=types,
+                    box_length=box)
+
+
 # MDL_S_* columns of the per-sample records returned by MDDriver.run
 _R = nat.S_INDEX
 _R_SB, _R_ST, _R_CB = nat.S_BORN_STRESS, nat.S_STRESS, nat.S_BORN_ELAST

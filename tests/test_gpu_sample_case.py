@@ -19,7 +19,7 @@ import pytest
 
 from mdelast_amd import get_potential
 from mdelast_amd.md_driver_module import MDDriver, Units, Vector3d
-from mdelast_amd.molecular_dynamics import Material, MolecularDynamics
+from mdelast_amd.molecular_dynamics import Material, MolecularDynamics, npt_to_nvt_material
 
 pytestmark = pytest.mark.gpu
 
@@ -109,18 +109,7 @@ def test_zn06_npt_to_nvt_statistics(tmp_path):
 
     # handoff (sample_case/run.py:254-283): the last NPT dump, scaled to the
     # production-mean box
-    dumps = sorted((tmp_path / "npt" / "dump").glob("*.xyz"), key=lambda p: int(p.stem))
-    species = {e[0]: k for k, e in enumerate(elements)}
-    npos, ntypes = [], []
-    for line in dumps[-1].read_text().splitlines()[2:]:
-        f = line.split()
-        if len(f) < 7:
-            continue
-        npos.append([float(v) * Units.angstrom for v in f[1:4]])
-        ntypes.append(species[f[0]])
-    scale = npt.series.V_V0.mean ** (1.0 / 3.0)
-    npos = [Vector3d(*(np.array(p) * scale)) for p in npos]
-    nvt = _stage(mk(npos, ntypes, L * scale), MDDriver.NVT, 0, tmp_path / "nvt")
+    nvt = _stage(npt_to_nvt_material(npt), MDDriver.NVT, 0, tmp_path / "nvt")
     got_nvt = _quantities(nvt, elastic=True)
 
     bad = _check(got_npt, "npt")
