@@ -211,6 +211,8 @@ struct Driver {
     }
 
     int gridL = 1;                  // blocks of k_list: no more than the device holds at once
+    int gridL_solo = 1;             // ... when this driver runs alone
+    int64_t list_cap = 1;           // k_list blocks the device holds at once
     unsigned *d_bar = nullptr, *d_ticket_fin = nullptr;
 
     // The per-step list check (and the rare build it finds due).
@@ -936,6 +938,8 @@ static mdl_status create(const mdl_config *c, Driver &d) {
                                  : ((int64_t)N * LIST_LANES + BLOCK - 1) / BLOCK;
         int64_t const cap = std::min<int64_t>(512, (int64_t)per_cu * cus);
         d.gridL = (int)std::max<int64_t>(1, std::min<int64_t>(want, cap));
+        d.gridL_solo = d.gridL;
+        d.list_cap = (int64_t)per_cu * cus;
     }
 
     // constructor: list, forces, thermodynamics (md_driver.cpp:112-118)
@@ -1299,6 +1303,23 @@ void joint_forget(uint64_t uid) {
 
 static Driver *D(mdl_driver *p) { return reinterpret_cast<Driver *>(p); }
 
+// k_list's in-kernel build waits at grid barriers until all of its blocks are
+// resident. Drivers run side by side on one device may all be inside that
+// build at once, so their k_list grids together must fit the device: with
+// `share` drivers on it, each takes at most 1/share of what it holds. The
+// list a build makes does not depend on the grid (each cell or atom row is
+// built whole by one block), so the results do not change. Graphs captured
+// with the old grid are dropped.
+static void share_list_grid(Driver *d, int share) {
+    int64_t const g = std::max<int64_t>(
+        1, std::min<int64_t>(d->gridL_solo, d->list_cap / std::max(share, 1)));
+    if (g != d->gridL) {
+        d->gridL = (int)g;
+        d->drop_graph();
+        joint_forget(d->uid);
+    }
+}
+
 extern "C" {
 
 mdl_status mdl_driver_create(const mdl_config *cfg, mdl_driver **out) {
@@ -1343,6 +1364,7 @@ mdl_status mdl_driver_integrate(mdl_driver *p, int64_t nsteps) {
     Driver *d = D(p);
     if (mdl_status s = guard(d))
         return s;
+    share_list_grid(d, 1);
     int const smp = d->sample != 0 ? 1 : 0;
     if (nsteps > 0)
         k_pre<<<1, BLOCK, 0, d->stream>>>(d->gs, smp);
@@ -2088,6 +2110,8 @@ mdl_status mdl_driver_run(mdl_driver *p, int64_t nsteps, int64_t sample_stride,
     r.rebuild = rebuild_every;
     r.records = records;
     r.n_records = n_records;
+    if (r.d)
+        share_list_grid(r.d, 1);
     if (mdl_status s = r.begin())
         return s;
     while (r.k < nsteps)
@@ -2112,6 +2136,11 @@ mdl_status mdl_drivers_run(mdl_driver *const *drivers, int32_t n, int64_t nsteps
     }
     if (sample_stride < 1)
         return bad("sample_stride must be >= 1");
+    std::map<int, int> on_device;
+    for (int32_t i = 0; i < n; ++i)
+        ++on_device[D(drivers[i])->device];
+    for (int32_t i = 0; i < n; ++i)
+        share_list_grid(D(drivers[i]), on_device[D(drivers[i])->device]);
     for (int32_t i = 0; i < n; ++i) {
         RunPlan &r = plans[(size_t)i];
         r.d = D(drivers[i]);

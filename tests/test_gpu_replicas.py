@@ -34,6 +34,34 @@ def test_side_by_side_equals_one_at_a_time(precision):
     assert len(set(np.round(T, 6))) == 3
 
 
+def test_side_by_side_self_decided_rebuilds():
+    """Eight 32,000-atom replicas side by side with a thin skin, so that the list
+    check decides rebuilds inside k_list on many steps, often on several
+    streams at once. Their k_list grids share the device (each would fill it
+    alone), and the records, forces and build counts equal the replicas run one
+    at a time, bit for bit."""
+    elements, pots, eams = get_potential.get_EAM_alloy_file("ZnCu.eam.alloy")
+    masses = [e[2] / U.Navogadro for e in elements]
+    specs = [dict(edge=20, N=32000, x_zn=0.05 * (k % 3), T=300.0 + 100.0 * k,
+                  place_seed=10 + k, vel_seed=20 + k) for k in range(8)]
+
+    def make():
+        return [R.make_replica_driver(s, pots, eams, masses, precision="mixed",
+                                      lanes_per_atom=16, skin=1.04) for s in specs]
+
+    a, b = make(), make()
+    builds0 = [d.neighbor_list_builds for d in a]
+    ra = R.run_replicas(a, 60, 5, 0)
+    rb = [d.run(60, 5, 0) for d in b]
+    for x, y in zip(ra, rb):
+        assert x.shape == (12, 72)
+        assert np.array_equal(x, y)
+    for da, db, b0 in zip(a, b, builds0):
+        assert np.array_equal(da.forces_array(), db.forces_array())
+        assert da.neighbor_list_builds == db.neighbor_list_builds
+        assert da.neighbor_list_builds - b0 >= 2      # decided by the check, none forced
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_graph_replay_equals_step_launches(precision):
     """Chunks of 20 steps replayed as captured graphs give the step-by-step
