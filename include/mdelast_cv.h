@@ -103,6 +103,32 @@ mdl_status mdl_cv_field_contract_backward(const float *J, const float *L, const 
                                           const float *gcF, const float *gcG, int64_t M,
                                           int32_t nfield, float *gw, float *gwt, void *stream);
 
+/* Stress proxy of the zero-variance fields (no counterpart in the reference).
+ * For `frames` frames of N atoms of two species (types 0/1; the same types for
+ * every frame), positions pos [frames][N][3] FP64 in A, box: 3 FP64 on the
+ * HOST (2 rc <= each edge), all pairs by brute force, FP64 throughout:
+ *
+ * mdl_cv_stress_basis_sums: part[frame][block][p][k][v] (blocks = nblocks) =
+ *   the share of block `block` of Z[p][k][v] = sum over ordered pairs (i, j),
+ *   r_ij < rc, t_i + t_j = p, of fc(r) g_k(r) n_a n_b, with n = (x_i - x_j)/r,
+ *   Voigt v = (a, b) in the order xx, yy, zz, yz, xz, xy,
+ *   g_k(r) = exp(-(r - mu0 - k dmu)^2 / (2 width^2)), fc = (cos(pi r / rc) + 1)/2,
+ *   k = 0..31. Summing over blocks gives Z (576 values per frame).
+ *
+ * mdl_cv_stress_proxy: with psi_p(r) = fc(r) sum_k coef[p][k] g_k(r) (coef:
+ *   [3][32] FP64 device), the proxy h_v = sum over ordered pairs of
+ *   psi_{t_i+t_j}(r) n_a n_b has per atom i: hl[frame][i][0..5] = its share
+ *   sum_j psi(r_ij) n_a n_b, hl[frame][i][6..11] = the Laplacian of h_v along
+ *   atom i's coordinates, grad[frame][i][v][c] = d h_v / d x_{i,c}. */
+mdl_status mdl_cv_stress_basis_sums(const double *pos, const int32_t *types, int32_t N,
+                                    int32_t frames, const double *box, double rc, double mu0,
+                                    double dmu, double width, double *part, int32_t nblocks,
+                                    void *stream);
+mdl_status mdl_cv_stress_proxy(const double *pos, const int32_t *types, int32_t N,
+                               int32_t frames, const double *box, double rc, double mu0,
+                               double dmu, double width, const double *coef, double *grad,
+                               double *hl, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -347,6 +347,205 @@ __global__ __launch_bounds__(256) void k_field_contract_bwd(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Stress proxy h_v(x) = off_v + sum over ordered pairs (i, j), r_ij < rc, of
+// psi_{t_i + t_j}(r_ij) n_a n_b (Voigt v = (a, b), n = d / r, d = x_i - x_j),
+// psi_p(r) = fc(r) sum_k c[p][k] g_k(r), g_k = exp(-(r - mu_k)^2 / (2 w^2)).
+// Its Stein fields need, per atom m, grad_m h_v = 2 sum_j T_v(d_mj) with
+//   T_v,c = psi' n_a n_b n_c + (psi / r)(d_ac n_b + d_bc n_a - 2 n_a n_b n_c),
+// and the Laplacian lap_m h_v = 2 sum_j [(psi'' + 2 psi'/r - 6 psi/r^2) n_a n_b
+// + 2 (psi / r^2) d_ab] (the Laplacian in d of psi(r) n_a n_b; both atoms of
+// a pair see it). Brute force over all pairs, FP64 throughout (the Stein sums
+// need the derivatives of exactly the function evaluated).
+// ---------------------------------------------------------------------------
+__constant__ int SP_VA[6] = {0, 1, 2, 1, 0, 0};
+__constant__ int SP_VB[6] = {0, 1, 2, 2, 2, 1};
+
+struct SpBasis {
+    double mu0, dmu, iw2, pirc, rc2;
+};
+
+// The basis sums of one frame: Z[p][k][v] = sum over ordered pairs of species
+// pair p = t_i + t_j of fc(r) g_k(r) n_a n_b. One wave per atom at a time
+// (lanes 0-31 and 32-63 take alternate pairs, lane & 31 = k), 18 sums per
+// lane; per block, the sums over its waves and halves in a fixed order.
+__global__ __launch_bounds__(256) void k_cv_stress_basis(const double *__restrict__ pos,
+                                                         const int *__restrict__ types, int N,
+                                                         double Lx, double Ly, double Lz,
+                                                         SpBasis b, double *__restrict__ part) {
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6, k = lane & 31, half = lane >> 5;
+    int const frame = blockIdx.y;
+    const double *P = pos + (size_t)frame * N * 3;
+    double acc[3][6];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int v = 0; v < 6; ++v)
+            acc[p][v] = 0.0;
+    double const mu = b.mu0 + b.dmu * k;
+    double const iLx = 1.0 / Lx, iLy = 1.0 / Ly, iLz = 1.0 / Lz;
+    for (int i = blockIdx.x * 4 + wave; i < N; i += gridDim.x * 4) {
+        double const xi = P[3 * i], yi = P[3 * i + 1], zi = P[3 * i + 2];
+        int const ti = types[i];
+        for (int j0 = 0; j0 < N; j0 += 64) {
+            int const j = j0 + lane;
+            double dx = 0., dy = 0., dz = 0., r2 = 1e300;
+            if (j < N && j != i) {
+                dx = xi - P[3 * j];
+                dy = yi - P[3 * j + 1];
+                dz = zi - P[3 * j + 2];
+                dx -= Lx * rint(dx * iLx);
+                dy -= Ly * rint(dy * iLy);
+                dz -= Lz * rint(dz * iLz);
+                r2 = dx * dx + dy * dy + dz * dz;
+            }
+            unsigned long long m = __ballot(r2 < b.rc2);
+            while (m) {
+                // the half-waves take the next two pairs
+                int const s0 = __ffsll(m) - 1;
+                m &= m - 1;
+                int s1 = -1;
+                if (m) {
+                    s1 = __ffsll(m) - 1;
+                    m &= m - 1;
+                }
+                int const src = half ? s1 : s0;
+                int const srcc = src < 0 ? s0 : src;
+                double const ex = __shfl(dx, srcc, 64), ey = __shfl(dy, srcc, 64),
+                             ez = __shfl(dz, srcc, 64), er2 = __shfl(r2, srcc, 64);
+                int const pt = ti + types[j0 + srcc];
+                if (src < 0)
+                    continue;
+                double const r = sqrt(er2), ir = 1.0 / r;
+                double const n[3] = {ex * ir, ey * ir, ez * ir};
+                double const u = r - mu;
+                double const g = exp(-0.5 * u * u * b.iw2) * 0.5 * (cos(b.pirc * r) + 1.0);
+#pragma unroll
+                for (int v = 0; v < 6; ++v) {
+                    double const w = g * n[SP_VA[v]] * n[SP_VB[v]];
+                    acc[0][v] += pt == 0 ? w : 0.0;
+                    acc[1][v] += pt == 1 ? w : 0.0;
+                    acc[2][v] += pt == 2 ? w : 0.0;
+                }
+            }
+        }
+    }
+    __shared__ double red[4][64][18];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int v = 0; v < 6; ++v)
+            red[wave][lane][p * 6 + v] = acc[p][v];
+    __syncthreads();
+    double *out = part + ((size_t)frame * gridDim.x + blockIdx.x) * 576;
+    for (int q = threadIdx.x; q < 576; q += 256) {
+        int const p = q / 192, kk = (q / 6) % 32, v = q % 6;
+        double t = 0.0;
+        for (int w = 0; w < 4; ++w)
+            t += red[w][kk][p * 6 + v] + red[w][kk + 32][p * 6 + v];
+        out[q] = t;   // [p][k][v]
+    }
+}
+
+// The proxy's per-atom terms with fitted coefficients c [3][32]. One wave per
+// atom i; the half-waves take alternate pairs (from a ballot over 64
+// candidates at a time) and lane k & 31 evaluates basis function k, so
+// psi, psi' and psi'' are sums over the 32 lanes of a half (a butterfly). Lane
+// q < 30 of each half then keeps one of the atom's 30 sums: q < 6 the share
+// of h_v, 6 <= q < 12 the Laplacian of h_v, 12 + 3 v + c the gradient
+// component d h_v / d x_{i,c}. The halves are added at the end: hl[i] = the
+// first 12 sums, grad[i][v][c] the other 18.
+__global__ __launch_bounds__(256) void k_cv_stress_proxy(const double *__restrict__ pos,
+                                                         const int *__restrict__ types, int N,
+                                                         double Lx, double Ly, double Lz,
+                                                         SpBasis b, const double *__restrict__ coef,
+                                                         double *__restrict__ grad,
+                                                         double *__restrict__ hl) {
+    int const lane = threadIdx.x & 63, k = lane & 31, half = lane >> 5;
+    int const i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= N)
+        return;   // a whole wave
+    int const frame = blockIdx.y;
+    const double *P = pos + (size_t)frame * N * 3;
+    double const iLx = 1.0 / Lx, iLy = 1.0 / Ly, iLz = 1.0 / Lz;
+    double const xi = P[3 * i], yi = P[3 * i + 1], zi = P[3 * i + 2];
+    int const ti = types[i];
+    double const ck[3] = {coef[k], coef[32 + k], coef[64 + k]};
+    double const mu = b.mu0 + b.dmu * k;
+    // this lane's sum: its kind and Voigt / axis indices
+    int const q = k;
+    int const v = q < 6 ? q : (q < 12 ? q - 6 : (q < 30 ? (q - 12) / 3 : 0));
+    int const ax = q >= 12 && q < 30 ? (q - 12) % 3 : 0;
+    int const va = SP_VA[v], vb = SP_VB[v];
+    double acc = 0.0;
+    for (int j0 = 0; j0 < N; j0 += 64) {
+        int const j = j0 + lane;
+        double dx = 0., dy = 0., dz = 0., r2 = 1e300;
+        if (j < N && j != i) {
+            dx = xi - P[3 * j];
+            dy = yi - P[3 * j + 1];
+            dz = zi - P[3 * j + 2];
+            dx -= Lx * rint(dx * iLx);
+            dy -= Ly * rint(dy * iLy);
+            dz -= Lz * rint(dz * iLz);
+            r2 = dx * dx + dy * dy + dz * dz;
+        }
+        unsigned long long m = __ballot(r2 < b.rc2);
+        while (m) {
+            int const s0 = __ffsll(m) - 1;
+            m &= m - 1;
+            int s1 = -1;
+            if (m) {
+                s1 = __ffsll(m) - 1;
+                m &= m - 1;
+            }
+            int const src = half ? s1 : s0;
+            int const srcc = src < 0 ? s0 : src;   // an idle half repeats s0 with weight 0
+            double const live = src < 0 ? 0.0 : 1.0;
+            double const ex = __shfl(dx, srcc, 64), ey = __shfl(dy, srcc, 64),
+                         ez = __shfl(dz, srcc, 64), er2 = __shfl(r2, srcc, 64);
+            int const pt = ti + types[j0 + srcc];
+            double const r = sqrt(er2), ir = 1.0 / r;
+            double const u = r - mu, qv = u * b.iw2;
+            double const cg = (pt == 0 ? ck[0] : (pt == 1 ? ck[1] : ck[2])) * exp(-0.5 * u * qv);
+            double a0 = cg, a1 = -cg * qv, a2 = cg * (qv * qv - b.iw2);
+#pragma unroll
+            for (int off = 16; off > 0; off >>= 1) {
+                a0 += __shfl_xor(a0, off, 32);
+                a1 += __shfl_xor(a1, off, 32);
+                a2 += __shfl_xor(a2, off, 32);
+            }
+            double const cr = cos(b.pirc * r), sr = sin(b.pirc * r);
+            double const fc = 0.5 * (cr + 1.0), dfc = -0.5 * b.pirc * sr,
+                         ddfc = -0.5 * b.pirc * b.pirc * cr;
+            double const psi = fc * a0, dpsi = dfc * a0 + fc * a1,
+                         ddpsi = ddfc * a0 + 2.0 * dfc * a1 + fc * a2;
+            double const n[3] = {ex * ir, ey * ir, ez * ir};
+            double const nab = n[va] * n[vb], pr = psi * ir;
+            double t;
+            if (q < 6) {
+                t = psi * nab;
+            } else if (q < 12) {
+                t = 2.0 * ((ddpsi + 2.0 * dpsi * ir - 6.0 * pr * ir) * nab +
+                           (va == vb ? 2.0 * pr * ir : 0.0));
+            } else {
+                t = 2.0 * (dpsi * nab * n[ax] +
+                           pr * ((va == ax ? n[vb] : 0.0) + (vb == ax ? n[va] : 0.0) -
+                                 2.0 * nab * n[ax]));
+            }
+            acc += live * t;
+        }
+    }
+    acc += __shfl_down(acc, 32, 64);
+    if (half == 0 && q < 30) {
+        size_t const o = (size_t)frame * N + i;
+        if (q < 12)
+            hl[o * 12 + q] = acc;
+        else
+            grad[o * 18 + (q - 12)] = acc;
+    }
+}
+
 mdl_status launch_ok(const char *what) {
     hipError_t const e = hipGetLastError();
     if (e != hipSuccess)
@@ -376,6 +575,60 @@ mdl_status mdl_cv_descriptors(const double *pos, const int32_t *types, int32_t N
     k_cv_descriptors<<<blocks, 256, 0, (hipStream_t)stream>>>(
         pos, types, N, box[0], box[1], box[2], mdl::cv_basis(rc, mu0, dmu, width), D, J, L);
     return launch_ok("mdl_cv_descriptors");
+}
+
+static bool sp_args(const char *who, const double *pos, const int32_t *types, int32_t N,
+                    int32_t frames, const double *box, double rc, double width) {
+    if (!pos || !types || !box || N < 2 || frames < 1 || frames > 65535 || !(rc > 0.) ||
+        !(width > 0.)) {
+        set_error("%s: bad arguments", who);
+        return false;
+    }
+    if (!(2.0 * rc <= box[0] && 2.0 * rc <= box[1] && 2.0 * rc <= box[2])) {
+        set_error("%s: the cut-off exceeds half the box", who);
+        return false;
+    }
+    return true;
+}
+
+static SpBasis sp_basis(double rc, double mu0, double dmu, double width) {
+    SpBasis b;
+    b.mu0 = mu0;
+    b.dmu = dmu;
+    b.iw2 = 1.0 / (width * width);
+    b.pirc = 3.14159265358979323846 / rc;
+    b.rc2 = rc * rc;
+    return b;
+}
+
+mdl_status mdl_cv_stress_basis_sums(const double *pos, const int32_t *types, int32_t N,
+                                    int32_t frames, const double *box, double rc, double mu0,
+                                    double dmu, double width, double *part, int32_t nblocks,
+                                    void *stream) {
+    if (!sp_args("mdl_cv_stress_basis_sums", pos, types, N, frames, box, rc, width))
+        return MDL_ERR_ARG;
+    if (!part || nblocks < 1 || nblocks > 65535) {
+        set_error("mdl_cv_stress_basis_sums: bad arguments");
+        return MDL_ERR_ARG;
+    }
+    k_cv_stress_basis<<<dim3(nblocks, frames), 256, 0, (hipStream_t)stream>>>(
+        pos, types, N, box[0], box[1], box[2], sp_basis(rc, mu0, dmu, width), part);
+    return launch_ok("mdl_cv_stress_basis_sums");
+}
+
+mdl_status mdl_cv_stress_proxy(const double *pos, const int32_t *types, int32_t N,
+                               int32_t frames, const double *box, double rc, double mu0,
+                               double dmu, double width, const double *coef, double *grad,
+                               double *hl, void *stream) {
+    if (!sp_args("mdl_cv_stress_proxy", pos, types, N, frames, box, rc, width))
+        return MDL_ERR_ARG;
+    if (!coef || !grad || !hl) {
+        set_error("mdl_cv_stress_proxy: bad arguments");
+        return MDL_ERR_ARG;
+    }
+    k_cv_stress_proxy<<<dim3((N + 3) / 4, frames), 256, 0, (hipStream_t)stream>>>(
+        pos, types, N, box[0], box[1], box[2], sp_basis(rc, mu0, dmu, width), coef, grad, hl);
+    return launch_ok("mdl_cv_stress_proxy");
 }
 
 mdl_status mdl_cv_gemm(const float *A, const float *W, int32_t transB, const float *bias,

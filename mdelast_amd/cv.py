@@ -29,15 +29,28 @@ gradients are known: s[H g] = H s[g] + sum_i grad_i H . g_i. The fields are
   linear function of the descriptors fitted to the instantaneous pressure
   (grad h follows from the descriptor derivatives alone).
 
-The coefficients of each head are fitted by least squares on frames of a
-plain run after the network has been trained on them (Adam on the residual
+* zero-variance fields of a stress proxy: h_v = sum over ordered pairs of
+  psi_{t_i t_j}(r) n_a n_b (one radial function per species pair, 32
+  Gaussians, shared by the six Voigt components v = (a, b), fitted to the
+  instantaneous Born stress, R^2 ~ 0.99), and the fields grad h_a and
+  (h_a - h0_a) grad h_b, whose Stein sums are lap h_a + beta grad h_a . F and
+  grad h_a . grad h_b + (h_a - h0_a)(lap h_b + beta grad h_b . F). In a
+  harmonic crystal with grad h along the soft directions the latter is
+  const - beta k (h_a - h0_a)(h_b - h0_b): it carries the products of stress
+  fluctuations that dominate the integrands, deviatoric parts included.
+
+The coefficients of each head are fitted by least squares (after the network,
+when it is used, has been trained on the same frames by Adam on the residual
 variance); a head whose held-out variance the variate does not cut is switched
-off, and that head keeps the plain estimator.
+off, and that head keeps the plain estimator. ``crossfit_controls`` needs no
+training run: each half of a run is controlled by a variate fitted on the
+other half.
 
 Device work is hand-written HIP (include/mdelast_cv.h): descriptors with
-their derivatives and Laplacians from the driver's neighbour list, and the
-dense layers with their tangents (Y = act(XW + b), Y'_a = act'(XW + b) T_a W)
-and backward on bf16 MFMA with fp32 accumulation. PyTorch holds the device
+their derivatives and Laplacians from the driver's neighbour list; the stress
+proxy's basis sums and its per-atom gradients and Laplacians (FP64, all pairs);
+and the dense layers with their tangents (Y = act(XW + b), Y'_a = act'(XW + b)
+T_a W) and backward on bf16 MFMA with fp32 accumulation. PyTorch holds the device
 memory, runs the optimiser and the per-atom contractions; ``TangentDense`` is
 the torch.autograd.Function over the kernels.
 """
@@ -118,6 +131,42 @@ def descriptors(pos_A, types, box_A, basis: Basis = Basis()):
                                         basis.rc, basis.mu0, basis.dmu, basis.width,
                                         _p(D), _p(J), _p(L), _stream()))
     return D, J, L
+
+
+NSTRESS = 42       # zero-variance fields of the stress proxy: 6 gradients, 36 weighted
+SP_BLOCKS = 64     # blocks per frame of the basis sums
+
+
+def stress_basis_sums(pos_A, types32, box_A, basis: Basis = Basis()):
+    """Z [B, 3, 32, 6] (FP64, device) of B frames of positions [B, N, 3] in A:
+    over ordered pairs of species pair p = t_i + t_j, sum fc g_k n_a n_b
+    (mdl_cv_stress_basis_sums)."""
+    torch = _torch()
+    pos = pos_A.contiguous()
+    B, N, _ = pos.shape
+    part = torch.empty((B, SP_BLOCKS, 3, NB, 6), dtype=torch.float64, device=pos.device)
+    box = (C.c_double * 3)(*[float(b) for b in box_A])
+    _check(nat.lib().mdl_cv_stress_basis_sums(_p(pos), _p(types32), N, B, box, basis.rc,
+                                              basis.mu0, basis.dmu, basis.width, _p(part),
+                                              SP_BLOCKS, _stream()))
+    return part.sum(1)
+
+
+def stress_proxy(pos_A, types32, box_A, basis: Basis, coef):
+    """For B frames [B, N, 3] (A) and the proxy's coefficients [3, 32]: h's pair
+    sum [B, 6], its Laplacian summed over the atoms [B, 6] and its per-atom
+    gradients [B, N, 6, 3] (mdl_cv_stress_proxy)."""
+    torch = _torch()
+    pos = pos_A.contiguous()
+    B, N, _ = pos.shape
+    grad = torch.empty((B, N, 6, 3), dtype=torch.float64, device=pos.device)
+    hl = torch.empty((B, N, 12), dtype=torch.float64, device=pos.device)
+    box = (C.c_double * 3)(*[float(b) for b in box_A])
+    _check(nat.lib().mdl_cv_stress_proxy(_p(pos), _p(types32), N, B, box, basis.rc, basis.mu0,
+                                         basis.dmu, basis.width, _p(coef.contiguous()),
+                                         _p(grad), _p(hl), _stream()))
+    s = hl.sum(1)
+    return s[:, :6], s[:, 6:], grad
 
 
 def driver_descriptors(drv, basis: Basis = Basis(), device="cuda:0", out=None, La=None):
@@ -534,16 +583,16 @@ def collect_frames(drv, nframes, stride, T, sites=None, basis: Basis = Basis(),
 class ControlVariate:
     """s_h(x) = theta_h . X_h(x) per head h. X_h: 27 global Stein sums (the
     displacement fields e_a u_ib as they are, weighted by U - U0 and weighted by
-    h - h0, scaled by their training spread) and the sums of the head's two
-    learned fields (plain, h-weighted). h = sum_i V[t_i] . D_i is a
-    linear function of the descriptors fitted to the instantaneous pressure.
-    Heads whose held-out variance the variate does not cut are switched off
-    (s_h = 0), so the controlled estimate falls back to the plain one."""
-
-    NX = NGLOBAL_USED + 2
+    h - h0), with ``use_stress`` 42 more from the zero-variance fields of a
+    fitted stress proxy (``stress_features``), all scaled by their training
+    spread, and the sums of the head's two learned fields (plain, h-weighted).
+    h = sum_i V[t_i] . D_i is a linear function of the descriptors fitted to the
+    instantaneous pressure. Heads whose held-out variance the variate does not
+    cut are switched off (s_h = 0), so the controlled estimate falls back to
+    the plain one."""
 
     def __init__(self, sites, box, T, types, device="cuda:0", seed=0, basis: Basis = Basis(),
-                 use_net=True):
+                 use_net=True, use_stress=False):
         torch = _torch()
         self.device = device
         self.sites = torch.as_tensor(np.asarray(sites), dtype=torch.float64, device=device)
@@ -553,7 +602,14 @@ class ControlVariate:
         self.basis = basis
         self.net = SteinNet(seed=seed, device=device)
         self.use_net = use_net
-        self.t_scale = np.ones(NGLOBAL_USED)
+        self.use_stress = use_stress
+        self.nshared = NGLOBAL_USED + (NSTRESS if use_stress else 0)
+        self.NX = self.nshared + 2
+        self.types32 = self.types.to(torch.int32).contiguous()
+        self.sp_coef = torch.zeros((3, NB), dtype=torch.float64, device=device)
+        self.sp_off = np.zeros(6)
+        self.sp_h0 = np.zeros(6)
+        self.t_scale = np.ones(self.nshared)
         self.U0 = 0.0
         self.h0 = 0.0
         self.V = torch.zeros((2, NF), dtype=torch.float64, device=device)
@@ -605,6 +661,48 @@ class ControlVariate:
             self._tables_of = self.V
         return self._tables
 
+    # -- the stress proxy and its zero-variance fields ---------------------------
+    def fit_stress_proxy(self, fr: Frames, batch=32, ridge=1e-10):
+        """The stress proxy h_v = off_v + sum over ordered pairs of
+        psi_{t_i+t_j}(r) n_a n_b (psi = fc sum_k c[p, k] g_k, one radial function
+        per species pair shared by the six Voigt components), by least squares
+        of the Born stress (GPa) on the basis sums of ``fr``'s stored frames;
+        returns R^2 per component."""
+        Z = np.concatenate([stress_basis_sums(fr.store["pos"][k:k + batch] / U.angstrom,
+                                              self.types32, self.box, self.basis)
+                            .cpu().numpy() for k in range(0, fr.n, batch)])
+        n = fr.n
+        A = np.concatenate([Z.transpose(0, 3, 1, 2).reshape(n * 6, 3 * NB),
+                            np.tile(np.eye(6), (n, 1))], axis=1)
+        y = fr.born_stress.reshape(n * 6)
+        G = A.T @ A
+        G += ridge * np.trace(G) / len(G) * np.eye(len(G))
+        c = np.linalg.solve(G, A.T @ y)
+        torch = _torch()
+        self.sp_coef = torch.as_tensor(c[:3 * NB].reshape(3, NB), dtype=torch.float64,
+                                       device=self.device)
+        self.sp_off = c[3 * NB:]
+        pred = (A @ c).reshape(n, 6)
+        h = pred
+        self.sp_h0 = h.mean(0)
+        res = fr.born_stress - pred
+        return 1.0 - res.var(0) / fr.born_stress.var(0)
+
+    def stress_features(self, pos, F):
+        """[B, 42] Stein sums of the zero-variance fields of the stress proxy h
+        (positions in CGS, forces in eV/A): the fields grad h_a (sums lap h_a +
+        beta sum_m grad_m h_a . F_m) and (h_a - h0_a) grad h_b (sums
+        sum_m grad_m h_a . grad_m h_b + (h_a - h0_a) x the former)."""
+        torch = _torch()
+        hp, lap, grad = stress_proxy(pos / U.angstrom, self.types32, self.box, self.basis,
+                                     self.sp_coef)
+        h = hp + torch.as_tensor(self.sp_off, dtype=hp.dtype, device=hp.device)
+        dh = h - torch.as_tensor(self.sp_h0, dtype=hp.dtype, device=hp.device)
+        A = lap + self.beta * torch.einsum("bmac,bmc->ba", grad, F)
+        GG = torch.einsum("bmac,bmdc->bad", grad, grad)
+        Bm = GG + dh[:, :, None] * A[:, None, :]
+        return torch.cat([A, Bm.reshape(-1, 36)], dim=1)
+
     # -- features ----------------------------------------------------------------
     def _x(self, D, J, L, pos, Fc, Ut, grad=False):
         """X [B, 3, NX] (torch, FP64) for B frames of per-atom inputs (positions
@@ -623,6 +721,8 @@ class ControlVariate:
         Th = dh[:, None, None] * S + (gh.unsqueeze(3) * u.unsqueeze(2)).sum(1)
         t = torch.cat([S.reshape(B, 9), (dU[:, None, None] * S - M).reshape(B, 9),
                        Th.reshape(B, 9)], dim=1)
+        if self.use_stress:
+            t = torch.cat([t, self.stress_features(pos, F)], dim=1)
         t = t / tsc
         if self.use_net:
             with torch.set_grad_enabled(grad):
@@ -633,7 +733,7 @@ class ControlVariate:
                 nx = net_features(A, Cs, E, dh)
         else:
             nx = torch.zeros((B, 3, 2), dtype=t.dtype, device=t.device)
-        return torch.cat([t[:, None, :].expand(B, 3, NGLOBAL_USED), nx], dim=2)
+        return torch.cat([t[:, None, :].expand(B, 3, self.nshared), nx], dim=2)
 
     def _batch(self, fr: Frames, sl, grad=False):
         torch = _torch()
@@ -642,7 +742,7 @@ class ControlVariate:
         return self._x(st["D"][sl], st["J"][sl], st["L"][sl], st["pos"][sl], st["F"][sl], Ut,
                        grad)
 
-    def features(self, fr: Frames, batch=8):
+    def features(self, fr: Frames, batch=64):
         out = np.zeros((fr.n, 3, self.NX))
         for k0 in range(0, fr.n, batch):
             sl = slice(k0, min(fr.n, k0 + batch))
@@ -677,14 +777,17 @@ class ControlVariate:
         self.U0 = float(train.epot.mean())
         r2 = self.fit_proxy(train)
         say(f"pressure proxy: R^2 {r2:.4f} on the training frames")
-        self.t_scale = np.ones(NGLOBAL_USED)
+        if self.use_stress:
+            r2s = self.fit_stress_proxy(train)
+            say(f"stress proxy: R^2 {np.round(r2s, 4)} on the training frames")
+        self.t_scale = np.ones(self.nshared)
         use_net, self.use_net = self.use_net, False
         X = self.features(train)
-        self.t_scale = np.maximum(X[:, 0, :NGLOBAL_USED].std(0), 1e-30)
-        X[:, :, :NGLOBAL_USED] /= self.t_scale
+        self.t_scale = np.maximum(X[:, 0, :self.nshared].std(0), 1e-30)
+        X[:, :, :self.nshared] /= self.t_scale
         y = train.y()
         yc = y - y.mean(0)
-        glob = np.concatenate([self._fit_linear(X[:, :, :NGLOBAL_USED], yc), np.zeros((3, 2))],
+        glob = np.concatenate([self._fit_linear(X[:, :, :self.nshared], yc), np.zeros((3, 2))],
                               axis=1)
         self.coef = glob
         yv = valid.y()
@@ -697,7 +800,7 @@ class ControlVariate:
         if self.use_net and epochs > 0 and heads.any():
             n_fit = int(round(train.n * (1.0 - stop_fraction)))
             fit_part, stop_part = subset(train, slice(0, n_fit)), subset(train, slice(n_fit, None))
-            self.coef = np.concatenate([glob[:, :NGLOBAL_USED], np.ones((3, 2))], axis=1)
+            self.coef = np.concatenate([glob[:, :self.nshared], np.ones((3, 2))], axis=1)
             params = self.net.parameters()
             opt = torch.optim.AdamW(params, lr=lr, weight_decay=weight_decay)
             sched = torch.optim.lr_scheduler.ExponentialLR(opt, gamma=0.1 ** (1.0 / epochs))
@@ -879,6 +982,71 @@ def frames_needed(se, n, target=1.0):
     return n * (2.0 * np.asarray(se) / target) ** 2
 
 
+def capture_frames(drv, nframes, stride, T, basis: Basis = Basis(), device="cuda:0"):
+    """One run of nframes x stride steps (one host call) that keeps every
+    stride-th step: its scalars from the run's records, and its per-atom inputs
+    of the controls (D, J, L, positions, forces) captured on the device inside
+    the run (mdl_driver_set_cv_capture with one slot per frame)."""
+    torch = _torch()
+    N = drv.N
+    store = dict(D=torch.empty((nframes, N, NF), device=device),
+                 J=torch.empty((nframes, 3, N, NF), device=device),
+                 L=torch.empty((nframes, N, NF), device=device),
+                 pos=torch.empty((nframes, N, 3), device=device, dtype=torch.float64),
+                 F=torch.empty((nframes, N, 3), device=device, dtype=torch.float64))
+    lib = nat.lib()
+    _check(lib.mdl_driver_set_cv_capture(drv.handle, basis.rc, basis.mu0, basis.dmu,
+                                         basis.width, *[_p(store[q]) for q in
+                                                        ("D", "J", "L", "pos", "F")], nframes))
+    try:
+        rec = drv.run(nframes * stride, sample_stride=stride)
+    finally:
+        lib.mdl_driver_set_cv_capture(drv.handle, 0.0, 0.0, 0.0, 0.0, None, None, None,
+                                      None, None, 0)
+    if len(rec) != nframes:
+        raise RuntimeError(f"capture_frames: {len(rec)} records for {nframes} frames")
+    ep = np.zeros(nframes)
+    sb = np.zeros((nframes, 6))
+    cb = np.zeros((nframes, 6, 6))
+    for k in range(nframes):
+        ep[k], sb[k], cb[k] = _record_scalars(rec[k], N)
+    box = np.diag(drv.box.h) / U.angstrom
+    return Frames(ep, sb, cb, box, T, N / float(np.prod(box)), store)
+
+
+def crossfit_controls(fr: Frames, sites, types, T, folds=2, valid_fraction=0.2,
+                      use_net=False, use_stress=True, epochs=0, device="cuda:0", log=None):
+    """Controls s [n, 3] of one run's stored frames by cross-fitting, with no
+    training run of its own: the run is cut into ``folds`` contiguous blocks, and
+    the controls of each block come from a variate fitted (proxy, coefficients,
+    head switches; the last ``valid_fraction`` of the fitting frames decides the
+    switches) on the other blocks only. Each block's coefficients are fixed with
+    respect to its own frames, so its controls keep their zero mean (up to the
+    correlation across the block boundary, a few frames of a long run). Returns s
+    and each fold's fit summary."""
+    n = fr.n
+    if folds != 2:
+        raise ValueError("crossfit_controls: two folds (each fits on the other half)")
+    half = n // 2
+    blocks = [slice(0, half), slice(half, n)]
+    s = np.zeros((n, 3))
+    fits = []
+    for k in range(2):
+        fit_on = subset(fr, blocks[1 - k])
+        m = fit_on.n
+        cut = int(round(m * (1.0 - valid_fraction)))
+        cvm = ControlVariate(sites, fr.box, T, types, device=device, use_net=use_net,
+                             use_stress=use_stress)
+        info = cvm.fit(subset(fit_on, slice(0, cut)), subset(fit_on, slice(cut, None)),
+                       epochs=epochs, log=log)
+        s[blocks[k]] = cvm.controls(subset(fr, blocks[k]))
+        fits.append(dict(enabled=cvm.enabled.copy(), uses_learned_fields=cvm.uses_fields.copy(),
+                         proxy_r2=info["proxy_r2"],
+                         var_ratio_valid=np.asarray(info["var_cv_valid"]) /
+                         np.asarray(info["var_plain_valid"])))
+    return s, fits
+
+
 # ---------------------------------------------------------------------------
 # the 4000-atom alpha-brass case
 # ---------------------------------------------------------------------------
@@ -999,18 +1167,85 @@ def brass_experiment(n_train=2000, n_valid=400, n_eval=4000, eval_runs=2, stride
     return out
 
 
+def brass_crossfit_study(n_frames=3000, state_seeds=(101, 102, 103), runs_per_state=3,
+                         stride=20, equil=20_000, edge=10, T=300.0, nblocks=32,
+                         use_stress=True, precision="mixed", log=None):
+    """The control variate as a production run would use it, with no training
+    run: on fresh alloy samples (placement seeds ``state_seeds``, none of them
+    the sample any default elsewhere uses) and fresh trajectories (velocity
+    seeds), each run's frames give the plain estimate and the cross-fitted
+    controlled estimate (``crossfit_controls``) of C11, C12, C44.
+
+    Device time per frame: the plain route pays stride MD steps (timed without
+    records); the controlled route pays the same steps with the control inputs
+    captured inside the run, plus both folds' fits and controls (all timed, so
+    the fitting is counted). GPU-s to 2 SE <= 1 GPa = frames needed x time per
+    frame, with the pooled SE over all runs (block bootstrap, ``nblocks``
+    blocks per run). The control means over the runs, in units of their own
+    SE, test the zero-mean property and the SE estimate together."""
+    torch = _torch()
+    say = log or (lambda *_: None)
+    runs = []
+    t_step = None
+    for p in state_seeds:
+        for r in range(runs_per_state):
+            d, sites, types = brass_driver(edge=edge, T=T, place_seed=p,
+                                           vel_seed=1000 * p + r, precision=precision)
+            d.run(equil, sample_stride=equil)
+            if t_step is None:
+                t_step = time_steps(d, 4000)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fr = capture_frames(d, n_frames, stride, T)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            s, fits = crossfit_controls(fr, sites, types, T, use_stress=use_stress)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            Cp, sep = estimate(fr, nblocks=nblocks)
+            Cc, sec = estimate(fr, s, nblocks=nblocks)
+            y = fr.y()
+            se_s = np.array([block_bootstrap_se(s[:, h], nblocks) for h in range(3)])
+            runs.append(dict(place_seed=p, vel_seed=1000 * p + r, C_plain=Cp, se_plain=sep,
+                             C_cv=Cc, se_cv=sec, var_ratio=(y - s).var(0) / y.var(0),
+                             mean_s=s.mean(0), se_mean_s=se_s,
+                             t_capture_frame=(t1 - t0) / n_frames,
+                             t_fit_frame=(t2 - t1) / n_frames,
+                             enabled=[f["enabled"].tolist() for f in fits]))
+            say(f"sample {p} run {r}: plain {np.round(Cp, 2)} +- {np.round(sep, 3)}; "
+                f"controlled {np.round(Cc, 2)} +- {np.round(sec, 3)}; "
+                f"capture {1e6 * (t1 - t0) / n_frames:.0f} us/frame, fits "
+                f"{1e6 * (t2 - t1) / n_frames:.0f} us/frame")
+            del fr, d
+    se_p = np.sqrt(np.mean([x["se_plain"] ** 2 for x in runs], 0))
+    se_c = np.sqrt(np.mean([x["se_cv"] ** 2 for x in runs], 0))
+    cost_p = stride * t_step
+    cost_c = float(np.mean([x["t_capture_frame"] + x["t_fit_frame"] for x in runs]))
+    gpu_s_plain = frames_needed(se_p, n_frames) * cost_p
+    gpu_s_cv = frames_needed(se_c, n_frames) * cost_c
+    z = np.array([x["mean_s"] / x["se_mean_s"] for x in runs])
+    out = dict(N=4 * edge ** 3, stride=stride, n_frames=n_frames, runs=runs,
+               state_seeds=list(state_seeds), runs_per_state=runs_per_state, nblocks=nblocks,
+               t_step_s=t_step, cost_plain_frame_s=cost_p, cost_cv_frame_s=cost_c,
+               se_plain=se_p, se_cv=se_c, gpu_s_plain=gpu_s_plain, gpu_s_cv=gpu_s_cv,
+               gpu_s_plain_worst=float(gpu_s_plain.max()), gpu_s_cv_worst=float(gpu_s_cv.max()),
+               control_z=z, control_z_rms=np.sqrt((z ** 2).mean(0)))
+    out["speedup"] = out["gpu_s_plain_worst"] / out["gpu_s_cv_worst"]
+    return out
+
+
 def profile_case(nframes=32, epochs=1, stride=20, equil=400, log=print):
     """A short pass over every control-variate kernel, for profilers (counter
     runs serialise each dispatch, so the full experiment is too long there):
-    the 4,000-atom brass cell, nframes stored training frames, the fit with
-    ``epochs`` epochs of the network (forced on), and the in-loop controls of
-    nframes more frames."""
+    the 4,000-atom brass cell, nframes stored training frames, the fit (stress
+    proxy included) with ``epochs`` epochs of the network, and the in-loop
+    controls of nframes more frames with the network forced on."""
     torch = _torch()
     drv, sites, types = brass_driver()
     drv.run(equil, sample_stride=equil)
     fr = collect_frames(drv, nframes + nframes // 4, stride, 300.0, sites=sites)
     train, valid = subset(fr, slice(0, nframes)), subset(fr, slice(nframes, None))
-    cvm = ControlVariate(sites, fr.box, 300.0, types)
+    cvm = ControlVariate(sites, fr.box, 300.0, types, use_stress=True)
     fit = cvm.fit(train, valid, epochs=epochs)
     cvm.use_net = True            # the controls pass through the network too
     cbat = ControlBatch(cvm, drv.N)

@@ -199,3 +199,97 @@ def global_stein(u, F, U, U0, beta, ndiv=None):
     T = dU * S - M
     Q = dU ** 2 * S - 2.0 * dU * M
     return np.concatenate([S.ravel(), T.ravel(), Q.ravel()])
+
+
+# ---------------------------------------------------------------------------
+# stress proxy and its zero-variance fields
+# ---------------------------------------------------------------------------
+VOIGT = ((0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1))
+
+
+def _ordered_pairs(pos, box, rc):
+    """Ordered pairs (i, j), i != j, r_ij < rc: i, j, d = x_i - x_j (minimum
+    image), r."""
+    pos = np.asarray(pos, dtype=float)
+    d = pos[:, None, :] - pos[None, :, :]
+    d = min_image(d, box)
+    r2 = (d * d).sum(-1)
+    np.fill_diagonal(r2, np.inf)
+    i, j = np.nonzero(r2 < rc * rc)
+    dv = d[i, j]
+    return i, j, dv, np.sqrt(r2[i, j])
+
+
+def _gauss(r, rc, mu0, dmu, width):
+    """g_k(r) [P, 32], fc, fc', fc''."""
+    mu = mu0 + dmu * np.arange(NB)
+    g = np.exp(-(r[:, None] - mu) ** 2 / (2.0 * width ** 2))
+    a = np.pi / rc
+    return g, 0.5 * (np.cos(a * r) + 1.0), -0.5 * a * np.sin(a * r), -0.5 * a * a * np.cos(a * r)
+
+
+def stress_basis_sums(pos, types, box, rc=6.8, mu0=2.0, dmu=0.15, width=0.15):
+    """Z [3, 32, 6]: over ordered pairs of species pair p = t_i + t_j,
+    sum fc(r) g_k(r) n_a n_b, Voigt order xx, yy, zz, yz, xz, xy."""
+    t = np.asarray(types)
+    i, j, d, r = _ordered_pairs(pos, box, rc)
+    g, fc, _, _ = _gauss(r, rc, mu0, dmu, width)
+    n = d / r[:, None]
+    Z = np.zeros((3, NB, 6))
+    p = t[i] + t[j]
+    for v, (a, b) in enumerate(VOIGT):
+        w = g * (fc * n[:, a] * n[:, b])[:, None]
+        for q in range(3):
+            Z[q, :, v] = w[p == q].sum(0)
+    return Z
+
+
+def stress_proxy(pos, types, box, coef, rc=6.8, mu0=2.0, dmu=0.15, width=0.15):
+    """Per atom: the share of h_v (sum_j psi n_a n_b) [N, 6], the Laplacian of
+    h_v along the atom's coordinates [N, 6] and grad h_v [N, 6, 3], for
+    h_v = sum over ordered pairs of psi_p(r) n_a n_b, psi_p = fc sum_k coef[p, k] g_k."""
+    pos = np.asarray(pos, dtype=float)
+    t = np.asarray(types)
+    N = len(pos)
+    i, j, d, r = _ordered_pairs(pos, box, rc)
+    g, fc, dfc, ddfc = _gauss(r, rc, mu0, dmu, width)
+    mu = mu0 + dmu * np.arange(NB)
+    q = (r[:, None] - mu) / width ** 2
+    c = np.asarray(coef, dtype=float)[t[i] + t[j]]
+    s0 = (c * g).sum(1)
+    s1 = -(c * g * q).sum(1)
+    s2 = (c * g * (q * q - 1.0 / width ** 2)).sum(1)
+    psi = fc * s0
+    dpsi = dfc * s0 + fc * s1
+    ddpsi = ddfc * s0 + 2.0 * dfc * s1 + fc * s2
+    n = d / r[:, None]
+    hp = np.zeros((N, 6))
+    lap = np.zeros((N, 6))
+    grad = np.zeros((N, 6, 3))
+    eye = np.eye(3)
+    for v, (a, b) in enumerate(VOIGT):
+        nab = n[:, a] * n[:, b]
+        np.add.at(hp[:, v], i, psi * nab)
+        lp = (ddpsi + 2.0 * dpsi / r - 6.0 * psi / r ** 2) * nab + (2.0 * psi / r ** 2) * eye[a, b]
+        # both atoms of an ordered pair see the Laplacian in d
+        np.add.at(lap[:, v], i, lp)
+        np.add.at(lap[:, v], j, lp)
+        T = (dpsi * nab)[:, None] * n + (psi / r)[:, None] * (
+            eye[a][None, :] * n[:, b:b + 1] + eye[b][None, :] * n[:, a:a + 1]
+            - 2.0 * nab[:, None] * n)
+        # d/dx_i of the pair term is T(d), d/dx_j is -T(d); T is odd in d
+        np.add.at(grad[:, v, :], i, T)
+        np.add.at(grad[:, v, :], j, -T)
+    return hp, lap, grad
+
+
+def zv_stein(h, h0, lap, grad, F, beta):
+    """Stein sums of the zero-variance fields of the stress proxy h [6] (h0 any
+    constant [6]; lap [6] = sum_m lap_m h, grad [N, 6, 3], F [N, 3] in eV/A):
+      grad h_a:          A_a  = lap h_a + beta sum_m grad_m h_a . F_m
+      (h_a - h0_a) grad h_b:  B_ab = sum_m grad_m h_a . grad_m h_b + (h_a - h0_a) A_b
+    Returns [42]: A (6), then B row-major (36)."""
+    A = np.asarray(lap) + beta * np.einsum("mac,mc->a", grad, F)
+    GG = np.einsum("mac,mbc->ab", grad, grad)
+    B = GG + (np.asarray(h) - np.asarray(h0))[:, None] * A[None, :]
+    return np.concatenate([A, B.ravel()])

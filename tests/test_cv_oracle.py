@@ -261,3 +261,87 @@ def test_block_bootstrap_of_white_noise():
     x = rs.normal(size=64 * 200)
     se = cv.block_bootstrap_se(x)
     assert abs(se / (1.0 / np.sqrt(len(x))) - 1.0) < 0.15
+
+
+# ---------------------------------------------------------------------------
+# the stress proxy and its zero-variance fields
+# ---------------------------------------------------------------------------
+def _sp_coef(seed=11):
+    return np.random.RandomState(seed).normal(size=(3, 32)) * 0.3
+
+
+def _h_total(pos, types, box, coef):
+    return O.stress_proxy(pos, types, box, coef)[0].sum(0)
+
+
+def test_stress_proxy_derivatives_are_finite_differences():
+    """grad h == central differences of h, and the per-atom Laplacian ==
+    central differences of grad h along the atom's own coordinates (a dropped
+    2 psi / r^2 term, a wrong factor or a sign error fails)."""
+    _, pos, types, box = _cell(seed=8)
+    coef = _sp_coef()
+    hp, lap, grad = O.stress_proxy(pos, types, box, coef)
+    scale_g = np.abs(grad).max()
+    for i in (0, 57, 201):
+        for a in range(3):
+            fd = _fd(lambda p: _h_total(p, types, box, coef), pos, i, a)
+            assert np.allclose(grad[i, :, a], fd, rtol=1e-6, atol=1e-8 * scale_g)
+        lp = sum(_fd(lambda p: O.stress_proxy(p, types, box, coef)[2][i, :, a], pos, i, a)
+                 for a in range(3))
+        assert np.allclose(lap[i], lp, rtol=1e-5, atol=1e-7 * np.abs(lap).max())
+    # translation invariance: the gradients sum to zero over the atoms
+    assert np.abs(grad.sum(0)).max() < 1e-10 * scale_g
+
+
+def test_stress_proxy_with_one_hot_coefficients_is_the_basis_sums():
+    _, pos, types, box = _cell(seed=9)
+    Z = O.stress_basis_sums(pos, types, box)
+    assert Z.shape == (3, 32, 6)
+    for p, k in ((0, 3), (1, 10), (2, 17), (0, 25)):
+        c = np.zeros((3, 32))
+        c[p, k] = 1.0
+        assert np.allclose(_h_total(pos, types, box, c), Z[p, k], rtol=1e-12,
+                           atol=1e-14 * np.abs(Z).max())
+    # the proxy is linear in the coefficients
+    c1, c2 = _sp_coef(1), _sp_coef(2)
+    assert np.allclose(_h_total(pos, types, box, c1 + 2 * c2),
+                       _h_total(pos, types, box, c1) + 2 * _h_total(pos, types, box, c2),
+                       rtol=1e-12)
+    # an isotropic sum: xx + yy + zz of the basis sums is the pair count weighted by g_k fc
+    i, j, d, r = O._ordered_pairs(pos, box, 6.8)
+    g, fc, _, _ = O._gauss(r, 6.8, 2.0, 0.15, 0.15)
+    t = np.asarray(types)
+    tot = np.stack([(g * fc[:, None])[t[i] + t[j] == p].sum(0) for p in range(3)])
+    assert np.allclose(Z[:, :, :3].sum(2), tot, rtol=1e-12)
+
+
+def test_stein_identity_of_the_stress_fields():
+    """E[A_a] = E[B_ab] = 0 by exact Monte Carlo on harmonic oscillators at
+    disordered sites (h0: h at the sites, a constant); dropping the grad . grad term of
+    B or the Laplacian of A moves the means by many standard errors."""
+    L = 14.0
+    sites = _random_sites(110, L, 2.1, seed=13)
+    box = np.full(3, L)
+    types = (np.arange(len(sites)) % 4 == 0).astype(np.int32)
+    coef = _sp_coef(5)
+    k, beta = 2.0, 1.0 / 0.0258
+    us = _harmonic_samples(300, len(sites), k, beta, seed=14)
+    # any constant keeps the mean at zero; h at the sites keeps the noise low
+    h0 = O.stress_proxy(sites, types, box, coef)[0].sum(0)
+    feats, no_gg, no_lap = [], [], []
+    for u in us:
+        hp, lap, grad = O.stress_proxy(sites + u, types, box, coef)
+        F = -k * u
+        f = O.zv_stein(hp.sum(0), h0, lap.sum(0), grad, F, beta)
+        feats.append(f)
+        gg = np.einsum("mac,mbc->ab", grad, grad).ravel()
+        no_gg.append(f[6:] - gg)
+        no_lap.append(f[:6] - lap.sum(0))
+    feats, no_gg, no_lap = map(np.asarray, (feats, no_gg, no_lap))
+    se = feats.std(0) / np.sqrt(len(feats))
+    assert np.all(np.abs(feats.mean(0)) < 4.5 * se)
+    diag = [6 * a + a for a in range(6)]
+    s2 = no_gg.std(0) / np.sqrt(len(no_gg))
+    assert np.all(np.abs(no_gg.mean(0))[diag] > 8 * s2[diag])
+    s3 = no_lap.std(0) / np.sqrt(len(no_lap))
+    assert np.any(np.abs(no_lap.mean(0)) > 8 * s3)

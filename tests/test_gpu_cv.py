@@ -314,3 +314,63 @@ def test_learned_fields_are_kept_only_where_they_help():
     s = cvm.controls(valid, enabled_only=False)
     assert np.allclose((valid.y() - s).var(0), vc, rtol=1e-6)
     assert len(info["history"]) == 4
+
+
+def test_stress_kernels_match_the_oracle():
+    """The stress proxy's basis sums, pair sums, Laplacians and per-atom
+    gradients on three frames at once against the oracle (FP64 both)."""
+    frames = [_cell(seed=s) for s in (21, 22, 23)]
+    types = frames[0][2]
+    box = frames[0][3]
+    pos = np.stack([f[1] for f in frames])
+    coef = np.random.RandomState(4).normal(size=(3, 32)) * 0.3
+    t = torch.tensor(pos, dtype=torch.float64, device=DEV)
+    ty = torch.tensor(types, dtype=torch.int32, device=DEV)
+    print("stress proxy kernels: 3 x 256 atoms", flush=True)
+    Z = cv.stress_basis_sums(t, ty, box).cpu().numpy()
+    hp, lap, grad = cv.stress_proxy(t, ty, box, cv.Basis(),
+                                    torch.tensor(coef, dtype=torch.float64, device=DEV))
+    hp, lap, grad = (x.cpu().numpy() for x in (hp, lap, grad))
+    for b in range(3):
+        Zo = O.stress_basis_sums(pos[b], types, box)
+        hpo, lapo, grado = O.stress_proxy(pos[b], types, box, coef)
+        assert _rel(Z[b], Zo) < 1e-12
+        assert _rel(hp[b], hpo.sum(0)) < 1e-12
+        assert _rel(lap[b], lapo.sum(0)) < 1e-12
+        assert _rel(grad[b], grado) < 1e-12
+
+
+def test_stress_fields_on_md_frames():
+    """500-atom alpha-brass NVT, frames captured inside one run: the stress
+    proxy fits the Born stress, the variate's 42 stress-field sums equal the
+    oracle's on stored frames, and the fitted variate keeps a zero mean on a
+    fresh trajectory (Stein)."""
+    print("brass 500 atoms: captured frames", flush=True)
+    drv, sites, types = cv.brass_driver(edge=5)
+    drv.run(4000, sample_stride=4000)
+    fr = cv.capture_frames(drv, 800, 20, 300.0)
+    train, valid = cv.subset(fr, slice(0, 640)), cv.subset(fr, slice(640, None))
+    cvm = cv.ControlVariate(sites, fr.box, 300.0, types, device=DEV, use_net=False,
+                            use_stress=True)
+    info = cvm.fit(train, valid, epochs=0)
+    r2 = cvm.fit_stress_proxy(train)
+    print("stress proxy R^2", r2, "held-out ratios",
+          np.asarray(info["var_cv_valid"]) / np.asarray(info["var_plain_valid"]), flush=True)
+    assert np.all(r2 > 0.9)
+    coef = cvm.sp_coef.cpu().numpy()
+    for k in (3, 500):
+        posA = fr.store["pos"][k].cpu().numpy() / cv.U.angstrom
+        F = fr.store["F"][k].cpu().numpy() * (cv.U.angstrom / cv.U.ev)
+        hp, lap, grad = O.stress_proxy(posA, types, fr.box, coef)
+        ref = O.zv_stein(hp.sum(0) + cvm.sp_off, cvm.sp_h0, lap.sum(0), grad, F, cvm.beta)
+        got = cvm.stress_features(fr.store["pos"][k:k + 1],
+                                  fr.store["F"][k:k + 1] * (cv.U.angstrom / cv.U.ev))
+        assert _rel(got.cpu().numpy()[0], ref) < 1e-10
+    print("brass 500 atoms: fresh trajectory", flush=True)
+    d, _, _ = cv.brass_driver(edge=5, vel_seed=999)
+    d.run(4000, sample_stride=4000)
+    ev = cv.capture_frames(d, 1280, 20, 300.0)
+    s_all = cvm.controls(ev, enabled_only=False)
+    se = np.array([cv.block_bootstrap_se(s_all[:, h], 32) for h in range(3)])
+    print("mean s", s_all.mean(0), "se", se, flush=True)
+    assert np.all(np.abs(s_all.mean(0)) < 4.5 * se)
