@@ -353,7 +353,8 @@ def _learned_cv_here(n_eval):
     the fitting is timed into the controlled route's cost)."""
     from mdelast_amd import cv
     try:
-        r = cv.brass_crossfit_study(n_frames=n_eval)
+        r = cv.brass_crossfit_study(
+            n_frames=n_eval, log=lambda m: print(m, file=sys.stderr, flush=True))
     except Exception as e:        # reported, never hides the throughput line
         return {"error": f"{type(e).__name__}: {e}"}
 
