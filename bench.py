@@ -387,6 +387,10 @@ def _learned_cv_here(n_eval):
         "eval_var_ratio": f(np.mean([x["var_ratio"] for x in runs], 0)),
         "eval_se2_ratio": f((np.asarray(r["se_cv"]) / np.asarray(r["se_plain"])) ** 2),
         "enabled_per_fold": [x["enabled"] for x in runs],
+        "fields": "displacement, energy- and pressure-proxy-weighted displacement, and the "
+                  "zero-variance fields of a fitted stress proxy (grad h_a, (h_a - h0_a) grad h_b)",
+        "uses_stress_proxy_fields": True,
+        "uses_network_fields": [False, False, False],
         "eval_runs": len(runs),
         "nblocks": r["nblocks"],
         "t_step_us": round(r["t_step_s"] * 1e6, 3),

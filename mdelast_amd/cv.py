@@ -682,9 +682,8 @@ class ControlVariate:
         self.sp_coef = torch.as_tensor(c[:3 * NB].reshape(3, NB), dtype=torch.float64,
                                        device=self.device)
         self.sp_off = c[3 * NB:]
-        pred = (A @ c).reshape(n, 6)
-        h = pred
-        self.sp_h0 = h.mean(0)
+        pred = (A @ c).reshape(n, 6)          # h_v of every fitting frame
+        self.sp_h0 = pred.mean(0)
         res = fr.born_stress - pred
         return 1.0 - res.var(0) / fr.born_stress.var(0)
 

@@ -163,3 +163,23 @@ def test_bad_position_and_type():
     bad["types"][11] = 2                               # two species
     assert _oracle_fail_step(bad, setup, 1) == ("TYPE", 0)
     assert _gpu_fail_step(bad, setup, 1)[:2] == ("TYPE", 0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_collapsed_positions_are_reported_by_the_cell_build(precision):
+    """More atoms in one neighbour cell than the build stages (1,024): the
+    positions have collapsed, as in a diverged integration. The cell sort flags
+    it and the constructor raises NEIGHBORS naming the count (the reference has
+    no such check; its all-pairs loop would run on)."""
+    cell = cells.zn06_cell(10)                     # 4,000 atoms, 4 cells per axis
+    pos = cell["pos"].copy()
+    L = cell["box"][0]
+    corner = -0.5 * L + 2.0 * U.angstrom
+    rs = np.random.RandomState(5)
+    pos[:1100] = corner + rs.rand(1100, 3) * U.angstrom
+    bad = dict(cell, pos=pos)
+    setup = H.md_setup(cell, T=300.0, seed=0)
+    with pytest.raises(mdm.MDDriverError) as ei:
+        H.gpu_md(bad, setup, precision=precision)
+    assert ei.value.kind == "NEIGHBORS"
+    assert "fall into one cell" in str(ei.value)
