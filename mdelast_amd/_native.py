@@ -113,6 +113,7 @@ SIGNATURES = {
     "mdl_driver_get_chain": (C.c_int32, [_vp, C.c_int32, _dp, _dp]),
     "mdl_driver_kernel_times": (C.c_int32, [_vp, _dp]),
     "mdl_driver_set_graph_steps": (C.c_int32, [_vp, C.c_int32]),
+    "mdl_expx_eval": (C.c_int32, [C.c_int32, C.c_int64, _dp, _dp, _dp]),
     # learned control variate (include/mdelast_cv.h); device pointers as c_void_p
     "mdl_cv_descriptors": (C.c_int32, [_vp, _vp, C.c_int32, _dp, C.c_float, C.c_float,
                                        C.c_float, C.c_float, _vp, _vp, _vp, _vp]),
@@ -184,3 +185,13 @@ def device_info():
     cu = C.c_int32(0)
     check(lib().mdl_device_info(name, 256, C.byref(cu)))
     return name.value.decode(), cu.value
+
+
+def expx_eval(x, form=0):
+    """exp(-x) and (1 - exp(-x))/x on the GPU as the integrator takes them: form 0
+    the barostat's operator, 1 the chain sweeps' wide form, 2 their series form."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    e = np.empty_like(x)
+    op = np.empty_like(x)
+    check(lib().mdl_expx_eval(int(form), x.size, dptr(x), dptr(e), dptr(op)))
+    return e, op

@@ -1115,6 +1115,29 @@ __global__ __launch_bounds__(GR_T) void k_pair_hist(const double4 *__restrict__ 
     }
 }
 
+// The exponential operator on n points: form 0 expx_operator (barostat,
+// chains longer than the register forms), 1 the chain sweeps' wide form
+// (expx_any), 2 their series form (op_series, |x| <= EXPX_SMALL).
+__global__ __launch_bounds__(BLOCK) void k_expx_eval(int form, int64_t n,
+                                                    const double *__restrict__ x,
+                                                    double *__restrict__ expx,
+                                                    double *__restrict__ op) {
+    int64_t const i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n)
+        return;
+    double e, o;
+    if (form == 0) {
+        expx_operator(x[i], e, o);
+    } else if (form == 1) {
+        expx_any(x[i], e, o);
+    } else {
+        o = op_series(x[i]);
+        e = fma(-x[i], o, 1.);
+    }
+    expx[i] = e;
+    op[i] = o;
+}
+
 __global__ __launch_bounds__(256) void k_int_to_double(const int *__restrict__ in, int n,
                                                        double *__restrict__ out) {
     int const i = blockIdx.x * 256 + threadIdx.x;
@@ -2505,6 +2528,29 @@ mdl_status mdl_driver_kernel_times(mdl_driver *p, double *out) {
     out[0] = d->t_dens;
     out[1] = d->t_force;
     out[2] = d->t_step;
+    return MDL_OK;
+}
+
+mdl_status mdl_expx_eval(int32_t form, int64_t n, const double *x, double *expx, double *op) {
+    if (form < 0 || form > 2 || n < 0 || (n > 0 && (!x || !expx || !op)))
+        return bad("mdl_expx_eval: bad argument");
+    if (n == 0)
+        return MDL_OK;
+    double *buf = nullptr;
+    MDL_HIP(hipMalloc(&buf, sizeof(double) * 3 * (size_t)n));
+    double *dx = buf, *de = buf + n, *dop = buf + 2 * n;
+    hipError_t e = hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        k_expx_eval<<<(unsigned)((n + BLOCK - 1) / BLOCK), BLOCK>>>(form, n, dx, de, dop);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpy(expx, de, sizeof(double) * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+        e = hipMemcpy(op, dop, sizeof(double) * n, hipMemcpyDeviceToHost);
+    (void)hipFree(buf);
+    if (e != hipSuccess)
+        return hip_fail(e, "mdl_expx_eval");
     return MDL_OK;
 }
 

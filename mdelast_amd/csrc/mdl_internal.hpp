@@ -114,6 +114,7 @@ struct DevState {
     double err_val[4];
     long long warn_virial, warn_rmin;
     long long nhc_general;   // chain propagations with an argument beyond the series range
+    int nhc_wide[2];         // thermostat, barostat chain: its last propagation had one
     int init_done;
 };
 

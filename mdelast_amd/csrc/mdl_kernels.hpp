@@ -59,29 +59,69 @@ struct Blk {
 };
 __device__ __forceinline__ Blk hw_blk() { return Blk{(int)blockIdx.x, (int)gridDim.x}; }
 
+// Wave sums of NV components per lane by recursive halving over the lane bits
+// 32, 16, ..., 1: at each step a lane keeps one half of its current array and
+// sends the other half to its partner across that bit, which keeps the other
+// one (halves of ceil(A/2), the odd one padded with zero). About NV exchanges
+// in all, where a butterfly per component takes 6 NV. Every component is summed
+// over the same tree as by that butterfly (pairs across bit 32 first, then 16,
+// ...; the additions commute exactly), so the sums are its bits. On return,
+// lane l holds the sum of component `idx`, or idx < 0 if its slot is padding.
+constexpr int halved(int n, int steps) {
+    for (int k = 0; k < steps; ++k)
+        n = (n + 1) / 2;
+    return n;
+}
+
+template <int NV, int S>
+__device__ __forceinline__ void halving_step(double (&v)[NV], int l64, int &lo, int &hi) {
+    constexpr int A = halved(NV, S), H = (A + 1) / 2, OFF = 32 >> S;
+    bool const up = (l64 & OFF) != 0;
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        double const a = v[i];
+        double const b = H + i < A ? v[H + i] : 0.;
+        double const keep = up ? b : a;
+        v[i] = keep + __shfl_xor(up ? a : b, OFF, 64);
+    }
+    int const mid = lo + H;
+    if (up)
+        lo = mid;
+    else
+        hi = min(hi, mid);
+}
+
+template <int NV>
+__device__ __forceinline__ double wave_sum_scatter(double (&v)[NV], int &idx) {
+    static_assert(NV >= 1 && NV <= 64, "one component per lane at most");
+    int const l64 = threadIdx.x & 63;
+    int lo = 0, hi = NV;
+    halving_step<NV, 0>(v, l64, lo, hi);
+    halving_step<NV, 1>(v, l64, lo, hi);
+    halving_step<NV, 2>(v, l64, lo, hi);
+    halving_step<NV, 3>(v, l64, lo, hi);
+    halving_step<NV, 4>(v, l64, lo, hi);
+    halving_step<NV, 5>(v, l64, lo, hi);
+    idx = lo < hi ? lo : -1;
+    return v[0];
+}
+
+// Block sums of NV components (wave sums, then the waves in order), stored as
+// out[k * PART_STRIDE + bid]. acc is consumed.
 template <int NV>
 __device__ __forceinline__ void block_sum_store(double (&acc)[NV], double *out, int bid) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        double v = acc[k];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1)
-            v += __shfl_xor(v, off, 64);
-        acc[k] = v;
-    }
+    int idx;
+    double const w = wave_sum_scatter<NV>(acc, idx);
     __shared__ double sm[BLOCK / 64][NV];
-    int const wave = threadIdx.x >> 6, l64 = threadIdx.x & 63;
-    if (l64 == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k)
-            sm[wave][k] = acc[k];
-    }
+    int const wave = threadIdx.x >> 6;
+    if (idx >= 0)
+        sm[wave][idx] = w;
     __syncthreads();
     if (threadIdx.x < NV) {
         double s = 0.;
 #pragma unroll
-        for (int w = 0; w < BLOCK / 64; ++w)
-            s += sm[w][threadIdx.x];
+        for (int k = 0; k < BLOCK / 64; ++k)
+            s += sm[k][threadIdx.x];
         out[(size_t)threadIdx.x * PART_STRIDE + bid] = s;
     }
 }
@@ -94,35 +134,30 @@ __device__ __forceinline__ void block_sum_store(double (&acc)[NV], double *out) 
 // block_sum_store without the components [S0, S1): a step that does not sample
 // leaves the sampled-only partials (Born and Laplacian terms) unreduced and
 // their slots untouched; post_step reads them only on sampled steps. The other
-// components are reduced exactly as by block_sum_store.
+// components are reduced exactly as by block_sum_store (the same tree per
+// component, whatever its place in the array).
 template <int NV, int S0, int S1>
 __device__ __forceinline__ void block_sum_store_skip(double (&acc)[NV], double *out, int bid) {
-    static_assert(0 <= S0 && S0 <= S1 && S1 <= NV, "skip range");
+    static_assert(0 <= S0 && S0 <= S1 && S1 <= NV && NV - (S1 - S0) >= 1, "skip range");
+    constexpr int NK = NV - (S1 - S0);
+    double v[NK];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        if (k >= S0 && k < S1)
-            continue;
-        double v = acc[k];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1)
-            v += __shfl_xor(v, off, 64);
-        acc[k] = v;
-    }
-    __shared__ double sm[BLOCK / 64][NV];
-    int const wave = threadIdx.x >> 6, l64 = threadIdx.x & 63;
-    if (l64 == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k)
-            if (k < S0 || k >= S1)
-                sm[wave][k] = acc[k];
-    }
+    for (int k = 0; k < NK; ++k)
+        v[k] = acc[k < S0 ? k : k + (S1 - S0)];
+    int idx;
+    double const w = wave_sum_scatter<NK>(v, idx);
+    __shared__ double sm[BLOCK / 64][NK];
+    int const wave = threadIdx.x >> 6;
+    if (idx >= 0)
+        sm[wave][idx] = w;
     __syncthreads();
-    if (threadIdx.x < NV && (threadIdx.x < S0 || threadIdx.x >= S1)) {
+    if (threadIdx.x < NK) {
         double s = 0.;
 #pragma unroll
-        for (int w = 0; w < BLOCK / 64; ++w)
-            s += sm[w][threadIdx.x];
-        out[(size_t)threadIdx.x * PART_STRIDE + bid] = s;
+        for (int k = 0; k < BLOCK / 64; ++k)
+            s += sm[k][threadIdx.x];
+        int const c = threadIdx.x < S0 ? threadIdx.x : threadIdx.x + (S1 - S0);
+        out[(size_t)c * PART_STRIDE + bid] = s;
     }
 }
 
@@ -178,19 +213,18 @@ __device__ __forceinline__ void min_image(const double4 &ci, const double4 &cj, 
 // 1/|x| ulp to cancellation and agrees to that).
 constexpr double EXPX_SMALL = 0.125;
 
+// Both polynomials below are evaluated by Estrin's scheme: the chain sweeps
+// run them on one lane, one after another, so their cost is the length of the
+// dependent chain (4 FMA levels after the powers instead of 10-12 in Horner form).
 __device__ __forceinline__ double op_series(double x) {
-    double const y = -x;
-    double o = 1. / 39916800.;
-    o = fma(o, y, 1. / 3628800.);
-    o = fma(o, y, 1. / 362880.);
-    o = fma(o, y, 1. / 40320.);
-    o = fma(o, y, 1. / 5040.);
-    o = fma(o, y, 1. / 720.);
-    o = fma(o, y, 1. / 120.);
-    o = fma(o, y, 1. / 24.);
-    o = fma(o, y, 1. / 6.);
-    o = fma(o, y, 0.5);
-    return fma(o, y, 1.);
+    // sum_{m=0}^{10} (-x)^m / (m+1)!
+    double const y = -x, y2 = y * y, y4 = y2 * y2, y8 = y4 * y4;
+    double const q0 = fma(y, 0.5, 1.), q1 = fma(y, 1. / 24., 1. / 6.),
+                 q2 = fma(y, 1. / 720., 1. / 120.), q3 = fma(y, 1. / 40320., 1. / 5040.),
+                 q4 = fma(y, 1. / 3628800., 1. / 362880.);
+    double const s0 = fma(q1, y2, q0), s1 = fma(q3, y2, q2),
+                 s2 = fma(1. / 39916800., y2, q4);
+    return fma(s2, y8, fma(s1, y4, s0));
 }
 
 // exp(-x) for any x: y = -x = k ln2 + r with |r| <= ln2/2 (Cody-Waite: ln2
@@ -203,20 +237,28 @@ __device__ __forceinline__ double exp_neg_general(double x) {
     double const k = rint(y * 1.4426950408889634);   // y / ln 2
     double r = fma(k, -0x1.62e42ff000000p-1, y);     // ln2 head (32 significant bits)
     r = fma(k, 0x1.718432a1b0e26p-35, r);            // -(ln2 tail)
-    double p = 1. / 6227020800.;
-    p = fma(p, r, 1. / 479001600.);
-    p = fma(p, r, 1. / 39916800.);
-    p = fma(p, r, 1. / 3628800.);
-    p = fma(p, r, 1. / 362880.);
-    p = fma(p, r, 1. / 40320.);
-    p = fma(p, r, 1. / 5040.);
-    p = fma(p, r, 1. / 720.);
-    p = fma(p, r, 1. / 120.);
-    p = fma(p, r, 1. / 24.);
-    p = fma(p, r, 1. / 6.);
-    p = fma(p, r, 0.5);
-    p = fma(p, r, 1.);
+    // p(r) = sum_{m=0}^{12} r^m / (m+1)!
+    double const r2 = r * r, r4 = r2 * r2, r8 = r4 * r4;
+    double const q0 = fma(r, 0.5, 1.), q1 = fma(r, 1. / 24., 1. / 6.),
+                 q2 = fma(r, 1. / 720., 1. / 120.), q3 = fma(r, 1. / 40320., 1. / 5040.),
+                 q4 = fma(r, 1. / 3628800., 1. / 362880.),
+                 q5 = fma(r, 1. / 479001600., 1. / 39916800.);
+    double const s0 = fma(q1, r2, q0), s1 = fma(q3, r2, q2), s2 = fma(q5, r2, q4);
+    double const t0 = fma(s1, r4, s0), t1 = fma(1. / 6227020800., r4, s2);
+    double const p = fma(t1, r8, t0);
     return ldexp(fma(r, p, 1.), (int)k);
+}
+
+// 1/x from the hardware estimate and two Newton steps (within 1 ulp for
+// normal x). It does not depend on exp(-x), so it runs beside that chain, and
+// (1 - exp(-x)) * (1/x) replaces the quotient that would follow it (one
+// rounding more).
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.);
+    return fma(r, e, r);
 }
 
 __device__ __forceinline__ double exp_neg(double x) {
@@ -228,8 +270,9 @@ __device__ __forceinline__ void expx_operator(double x, double &expx, double &op
         op = op_series(x);
         expx = fma(-x, op, 1.);
     } else {
+        double const ix = rcp_nr(x);
         expx = exp_neg_general(x);
-        op = (1.0 - expx) / x;
+        op = (1.0 - expx) * ix;
     }
 }
 
@@ -313,45 +356,60 @@ template <int M> struct RegChain {
     }
 };
 
-// Lane j's value of v, to every lane, through scalar registers (v_readlane:
-// no LDS round trip on the chain's critical path, unlike __shfl). The lane
-// must be active; j is a compile-time constant where this is used.
-__device__ __forceinline__ double lane_value(double v, int j) {
-    long long const b = __double_as_longlong(v);
-    unsigned const lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, j);
-    unsigned const hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), j);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+// exp(-x) and op(x) for any x with no branch: y = -x = k ln2 + r as in
+// exp_neg_general, e^r = 1 + r p(r). For k = 0 (|x| < ln2/2) op(x) is p(r)
+// itself (r = -x exactly, p is the series of (e^r - 1)/r); otherwise op(x) =
+// -expm1(-x)/x with expm1(y) = 2^k r p(r) + (2^k - 1), which does not cancel
+// for |y| >= ln2/2, times the Newton reciprocal of x. Within a few ulp of
+// exp(-x) and op(x) everywhere (x = 0 included: op = 1).
+__device__ __forceinline__ void expx_any(double x, double &expx, double &op) {
+    double const y = -x;
+    double const k = rint(y * 1.4426950408889634);
+    double r = fma(k, -0x1.62e42ff000000p-1, y);
+    r = fma(k, 0x1.718432a1b0e26p-35, r);
+    double const r2 = r * r, r4 = r2 * r2, r8 = r4 * r4;
+    double const q0 = fma(r, 0.5, 1.), q1 = fma(r, 1. / 24., 1. / 6.),
+                 q2 = fma(r, 1. / 720., 1. / 120.), q3 = fma(r, 1. / 40320., 1. / 5040.),
+                 q4 = fma(r, 1. / 3628800., 1. / 362880.),
+                 q5 = fma(r, 1. / 479001600., 1. / 39916800.);
+    double const s0 = fma(q1, r2, q0), s1 = fma(q3, r2, q2), s2 = fma(q5, r2, q4);
+    double const t0 = fma(s1, r4, s0), t1 = fma(1. / 6227020800., r4, s2);
+    double const p = fma(t1, r8, t0);
+    int const ki = (int)k;
+    expx = ldexp(fma(r, p, 1.), ki);
+    double const em1 = ldexp(r * p, ki) + (ldexp(1., ki) - 1.);
+    double const o = -em1 * rcp_nr(x);
+    op = k == 0. ? p : o;
 }
 
-// expx_operator for code that one lane runs, reporting an argument beyond
-// EXPX_SMALL in `general`. The branch is made uniform (readfirstlane), so the
-// path not taken is jumped over instead of issued with an empty exec mask.
-__device__ __forceinline__ void expx_flag(double x, double &expx, double &op, bool &general) {
-    if (__builtin_amdgcn_readfirstlane(fabs(x) <= EXPX_SMALL ? 1 : 0)) {
-        op = op_series(x);
-        expx = fma(-x, op, 1.);
-    } else {
-        expx = exp_neg_general(x);
-        op = (1.0 - expx) / x;
-        general = true;
-    }
-}
-
-// One U_NHC propagation (7 Suzuki-Yoshida weights; md_driver.cpp:539-561) of a
-// chain of compile-time length M in registers. A lone lane runs this, so its
-// cost is its instruction count: the inward sweep's G_j read p_{j-1} before its
-// update and are formed first, the outward sweep's exponentials read p_{j+1}
-// before its update and are formed first, and dt*(p^2/q - kT) is formed as
+// The sweeps of one U_NHC propagation (7 Suzuki-Yoshida weights;
+// md_driver.cpp:539-561) of a chain of compile-time length M in registers. A
+// lone lane's worth of work (the wave runs it with identical values), so its
+// cost is its instruction count and its dependent chain: the inward sweep's
+// G_j read p_{j-1} before its update and are formed first, the outward sweep's
+// factors read p_{j+1} before its update and are formed first (independent of
+// each other, they overlap), and dt*(p^2/q - kT) is formed as
 // p^2*(dt/q) - dt*kT (thermostat.cpp:40-66 otherwise, element by element).
 // BARO: the coupling is p_eps^2/(2W) and U3 scales p_eps (held in kin); else
 // the coupling is cw*K + c0 (md_driver.cpp:512-531) and U3 scales K by f^2 and
-// the running velocity factor by f. Returns whether an exponential argument
-// exceeded EXPX_SMALL (took the library path).
-template <int M, bool BARO>
-__device__ __forceinline__ bool nhc_prop(double (&p)[M], double (&eta)[M], const double (&iq)[M],
-                                         double Nf, double kT, double dt, double &kin,
-                                         double &scale, double cw, double c0, double inv_W) {
-    bool general = false;
+// the running velocity factor by f.
+// WIDE = false: every factor from the series form (op_series), which holds for
+// |x| <= EXPX_SMALL only; WIDE = true: every factor from expx_any. Both return
+// whether an argument exceeded EXPX_SMALL; no branch depends on the values.
+template <int M, bool BARO, bool WIDE>
+__device__ __forceinline__ bool nhc_sweeps(double (&p)[M], double (&eta)[M], const double (&iq)[M],
+                                           double Nf, double kT, double dt, double &kin,
+                                           double &scale, double cw, double c0, double inv_W) {
+    bool big = false;
+    auto factor = [&big](double x, double &e, double &o) {
+        big |= !(fabs(x) <= EXPX_SMALL);
+        if (WIDE) {
+            expx_any(x, e, o);
+        } else {
+            o = op_series(x);
+            e = fma(-x, o, 1.);
+        }
+    };
 #pragma unroll 1
     for (int k = 0; k < 7; ++k) {
         double const dtw = WSY7[k] * dt, dt4 = 0.5 * dtw;
@@ -373,13 +431,13 @@ __device__ __forceinline__ bool nhc_prop(double (&p)[M], double (&eta)[M], const
 #pragma unroll
         for (int j = M - 2; j >= 0; --j) {
             double e, o;
-            expx_flag(c[j + 1] * p[j + 1], e, o, general);
+            factor(c[j + 1] * p[j + 1], e, o);
             p[j] = fma(p[j], e, tG[j] * o);
         }
         // U3
         {
             double f, unused;
-            expx_flag((2. * c[0]) * p[0], f, unused, general);
+            factor((2. * c[0]) * p[0], f, unused);
 #pragma unroll
             for (int j = 0; j < M; ++j)
                 eta[j] = fma(2. * c[j], p[j], eta[j]);
@@ -390,34 +448,11 @@ __device__ __forceinline__ bool nhc_prop(double (&p)[M], double (&eta)[M], const
                 kin *= f * f;
             }
         }
-        // U4, outward: the M - 1 factors depend only on the momenta after U3,
-        // so lane j of the (fully active, identically loaded) wave evaluates
-        // factor j and the others read it back; the same operations as one
-        // lane evaluating all of them
+        // U4, outward
         double E[M], O[M];
-        {
-            int const lane = threadIdx.x & 63;
-            double xl = 0.;
 #pragma unroll
-            for (int j = 0; j + 1 < M; ++j)
-                xl = lane == j ? c[j + 1] * p[j + 1] : xl;
-            double el, ol;
-            bool const big = !(fabs(xl) <= EXPX_SMALL);
-            if (big) {
-                el = exp_neg_general(xl);
-                ol = (1.0 - el) / xl;
-            } else {
-                ol = op_series(xl);
-                el = fma(-xl, ol, 1.);
-            }
-            if (__ballot(big && lane + 1 < M) != 0ull)
-                general = true;
-#pragma unroll
-            for (int j = 0; j + 1 < M; ++j) {
-                E[j] = lane_value(el, j);
-                O[j] = lane_value(ol, j);
-            }
-        }
+        for (int j = 0; j + 1 < M; ++j)
+            factor(c[j + 1] * p[j + 1], E[j], O[j]);
         double const e = BARO ? 0.5 * (kin * kin) * inv_W : cw * kin + c0;
         double const tG0 = dt4 * (2. * e - Nf * kT);
         if (M > 1)
@@ -433,7 +468,42 @@ __device__ __forceinline__ bool nhc_prop(double (&p)[M], double (&eta)[M], const
                 p[j] += t;
         }
     }
-    return general;
+    return big;
+}
+
+// One U_NHC propagation. `wide` says whether the chain's previous propagation
+// had an argument beyond the series range. If not, the series sweeps run
+// first and, should an argument leave the range after all, the chain is
+// restored and the wide sweeps run instead; one uniform branch per
+// propagation, none per factor. `wide` is updated for the next propagation.
+// Returns whether the result came from the wide sweeps.
+template <int M, bool BARO>
+__device__ __forceinline__ bool nhc_prop(double (&p)[M], double (&eta)[M], const double (&iq)[M],
+                                         double Nf, double kT, double dt, double &kin,
+                                         double &scale, double cw, double c0, double inv_W,
+                                         int &wide) {
+    if (!wide) {
+        double p0[M], eta0[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            p0[j] = p[j];
+            eta0[j] = eta[j];
+        }
+        double const kin0 = kin, scale0 = scale;
+        bool const big = nhc_sweeps<M, BARO, false>(p, eta, iq, Nf, kT, dt, kin, scale, cw, c0,
+                                                    inv_W);
+        if (!__builtin_amdgcn_readfirstlane(big ? 1 : 0))
+            return false;
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            p[j] = p0[j];
+            eta[j] = eta0[j];
+        }
+        kin = kin0;
+        scale = scale0;
+    }
+    wide = nhc_sweeps<M, BARO, true>(p, eta, iq, Nf, kT, dt, kin, scale, cw, c0, inv_W) ? 1 : 0;
+    return true;
 }
 
 // U_NHC_thermo_propagator (md_driver.cpp:539-549) with U3 acting on a running
@@ -449,7 +519,8 @@ template <int M> __device__ __forceinline__ double nhc_thermo_m(DevState &s, dou
         c0 = s.coupling == MDL_CONFIGURATIONAL ? econf : 0.5 * econf;
     }
     double kin = s.ekin_tot, scale = 1.;
-    if (nhc_prop<M, false>(c.p, c.eta, c.iq, s.th.Nf, kT, dt, kin, scale, cw, c0, 0.))
+    if (nhc_prop<M, false>(c.p, c.eta, c.iq, s.th.Nf, kT, dt, kin, scale, cw, c0, 0.,
+                           s.nhc_wide[0]))
         s.nhc_general++;
     s.ekin_tot = kin;
     c.store(s.th);
@@ -477,7 +548,7 @@ template <int M> __device__ __forceinline__ void nhc_baro_m(DevState &s, double 
     double const kT = K_BOLTZ * s.bth.T;
     double peps = s.peps, unused = 1.;
     if (nhc_prop<M, true>(c.p, c.eta, c.iq, s.bth.Nf, kT, dt, peps, unused, 0., 0.,
-                          1. / s.bar_W))
+                          1. / s.bar_W, s.nhc_wide[1]))
         s.nhc_general++;
     s.peps = peps;
     c.store(s.bth);

@@ -779,8 +779,9 @@ class MDDriver:
 
     @property
     def nhc_general_path_count(self):
-        """Chain propagations that took the general (range-reduced exp) path: a chain
-        argument dt p/q beyond 1/8 (diagnostic; no counterpart in the reference)."""
+        """Chain propagations done in the wide form (range-reduced exp): one with a
+        chain argument dt p/q beyond 1/8, or following one (diagnostic; no
+        counterpart in the reference)."""
         return int(self._scalars()[nat.S_NHC_GENERAL])
 
     @property
