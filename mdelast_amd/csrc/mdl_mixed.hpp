@@ -279,9 +279,23 @@ __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTab
             }
         }
 #pragma unroll
-        for (int off = LPA / 2; off > 0; off >>= 1) {
+        for (int off = LPA / 2; off > 0; off >>= 1)
             rho += __shfl_xor(rho, off, LPA);
-            if (SAMPLE) {
+        // sampled steps: g (6) and h (3) of the atom, each summed into one lane
+        // of its group (groups of 16 lanes or more), which stores its g
+        // component; |h|^2 is summed from the h lanes' squares
+        float gh = 0.f, h2 = 0.f;
+        int c = -1;
+        if constexpr (SAMPLE && LPA >= 16) {
+            float gs9[9] = {g0, g1, g2, g3, g4, g5, hx, hy, hz};
+            gh = group_sum_scatter<LPA, 9>(gs9, c);
+            h2 = c >= 6 ? gh * gh : 0.f;
+#pragma unroll
+            for (int off = LPA / 2; off > 0; off >>= 1)
+                h2 += __shfl_xor(h2, off, LPA);
+        } else if constexpr (SAMPLE) {
+#pragma unroll
+            for (int off = LPA / 2; off > 0; off >>= 1) {
                 g0 += __shfl_xor(g0, off, LPA);
                 g1 += __shfl_xor(g1, off, LPA);
                 g2 += __shfl_xor(g2, off, LPA);
@@ -292,6 +306,11 @@ __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTab
                 hy += __shfl_xor(hy, off, LPA);
                 hz += __shfl_xor(hz, off, LPA);
             }
+            h2 = hx * hx + hy * hy + hz * hz;
+        }
+        if constexpr (SAMPLE && LPA >= 16) {
+            if (active && c >= 0 && c < 6)
+                reinterpret_cast<double *>(a.geam + 4 * (size_t)i)[c] = (double)gh;
         }
         if (active && lane == 0) {
             l.ccnt[i] = nin;
@@ -329,10 +348,12 @@ __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTab
             acc[PA_EF] += (double)F;
             if (SAMPLE) {
                 double2 *gp = a.geam + 4 * (size_t)i;
-                gp[0] = make_double2(g0, g1);
-                gp[1] = make_double2(g2, g3);
-                gp[2] = make_double2(g4, g5);
-                gp[3] = make_double2((double)(hx * hx + hy * hy + hz * hz), 0.);
+                if constexpr (LPA < 16) {
+                    gp[0] = make_double2(g0, g1);
+                    gp[1] = make_double2(g2, g3);
+                    gp[2] = make_double2(g4, g5);
+                }
+                gp[3] = make_double2((double)h2, 0.);
             }
         }
     }
@@ -521,20 +542,22 @@ __device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTable
                     physics(g, pp, Fj);
             }
         }
-#pragma unroll
-        for (int off = LPA / 2; off > 0; off >>= 1) {
-            fx += __shfl_xor(fx, off, LPA);
-            fy += __shfl_xor(fy, off, LPA);
-            fz += __shfl_xor(fz, off, LPA);
-            e += __shfl_xor(e, off, LPA);
-        }
-        if (active && lane == 0) {
-            a.fx[i] = fx * F_CGS;
-            a.fy[i] = fy * F_CGS;
-            a.fz[i] = fz * F_CGS;
-            double const ei = a.Femb[i] + e;   // eV
-            a.epot[i] = ei * E_CGS;
-            epot += ei;
+        // the atom's sums: fx, fy, fz and e each land in one lane of its group
+        double fs[4] = {fx, fy, fz, e};
+        int c;
+        double const v = group_sum_scatter<LPA, 4>(fs, c);
+        if (active && c >= 0) {
+            if (c == 0) {
+                a.fx[i] = v * F_CGS;
+            } else if (c == 1) {
+                a.fy[i] = v * F_CGS;
+            } else if (c == 2) {
+                a.fz[i] = v * F_CGS;
+            } else {
+                double const ei = a.Femb[i] + v;   // eV
+                a.epot[i] = ei * E_CGS;
+                epot += ei;
+            }
         }
     }
     double acc[PB_N];
