@@ -1703,8 +1703,8 @@ __global__ __launch_bounds__(BLOCK) void k_density_rep(const RepArgs *__restrict
 }
 
 template <int KIND, int LPA, bool SAMPLE>
-__global__ __launch_bounds__(BLOCK) void k_force_rep(const RepArgs *__restrict__ R,
-                                                    const int2 *__restrict__ map) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+void k_force_rep(const RepArgs *__restrict__ R, const int2 *__restrict__ map) {
     int2 const m = map[blockIdx.x];
     RepArgs const &r = R[m.x];
     force_mx_body<KIND, LPA, SAMPLE>(r.a, r.l, r.rt, r.N, r.npair, r.rmin, r.gs, r.pB,

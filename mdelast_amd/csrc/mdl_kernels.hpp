@@ -1696,8 +1696,8 @@ __device__ __forceinline__ void finish_tail(DevState *__restrict__ gs,
         tot[c] = __hip_atomic_load(&totals[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __shared__ DevState s;
     state_load(s, gs);
-    // the whole first wave runs the scalar step with identical values (the
-    // chain sweeps spread independent factors over its lanes)
+    bool const copy = rec_out && s.sample != 0;   // this step's record, as post_step samples
+    // the whole first wave runs the scalar step with identical values
     if (threadIdx.x < 64) {
         post_step(s, tot, tot + PA_N, tot + PA_N + PB_N, mode, rec_out);
         if (next_sample >= 0 && !s.err)
@@ -1705,6 +1705,11 @@ __device__ __forceinline__ void finish_tail(DevState *__restrict__ gs,
         if (ticket && threadIdx.x == 0)
             *ticket = 0u;
     }
+    __syncthreads();
+    // the sampled record out, one element per thread (pre_step leaves it as is)
+    if (copy)
+        for (int k = threadIdx.x; k < MDL_NRECORD; k += BLOCK)
+            rec_out[k] = s.rec[k];
     state_store(gs, s);
 }
 
@@ -1797,7 +1802,10 @@ __device__ __forceinline__ void post_step(DevState &s, const double *tA, const d
         s.vscale = scale;
     }
     // calculate_thermodynamics (md_driver.cpp:648-730) on the scaled velocities
-    double const ekin_atom = s.ekin_tot / N;
+    // the per-atom and per-volume records as products with one reciprocal each
+    // (within an ulp of the quotients)
+    double const iN = 1. / N;
+    double const ekin_atom = s.ekin_tot * iN;
     double mom[3];
     for (int c = 0; c < 3; ++c)
         mom[c] = scale * tK[PK_MOM + c];
@@ -1810,12 +1818,12 @@ __device__ __forceinline__ void post_step(DevState &s, const double *tA, const d
         s.err_val[2] = mom[2];
         s.err_val[3] = sqrt(sc2);
     }
-    double const epot_atom = tB[PB_EPOT] * uE / N;
-    double const epot_eam_atom = s.has_eam ? tA[PA_EF] * uE / N : 0.;
+    double const epot_atom = tB[PB_EPOT] * uE * iN;
+    double const epot_eam_atom = s.has_eam ? tA[PA_EF] * uE * iN : 0.;
     double const etot_atom = ekin_atom + epot_atom;
     double const old = s.etot_cons;
     double ec = etot_atom * N;
-    double const V = box_volume(s.L);
+    double const V = box_volume(s.L), iV = 1. / V;
     if (thermo)
         ec += chain_etot(s.th);
     if (npt)
@@ -1824,11 +1832,11 @@ __device__ __forceinline__ void post_step(DevState &s, const double *tA, const d
     s.etot_cons_err = fabs(ec / s.etot_cons_initial - 1.);
     double const err_cur = fabs(ec / old - 1.);
     double const Tk = 2. * s.ekin_tot / (K_BOLTZ * s.Nf);
-    double const P = (s.ekin_tot * 2. / 3. + s.virial) / V;
+    double const P = (s.ekin_tot * 2. / 3. + s.virial) * iV;
     double st[6];
     double const s2 = scale * scale;
     for (int v = 0; v < 6; ++v)
-        st[v] = (-sB[v] + s2 * tK[PK_MVV + v]) / V;
+        st[v] = (-sB[v] + s2 * tK[PK_MVV + v]) * iV;
     double const ptr = st[0] + st[1] + st[2];
     double const perr = fabs(P / (ptr / 3.) - 1.);
     if (perr > 1e-5) {
@@ -1872,23 +1880,20 @@ __device__ __forceinline__ void post_step(DevState &s, const double *tA, const d
     rec[MDL_S_BAR_EPS] = s.eps;
     rec[MDL_S_BAR_PEPS] = s.peps;
     for (int v = 0; v < 6; ++v) {
-        rec[MDL_S_BORN_STRESS + v] = sB[v] / V;
+        rec[MDL_S_BORN_STRESS + v] = sB[v] * iV;
         rec[MDL_S_STRESS + v] = st[v];
     }
     if (sample) {
         int q = 0;
         for (int v = 0; v < 6; ++v)
             for (int w = 0; w < 6; ++w)
-                rec[MDL_S_BORN_ELAST + 6 * v + w] = w >= v ? CB[q++] / V : 0.;
+                rec[MDL_S_BORN_ELAST + 6 * v + w] = w >= v ? CB[q++] * iV : 0.;
     }
     rec[MDL_S_NLIST_BUILDS] = (double)s.nbuilds;
     rec[MDL_S_WARN_VIRIAL] = (double)s.warn_virial;
     rec[MDL_S_WARN_RMIN] = (double)s.warn_rmin;
     rec[MDL_S_NLIST_ENTRIES] = (double)s.nlist_entries;
     rec[MDL_S_NHC_GENERAL] = (double)s.nhc_general;
-    if (rec_out && sample)
-        for (int k = 0; k < MDL_NRECORD; ++k)
-            rec_out[k] = rec[k];
 }
 
 // Host-visible per-atom arrays (md_driver_module.cpp:103-109)
