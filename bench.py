@@ -418,9 +418,9 @@ def main():
     ap.add_argument("--replica-edge", type=int, nargs=2, default=[5, 20], metavar=("LO", "HI"),
                     help="replicas: FCC edges (cells) drawn from LO..HI (4 edge^3 atoms)")
     ap.add_argument("--graph-steps", type=int, default=20,
-                    help="replicas: replay chunks of this many steps as one captured HIP graph "
-                         "(per replica, or one graph of batched steps with --batched; 0: launch "
-                         "step by step)")
+                    help="replay aligned chunks of this many steps as one captured HIP graph "
+                         "(the single system; per replica, or one graph of batched steps with "
+                         "--batched; 0: launch step by step)")
     ap.add_argument("--batched", action="store_true",
                     help="replicas: each pass of a step as one launch over every replica "
                          "(mdl_replicas_run) instead of per-replica launches")
@@ -453,6 +453,8 @@ def main():
         return
     w = build_workload(nside=args.nside)
     d, keep = make_driver(w, args.precision, args.lanes_per_atom)
+    if args.graph_steps > 0:
+        d.graph_steps = args.graph_steps
     d.run(args.warmup, 1, 20, records=False)
     mid0 = d.mid_list_builds
     t0 = time.perf_counter()
@@ -516,6 +518,7 @@ def main():
             "mid_list_filterings_per_step": mid_frac,
             "list_builds": d.neighbor_list_builds,
             "lanes_per_atom": d.lanes_per_atom_used,
+            "graph_steps": args.graph_steps,
             "nhc_general_path": d.nhc_general_path_count,
             "kernel_ms": {k: round(v, 5) for k, v in kt.items()},
             "final_T_kin": float(rec[-1][11]),
