@@ -1358,9 +1358,25 @@ __global__ __launch_bounds__(BLOCK) void k_density(AtomArrays a, ListArrays l, R
             nin += __popcll(bal);
         }
 #pragma unroll
-        for (int off = LPA / 2; off > 0; off >>= 1) {
+        for (int off = LPA / 2; off > 0; off >>= 1)
             rho += __shfl_xor(rho, off, LPA);
-            if (SAMPLE) {
+        // sampled steps: g (6) and h (3) of the atom, each summed into one lane
+        // of its group (groups of 16 lanes or more), which stores its g
+        // component; |h|^2 is summed from the h lanes' squares
+        double h2 = 0.;
+        if constexpr (SAMPLE && LPA >= 16) {
+            double gs9[9] = {g0, g1, g2, g3, g4, g5, hx, hy, hz};
+            int c;
+            double const v = group_sum_scatter<LPA, 9>(gs9, c);
+            h2 = c >= 6 ? v * v : 0.;
+#pragma unroll
+            for (int off = LPA / 2; off > 0; off >>= 1)
+                h2 += __shfl_xor(h2, off, LPA);
+            if (active && c >= 0 && c < 6)
+                reinterpret_cast<double *>(a.geam + 4 * (size_t)i)[c] = v;
+        } else if constexpr (SAMPLE) {
+#pragma unroll
+            for (int off = LPA / 2; off > 0; off >>= 1) {
                 g0 += __shfl_xor(g0, off, LPA);
                 g1 += __shfl_xor(g1, off, LPA);
                 g2 += __shfl_xor(g2, off, LPA);
@@ -1371,6 +1387,7 @@ __global__ __launch_bounds__(BLOCK) void k_density(AtomArrays a, ListArrays l, R
                 hy += __shfl_xor(hy, off, LPA);
                 hz += __shfl_xor(hz, off, LPA);
             }
+            h2 = hx * hx + hy * hy + hz * hz;
         }
         if (active && lane == 0) {
             l.ccnt[i] = nin;
@@ -1396,10 +1413,12 @@ __global__ __launch_bounds__(BLOCK) void k_density(AtomArrays a, ListArrays l, R
             acc[PA_EF] += F;
             if (SAMPLE) {
                 double2 *gp = a.geam + 4 * (size_t)i;
-                gp[0] = make_double2(g0, g1);
-                gp[1] = make_double2(g2, g3);
-                gp[2] = make_double2(g4, g5);
-                gp[3] = make_double2(hx * hx + hy * hy + hz * hz, 0.);
+                if constexpr (LPA < 16) {
+                    gp[0] = make_double2(g0, g1);
+                    gp[1] = make_double2(g2, g3);
+                    gp[2] = make_double2(g4, g5);
+                }
+                gp[3] = make_double2(h2, 0.);
             }
         }
     }
