@@ -366,9 +366,13 @@ struct SpBasis {
 };
 
 // The basis sums of one frame: Z[p][k][v] = sum over ordered pairs of species
-// pair p = t_i + t_j of fc(r) g_k(r) n_a n_b. One wave per atom at a time
-// (lanes 0-31 and 32-63 take alternate pairs, lane & 31 = k), 18 sums per
+// pair p = t_i + t_j of fc(r) g_k(r) n_a n_b. One wave per atom at a time: the
+// lanes that find a neighbour within rc (64 candidates at a time) write its
+// geometry (n, r, fc and the species pair) to LDS once, in index order; the
+// half-waves then take alternate pairs from there (lane & 31 = k), 18 sums per
 // lane; per block, the sums over its waves and halves in a fixed order.
+constexpr int SB_NBR = 128;   // neighbours buffered per wave
+
 __global__ __launch_bounds__(256) void k_cv_stress_basis(const double *__restrict__ pos,
                                                          const int *__restrict__ types, int N,
                                                          double Lx, double Ly, double Lz,
@@ -376,6 +380,18 @@ __global__ __launch_bounds__(256) void k_cv_stress_basis(const double *__restric
     int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6, k = lane & 31, half = lane >> 5;
     int const frame = blockIdx.y;
     const double *P = pos + (size_t)frame * N * 3;
+    // the neighbour geometry of the pair loop and the final per-lane sums share
+    // the block's LDS
+    __shared__ union {
+        struct {
+            double geo[4][SB_NBR][5];
+            int pt[4][SB_NBR];
+        } nb;
+        double red[4][64][18];
+    } sh;
+    auto &geo = sh.nb.geo;
+    auto &gpt = sh.nb.pt;
+    unsigned long long const below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     double acc[3][6];
 #pragma unroll
     for (int p = 0; p < 3; ++p)
@@ -387,6 +403,7 @@ __global__ __launch_bounds__(256) void k_cv_stress_basis(const double *__restric
     for (int i = blockIdx.x * 4 + wave; i < N; i += gridDim.x * 4) {
         double const xi = P[3 * i], yi = P[3 * i + 1], zi = P[3 * i + 2];
         int const ti = types[i];
+        int cnt = 0;
         for (int j0 = 0; j0 < N; j0 += 64) {
             int const j = j0 + lane;
             double dx = 0., dy = 0., dz = 0., r2 = 1e300;
@@ -399,27 +416,29 @@ __global__ __launch_bounds__(256) void k_cv_stress_basis(const double *__restric
                 dz -= Lz * rint(dz * iLz);
                 r2 = dx * dx + dy * dy + dz * dz;
             }
-            unsigned long long m = __ballot(r2 < b.rc2);
-            while (m) {
-                // the half-waves take the next two pairs
-                int const s0 = __ffsll(m) - 1;
-                m &= m - 1;
-                int s1 = -1;
-                if (m) {
-                    s1 = __ffsll(m) - 1;
-                    m &= m - 1;
-                }
-                int const src = half ? s1 : s0;
-                int const srcc = src < 0 ? s0 : src;
-                double const ex = __shfl(dx, srcc, 64), ey = __shfl(dy, srcc, 64),
-                             ez = __shfl(dz, srcc, 64), er2 = __shfl(r2, srcc, 64);
-                int const pt = ti + types[j0 + srcc];
-                if (src < 0)
-                    continue;
-                double const r = sqrt(er2), ir = 1.0 / r;
-                double const n[3] = {ex * ir, ey * ir, ez * ir};
-                double const u = r - mu;
-                double const g = exp(-0.5 * u * u * b.iw2) * 0.5 * (cos(b.pirc * r) + 1.0);
+            bool const hit = r2 < b.rc2;
+            unsigned long long const m = __ballot(hit);
+            if (hit) {
+                int const s = cnt + __popcll(m & below);
+                double const r = sqrt(r2), ir = 1.0 / r;
+                geo[wave][s][0] = dx * ir;
+                geo[wave][s][1] = dy * ir;
+                geo[wave][s][2] = dz * ir;
+                geo[wave][s][3] = r;
+                geo[wave][s][4] = 0.5 * (cos(b.pirc * r) + 1.0);
+                gpt[wave][s] = ti + types[j];
+            }
+            cnt += __popcll(m);
+            if (cnt <= SB_NBR - 64 && j0 + 64 < N)
+                continue;
+            // this wave's LDS writes before any lane reads them
+            __builtin_amdgcn_s_waitcnt(0xC07F);
+            __builtin_amdgcn_wave_barrier();
+            for (int t = half; t < cnt; t += 2) {
+                double const n[3] = {geo[wave][t][0], geo[wave][t][1], geo[wave][t][2]};
+                double const u = geo[wave][t][3] - mu;
+                double const g = exp(-0.5 * u * u * b.iw2) * geo[wave][t][4];
+                int const pt = gpt[wave][t];
 #pragma unroll
                 for (int v = 0; v < 6; ++v) {
                     double const w = g * n[SP_VA[v]] * n[SP_VB[v]];
@@ -428,9 +447,14 @@ __global__ __launch_bounds__(256) void k_cv_stress_basis(const double *__restric
                     acc[2][v] += pt == 2 ? w : 0.0;
                 }
             }
+            cnt = 0;
+            // the batch is read out before the next one is written
+            __builtin_amdgcn_s_waitcnt(0xC07F);
+            __builtin_amdgcn_wave_barrier();
         }
     }
-    __shared__ double red[4][64][18];
+    __syncthreads();   // every wave is done with the neighbour geometry
+    auto &red = sh.red;
 #pragma unroll
     for (int p = 0; p < 3; ++p)
 #pragma unroll
@@ -448,72 +472,87 @@ __global__ __launch_bounds__(256) void k_cv_stress_basis(const double *__restric
 }
 
 // The proxy's per-atom terms with fitted coefficients c [3][32]. One wave per
-// atom i; the half-waves take alternate pairs (from a ballot over 64
-// candidates at a time) and lane k & 31 evaluates basis function k, so
-// psi, psi' and psi'' are sums over the 32 lanes of a half (a butterfly). Lane
-// q < 30 of each half then keeps one of the atom's 30 sums: q < 6 the share
-// of h_v, 6 <= q < 12 the Laplacian of h_v, 12 + 3 v + c the gradient
-// component d h_v / d x_{i,c}. The halves are added at the end: hl[i] = the
-// first 12 sums, grad[i][v][c] the other 18.
+// atom i. The wave screens all atoms 64 at a time and collects the neighbours
+// within rc in LDS, in index order (batches of up to SP_NBR); then each lane
+// takes one neighbour and evaluates psi, psi' and psi'' of its pair over the 32
+// basis functions itself, so nothing is summed across lanes per pair, and adds
+// the pair's terms to the atom's 30 sums: q < 6 the share of h_v, 6 <= q < 12
+// the Laplacian of h_v, 12 + 3 v + c the gradient component d h_v / d x_{i,c}.
+// The wave sums them at the end (recursive halving, one sum per lane):
+// hl[i] = the first 12 sums, grad[i][v][c] the other 18.
+constexpr int SP_NBR = 256;   // neighbours buffered per wave
+
+__device__ __forceinline__ void sp_min_image(const double *P, int i, int j, double Lx, double Ly,
+                                             double Lz, double iLx, double iLy, double iLz,
+                                             double &dx, double &dy, double &dz) {
+    dx = P[3 * i] - P[3 * j];
+    dy = P[3 * i + 1] - P[3 * j + 1];
+    dz = P[3 * i + 2] - P[3 * j + 2];
+    dx -= Lx * rint(dx * iLx);
+    dy -= Ly * rint(dy * iLy);
+    dz -= Lz * rint(dz * iLz);
+}
+
 __global__ __launch_bounds__(256) void k_cv_stress_proxy(const double *__restrict__ pos,
                                                          const int *__restrict__ types, int N,
                                                          double Lx, double Ly, double Lz,
                                                          SpBasis b, const double *__restrict__ coef,
                                                          double *__restrict__ grad,
                                                          double *__restrict__ hl) {
-    int const lane = threadIdx.x & 63, k = lane & 31, half = lane >> 5;
-    int const i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    __shared__ double cs[96];
+    __shared__ int nbr[4][SP_NBR];
+    if (threadIdx.x < 96)
+        cs[threadIdx.x] = coef[threadIdx.x];
+    __syncthreads();
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int const i = blockIdx.x * 4 + wave;
     if (i >= N)
         return;   // a whole wave
     int const frame = blockIdx.y;
     const double *P = pos + (size_t)frame * N * 3;
     double const iLx = 1.0 / Lx, iLy = 1.0 / Ly, iLz = 1.0 / Lz;
-    double const xi = P[3 * i], yi = P[3 * i + 1], zi = P[3 * i + 2];
     int const ti = types[i];
-    double const ck[3] = {coef[k], coef[32 + k], coef[64 + k]};
-    double const mu = b.mu0 + b.dmu * k;
-    // this lane's sum: its kind and Voigt / axis indices
-    int const q = k;
-    int const v = q < 6 ? q : (q < 12 ? q - 6 : (q < 30 ? (q - 12) / 3 : 0));
-    int const ax = q >= 12 && q < 30 ? (q - 12) % 3 : 0;
-    int const va = SP_VA[v], vb = SP_VB[v];
-    double acc = 0.0;
+    int *nb = nbr[wave];
+    unsigned long long const below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    double acc[30];
+#pragma unroll
+    for (int q = 0; q < 30; ++q)
+        acc[q] = 0.0;
+    int cnt = 0;
     for (int j0 = 0; j0 < N; j0 += 64) {
         int const j = j0 + lane;
-        double dx = 0., dy = 0., dz = 0., r2 = 1e300;
+        bool hit = false;
         if (j < N && j != i) {
-            dx = xi - P[3 * j];
-            dy = yi - P[3 * j + 1];
-            dz = zi - P[3 * j + 2];
-            dx -= Lx * rint(dx * iLx);
-            dy -= Ly * rint(dy * iLy);
-            dz -= Lz * rint(dz * iLz);
-            r2 = dx * dx + dy * dy + dz * dz;
+            double dx, dy, dz;
+            sp_min_image(P, i, j, Lx, Ly, Lz, iLx, iLy, iLz, dx, dy, dz);
+            hit = dx * dx + dy * dy + dz * dz < b.rc2;
         }
-        unsigned long long m = __ballot(r2 < b.rc2);
-        while (m) {
-            int const s0 = __ffsll(m) - 1;
-            m &= m - 1;
-            int s1 = -1;
-            if (m) {
-                s1 = __ffsll(m) - 1;
-                m &= m - 1;
-            }
-            int const src = half ? s1 : s0;
-            int const srcc = src < 0 ? s0 : src;   // an idle half repeats s0 with weight 0
-            double const live = src < 0 ? 0.0 : 1.0;
-            double const ex = __shfl(dx, srcc, 64), ey = __shfl(dy, srcc, 64),
-                         ez = __shfl(dz, srcc, 64), er2 = __shfl(r2, srcc, 64);
-            int const pt = ti + types[j0 + srcc];
-            double const r = sqrt(er2), ir = 1.0 / r;
-            double const u = r - mu, qv = u * b.iw2;
-            double const cg = (pt == 0 ? ck[0] : (pt == 1 ? ck[1] : ck[2])) * exp(-0.5 * u * qv);
-            double a0 = cg, a1 = -cg * qv, a2 = cg * (qv * qv - b.iw2);
-#pragma unroll
-            for (int off = 16; off > 0; off >>= 1) {
-                a0 += __shfl_xor(a0, off, 32);
-                a1 += __shfl_xor(a1, off, 32);
-                a2 += __shfl_xor(a2, off, 32);
+        unsigned long long const m = __ballot(hit);
+        if (hit)
+            nb[cnt + __popcll(m & below)] = j;
+        cnt += __popcll(m);
+        if (cnt <= SP_NBR - 64 && j0 + 64 < N)
+            continue;
+        // this wave's LDS writes before any lane reads them
+        __builtin_amdgcn_s_waitcnt(0xC07F);
+        __builtin_amdgcn_wave_barrier();
+        for (int t0 = 0; t0 < cnt; t0 += 64) {
+            int const t = t0 + lane;
+            if (t >= cnt)
+                continue;
+            int const jj = nb[t];
+            double ex, ey, ez;
+            sp_min_image(P, i, jj, Lx, Ly, Lz, iLx, iLy, iLz, ex, ey, ez);
+            double const r = sqrt(ex * ex + ey * ey + ez * ez), ir = 1.0 / r;
+            const double *c = cs + 32 * (ti + types[jj]);
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll 8
+            for (int k = 0; k < 32; ++k) {
+                double const u = r - (b.mu0 + b.dmu * k), qv = u * b.iw2;
+                double const cg = c[k] * exp(-0.5 * u * qv);
+                a0 += cg;
+                a1 += -cg * qv;
+                a2 += cg * (qv * qv - b.iw2);
             }
             double const cr = cos(b.pirc * r), sr = sin(b.pirc * r);
             double const fc = 0.5 * (cr + 1.0), dfc = -0.5 * b.pirc * sr,
@@ -521,28 +560,35 @@ __global__ __launch_bounds__(256) void k_cv_stress_proxy(const double *__restric
             double const psi = fc * a0, dpsi = dfc * a0 + fc * a1,
                          ddpsi = ddfc * a0 + 2.0 * dfc * a1 + fc * a2;
             double const n[3] = {ex * ir, ey * ir, ez * ir};
-            double const nab = n[va] * n[vb], pr = psi * ir;
-            double t;
-            if (q < 6) {
-                t = psi * nab;
-            } else if (q < 12) {
-                t = 2.0 * ((ddpsi + 2.0 * dpsi * ir - 6.0 * pr * ir) * nab +
-                           (va == vb ? 2.0 * pr * ir : 0.0));
-            } else {
-                t = 2.0 * (dpsi * nab * n[ax] +
-                           pr * ((va == ax ? n[vb] : 0.0) + (vb == ax ? n[va] : 0.0) -
-                                 2.0 * nab * n[ax]));
+            double const pr = psi * ir;
+            double const lapc = ddpsi + 2.0 * dpsi * ir - 6.0 * pr * ir;
+#pragma unroll
+            for (int v = 0; v < 6; ++v) {
+                int const va = v < 3 ? v : (v == 3 ? 1 : 0), vb = v < 3 ? v : (v == 5 ? 1 : 2);
+                double const nab = n[va] * n[vb];
+                acc[v] += psi * nab;
+                acc[6 + v] += 2.0 * (lapc * nab + (va == vb ? 2.0 * pr * ir : 0.0));
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax)
+                    acc[12 + 3 * v + ax] +=
+                        2.0 * (dpsi * nab * n[ax] +
+                               pr * ((va == ax ? n[vb] : 0.0) + (vb == ax ? n[va] : 0.0) -
+                                     2.0 * nab * n[ax]));
             }
-            acc += live * t;
         }
+        cnt = 0;
+        // the batch is read out before the next one is written
+        __builtin_amdgcn_s_waitcnt(0xC07F);
+        __builtin_amdgcn_wave_barrier();
     }
-    acc += __shfl_down(acc, 32, 64);
-    if (half == 0 && q < 30) {
+    int q;
+    double const sum = mdl::group_sum_scatter<64, 30>(acc, q);
+    if (q >= 0) {
         size_t const o = (size_t)frame * N + i;
         if (q < 12)
-            hl[o * 12 + q] = acc;
+            hl[o * 12 + q] = sum;
         else
-            grad[o * 18 + (q - 12)] = acc;
+            grad[o * 18 + (q - 12)] = sum;
     }
 }
 

@@ -59,53 +59,6 @@ struct Blk {
 };
 __device__ __forceinline__ Blk hw_blk() { return Blk{(int)blockIdx.x, (int)gridDim.x}; }
 
-// Sums over aligned groups of W lanes (W a power of two, up to the wave) of NV
-// components per lane, by recursive halving over the lane bits W/2, W/4, ...,
-// 1: at each step a lane keeps one half of its current array and sends the
-// other half to its partner across that bit, which keeps the other one
-// (halves of ceil(A/2), the odd one padded with zero). About NV exchanges in
-// all, where a butterfly per component takes NV log2(W). Every component is
-// summed over the same tree as by that butterfly (pairs across bit W/2 first,
-// then W/4, ...; the additions commute exactly), so the sums are its bits. On
-// return, each lane holds the group's sum of component `idx`, or idx < 0 if
-// its slot is padding; every component sits in exactly one lane of the group.
-constexpr int halved(int n, int steps) {
-    for (int k = 0; k < steps; ++k)
-        n = (n + 1) / 2;
-    return n;
-}
-
-template <int W, int NV, int S, typename T>
-__device__ __forceinline__ void halving_steps(T (&v)[NV], int lw, int &lo, int &hi) {
-    if constexpr ((W >> (S + 1)) >= 1) {
-        constexpr int A = halved(NV, S), H = (A + 1) / 2, OFF = (W / 2) >> S;
-        bool const up = (lw & OFF) != 0;
-#pragma unroll
-        for (int i = 0; i < H; ++i) {
-            T const a = v[i];
-            T const b = H + i < A ? v[H + i] : T(0);
-            T const keep = up ? b : a;
-            v[i] = keep + __shfl_xor(up ? a : b, OFF, W);
-        }
-        int const mid = lo + H;
-        if (up)
-            lo = mid;
-        else
-            hi = min(hi, mid);
-        halving_steps<W, NV, S + 1>(v, lw, lo, hi);
-    }
-}
-
-template <int W, int NV, typename T>
-__device__ __forceinline__ T group_sum_scatter(T (&v)[NV], int &idx) {
-    static_assert(W >= 1 && W <= 64 && (W & (W - 1)) == 0, "group of 2^k lanes");
-    static_assert(NV >= 1 && NV <= W, "one component per lane at most");
-    int lo = 0, hi = NV;
-    halving_steps<W, NV, 0>(v, (int)threadIdx.x & (W - 1), lo, hi);
-    idx = lo < hi ? lo : -1;
-    return v[0];
-}
-
 template <int NV>
 __device__ __forceinline__ double wave_sum_scatter(double (&v)[NV], int &idx) {
     return group_sum_scatter<64, NV>(v, idx);
