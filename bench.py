@@ -378,6 +378,10 @@ def _learned_cv_here(n_eval):
         "training_gpu_s": 0.0,
         "cost_plain_us_per_frame": round(1e6 * r["cost_plain_frame_s"], 2),
         "cost_cv_us_per_frame": round(1e6 * r["cost_cv_frame_s"], 2),
+        "cost_cv_capture_us_per_frame": round(
+            1e6 * float(np.mean([x["t_capture_frame"] for x in runs])), 2),
+        "cost_cv_fits_us_per_frame": round(
+            1e6 * float(np.mean([x["t_fit_frame"] for x in runs])), 2),
         "C_plain_GPa": [f(x["C_plain"]) for x in runs],
         "C_cv_GPa": [f(x["C_cv"]) for x in runs],
         "se_plain_GPa": f(r["se_plain"]),

@@ -129,6 +129,18 @@ mdl_status mdl_cv_stress_proxy(const double *pos, const int32_t *types, int32_t 
                                double dmu, double width, const double *coef, double *grad,
                                double *hl, void *stream);
 
+/* mdl_cv_stress_fields: the per-frame sums of the proxy's zero-variance fields
+ * without the per-atom arrays, over the frame's atoms m: part[frame][block][q]
+ * (blocks = nblocks; summing over blocks gives the frame's 39 values) with
+ * q < 6 the share of h_v, 6 <= q < 12 the Laplacian lap_m h_v, 12 <= q < 18
+ * grad_m h_v . (fscale F_m) (F [frames][N][3] FP64 device), and 18 + the index
+ * of (a, b), a <= b, in row order of grad_m h_a . grad_m h_b. */
+mdl_status mdl_cv_stress_fields(const double *pos, const double *F, double fscale,
+                                const int32_t *types, int32_t N, int32_t frames,
+                                const double *box, double rc, double mu0, double dmu,
+                                double width, const double *coef, double *part, int32_t nblocks,
+                                void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -123,6 +123,9 @@ SIGNATURES = {
     "mdl_cv_stress_proxy": (C.c_int32, [_vp, _vp, C.c_int32, C.c_int32, _dp, C.c_double,
                                         C.c_double, C.c_double, C.c_double, _vp, _vp, _vp,
                                         _vp]),
+    "mdl_cv_stress_fields": (C.c_int32, [_vp, _vp, C.c_double, _vp, C.c_int32, C.c_int32, _dp,
+                                         C.c_double, C.c_double, C.c_double, C.c_double, _vp,
+                                         _vp, C.c_int32, _vp]),
     "mdl_driver_cv_descriptors": (C.c_int32, [_vp, C.c_float, C.c_float, C.c_float, C.c_float,
                                               _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "mdl_driver_set_cv_capture": (C.c_int32, [_vp, C.c_float, C.c_float, C.c_float, C.c_float,

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 
@@ -363,6 +364,7 @@ __constant__ int SP_VB[6] = {0, 1, 2, 2, 2, 1};
 
 struct SpBasis {
     double mu0, dmu, iw2, pirc, rc2;
+    double q;   // exp(-dmu^2 / w^2): the ratio of successive Gaussian ratios
 };
 
 // The basis sums of one frame: Z[p][k][v] = sum over ordered pairs of species
@@ -478,9 +480,9 @@ __global__ __launch_bounds__(256) void k_cv_stress_basis(const double *__restric
 // basis functions itself, so nothing is summed across lanes per pair, and adds
 // the pair's terms to the atom's 30 sums: q < 6 the share of h_v, 6 <= q < 12
 // the Laplacian of h_v, 12 + 3 v + c the gradient component d h_v / d x_{i,c}.
-// The wave sums them at the end (recursive halving, one sum per lane):
-// hl[i] = the first 12 sums, grad[i][v][c] the other 18.
+// The wave sums them (recursive halving, one sum per lane).
 constexpr int SP_NBR = 256;   // neighbours buffered per wave
+constexpr int SP_FIELD = 39;  // per-frame field sums of k_cv_stress_fields
 
 __device__ __forceinline__ void sp_min_image(const double *P, int i, int j, double Lx, double Ly,
                                              double Lz, double iLx, double iLy, double iLz,
@@ -493,28 +495,50 @@ __device__ __forceinline__ void sp_min_image(const double *P, int i, int j, doub
     dz -= Lz * rint(dz * iLz);
 }
 
-__global__ __launch_bounds__(256) void k_cv_stress_proxy(const double *__restrict__ pos,
-                                                         const int *__restrict__ types, int N,
-                                                         double Lx, double Ly, double Lz,
-                                                         SpBasis b, const double *__restrict__ coef,
-                                                         double *__restrict__ grad,
-                                                         double *__restrict__ hl) {
-    __shared__ double cs[96];
-    __shared__ int nbr[4][SP_NBR];
-    if (threadIdx.x < 96)
-        cs[threadIdx.x] = coef[threadIdx.x];
-    __syncthreads();
-    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int const i = blockIdx.x * 4 + wave;
-    if (i >= N)
-        return;   // a whole wave
-    int const frame = blockIdx.y;
-    const double *P = pos + (size_t)frame * N * 3;
+// sum_k c[k] g_k(r) (times 1, -q_k and q_k^2 - 1/w^2, q_k = (r - mu_k)/w^2:
+// psi's three radial sums). RECUR: the Gaussians from four anchors (k = 0, 8,
+// 16, 24, each an exp) by their ratio g_{k+1}/g_k = exp((r - mu_k) dmu / w^2 -
+// dmu^2 / (2 w^2)), whose own ratio is the constant b.q; within 7 steps of an
+// anchor the product stays within a few ulp. The host takes this form only
+// where no anchor can underflow; else one exp per basis function.
+template <bool RECUR>
+__device__ __forceinline__ void sp_radial(double r, const double *__restrict__ c, const SpBasis &b,
+                                          double &a0, double &a1, double &a2) {
+    a0 = a1 = a2 = 0.0;
+#pragma unroll
+    for (int k0 = 0; k0 < 32; k0 += 8) {
+        double g = 0.0, R = 0.0;
+        if (RECUR) {
+            double const u0 = r - (b.mu0 + b.dmu * k0);
+            g = exp(-0.5 * u0 * u0 * b.iw2);
+            R = exp(b.iw2 * (u0 * b.dmu - 0.5 * b.dmu * b.dmu));
+        }
+#pragma unroll
+        for (int k = k0; k < k0 + 8; ++k) {
+            double const u = r - (b.mu0 + b.dmu * k), qv = u * b.iw2;
+            double const gk = RECUR ? g : exp(-0.5 * u * qv);
+            double const cg = c[k] * gk;
+            a0 += cg;
+            a1 += -cg * qv;
+            a2 += cg * (qv * qv - b.iw2);
+            if (RECUR) {
+                g *= R;
+                R *= b.q;
+            }
+        }
+    }
+}
+
+// The 30 per-atom sums of atom i in `acc` (each lane its own pairs' share).
+template <bool RECUR>
+__device__ __forceinline__ void sp_atom_terms(const double *__restrict__ P,
+                                              const int *__restrict__ types, int N, int i,
+                                              double Lx, double Ly, double Lz, const SpBasis &b,
+                                              const double *cs, int *nb, double (&acc)[30]) {
+    int const lane = threadIdx.x & 63;
     double const iLx = 1.0 / Lx, iLy = 1.0 / Ly, iLz = 1.0 / Lz;
     int const ti = types[i];
-    int *nb = nbr[wave];
     unsigned long long const below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    double acc[30];
 #pragma unroll
     for (int q = 0; q < 30; ++q)
         acc[q] = 0.0;
@@ -544,16 +568,8 @@ __global__ __launch_bounds__(256) void k_cv_stress_proxy(const double *__restric
             double ex, ey, ez;
             sp_min_image(P, i, jj, Lx, Ly, Lz, iLx, iLy, iLz, ex, ey, ez);
             double const r = sqrt(ex * ex + ey * ey + ez * ez), ir = 1.0 / r;
-            const double *c = cs + 32 * (ti + types[jj]);
-            double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-#pragma unroll 8
-            for (int k = 0; k < 32; ++k) {
-                double const u = r - (b.mu0 + b.dmu * k), qv = u * b.iw2;
-                double const cg = c[k] * exp(-0.5 * u * qv);
-                a0 += cg;
-                a1 += -cg * qv;
-                a2 += cg * (qv * qv - b.iw2);
-            }
+            double a0, a1, a2;
+            sp_radial<RECUR>(r, cs + 32 * (ti + types[jj]), b, a0, a1, a2);
             double const cr = cos(b.pirc * r), sr = sin(b.pirc * r);
             double const fc = 0.5 * (cr + 1.0), dfc = -0.5 * b.pirc * sr,
                          ddfc = -0.5 * b.pirc * b.pirc * cr;
@@ -581,6 +597,29 @@ __global__ __launch_bounds__(256) void k_cv_stress_proxy(const double *__restric
         __builtin_amdgcn_s_waitcnt(0xC07F);
         __builtin_amdgcn_wave_barrier();
     }
+}
+
+// Per atom: hl[i] = the first 12 sums, grad[i][v][c] the other 18.
+template <bool RECUR>
+__global__ __launch_bounds__(256) void k_cv_stress_proxy(const double *__restrict__ pos,
+                                                         const int *__restrict__ types, int N,
+                                                         double Lx, double Ly, double Lz,
+                                                         SpBasis b, const double *__restrict__ coef,
+                                                         double *__restrict__ grad,
+                                                         double *__restrict__ hl) {
+    __shared__ double cs[96];
+    __shared__ int nbr[4][SP_NBR];
+    if (threadIdx.x < 96)
+        cs[threadIdx.x] = coef[threadIdx.x];
+    __syncthreads();
+    int const wave = threadIdx.x >> 6;
+    int const i = blockIdx.x * 4 + wave;
+    if (i >= N)
+        return;   // a whole wave
+    int const frame = blockIdx.y;
+    double acc[30];
+    sp_atom_terms<RECUR>(pos + (size_t)frame * N * 3, types, N, i, Lx, Ly, Lz, b, cs, nbr[wave],
+                         acc);
     int q;
     double const sum = mdl::group_sum_scatter<64, 30>(acc, q);
     if (q >= 0) {
@@ -590,6 +629,78 @@ __global__ __launch_bounds__(256) void k_cv_stress_proxy(const double *__restric
         else
             grad[o * 18 + (q - 12)] = sum;
     }
+}
+
+// The per-frame sums the variate's stress fields need, without the per-atom
+// arrays: over the frame's atoms m, the share of h_v (6), lap_m h_v (6),
+// grad_m h_v . F_m (6, F = fscale times the forces given) and
+// grad_m h_a . grad_m h_b (21, a <= b, row by row). Each wave takes atoms in
+// turn; lane q keeps the running sum of quantity q; per block, the waves in
+// order: part[frame][block][39].
+template <bool RECUR>
+__global__ __launch_bounds__(256) void k_cv_stress_fields(const double *__restrict__ pos,
+                                                          const double *__restrict__ F,
+                                                          double fscale,
+                                                          const int *__restrict__ types, int N,
+                                                          double Lx, double Ly, double Lz,
+                                                          SpBasis b,
+                                                          const double *__restrict__ coef,
+                                                          double *__restrict__ part) {
+    __shared__ double cs[96];
+    __shared__ int nbr[4][SP_NBR];
+    __shared__ double sv[4][32];
+    __shared__ double red[4][SP_FIELD];
+    if (threadIdx.x < 96)
+        cs[threadIdx.x] = coef[threadIdx.x];
+    __syncthreads();
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int const frame = blockIdx.y;
+    const double *P = pos + (size_t)frame * N * 3;
+    const double *Ff = F + (size_t)frame * N * 3;
+    // lane's quantity: 18 + index of the pair (a, b), a <= b, for lanes 18..38
+    int pa = 0, pb = 0;
+    if (lane >= 18 && lane < SP_FIELD) {
+        int t = lane - 18;
+        while (t >= 6 - pa) {
+            t -= 6 - pa;
+            ++pa;
+        }
+        pb = pa + t;
+    }
+    double run = 0.0;
+    for (int i = blockIdx.x * 4 + wave; i < N; i += gridDim.x * 4) {
+        double acc[30];
+        sp_atom_terms<RECUR>(P, types, N, i, Lx, Ly, Lz, b, cs, nbr[wave], acc);
+        int q;
+        double const sum = mdl::group_sum_scatter<64, 30>(acc, q);
+        if (q >= 0)
+            sv[wave][q] = sum;
+        __builtin_amdgcn_s_waitcnt(0xC07F);
+        __builtin_amdgcn_wave_barrier();
+        const double *g = sv[wave] + 12;   // g[3 v + c] = d h_v / d x_{i,c}
+        double val = 0.0;
+        if (lane < 12) {
+            val = sv[wave][lane];
+        } else if (lane < 18) {
+            int const v = lane - 12;
+            val = fscale * (g[3 * v] * Ff[3 * i] + g[3 * v + 1] * Ff[3 * i + 1] +
+                            g[3 * v + 2] * Ff[3 * i + 2]);
+        } else if (lane < SP_FIELD) {
+            val = g[3 * pa] * g[3 * pb] + g[3 * pa + 1] * g[3 * pb + 1] +
+                  g[3 * pa + 2] * g[3 * pb + 2];
+        }
+        run += val;
+        // the atom's sums are read out before the next atom's are written
+        __builtin_amdgcn_s_waitcnt(0xC07F);
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (lane < SP_FIELD)
+        red[wave][lane] = run;
+    __syncthreads();
+    if (threadIdx.x < SP_FIELD)
+        part[((size_t)frame * gridDim.x + blockIdx.x) * SP_FIELD + threadIdx.x] =
+            ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) +
+            red[3][threadIdx.x];
 }
 
 mdl_status launch_ok(const char *what) {
@@ -644,7 +755,16 @@ static SpBasis sp_basis(double rc, double mu0, double dmu, double width) {
     b.iw2 = 1.0 / (width * width);
     b.pirc = 3.14159265358979323846 / rc;
     b.rc2 = rc * rc;
+    b.q = std::exp(-dmu * dmu * b.iw2);
     return b;
+}
+
+// The Gaussians may come from anchors and ratios when, for any r in [0, rc],
+// no anchor underflows and no ratio overflows: |r - mu_k0| <= max(rc - mu0,
+// mu0 + 24 dmu) over the anchors k0 = 0, 8, 16, 24 (e^-700 is a normal double).
+static bool sp_recur(const SpBasis &b, double rc) {
+    double const u = std::max(std::fabs(rc - b.mu0), std::fabs(b.mu0 + 24.0 * b.dmu));
+    return b.dmu > 0.0 && 0.5 * u * u * b.iw2 < 700.0 && u * b.dmu * b.iw2 < 700.0;
 }
 
 mdl_status mdl_cv_stress_basis_sums(const double *pos, const int32_t *types, int32_t N,
@@ -672,9 +792,35 @@ mdl_status mdl_cv_stress_proxy(const double *pos, const int32_t *types, int32_t 
         set_error("mdl_cv_stress_proxy: bad arguments");
         return MDL_ERR_ARG;
     }
-    k_cv_stress_proxy<<<dim3((N + 3) / 4, frames), 256, 0, (hipStream_t)stream>>>(
-        pos, types, N, box[0], box[1], box[2], sp_basis(rc, mu0, dmu, width), coef, grad, hl);
+    SpBasis const b = sp_basis(rc, mu0, dmu, width);
+    if (sp_recur(b, rc))
+        k_cv_stress_proxy<true><<<dim3((N + 3) / 4, frames), 256, 0, (hipStream_t)stream>>>(
+            pos, types, N, box[0], box[1], box[2], b, coef, grad, hl);
+    else
+        k_cv_stress_proxy<false><<<dim3((N + 3) / 4, frames), 256, 0, (hipStream_t)stream>>>(
+            pos, types, N, box[0], box[1], box[2], b, coef, grad, hl);
     return launch_ok("mdl_cv_stress_proxy");
+}
+
+mdl_status mdl_cv_stress_fields(const double *pos, const double *F, double fscale,
+                                const int32_t *types, int32_t N, int32_t frames,
+                                const double *box, double rc, double mu0, double dmu,
+                                double width, const double *coef, double *part, int32_t nblocks,
+                                void *stream) {
+    if (!sp_args("mdl_cv_stress_fields", pos, types, N, frames, box, rc, width))
+        return MDL_ERR_ARG;
+    if (!F || !coef || !part || nblocks < 1 || nblocks > 65535) {
+        set_error("mdl_cv_stress_fields: bad arguments");
+        return MDL_ERR_ARG;
+    }
+    SpBasis const b = sp_basis(rc, mu0, dmu, width);
+    if (sp_recur(b, rc))
+        k_cv_stress_fields<true><<<dim3(nblocks, frames), 256, 0, (hipStream_t)stream>>>(
+            pos, F, fscale, types, N, box[0], box[1], box[2], b, coef, part);
+    else
+        k_cv_stress_fields<false><<<dim3(nblocks, frames), 256, 0, (hipStream_t)stream>>>(
+            pos, F, fscale, types, N, box[0], box[1], box[2], b, coef, part);
+    return launch_ok("mdl_cv_stress_fields");
 }
 
 mdl_status mdl_cv_gemm(const float *A, const float *W, int32_t transB, const float *bias,

@@ -75,7 +75,8 @@ NOUT = NFIELD * NF
 HEADS = ("C11", "C12", "C44")
 NGLOBAL = 27       # global_features: S, T, Q
 NGLOBAL_USED = 27
-FIELD_GAIN = 0.05     # least held-out variance cut that keeps the learned fields of a head  # the controls use S, T and the h-weighted displacement sums
+FIELD_GAIN = 0.05     # least held-out variance cut that keeps the learned fields of a head
+FRAME_BATCH = 256     # stored frames per pass of the feature and proxy-fit kernels
 
 
 def _torch():
@@ -167,6 +168,30 @@ def stress_proxy(pos_A, types32, box_A, basis: Basis, coef):
                                          _p(grad), _p(hl), _stream()))
     s = hl.sum(1)
     return s[:, :6], s[:, 6:], grad
+
+
+SF_BLOCKS = 64     # blocks per frame of the field sums
+SF_N = 39          # per-frame field sums
+# (a, b), a <= b, row by row: the order of the 21 gradient products
+SF_PAIRS = [(a, b) for a in range(6) for b in range(a, 6)]
+
+
+def stress_field_sums(pos_A, F, types32, box_A, basis: Basis, coef):
+    """[B, 39] (FP64, device) for B frames of positions [B, N, 3] in A and
+    forces F [B, N, 3] (in the units the sums want): over the atoms m, the
+    proxy's pair sum h_v (6), its Laplacian lap_m h_v (6), grad_m h_v . F_m (6)
+    and grad_m h_a . grad_m h_b for a <= b (21) (mdl_cv_stress_fields)."""
+    torch = _torch()
+    pos = pos_A.contiguous()
+    Fc = F.contiguous()
+    B, N, _ = pos.shape
+    part = torch.empty((B, SF_BLOCKS, SF_N), dtype=torch.float64, device=pos.device)
+    box = (C.c_double * 3)(*[float(b) for b in box_A])
+    _check(nat.lib().mdl_cv_stress_fields(_p(pos), _p(Fc), 1.0, _p(types32), N, B, box,
+                                          basis.rc, basis.mu0, basis.dmu, basis.width,
+                                          _p(coef.contiguous()), _p(part), SF_BLOCKS,
+                                          _stream()))
+    return part.sum(1)
 
 
 def driver_descriptors(drv, basis: Basis = Basis(), device="cuda:0", out=None, La=None):
@@ -623,7 +648,7 @@ class ControlVariate:
         oh = torch.nn.functional.one_hot(self.types, 2).to(D.dtype)      # [N, 2]
         return torch.einsum("bnk,nc->bck", D, oh)
 
-    def fit_proxy(self, fr: Frames, batch=64, ridge=1e-8):
+    def fit_proxy(self, fr: Frames, batch=FRAME_BATCH, ridge=1e-8):
         """V by least squares of the pressure -(s_1 + s_2 + s_3)/3 (GPa) on the
         species sums of the descriptors; returns R^2 on ``fr``."""
         torch = _torch()
@@ -662,7 +687,7 @@ class ControlVariate:
         return self._tables
 
     # -- the stress proxy and its zero-variance fields ---------------------------
-    def fit_stress_proxy(self, fr: Frames, batch=32, ridge=1e-10):
+    def fit_stress_proxy(self, fr: Frames, batch=FRAME_BATCH, ridge=1e-10):
         """The stress proxy h_v = off_v + sum over ordered pairs of
         psi_{t_i+t_j}(r) n_a n_b (psi = fc sum_k c[p, k] g_k, one radial function
         per species pair shared by the six Voigt components), by least squares
@@ -691,14 +716,18 @@ class ControlVariate:
         """[B, 42] Stein sums of the zero-variance fields of the stress proxy h
         (positions in CGS, forces in eV/A): the fields grad h_a (sums lap h_a +
         beta sum_m grad_m h_a . F_m) and (h_a - h0_a) grad h_b (sums
-        sum_m grad_m h_a . grad_m h_b + (h_a - h0_a) x the former)."""
+        sum_m grad_m h_a . grad_m h_b + (h_a - h0_a) x the former), from the
+        per-frame sums of one kernel pass (no per-atom arrays)."""
         torch = _torch()
-        hp, lap, grad = stress_proxy(pos / U.angstrom, self.types32, self.box, self.basis,
-                                     self.sp_coef)
-        h = hp + torch.as_tensor(self.sp_off, dtype=hp.dtype, device=hp.device)
-        dh = h - torch.as_tensor(self.sp_h0, dtype=hp.dtype, device=hp.device)
-        A = lap + self.beta * torch.einsum("bmac,bmc->ba", grad, F)
-        GG = torch.einsum("bmac,bmdc->bad", grad, grad)
+        s = stress_field_sums(pos / U.angstrom, F, self.types32, self.box, self.basis,
+                              self.sp_coef)
+        h = s[:, :6] + torch.as_tensor(self.sp_off, dtype=s.dtype, device=s.device)
+        dh = h - torch.as_tensor(self.sp_h0, dtype=s.dtype, device=s.device)
+        A = s[:, 6:12] + self.beta * s[:, 12:18]
+        GG = torch.empty((s.shape[0], 6, 6), dtype=s.dtype, device=s.device)
+        for q, (a, b) in enumerate(SF_PAIRS):
+            GG[:, a, b] = s[:, 18 + q]
+            GG[:, b, a] = s[:, 18 + q]
         Bm = GG + dh[:, :, None] * A[:, None, :]
         return torch.cat([A, Bm.reshape(-1, 36)], dim=1)
 
@@ -741,7 +770,7 @@ class ControlVariate:
         return self._x(st["D"][sl], st["J"][sl], st["L"][sl], st["pos"][sl], st["F"][sl], Ut,
                        grad)
 
-    def features(self, fr: Frames, batch=64):
+    def features(self, fr: Frames, batch=FRAME_BATCH):
         out = np.zeros((fr.n, 3, self.NX))
         for k0 in range(0, fr.n, batch):
             sl = slice(k0, min(fr.n, k0 + batch))

@@ -338,6 +338,20 @@ def test_stress_kernels_match_the_oracle():
         assert _rel(hp[b], hpo.sum(0)) < 1e-12
         assert _rel(lap[b], lapo.sum(0)) < 1e-12
         assert _rel(grad[b], grado) < 1e-12
+    # the per-frame field sums of one pass (mdl_cv_stress_fields) against the
+    # oracle's per-atom terms contracted on the host
+    Fr = np.random.RandomState(5).normal(size=pos.shape)
+    sf = cv.stress_field_sums(t, torch.tensor(Fr, dtype=torch.float64, device=DEV), ty, box,
+                              cv.Basis(), torch.tensor(coef, dtype=torch.float64, device=DEV))
+    sf = sf.cpu().numpy()
+    for b in range(3):
+        hpo, lapo, grado = O.stress_proxy(pos[b], types, box, coef)
+        gF = np.einsum("mvc,mc->v", grado, Fr[b])
+        gg = np.einsum("mac,mbc->ab", grado, grado)
+        assert _rel(sf[b, :6], hpo.sum(0)) < 1e-12
+        assert _rel(sf[b, 6:12], lapo.sum(0)) < 1e-12
+        assert _rel(sf[b, 12:18], gF) < 1e-12
+        assert _rel(sf[b, 18:], np.array([gg[a, c] for a, c in cv.SF_PAIRS])) < 1e-12
 
 
 def test_stress_fields_on_md_frames():
