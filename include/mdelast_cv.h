@@ -57,6 +57,15 @@ mdl_status mdl_driver_set_cv_capture(mdl_driver *d, float rc, float mu0, float d
                                      float *D, float *J, float *L, double *pos_out,
                                      double *force_out, int32_t slots);
 
+/* With a capture set, also the pairs of each captured step: nbr_out
+ * [slots][N][width] and count_out [slots][N] (int32 device) receive, per atom,
+ * its pairs within the capture's cut-off in the driver's list order (the
+ * compacted pairs within r_c, or the full list when rc exceeds r_c) and their
+ * number; a count above width means the row was cut. NULL stops it; a new
+ * mdl_driver_set_cv_capture clears it. No counterpart in the reference. */
+mdl_status mdl_driver_set_cv_capture_pairs(mdl_driver *d, int32_t *nbr_out, int32_t *count_out,
+                                           int32_t width);
+
 /* The driver's stream (a hipStream_t) into *out. */
 mdl_status mdl_driver_stream(mdl_driver *d, void **out);
 
@@ -106,7 +115,12 @@ mdl_status mdl_cv_field_contract_backward(const float *J, const float *L, const 
 /* Stress proxy of the zero-variance fields (no counterpart in the reference).
  * For `frames` frames of N atoms of two species (types 0/1; the same types for
  * every frame), positions pos [frames][N][3] FP64 in A, box: 3 FP64 on the
- * HOST (2 rc <= each edge), all pairs by brute force, FP64 throughout:
+ * HOST (2 rc <= each edge), FP64 throughout. The pairs are found among all
+ * atoms, or, for mdl_cv_stress_basis_sums and mdl_cv_stress_fields with
+ * `pairs` non-NULL, taken from per-frame pair lists as
+ * mdl_driver_set_cv_capture_pairs writes them (pairs [frames][N][pair_width],
+ * pair_count [frames][N], int32 device; each pair re-checked against rc; a
+ * row's first min(count, pair_width) entries):
  *
  * mdl_cv_stress_basis_sums: part[frame][block][p][k][v] (blocks = nblocks) =
  *   the share of block `block` of Z[p][k][v] = sum over ordered pairs (i, j),
@@ -122,8 +136,9 @@ mdl_status mdl_cv_field_contract_backward(const float *J, const float *L, const 
  *   atom i's coordinates, grad[frame][i][v][c] = d h_v / d x_{i,c}. */
 mdl_status mdl_cv_stress_basis_sums(const double *pos, const int32_t *types, int32_t N,
                                     int32_t frames, const double *box, double rc, double mu0,
-                                    double dmu, double width, double *part, int32_t nblocks,
-                                    void *stream);
+                                    double dmu, double width, const int32_t *pairs,
+                                    const int32_t *pair_count, int32_t pair_width, double *part,
+                                    int32_t nblocks, void *stream);
 mdl_status mdl_cv_stress_proxy(const double *pos, const int32_t *types, int32_t N,
                                int32_t frames, const double *box, double rc, double mu0,
                                double dmu, double width, const double *coef, double *grad,
@@ -138,8 +153,9 @@ mdl_status mdl_cv_stress_proxy(const double *pos, const int32_t *types, int32_t 
 mdl_status mdl_cv_stress_fields(const double *pos, const double *F, double fscale,
                                 const int32_t *types, int32_t N, int32_t frames,
                                 const double *box, double rc, double mu0, double dmu,
-                                double width, const double *coef, double *part, int32_t nblocks,
-                                void *stream);
+                                double width, const double *coef, const int32_t *pairs,
+                                const int32_t *pair_count, int32_t pair_width, double *part,
+                                int32_t nblocks, void *stream);
 
 #ifdef __cplusplus
 }

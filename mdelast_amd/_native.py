@@ -118,18 +118,19 @@ SIGNATURES = {
     "mdl_cv_descriptors": (C.c_int32, [_vp, _vp, C.c_int32, _dp, C.c_float, C.c_float,
                                        C.c_float, C.c_float, _vp, _vp, _vp, _vp]),
     "mdl_cv_stress_basis_sums": (C.c_int32, [_vp, _vp, C.c_int32, C.c_int32, _dp, C.c_double,
-                                             C.c_double, C.c_double, C.c_double, _vp,
-                                             C.c_int32, _vp]),
+                                             C.c_double, C.c_double, C.c_double, _vp, _vp,
+                                             C.c_int32, _vp, C.c_int32, _vp]),
     "mdl_cv_stress_proxy": (C.c_int32, [_vp, _vp, C.c_int32, C.c_int32, _dp, C.c_double,
                                         C.c_double, C.c_double, C.c_double, _vp, _vp, _vp,
                                         _vp]),
     "mdl_cv_stress_fields": (C.c_int32, [_vp, _vp, C.c_double, _vp, C.c_int32, C.c_int32, _dp,
                                          C.c_double, C.c_double, C.c_double, C.c_double, _vp,
-                                         _vp, C.c_int32, _vp]),
+                                         _vp, _vp, C.c_int32, _vp, C.c_int32, _vp]),
     "mdl_driver_cv_descriptors": (C.c_int32, [_vp, C.c_float, C.c_float, C.c_float, C.c_float,
                                               _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "mdl_driver_set_cv_capture": (C.c_int32, [_vp, C.c_float, C.c_float, C.c_float, C.c_float,
                                               _vp, _vp, _vp, _vp, _vp, C.c_int32]),
+    "mdl_driver_set_cv_capture_pairs": (C.c_int32, [_vp, _vp, _vp, C.c_int32]),
     "mdl_driver_stream": (C.c_int32, [_vp, C.POINTER(_vp)]),
     "mdl_driver_copy_atoms": (C.c_int32, [_vp, C.c_int32, _vp]),
     "mdl_cv_gemm": (C.c_int32, [_vp, _vp, C.c_int32, _vp, _vp, C.c_int64, C.c_int32, C.c_int32,

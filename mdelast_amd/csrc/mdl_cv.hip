@@ -378,7 +378,10 @@ constexpr int SB_NBR = 128;   // neighbours buffered per wave
 __global__ __launch_bounds__(256) void k_cv_stress_basis(const double *__restrict__ pos,
                                                          const int *__restrict__ types, int N,
                                                          double Lx, double Ly, double Lz,
-                                                         SpBasis b, double *__restrict__ part) {
+                                                         SpBasis b,
+                                                         const int *__restrict__ plist,
+                                                         const int *__restrict__ pcount,
+                                                         int width, double *__restrict__ part) {
     int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6, k = lane & 31, half = lane >> 5;
     int const frame = blockIdx.y;
     const double *P = pos + (size_t)frame * N * 3;
@@ -406,10 +409,19 @@ __global__ __launch_bounds__(256) void k_cv_stress_basis(const double *__restric
         double const xi = P[3 * i], yi = P[3 * i + 1], zi = P[3 * i + 2];
         int const ti = types[i];
         int cnt = 0;
-        for (int j0 = 0; j0 < N; j0 += 64) {
-            int const j = j0 + lane;
+        // candidates: every atom, or the atom's captured pair list in list order
+        const int *pl = nullptr;
+        int nc = N;
+        if (plist) {
+            size_t const o = (size_t)frame * N + i;
+            pl = plist + o * width;
+            nc = min(pcount[o], width);
+        }
+        for (int j0 = 0; j0 < nc; j0 += 64) {
+            int const t = j0 + lane;
+            int const j = pl ? (t < nc ? pl[t] : i) : t;
             double dx = 0., dy = 0., dz = 0., r2 = 1e300;
-            if (j < N && j != i) {
+            if (t < nc && j != i) {
                 dx = xi - P[3 * j];
                 dy = yi - P[3 * j + 1];
                 dz = zi - P[3 * j + 2];
@@ -431,7 +443,7 @@ __global__ __launch_bounds__(256) void k_cv_stress_basis(const double *__restric
                 gpt[wave][s] = ti + types[j];
             }
             cnt += __popcll(m);
-            if (cnt <= SB_NBR - 64 && j0 + 64 < N)
+            if (cnt <= SB_NBR - 64 && j0 + 64 < nc)
                 continue;
             // this wave's LDS writes before any lane reads them
             __builtin_amdgcn_s_waitcnt(0xC07F);
@@ -530,11 +542,15 @@ __device__ __forceinline__ void sp_radial(double r, const double *__restrict__ c
 }
 
 // The 30 per-atom sums of atom i in `acc` (each lane its own pairs' share).
+// Candidates: every atom (screened 64 at a time, the neighbours collected in
+// LDS in index order), or with `pl` the atom's captured pair list (pl[0..pc),
+// in list order, re-checked against rc).
 template <bool RECUR>
 __device__ __forceinline__ void sp_atom_terms(const double *__restrict__ P,
                                               const int *__restrict__ types, int N, int i,
                                               double Lx, double Ly, double Lz, const SpBasis &b,
-                                              const double *cs, int *nb, double (&acc)[30]) {
+                                              const double *cs, int *nb, const int *pl, int pc,
+                                              double (&acc)[30]) {
     int const lane = threadIdx.x & 63;
     double const iLx = 1.0 / Lx, iLy = 1.0 / Ly, iLz = 1.0 / Lz;
     int const ti = types[i];
@@ -542,6 +558,43 @@ __device__ __forceinline__ void sp_atom_terms(const double *__restrict__ P,
 #pragma unroll
     for (int q = 0; q < 30; ++q)
         acc[q] = 0.0;
+    // the terms of neighbour jj (within rc) into this lane's sums
+    auto pair = [&](int jj, double ex, double ey, double ez) {
+        double const r = sqrt(ex * ex + ey * ey + ez * ez), ir = 1.0 / r;
+        double a0, a1, a2;
+        sp_radial<RECUR>(r, cs + 32 * (ti + types[jj]), b, a0, a1, a2);
+        double const cr = cos(b.pirc * r), sr = sin(b.pirc * r);
+        double const fc = 0.5 * (cr + 1.0), dfc = -0.5 * b.pirc * sr,
+                     ddfc = -0.5 * b.pirc * b.pirc * cr;
+        double const psi = fc * a0, dpsi = dfc * a0 + fc * a1,
+                     ddpsi = ddfc * a0 + 2.0 * dfc * a1 + fc * a2;
+        double const n[3] = {ex * ir, ey * ir, ez * ir};
+        double const pr = psi * ir;
+        double const lapc = ddpsi + 2.0 * dpsi * ir - 6.0 * pr * ir;
+#pragma unroll
+        for (int v = 0; v < 6; ++v) {
+            int const va = v < 3 ? v : (v == 3 ? 1 : 0), vb = v < 3 ? v : (v == 5 ? 1 : 2);
+            double const nab = n[va] * n[vb];
+            acc[v] += psi * nab;
+            acc[6 + v] += 2.0 * (lapc * nab + (va == vb ? 2.0 * pr * ir : 0.0));
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax)
+                acc[12 + 3 * v + ax] +=
+                    2.0 * (dpsi * nab * n[ax] +
+                           pr * ((va == ax ? n[vb] : 0.0) + (vb == ax ? n[va] : 0.0) -
+                                 2.0 * nab * n[ax]));
+        }
+    };
+    if (pl) {
+        for (int t = lane; t < pc; t += 64) {
+            int const jj = pl[t];
+            double ex, ey, ez;
+            sp_min_image(P, i, jj, Lx, Ly, Lz, iLx, iLy, iLz, ex, ey, ez);
+            if (jj != i && ex * ex + ey * ey + ez * ez < b.rc2)
+                pair(jj, ex, ey, ez);
+        }
+        return;
+    }
     int cnt = 0;
     for (int j0 = 0; j0 < N; j0 += 64) {
         int const j = j0 + lane;
@@ -567,30 +620,7 @@ __device__ __forceinline__ void sp_atom_terms(const double *__restrict__ P,
             int const jj = nb[t];
             double ex, ey, ez;
             sp_min_image(P, i, jj, Lx, Ly, Lz, iLx, iLy, iLz, ex, ey, ez);
-            double const r = sqrt(ex * ex + ey * ey + ez * ez), ir = 1.0 / r;
-            double a0, a1, a2;
-            sp_radial<RECUR>(r, cs + 32 * (ti + types[jj]), b, a0, a1, a2);
-            double const cr = cos(b.pirc * r), sr = sin(b.pirc * r);
-            double const fc = 0.5 * (cr + 1.0), dfc = -0.5 * b.pirc * sr,
-                         ddfc = -0.5 * b.pirc * b.pirc * cr;
-            double const psi = fc * a0, dpsi = dfc * a0 + fc * a1,
-                         ddpsi = ddfc * a0 + 2.0 * dfc * a1 + fc * a2;
-            double const n[3] = {ex * ir, ey * ir, ez * ir};
-            double const pr = psi * ir;
-            double const lapc = ddpsi + 2.0 * dpsi * ir - 6.0 * pr * ir;
-#pragma unroll
-            for (int v = 0; v < 6; ++v) {
-                int const va = v < 3 ? v : (v == 3 ? 1 : 0), vb = v < 3 ? v : (v == 5 ? 1 : 2);
-                double const nab = n[va] * n[vb];
-                acc[v] += psi * nab;
-                acc[6 + v] += 2.0 * (lapc * nab + (va == vb ? 2.0 * pr * ir : 0.0));
-#pragma unroll
-                for (int ax = 0; ax < 3; ++ax)
-                    acc[12 + 3 * v + ax] +=
-                        2.0 * (dpsi * nab * n[ax] +
-                               pr * ((va == ax ? n[vb] : 0.0) + (vb == ax ? n[va] : 0.0) -
-                                     2.0 * nab * n[ax]));
-            }
+            pair(jj, ex, ey, ez);
         }
         cnt = 0;
         // the batch is read out before the next one is written
@@ -619,7 +649,7 @@ __global__ __launch_bounds__(256) void k_cv_stress_proxy(const double *__restric
     int const frame = blockIdx.y;
     double acc[30];
     sp_atom_terms<RECUR>(pos + (size_t)frame * N * 3, types, N, i, Lx, Ly, Lz, b, cs, nbr[wave],
-                         acc);
+                         nullptr, 0, acc);
     int q;
     double const sum = mdl::group_sum_scatter<64, 30>(acc, q);
     if (q >= 0) {
@@ -645,6 +675,9 @@ __global__ __launch_bounds__(256) void k_cv_stress_fields(const double *__restri
                                                           double Lx, double Ly, double Lz,
                                                           SpBasis b,
                                                           const double *__restrict__ coef,
+                                                          const int *__restrict__ plist,
+                                                          const int *__restrict__ pcount,
+                                                          int width,
                                                           double *__restrict__ part) {
     __shared__ double cs[96];
     __shared__ int nbr[4][SP_NBR];
@@ -670,7 +703,14 @@ __global__ __launch_bounds__(256) void k_cv_stress_fields(const double *__restri
     double run = 0.0;
     for (int i = blockIdx.x * 4 + wave; i < N; i += gridDim.x * 4) {
         double acc[30];
-        sp_atom_terms<RECUR>(P, types, N, i, Lx, Ly, Lz, b, cs, nbr[wave], acc);
+        const int *pl = nullptr;
+        int pc = 0;
+        if (plist) {
+            size_t const o = (size_t)frame * N + i;
+            pl = plist + o * width;
+            pc = min(pcount[o], width);
+        }
+        sp_atom_terms<RECUR>(P, types, N, i, Lx, Ly, Lz, b, cs, nbr[wave], pl, pc, acc);
         int q;
         double const sum = mdl::group_sum_scatter<64, 30>(acc, q);
         if (q >= 0)
@@ -769,16 +809,18 @@ static bool sp_recur(const SpBasis &b, double rc) {
 
 mdl_status mdl_cv_stress_basis_sums(const double *pos, const int32_t *types, int32_t N,
                                     int32_t frames, const double *box, double rc, double mu0,
-                                    double dmu, double width, double *part, int32_t nblocks,
-                                    void *stream) {
+                                    double dmu, double width, const int32_t *pairs,
+                                    const int32_t *pair_count, int32_t pair_width, double *part,
+                                    int32_t nblocks, void *stream) {
     if (!sp_args("mdl_cv_stress_basis_sums", pos, types, N, frames, box, rc, width))
         return MDL_ERR_ARG;
-    if (!part || nblocks < 1 || nblocks > 65535) {
+    if (!part || nblocks < 1 || nblocks > 65535 || (pairs && (!pair_count || pair_width < 1))) {
         set_error("mdl_cv_stress_basis_sums: bad arguments");
         return MDL_ERR_ARG;
     }
     k_cv_stress_basis<<<dim3(nblocks, frames), 256, 0, (hipStream_t)stream>>>(
-        pos, types, N, box[0], box[1], box[2], sp_basis(rc, mu0, dmu, width), part);
+        pos, types, N, box[0], box[1], box[2], sp_basis(rc, mu0, dmu, width), pairs, pair_count,
+        pair_width, part);
     return launch_ok("mdl_cv_stress_basis_sums");
 }
 
@@ -805,21 +847,25 @@ mdl_status mdl_cv_stress_proxy(const double *pos, const int32_t *types, int32_t 
 mdl_status mdl_cv_stress_fields(const double *pos, const double *F, double fscale,
                                 const int32_t *types, int32_t N, int32_t frames,
                                 const double *box, double rc, double mu0, double dmu,
-                                double width, const double *coef, double *part, int32_t nblocks,
-                                void *stream) {
+                                double width, const double *coef, const int32_t *pairs,
+                                const int32_t *pair_count, int32_t pair_width, double *part,
+                                int32_t nblocks, void *stream) {
     if (!sp_args("mdl_cv_stress_fields", pos, types, N, frames, box, rc, width))
         return MDL_ERR_ARG;
-    if (!F || !coef || !part || nblocks < 1 || nblocks > 65535) {
+    if (!F || !coef || !part || nblocks < 1 || nblocks > 65535 ||
+        (pairs && (!pair_count || pair_width < 1))) {
         set_error("mdl_cv_stress_fields: bad arguments");
         return MDL_ERR_ARG;
     }
     SpBasis const b = sp_basis(rc, mu0, dmu, width);
     if (sp_recur(b, rc))
         k_cv_stress_fields<true><<<dim3(nblocks, frames), 256, 0, (hipStream_t)stream>>>(
-            pos, F, fscale, types, N, box[0], box[1], box[2], b, coef, part);
+            pos, F, fscale, types, N, box[0], box[1], box[2], b, coef, pairs, pair_count,
+            pair_width, part);
     else
         k_cv_stress_fields<false><<<dim3(nblocks, frames), 256, 0, (hipStream_t)stream>>>(
-            pos, F, fscale, types, N, box[0], box[1], box[2], b, coef, part);
+            pos, F, fscale, types, N, box[0], box[1], box[2], b, coef, pairs, pair_count,
+            pair_width, part);
     return launch_ok("mdl_cv_stress_fields");
 }
 

@@ -166,6 +166,8 @@ struct Driver {
     struct CvCapture {
         float *D = nullptr, *J = nullptr, *L = nullptr;
         double *pos = nullptr, *force = nullptr;
+        int *nbr = nullptr, *ncnt = nullptr;   // optional pair lists (mdl_driver_set_cv_capture_pairs)
+        int width = 0;
         int slots = 0, next = 0, use_full = 0;
         CvBasis basis{};
     } cvc;
@@ -1019,9 +1021,30 @@ __global__ __launch_bounds__(256) void k_cv_desc_list(AtomArrays a, ListArrays l
         acc.store(i, k, N, D, J, Lp, La);
 }
 
+// Atom i's pairs within the cut-off of the step just launched (the compacted
+// list, or the full list when the capture's cut-off exceeds r_c), in list order:
+// count[i] = their number and nbr[i][0..min(count, width)); a count beyond
+// width tells the caller that the row was cut. One wave per atom.
+__global__ __launch_bounds__(256) void k_cv_capture_pairs(ListArrays l, int N, int use_full,
+                                                         int width, int *__restrict__ nbr,
+                                                         int *__restrict__ count) {
+    int const lane = threadIdx.x & 63;
+    int const i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= N)
+        return;
+    int const cnt = use_full ? l.nnbr[i] : l.ccnt[i];
+    const int *src = (use_full ? l.nbr : l.cnbr) + (size_t)i * l.stride;
+    int *dst = nbr + (size_t)i * width;
+    int const m = min(cnt, width);
+    for (int t = lane; t < m; t += 64)
+        dst[t] = src[t];
+    if (lane == 0)
+        count[i] = cnt;
+}
+
 // The control-variate inputs of the step just launched, into the next frame
-// slot (descriptors from the compacted list, positions and forces), on the
-// driver's stream.
+// slot (descriptors from the compacted list, positions and forces, and the pair
+// lists when asked for), on the driver's stream.
 static void cv_capture(Driver &d) {
     Driver::CvCapture &c = d.cvc;
     int const j = c.next % c.slots;
@@ -1036,6 +1059,10 @@ static void cv_capture(Driver &d) {
                                            c.pos + (size_t)j * 3 * d.N);
     k_materialize<<<g, 256, 0, d.stream>>>(d.a, d.N, d.gs, MDL_A_FORCES,
                                            c.force + (size_t)j * 3 * d.N);
+    if (c.nbr)
+        k_cv_capture_pairs<<<(d.N + 3) / 4, 256, 0, d.stream>>>(
+            d.l, d.N, c.use_full, c.width, c.nbr + (size_t)j * d.N * c.width,
+            c.ncnt + (size_t)j * d.N);
 }
 
 // ---------------------------------------------------------------------------
@@ -2477,6 +2504,19 @@ mdl_status mdl_driver_set_cv_capture(mdl_driver *p, float rc, float mu0, float d
     c.use_full = (double)rc * rc * 1e-16 > d->rc2 * (1.0 + 1e-6) ? 1 : 0;
     c.basis = cv_basis(rc, mu0, dmu, width);
     d->cvc = c;
+    return MDL_OK;
+}
+
+mdl_status mdl_driver_set_cv_capture_pairs(mdl_driver *p, int32_t *nbr_out, int32_t *count_out,
+                                           int32_t width) {
+    Driver *d = D(p);
+    if (!d || (nbr_out && (!count_out || width < 1)))
+        return bad("mdl_driver_set_cv_capture_pairs: bad arguments");
+    if (nbr_out && d->cvc.slots == 0)
+        return bad("mdl_driver_set_cv_capture_pairs: no capture is set");
+    d->cvc.nbr = nbr_out;
+    d->cvc.ncnt = nbr_out ? count_out : nullptr;
+    d->cvc.width = nbr_out ? width : 0;
     return MDL_OK;
 }
 

@@ -77,6 +77,7 @@ NGLOBAL = 27       # global_features: S, T, Q
 NGLOBAL_USED = 27
 FIELD_GAIN = 0.05     # least held-out variance cut that keeps the learned fields of a head
 FRAME_BATCH = 256     # stored frames per pass of the feature and proxy-fit kernels
+PAIR_WIDTH = 192      # pairs kept per atom and frame (a row that needs more is an error)
 
 
 def _torch():
@@ -138,19 +139,36 @@ NSTRESS = 42       # zero-variance fields of the stress proxy: 6 gradients, 36 w
 SP_BLOCKS = 64     # blocks per frame of the basis sums
 
 
-def stress_basis_sums(pos_A, types32, box_A, basis: Basis = Basis()):
+def _pair_args(pairs):
+    """(pairs, counts, width) device pointers of captured pair lists, or nulls."""
+    if pairs is None:
+        return None, None, 0
+    nbr, cnt = pairs
+    return _p(nbr.contiguous()), _p(cnt.contiguous()), int(nbr.shape[-1])
+
+
+def stress_basis_sums(pos_A, types32, box_A, basis: Basis = Basis(), pairs=None):
     """Z [B, 3, 32, 6] (FP64, device) of B frames of positions [B, N, 3] in A:
     over ordered pairs of species pair p = t_i + t_j, sum fc g_k n_a n_b
-    (mdl_cv_stress_basis_sums)."""
+    (mdl_cv_stress_basis_sums); ``pairs`` = (lists [B, N, W], counts [B, N]) as
+    capture_frames keeps them, else the pairs are found among all atoms."""
     torch = _torch()
     pos = pos_A.contiguous()
     B, N, _ = pos.shape
     part = torch.empty((B, SP_BLOCKS, 3, NB, 6), dtype=torch.float64, device=pos.device)
     box = (C.c_double * 3)(*[float(b) for b in box_A])
     _check(nat.lib().mdl_cv_stress_basis_sums(_p(pos), _p(types32), N, B, box, basis.rc,
-                                              basis.mu0, basis.dmu, basis.width, _p(part),
-                                              SP_BLOCKS, _stream()))
+                                              basis.mu0, basis.dmu, basis.width,
+                                              *_pair_args(pairs), _p(part), SP_BLOCKS,
+                                              _stream()))
     return part.sum(1)
+
+
+def _frame_pairs(store, sl):
+    """The captured pair lists of frames ``sl`` of a store, or None."""
+    if store is None or "nbr" not in store:
+        return None
+    return store["nbr"][sl], store["ncnt"][sl]
 
 
 def stress_proxy(pos_A, types32, box_A, basis: Basis, coef):
@@ -176,11 +194,12 @@ SF_N = 39          # per-frame field sums
 SF_PAIRS = [(a, b) for a in range(6) for b in range(a, 6)]
 
 
-def stress_field_sums(pos_A, F, types32, box_A, basis: Basis, coef):
+def stress_field_sums(pos_A, F, types32, box_A, basis: Basis, coef, pairs=None):
     """[B, 39] (FP64, device) for B frames of positions [B, N, 3] in A and
     forces F [B, N, 3] (in the units the sums want): over the atoms m, the
     proxy's pair sum h_v (6), its Laplacian lap_m h_v (6), grad_m h_v . F_m (6)
-    and grad_m h_a . grad_m h_b for a <= b (21) (mdl_cv_stress_fields)."""
+    and grad_m h_a . grad_m h_b for a <= b (21) (mdl_cv_stress_fields);
+    ``pairs`` as for stress_basis_sums."""
     torch = _torch()
     pos = pos_A.contiguous()
     Fc = F.contiguous()
@@ -189,8 +208,8 @@ def stress_field_sums(pos_A, F, types32, box_A, basis: Basis, coef):
     box = (C.c_double * 3)(*[float(b) for b in box_A])
     _check(nat.lib().mdl_cv_stress_fields(_p(pos), _p(Fc), 1.0, _p(types32), N, B, box,
                                           basis.rc, basis.mu0, basis.dmu, basis.width,
-                                          _p(coef.contiguous()), _p(part), SF_BLOCKS,
-                                          _stream()))
+                                          _p(coef.contiguous()), *_pair_args(pairs), _p(part),
+                                          SF_BLOCKS, _stream()))
     return part.sum(1)
 
 
@@ -694,7 +713,8 @@ class ControlVariate:
         of the Born stress (GPa) on the basis sums of ``fr``'s stored frames;
         returns R^2 per component."""
         Z = np.concatenate([stress_basis_sums(fr.store["pos"][k:k + batch] / U.angstrom,
-                                              self.types32, self.box, self.basis)
+                                              self.types32, self.box, self.basis,
+                                              _frame_pairs(fr.store, slice(k, k + batch)))
                             .cpu().numpy() for k in range(0, fr.n, batch)])
         n = fr.n
         A = np.concatenate([Z.transpose(0, 3, 1, 2).reshape(n * 6, 3 * NB),
@@ -712,7 +732,7 @@ class ControlVariate:
         res = fr.born_stress - pred
         return 1.0 - res.var(0) / fr.born_stress.var(0)
 
-    def stress_features(self, pos, F):
+    def stress_features(self, pos, F, pairs=None):
         """[B, 42] Stein sums of the zero-variance fields of the stress proxy h
         (positions in CGS, forces in eV/A): the fields grad h_a (sums lap h_a +
         beta sum_m grad_m h_a . F_m) and (h_a - h0_a) grad h_b (sums
@@ -720,7 +740,7 @@ class ControlVariate:
         per-frame sums of one kernel pass (no per-atom arrays)."""
         torch = _torch()
         s = stress_field_sums(pos / U.angstrom, F, self.types32, self.box, self.basis,
-                              self.sp_coef)
+                              self.sp_coef, pairs)
         h = s[:, :6] + torch.as_tensor(self.sp_off, dtype=s.dtype, device=s.device)
         dh = h - torch.as_tensor(self.sp_h0, dtype=s.dtype, device=s.device)
         A = s[:, 6:12] + self.beta * s[:, 12:18]
@@ -732,7 +752,7 @@ class ControlVariate:
         return torch.cat([A, Bm.reshape(-1, 36)], dim=1)
 
     # -- features ----------------------------------------------------------------
-    def _x(self, D, J, L, pos, Fc, Ut, grad=False):
+    def _x(self, D, J, L, pos, Fc, Ut, grad=False, pairs=None):
         """X [B, 3, NX] (torch, FP64) for B frames of per-atom inputs (positions
         and forces in CGS)."""
         torch = _torch()
@@ -750,7 +770,7 @@ class ControlVariate:
         t = torch.cat([S.reshape(B, 9), (dU[:, None, None] * S - M).reshape(B, 9),
                        Th.reshape(B, 9)], dim=1)
         if self.use_stress:
-            t = torch.cat([t, self.stress_features(pos, F)], dim=1)
+            t = torch.cat([t, self.stress_features(pos, F, pairs)], dim=1)
         t = t / tsc
         if self.use_net:
             with torch.set_grad_enabled(grad):
@@ -768,7 +788,7 @@ class ControlVariate:
         st = fr.store
         Ut = torch.as_tensor(fr.epot[sl], dtype=torch.float64, device=self.device)
         return self._x(st["D"][sl], st["J"][sl], st["L"][sl], st["pos"][sl], st["F"][sl], Ut,
-                       grad)
+                       grad, _frame_pairs(st, sl))
 
     def features(self, fr: Frames, batch=FRAME_BATCH):
         out = np.zeros((fr.n, 3, self.NX))
@@ -1010,11 +1030,15 @@ def frames_needed(se, n, target=1.0):
     return n * (2.0 * np.asarray(se) / target) ** 2
 
 
-def capture_frames(drv, nframes, stride, T, basis: Basis = Basis(), device="cuda:0"):
+def capture_frames(drv, nframes, stride, T, basis: Basis = Basis(), device="cuda:0",
+                   pair_width=PAIR_WIDTH):
     """One run of nframes x stride steps (one host call) that keeps every
     stride-th step: its scalars from the run's records, and its per-atom inputs
     of the controls (D, J, L, positions, forces) captured on the device inside
-    the run (mdl_driver_set_cv_capture with one slot per frame)."""
+    the run (mdl_driver_set_cv_capture with one slot per frame); with
+    ``pair_width`` also each atom's pairs within the cut-off, in the driver's
+    list order (mdl_driver_set_cv_capture_pairs), which the stress proxy's
+    kernels then take instead of searching all atoms."""
     torch = _torch()
     N = drv.N
     store = dict(D=torch.empty((nframes, N, NF), device=device),
@@ -1022,17 +1046,28 @@ def capture_frames(drv, nframes, stride, T, basis: Basis = Basis(), device="cuda
                  L=torch.empty((nframes, N, NF), device=device),
                  pos=torch.empty((nframes, N, 3), device=device, dtype=torch.float64),
                  F=torch.empty((nframes, N, 3), device=device, dtype=torch.float64))
+    if pair_width:
+        store["nbr"] = torch.empty((nframes, N, pair_width), device=device, dtype=torch.int32)
+        store["ncnt"] = torch.empty((nframes, N), device=device, dtype=torch.int32)
     lib = nat.lib()
     _check(lib.mdl_driver_set_cv_capture(drv.handle, basis.rc, basis.mu0, basis.dmu,
                                          basis.width, *[_p(store[q]) for q in
                                                         ("D", "J", "L", "pos", "F")], nframes))
     try:
+        if pair_width:
+            _check(lib.mdl_driver_set_cv_capture_pairs(drv.handle, _p(store["nbr"]),
+                                                       _p(store["ncnt"]), pair_width))
         rec = drv.run(nframes * stride, sample_stride=stride)
     finally:
         lib.mdl_driver_set_cv_capture(drv.handle, 0.0, 0.0, 0.0, 0.0, None, None, None,
                                       None, None, 0)
     if len(rec) != nframes:
         raise RuntimeError(f"capture_frames: {len(rec)} records for {nframes} frames")
+    if pair_width:
+        most = int(store["ncnt"].max())
+        if most > pair_width:
+            raise RuntimeError(f"capture_frames: an atom has {most} pairs within the cut-off, "
+                               f"beyond the pair width {pair_width}")
     ep = np.zeros(nframes)
     sb = np.zeros((nframes, 6))
     cb = np.zeros((nframes, 6, 6))

@@ -354,6 +354,31 @@ def test_stress_kernels_match_the_oracle():
         assert _rel(sf[b, 18:], np.array([gg[a, c] for a, c in cv.SF_PAIRS])) < 1e-12
 
 
+def test_captured_pair_lists_give_the_all_pairs_sums():
+    """The stress kernels on the pair lists captured in the driver's run (its
+    compacted pairs within r_c, list order) against the same kernels searching
+    all atoms: the same pairs, another summation order."""
+    print("brass 500 atoms: 24 frames with pair lists", flush=True)
+    drv, sites, types = cv.brass_driver(edge=5)
+    drv.run(2000, sample_stride=2000)
+    fr = cv.capture_frames(drv, 24, 20, 300.0)
+    cnt = fr.store["ncnt"].cpu().numpy()
+    assert cnt.min() > 50 and cnt.max() <= cv.PAIR_WIDTH
+    posA = fr.store["pos"] / cv.U.angstrom
+    F = fr.store["F"] * (cv.U.angstrom / cv.U.ev)
+    ty = torch.tensor(types, dtype=torch.int32, device=DEV)
+    pairs = (fr.store["nbr"], fr.store["ncnt"])
+    Z0 = cv.stress_basis_sums(posA, ty, fr.box).cpu().numpy()
+    Z1 = cv.stress_basis_sums(posA, ty, fr.box, pairs=pairs).cpu().numpy()
+    assert _rel(Z1, Z0) < 1e-12
+    coef = torch.tensor(np.random.RandomState(6).normal(size=(3, 32)) * 0.3, dtype=torch.float64,
+                        device=DEV)
+    s0 = cv.stress_field_sums(posA, F, ty, fr.box, cv.Basis(), coef).cpu().numpy()
+    s1 = cv.stress_field_sums(posA, F, ty, fr.box, cv.Basis(), coef, pairs).cpu().numpy()
+    for a, b in ((0, 6), (6, 12), (12, 18), (18, 39)):
+        assert _rel(s1[:, a:b], s0[:, a:b]) < 1e-12
+
+
 def test_stress_fields_on_md_frames():
     """500-atom alpha-brass NVT, frames captured inside one run: the stress
     proxy fits the Born stress, the variate's 42 stress-field sums equal the
