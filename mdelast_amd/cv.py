@@ -929,7 +929,7 @@ class ControlBatch:
     """Controls of a running driver, gathered frame by frame into device
     buffers and evaluated B frames at a time (one network pass per batch)."""
 
-    def __init__(self, cvm: ControlVariate, N, B=16):
+    def __init__(self, cvm: ControlVariate, N, B=16, pair_width=PAIR_WIDTH):
         torch = _torch()
         dev = cvm.device
         self.cvm, self.B, self.k = cvm, B, 0
@@ -937,6 +937,11 @@ class ControlBatch:
                     torch.empty((B, N, NF), device=dev),
                     torch.empty((B, N, 3), device=dev, dtype=torch.float64),
                     torch.empty((B, N, 3), device=dev, dtype=torch.float64))
+        # pair lists captured by run() (the stress proxy's kernels take them)
+        self.width = pair_width
+        self.pairs = ((torch.empty((B, N, pair_width), device=dev, dtype=torch.int32),
+                       torch.empty((B, N), device=dev, dtype=torch.int32))
+                      if pair_width and cvm.use_stress else None)
         self.U = np.zeros(B)
         self.out = []
         # the controls run on the stream current now; the per-frame call takes
@@ -978,7 +983,9 @@ class ControlBatch:
         frame's inputs inside its run (one run per B frames,
         mdl_driver_set_cv_capture) and the controls of those frames are queued
         on the driver's stream right after the run, before the next one
-        overwrites the slots. Returns the frames' records (MDL_S_* rows)."""
+        overwrites the slots. With the stress proxy on, the run also captures
+        each frame's pair lists (mdl_driver_set_cv_capture_pairs) for its
+        kernels. Returns the frames' records (MDL_S_* rows)."""
         torch = _torch()
         lib = nat.lib()
         ptr = C.c_void_p()
@@ -990,13 +997,24 @@ class ControlBatch:
             for f0 in range(0, nframes, self.B):
                 nb = min(self.B, nframes - f0)
                 _check(lib.mdl_driver_set_cv_capture(drv.handle, *self._basis, *slots, self.B))
+                pairs = None
+                if self.pairs is not None:
+                    _check(lib.mdl_driver_set_cv_capture_pairs(
+                        drv.handle, _p(self.pairs[0]), _p(self.pairs[1]), self.width))
                 rec = drv.run(nb * stride, sample_stride=stride)
                 recs.append(rec)
                 U = np.array([_record_scalars(r, drv.N)[0] for r in rec])
                 with torch.cuda.stream(self._ext):
+                    if self.pairs is not None:
+                        most = int(self.pairs[1][:nb].max())
+                        if most > self.width:
+                            raise RuntimeError(f"ControlBatch.run: an atom has {most} pairs "
+                                               f"within the cut-off, beyond the pair width "
+                                               f"{self.width}")
+                        pairs = (self.pairs[0][:nb], self.pairs[1][:nb])
                     Ut = torch.tensor(U, dtype=torch.float64).pin_memory().to(
                         self.cvm.device, non_blocking=True)
-                    X = self.cvm._x(*[b[:nb] for b in self.buf], Ut)
+                    X = self.cvm._x(*[b[:nb] for b in self.buf], Ut, pairs=pairs)
                     co = self.cvm._device_consts()[2]
                     self.out.append((X * co[None]).sum(2))
         finally:

@@ -282,6 +282,41 @@ def test_stein_identity_and_fallback_on_md_frames():
     assert np.array_equal(Cp, Cc) and np.array_equal(sep, sec)
 
 
+def test_in_loop_controls_with_stress_fields_use_the_captured_pairs():
+    """A variate with the stress proxy, in the loop of a run (ControlBatch.run:
+    inputs and pair lists captured inside the run) against the same variate on
+    the stored frames of an identical trajectory (all-pairs stress kernels)."""
+    print("brass 500 atoms: stress variate in the loop", flush=True)
+    drv, sites, types = cv.brass_driver(edge=5)
+    drv.run(2000, sample_stride=2000)
+    fr = cv.collect_frames(drv, 300, 20, 300.0, sites=sites)
+    cvm = cv.ControlVariate(sites, fr.box, 300.0, types, device=DEV, use_net=False,
+                            use_stress=True)
+    cvm.fit(cv.subset(fr, slice(0, 240)), cv.subset(fr, slice(240, None)))
+    d1, _, _ = cv.brass_driver(edge=5, vel_seed=555)
+    d2, _, _ = cv.brass_driver(edge=5, vel_seed=555)
+    d1.run(200, sample_stride=200)
+    d2.run(200, sample_stride=200)
+    cap = cv.ControlBatch(cvm, d1.N, B=4)
+    assert cap.pairs is not None
+    recs = cap.run(d1, 10, 20)
+    s_cap = cap.result()
+    st = {q: [] for q in ("D", "J", "L", "pos", "F")}
+    recs2 = []
+    for k in range(10):
+        recs2.append(d2.run(20, sample_stride=20)[-1])
+        for q, v in zip(("D", "J", "L", "pos", "F"), cv.frame_inputs(d2)):
+            st[q].append(v)
+    assert np.array_equal(recs, np.array(recs2))
+    ref = cv.collect_frames(d2, 0, 20, 300.0)
+    ref.store = {q: torch.stack(v) for q, v in st.items()}
+    ref.epot = np.array([cv._record_scalars(r, d2.N)[0] for r in recs2])
+    s_ref = cvm.controls(ref, enabled_only=False)
+    print("captured", s_cap.ravel(), "stored", s_ref.ravel(), flush=True)
+    assert s_cap.shape == (10, 3)
+    assert _rel(s_cap, s_ref) < 1e-5
+
+
 def test_learned_fields_are_kept_only_where_they_help():
     """The fit measures, per head and on held-out frames, the plain variance,
     the variance with the global fields and the variance with the learned
