@@ -480,6 +480,8 @@ static mdl_status autotune(Driver &d) {
         k_autotune_jitter<<<(unsigned)((d.N + BLOCK - 1) / BLOCK), BLOCK, 0, d.stream>>>(
             d.a.pos, d.N, amp / d.hs.L[0], amp / d.hs.L[1], amp / d.hs.L[2]);
         MDL_HIP(hipGetLastError());
+        k_quantize<<<(unsigned)((d.N + BLOCK - 1) / BLOCK), BLOCK, 0, d.stream>>>(d.a, d.N);
+        MDL_HIP(hipGetLastError());
     }
     double best = 1e300;
     int best_lpa = 16;
@@ -504,6 +506,8 @@ static mdl_status autotune(Driver &d) {
     (void)saved;
     MDL_HIP(hipMemcpyAsync(d.a.pos, saved_pos, sizeof(double4) * (size_t)d.N,
                            hipMemcpyDeviceToDevice, d.stream));
+    k_quantize<<<(unsigned)((d.N + BLOCK - 1) / BLOCK), BLOCK, 0, d.stream>>>(d.a, d.N);
+    MDL_HIP(hipGetLastError());
     {
         int const zero = 0;   // a displaced density above rho_max is no error of the run
         MDL_HIP(hipMemcpyAsync(&d.gs->err, &zero, sizeof(int), hipMemcpyHostToDevice, d.stream));
@@ -778,6 +782,7 @@ static mdl_status create(const mdl_config *c, Driver &d) {
     if (mdl_status s_ = d.alloc(&(p), (n)))                                                    \
         return s_;
     ALLOC(a.pos, N);
+    ALLOC(a.qpos, N);
     ALLOC(a.vx, N);
     ALLOC(a.vy, N);
     ALLOC(a.vz, N);
@@ -859,6 +864,8 @@ static mdl_status create(const mdl_config *c, Driver &d) {
     MDL_HIP(hipMemcpy(a.vy, vy.data(), sizeof(double) * N, hipMemcpyHostToDevice));
     MDL_HIP(hipMemcpy(a.vz, vz.data(), sizeof(double) * N, hipMemcpyHostToDevice));
     MDL_HIP(hipMemcpy(a.type, typ.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+    k_quantize<<<(unsigned)((N + BLOCK - 1) / BLOCK), BLOCK, 0, d.stream>>>(a, N);
+    MDL_HIP(hipGetLastError());
 
     DevState &s = d.hs;
     memset(&s, 0, sizeof s);

@@ -17,6 +17,8 @@ namespace mdl {
 
 struct AtomArrays {
     double4 *pos;      // unwrapped scaled coordinates c (md_driver.hpp:97), type in .w
+    int4 *qpos;        // the same as 32-bit fixed point (quantize), type in .w: the
+                       // mixed passes' pair geometry
     double *vx, *vy, *vz;
     double *fx, *fy, *fz;
     int *type;
@@ -150,6 +152,28 @@ __device__ __forceinline__ unsigned long long group_mask() {
     int const l64 = threadIdx.x & 63;
     unsigned long long const m = LPA == 64 ? ~0ull : ((1ull << LPA) - 1ull);
     return m << (l64 & ~(LPA - 1));
+}
+
+// A scaled coordinate as 32-bit fixed point modulo the box: c 2^32 rounded to
+// an integer, modulo 2^32. The wrapping difference of two atoms' values is then
+// their minimum-image separation in units of L 2^-32 (a quantum of 2.5e-8 A in
+// a 100 A box), with no rounding step.
+__device__ __forceinline__ int quantize(double c) {
+    return (int)(unsigned)(unsigned long long)__double2ll_rn(c * 4294967296.0);
+}
+__device__ __forceinline__ int4 quantize(const double4 &c) {
+    return make_int4(quantize(c.x), quantize(c.y), quantize(c.z), (int)c.w);
+}
+
+// Pair vector d = x_i - x_j from fixed-point positions: the wrapping integer
+// difference (the minimum image, exact), one rounding to float and one product
+// by S = L 2^-32 (A). Exactly antisymmetric in (i, j), so a pair's two visits
+// see the same r and opposite vectors.
+__device__ __forceinline__ void qmin_image(const int4 &qi, const int4 &qj, const float *S,
+                                           float &x, float &y, float &z) {
+    x = (float)(int)((unsigned)qi.x - (unsigned)qj.x) * S[0];
+    y = (float)(int)((unsigned)qi.y - (unsigned)qj.y) * S[1];
+    z = (float)(int)((unsigned)qi.z - (unsigned)qj.z) * S[2];
 }
 
 // Minimum-image pair vector d = (s - nint(s)) L of s = c_i - c_j
@@ -673,6 +697,7 @@ __device__ __forceinline__ void kick_drift_body(AtomArrays a, int N, DevState *_
         c.y += cf * vy / Ld[1];
         c.z += cf * vz / Ld[2];
         a.pos[i] = c;
+        a.qpos[i] = quantize(c);
         double4 const r = a.cref[i];
         double const ux = (c.x - r.x) * Ln[0], uy = (c.y - r.y) * Ln[1], uz = (c.z - r.z) * Ln[2];
         dmax = fmax(dmax, ux * ux + uy * uy + uz * uz);
@@ -1866,6 +1891,13 @@ __device__ __forceinline__ void post_step(DevState &s, const double *tA, const d
     rec[MDL_S_WARN_RMIN] = (double)s.warn_rmin;
     rec[MDL_S_NLIST_ENTRIES] = (double)s.nlist_entries;
     rec[MDL_S_NHC_GENERAL] = (double)s.nhc_general;
+}
+
+// The fixed-point positions from the FP64 ones, after the host wrote those.
+__global__ __launch_bounds__(BLOCK) void k_quantize(AtomArrays a, int N) {
+    int const i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < N)
+        a.qpos[i] = quantize(a.pos[i]);
 }
 
 // Host-visible per-atom arrays (md_driver_module.cpp:103-109)

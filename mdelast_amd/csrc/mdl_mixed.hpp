@@ -179,8 +179,10 @@ __device__ __forceinline__ float sym_dedr(float dphi, float Fi, float drj, float
 }
 
 // ---------------------------------------------------------------------------
-// Pass A, mixed: as k_density (FP64 geometry, compaction, FP64 rho), with the
-// rho(r) and F(rho) polynomials and the g/h sums in FP32.
+// Pass A, mixed: as k_density (compaction, FP64 rho), with the pair geometry
+// from the fixed-point positions and the rho(r) and F(rho) polynomials and the
+// g/h sums in FP32. The r_c and r_mid decisions are taken on the float r^2, the
+// same for both visits of a pair, so the compacted lists stay symmetric.
 // ---------------------------------------------------------------------------
 template <int KIND, int LPA, bool SAMPLE>
 __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTablesF rt,
@@ -195,10 +197,14 @@ __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTab
     int const l64 = threadIdx.x & 63;
     unsigned long long const gmask = group_mask<LPA>();
     unsigned long long const below = (l64 == 0) ? 0ull : (~0ull >> (64 - l64));
-    double const L[3] = {gs->L[0] * 1e8, gs->L[1] * 1e8, gs->L[2] * 1e8};   // A
+    float const S[3] = {(float)(gs->L[0] * 1e8) * 0x1p-32f, (float)(gs->L[1] * 1e8) * 0x1p-32f,
+                        (float)(gs->L[2] * 1e8) * 0x1p-32f};   // A per fixed-point unit
     // scan the mid list, or the full list while the mid list is refiltered
     bool const mid_build = gs->mid_build != 0;
-    double const rm2 = (gs->r_mid * 1e8) * (gs->r_mid * 1e8);   // A^2
+    // r_mid widened by 2e-6 relative: the float r of a pair errs by far less,
+    // so every pair the exact test keeps is kept
+    float const rm2 = (float)((gs->r_mid * 1e8) * (gs->r_mid * 1e8) * (1. + 4e-6));   // A^2
+    float const rc2f = (float)rc2;
     int const rowS = rt.nf * P::S;           // floats per interval (fits int)
     float const nm1 = (float)(rt.n - 1);
     double acc[PA_N];
@@ -212,10 +218,10 @@ __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTab
         double rho = 0.;
         float g0 = 0.f, g1 = 0.f, g2 = 0.f, g3 = 0.f, g4 = 0.f, g5 = 0.f;
         float hx = 0.f, hy = 0.f, hz = 0.f;
-        double4 ci = make_double4(0., 0., 0., 0.);
+        int4 qi = make_int4(0, 0, 0, 0);
         int cnt = 0;
         if (active) {
-            ci = a.pos[i];
+            qi = a.qpos[i];
             cnt = mid_build ? l.nnbr[i] : l.mcnt[i];
         }
         int const ia = active ? i : 0;
@@ -230,15 +236,15 @@ __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTab
         // as a harmless dummy; every contribution is selected by `in`.
         int j0 = lane < cnt ? nb[lane] : ia;
         int j1 = LPA + lane < cnt ? nb[LPA + lane] : ia;
-        double4 c0 = a.pos[j0];
+        int4 c0 = a.qpos[j0];
         for (int k0 = 0; k0 < cnt; k0 += LPA) {
             int const k = k0 + lane;
             int const j = j0;
-            double4 const cj = c0;
-            double dx, dy, dz;
-            min_image(ci, cj, L, dx, dy, dz);
-            double const r2 = dx * dx + dy * dy + dz * dz;
-            bool const in = k < cnt && !(r2 > rc2);
+            int4 const cj = c0;
+            float fx, fy, fz;
+            qmin_image(qi, cj, S, fx, fy, fz);
+            float const r2 = fx * fx + fy * fy + fz * fz;
+            bool const in = k < cnt && !(r2 > rc2f);
             if (mid_build) {
                 bool const inm = k < cnt && !(r2 > rm2);
                 unsigned long long const balm = __ballot(inm) & gmask;
@@ -246,9 +252,8 @@ __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTab
                     mb[nmid + __popcll(balm & below)] = j;
                 nmid += __popcll(balm);
             }
-            float const fx = (float)dx, fy = (float)dy, fz = (float)dz;
-            float const r = __builtin_amdgcn_sqrtf(fx * fx + fy * fy + fz * fz);
-            int const tj = (int)cj.w;
+            float const r = __builtin_amdgcn_sqrtf(r2);
+            int const tj = cj.w;
             float t;
             int const it = tab_locate_f(r, rt.inv_h, nm1, t);
             // about half of the candidates lie beyond r_c: their lanes request
@@ -256,7 +261,7 @@ __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTab
             P p;
             if (in)
                 p.load(rt.rec + (it * rowS + (npair + tj) * P::S));
-            c0 = a.pos[j1];
+            c0 = a.qpos[j1];
             j0 = j1;
             j1 = k + 2 * LPA < cnt ? nb[k + 2 * LPA] : ia;
             unsigned long long const bal = __ballot(in) & gmask;
@@ -316,9 +321,9 @@ __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTab
             l.ccnt[i] = nin;
             if (mid_build) {
                 l.mcnt[i] = nmid;
-                a.mref[i] = ci;
+                a.mref[i] = a.pos[i];
             }
-            int const ti = (int)ci.w;
+            int const ti = qi.w;
             // selected, not indexed: a dynamic index into the table struct
             // would put a copy of it in scratch memory
             double rmax = ft.rhomax[0];
@@ -369,7 +374,8 @@ __global__ __launch_bounds__(BLOCK) void k_density_mx(AtomArrays a, ListArrays l
 }
 
 // ---------------------------------------------------------------------------
-// Pass B, mixed: forces (FP64 sums of symmetric FP32 pair terms), energies,
+// Pass B, mixed: pair geometry from the fixed-point positions; forces (FP64
+// sums of FP32 pair terms, symmetric in the pair), energies,
 // Born stress, and when sampled the Born monomials and Laplacian (FP32 per
 // lane). The virial is taken as the trace of the Born stress in k_post.
 // Outputs per atom in CGS; block sums in eV and Angstrom (scaled in k_post).
@@ -385,7 +391,8 @@ __device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTable
     constexpr double F_CGS = 1.6021773e-12 / 1e-8;   // eV/A -> dyn
     constexpr double E_CGS = 1.6021773e-12;          // eV -> erg
     int const lane = threadIdx.x % LPA, slot = threadIdx.x / LPA;
-    double const L[3] = {gs->L[0] * 1e8, gs->L[1] * 1e8, gs->L[2] * 1e8};
+    float const S[3] = {(float)(gs->L[0] * 1e8) * 0x1p-32f, (float)(gs->L[1] * 1e8) * 0x1p-32f,
+                        (float)(gs->L[2] * 1e8) * 0x1p-32f};   // A per fixed-point unit
     float const nm1 = (float)(rt.n - 1);
     float sb0 = 0.f, sb1 = 0.f, sb2 = 0.f, sb3 = 0.f, sb4 = 0.f, sb5 = 0.f;
     float cb[15];
@@ -427,25 +434,21 @@ __device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTable
         for (int off = 32; off > 0; off >>= 1)
             wmax = max(wmax, __shfl_xor(wmax, off, 64));
         {
-            double4 const ci = a.pos[ia];
-            int const ti = (int)ci.w;
+            int4 const ci = a.qpos[ia];
+            int const ti = ci.w;
             float2 const Fi = a.fprime_f[ia];
             const int *nb = l.cnbr + (size_t)ia * l.stride;
             struct Geo {
-                double dx, dy, dz;
                 float x, y, z, r, t;
                 int tj;
                 bool act;
             };
             // pair vector, interval and record of one pair
-            auto geometry = [&](const double4 &cj, bool act, Geo &g) -> int {
-                min_image(ci, cj, L, g.dx, g.dy, g.dz);
-                g.x = (float)g.dx;
-                g.y = (float)g.dy;
-                g.z = (float)g.dz;
+            auto geometry = [&](const int4 &cj, bool act, Geo &g) -> int {
+                qmin_image(ci, cj, S, g.x, g.y, g.z);
                 float const xx = g.x * g.x, yy = g.y * g.y, zz = g.z * g.z;
                 g.r = __builtin_amdgcn_sqrtf(xx + yy + zz);
-                g.tj = (int)cj.w;
+                g.tj = cj.w;
                 g.act = act;
                 int const tmax = max(ti, g.tj), tmin = min(ti, g.tj);
                 int const pslot = ((tmax * (tmax + 1)) >> 1) + tmin;
@@ -467,7 +470,6 @@ __device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTable
             };
             auto physics = [&](const Geo &g, const Rec &pp, float2 Fj) {
                 float const x = g.x, y = g.y, z = g.z, r = g.r, t = g.t;
-                double const dx = g.dx, dy = g.dy, dz = g.dz;
                 float const xx = x * x, yy = y * y, zz = z * z;
                 nrmin += r < rmin;
                 // rho of species i and j: the record's lower or higher one
@@ -479,9 +481,9 @@ __device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTable
                 float const ir = __builtin_amdgcn_rcpf(r);
                 float const dedr = sym_dedr(dV, Fi.x, drj, Fj.x, dri);
                 float const s = -dedr * ir;
-                fx += (double)s * dx;
-                fy += (double)s * dy;
-                fz += (double)s * dz;
+                fx += (double)s * (double)x;   // exact products of two floats
+                fy += (double)s * (double)y;
+                fz += (double)s * (double)z;
                 e += (double)(0.5f * V);
                 float const w = dedr * ir;
                 sb0 += w * xx;
@@ -524,7 +526,7 @@ __device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTable
             // are requested after this pair's table record.
             int j0 = lane < cnt ? nb[lane] : ia;
             int j1 = LPA + lane < cnt ? nb[LPA + lane] : ia;
-            double4 c0 = a.pos[j0];
+            int4 c0 = a.qpos[j0];
             for (int k = lane; k - lane < wmax; k += LPA) {
                 Geo g;
                 int const rec = geometry(c0, k < cnt, g);
@@ -532,7 +534,7 @@ __device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTable
                 mdl_wait_lds();   // the stage was read out for the pair before
                 request(stage, rec);
                 float2 const Fj = a.fprime_f[j];
-                c0 = a.pos[j1];
+                c0 = a.qpos[j1];
                 j0 = j1;
                 j1 = k + 2 * LPA < cnt ? nb[k + 2 * LPA] : ia;
                 mdl_wait_vmem();
