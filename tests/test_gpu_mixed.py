@@ -1,5 +1,6 @@
-"""Mixed-precision path (FP32 tables and pair algebra, FP64 geometry and sums)
-against the FP64 oracle, with the per-quantity tolerances this path is held to.
+"""Mixed-precision path (FP32 tables and pair algebra, pair vectors from 32-bit
+fixed-point positions, FP64 positions and sums) against the FP64 oracle, with the
+per-quantity tolerances this path is held to.
 
 The reference changed its own tabulation by these amounts and treated the change
 as negligible (FIXES.md:19-22 of the reference): forces 5e-6, Born 4e-8 and
