@@ -368,9 +368,13 @@ __device__ __forceinline__ void expx_any(double x, double &expx, double &op) {
 // md_driver.cpp:539-561) of a chain of compile-time length M in registers. A
 // lone lane's worth of work (the wave runs it with identical values), so its
 // cost is its instruction count and its dependent chain: the inward sweep's
-// G_j read p_{j-1} before its update and are formed first, the outward sweep's
-// factors read p_{j+1} before its update and are formed first (independent of
-// each other, they overlap), and dt*(p^2/q - kT) is formed as
+// G_j read p_{j-1} before its update and are formed first. The outward sweep's
+// factor of element j has the argument dt p_{j+1}/q_{j+1} with p_{j+1} as the
+// inward sweep left it (U3 changes no p, and the outward sweep reads p_{j+1}
+// before updating it), which is the argument of the inward sweep's factor of
+// element j: the inward factors are kept and used again, bit for bit the
+// values the outward sweep would form (thermostat.cpp:40-66 forms them twice).
+// dt*(p^2/q - kT) is formed as
 // p^2*(dt/q) - dt*kT (thermostat.cpp:40-66 otherwise, element by element).
 // BARO: the coupling is p_eps^2/(2W) and U3 scales p_eps (held in kin); else
 // the coupling is cw*K + c0 (md_driver.cpp:512-531) and U3 scales K by f^2 and
@@ -400,7 +404,8 @@ __device__ __forceinline__ bool nhc_sweeps(double (&p)[M], double (&eta)[M], con
         for (int j = 0; j < M; ++j)
             c[j] = dt4 * iq[j];
         double const dtkT = dt4 * kT;
-        // U4, inward
+        // U4, inward (its factors E, O kept for the outward sweep)
+        double E[M], O[M];
         double tG[M];
         {
             double const e = BARO ? 0.5 * (kin * kin) * inv_W : cw * kin + c0;
@@ -412,9 +417,8 @@ __device__ __forceinline__ bool nhc_sweeps(double (&p)[M], double (&eta)[M], con
         p[M - 1] += tG[M - 1];
 #pragma unroll
         for (int j = M - 2; j >= 0; --j) {
-            double e, o;
-            factor(c[j + 1] * p[j + 1], e, o);
-            p[j] = fma(p[j], e, tG[j] * o);
+            factor(c[j + 1] * p[j + 1], E[j], O[j]);
+            p[j] = fma(p[j], E[j], tG[j] * O[j]);
         }
         // U3
         {
@@ -431,10 +435,6 @@ __device__ __forceinline__ bool nhc_sweeps(double (&p)[M], double (&eta)[M], con
             }
         }
         // U4, outward
-        double E[M], O[M];
-#pragma unroll
-        for (int j = 0; j + 1 < M; ++j)
-            factor(c[j + 1] * p[j + 1], E[j], O[j]);
         double const e = BARO ? 0.5 * (kin * kin) * inv_W : cw * kin + c0;
         double const tG0 = dt4 * (2. * e - Nf * kT);
         if (M > 1)
