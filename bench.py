@@ -90,20 +90,22 @@ def pass_bytes(N, pairs_rc, mid_entries, list_entries, mid_frac):
     """Algorithmic HBM bytes of one launch of each pair pass in this layout: each
     atom's own data read once and its results written once; neighbour positions,
     their F' and the tables are gathers of data other atoms read too (re-use).
+    Both mixed passes scan the mid list (pairs within r_c + 0.15 A); pairs_rc is
+    reported beside them.
 
     force pass (k_force_mx): fixed-point position 16 B, F'/F'' 8 B, F 8 B, list
       length 4 B read, forces 3 x 8 B and energy 8 B written = 68 B per atom,
-      plus 4 B per entry of its list of pairs within r_c;
+      plus 4 B per mid-list entry;
     density pass (k_density_mx): fixed-point position 16 B and list length 4 B
-      read; list length 4 B, F 8 B, rho 8 B, F'/F'' 16 + 8 B and the sampled g,
-      |h|^2 64 B written = 128 B per atom, plus 4 B per written r_c entry and 4 B
-      per mid-list entry scanned; on the fraction mid_frac of steps that refilter
-      the mid list it also reads the full list (4 B per entry) and the FP64
-      position (32 B) and writes the mid-list length and reference position
-      (4 + 32 B)."""
-    force = 68.0 * N + 4.0 * pairs_rc
-    density = (128.0 * N + 4.0 * pairs_rc + 4.0 * mid_entries
-               + mid_frac * (4.0 * list_entries + 68.0 * N))
+      read; F 8 B, rho 8 B, F'/F'' 16 + 8 B and the sampled g, |h|^2 64 B
+      written = 124 B per atom, plus 4 B per mid-list entry scanned; on the
+      fraction mid_frac of steps that refilter the mid list it reads the full
+      list (4 B per entry), its length and the FP64 position (4 + 32 B) and
+      writes the mid list (4 B per entry), its length and the reference
+      position (4 + 32 B)."""
+    force = 68.0 * N + 4.0 * mid_entries
+    density = (124.0 * N + 4.0 * mid_entries
+               + mid_frac * (4.0 * list_entries + 4.0 * mid_entries + 72.0 * N))
     return {"k_density_mx": density, "k_force_mx": force}
 
 

@@ -47,7 +47,8 @@ template <int KIND, bool S, bool E> static ForceFn force_fn(int lpa) {
 }
 using DensMxFn = void (*)(AtomArrays, ListArrays, RTablesF, FTablesF, int, int, int, double,
                           DevState *, double *);
-using ForceMxFn = void (*)(AtomArrays, ListArrays, RTablesF, int, int, float, DevState *, double *);
+using ForceMxFn = void (*)(AtomArrays, ListArrays, RTablesF, int, int, float, float, DevState *,
+                           double *);
 template <int KIND, bool S> static DensMxFn dens_mx_fn(int lpa) {
     switch (lpa) {
     case 4: return k_density_mx<KIND, 4, S>;
@@ -168,7 +169,7 @@ struct Driver {
         double *pos = nullptr, *force = nullptr;
         int *nbr = nullptr, *ncnt = nullptr;   // optional pair lists (mdl_driver_set_cv_capture_pairs)
         int width = 0;
-        int slots = 0, next = 0, use_full = 0;
+        int slots = 0, next = 0, list = 0;   // list: pair_list's choice
         CvBasis basis{};
     } cvc;
     // run() replays graph_steps consecutive steps as one captured graph (0: off)
@@ -254,7 +255,8 @@ struct Driver {
             if (e1)
                 (void)hipEventRecord(e1, stream);
             ForceMxFn fb = pick_force_mx(kind, lpa, s);
-            fb<<<gridB, BLOCK, 0, stream>>>(a, l, rtf, N, npair, (float)(rmin * 1e8), gs, pB);
+            fb<<<gridB, BLOCK, 0, stream>>>(a, l, rtf, N, npair, (float)(rmin * 1e8),
+                                            (float)rc2A, gs, pB);
             if (e2)
                 (void)hipEventRecord(e2, stream);
             return;
@@ -912,7 +914,8 @@ static mdl_status create(const mdl_config *c, Driver &d) {
     s.skin = r_list - rc;
     // mid list: 0.3 A beyond r_c (half the skin when that is thinner); it is
     // refiltered when an atom may have moved 0.15 A since the last filtering
-    s.r_mid = rc + std::min(0.3e-8, 0.5 * (r_list - rc));
+    // mixed: pass B scans the mid list itself, so its shell is kept thin
+    s.r_mid = rc + std::min(d.precision == MDL_MIXED ? 0.15e-8 : 0.3e-8, 0.5 * (r_list - rc));
     s.mid_build = 1;
     for (int k = 0; k < 3; ++k)
         s.L_mid[k] = c->box[k];
@@ -975,14 +978,45 @@ static mdl_status create(const mdl_config *c, Driver &d) {
 
 using namespace mdl;
 
-// Control-variate descriptors of the current configuration from the pairs
-// that pass A of the last force evaluation kept within the potential's
-// cut-off (cnbr; use_full: the full list, before any force evaluation).
+// Mixed path: atom i's pairs within r_c, counted over its mid list with the
+// passes' own decision (pair_r2 of the fixed-point pair vector against rc2f).
+__global__ __launch_bounds__(256) void k_count_within_rc(AtomArrays a, ListArrays l, int N,
+                                                        const DevState *__restrict__ gs,
+                                                        float rc2f, double *__restrict__ out) {
+    int const i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= N)
+        return;
+    float const S[3] = {(float)(gs->L[0] * 1e8) * 0x1p-32f, (float)(gs->L[1] * 1e8) * 0x1p-32f,
+                        (float)(gs->L[2] * 1e8) * 0x1p-32f};
+    int4 const qi = a.qpos[i];
+    const int *nb = l.mnbr + (size_t)i * l.stride;
+    int const m = l.mcnt[i];
+    int n = 0;
+    for (int t = 0; t < m; ++t) {
+        float x, y, z;
+        qmin_image(qi, a.qpos[nb[t]], S, x, y, z);
+        n += pair_r2(x, y, z) > rc2f ? 0 : 1;
+    }
+    out[i] = (double)n;
+}
+
+// The pair list a control-variate kernel reads: 0 the pairs pass A kept within
+// the potential's cut-off (FP64 path), 1 the full list (before any force
+// evaluation, or for a cut-off beyond the potential's), 2 the mid list (mixed
+// path, whose passes take the r_c decision themselves). Each holds every pair
+// within the potential's cut-off, in list order; the readers test r.
+__device__ __forceinline__ int pair_list(const ListArrays &l, int which, int i, const int *&nb) {
+    nb = (which == 1 ? l.nbr : which == 2 ? l.mnbr : l.cnbr) + (size_t)i * l.stride;
+    return which == 1 ? l.nnbr[i] : which == 2 ? l.mcnt[i] : l.ccnt[i];
+}
+
+// Control-variate descriptors of the current configuration from the pairs of
+// list `which` (pair_list) within the basis cut-off.
 // One wave per two atoms, lane k%32 = radial function k; lengths in A.
 template <bool AXES>
 __global__ __launch_bounds__(256) void k_cv_desc_list(AtomArrays a, ListArrays l, int N,
                                                       const DevState *__restrict__ gs,
-                                                      double to_A, CvBasis basis, int use_full,
+                                                      double to_A, CvBasis basis, int which,
                                                       float *__restrict__ D,
                                                       float *__restrict__ J,
                                                       float *__restrict__ Lp,
@@ -993,8 +1027,9 @@ __global__ __launch_bounds__(256) void k_cv_desc_list(AtomArrays a, ListArrays l
     int const ia = active ? i : 0;
     double const L[3] = {gs->L[0], gs->L[1], gs->L[2]};
     double4 const ci = a.pos[ia];
-    int const cnt = active ? (use_full ? l.nnbr[ia] : l.ccnt[ia]) : 0;
-    const int *nb = (use_full ? l.nbr : l.cnbr) + (size_t)ia * l.stride;
+    const int *nb;
+    int const cnt0 = pair_list(l, which, ia, nb);
+    int const cnt = active ? cnt0 : 0;
     CvAcc<AXES> acc;
     // the 32 lanes of an atom gather 32 pairs at once (one load round trip per
     // 32 pairs), then take them one by one from the lane that holds each, in
@@ -1028,19 +1063,19 @@ __global__ __launch_bounds__(256) void k_cv_desc_list(AtomArrays a, ListArrays l
         acc.store(i, k, N, D, J, Lp, La);
 }
 
-// Atom i's pairs within the cut-off of the step just launched (the compacted
-// list, or the full list when the capture's cut-off exceeds r_c), in list order:
-// count[i] = their number and nbr[i][0..min(count, width)); a count beyond
+// Atom i's candidate pairs of the step just launched, list `which` (pair_list:
+// a superset of its pairs within the cut-off, in list order; the readers test
+// r): count[i] = their number and nbr[i][0..min(count, width)); a count beyond
 // width tells the caller that the row was cut. One wave per atom.
-__global__ __launch_bounds__(256) void k_cv_capture_pairs(ListArrays l, int N, int use_full,
+__global__ __launch_bounds__(256) void k_cv_capture_pairs(ListArrays l, int N, int which,
                                                          int width, int *__restrict__ nbr,
                                                          int *__restrict__ count) {
     int const lane = threadIdx.x & 63;
     int const i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= N)
         return;
-    int const cnt = use_full ? l.nnbr[i] : l.ccnt[i];
-    const int *src = (use_full ? l.nbr : l.cnbr) + (size_t)i * l.stride;
+    const int *src;
+    int const cnt = pair_list(l, which, i, src);
     int *dst = nbr + (size_t)i * width;
     int const m = min(cnt, width);
     for (int t = lane; t < m; t += 64)
@@ -1059,7 +1094,7 @@ static void cv_capture(Driver &d) {
     size_t const nf = (size_t)d.N * 2 * CV_NB;
     int const waves = (d.N + 1) / 2;
     k_cv_desc_list<false><<<(waves * 64 + 255) / 256, 256, 0, d.stream>>>(
-        d.a, d.l, d.N, d.gs, 1e8, c.basis, c.use_full, c.D + j * nf, c.J + j * 3 * nf,
+        d.a, d.l, d.N, d.gs, 1e8, c.basis, c.list, c.D + j * nf, c.J + j * 3 * nf,
         c.L + j * nf, nullptr);
     unsigned const g = (d.N + 255) / 256;
     k_materialize<<<g, 256, 0, d.stream>>>(d.a, d.N, d.gs, MDL_A_POSITIONS,
@@ -1068,7 +1103,7 @@ static void cv_capture(Driver &d) {
                                            c.force + (size_t)j * 3 * d.N);
     if (c.nbr)
         k_cv_capture_pairs<<<(d.N + 3) / 4, 256, 0, d.stream>>>(
-            d.l, d.N, c.use_full, c.width, c.nbr + (size_t)j * d.N * c.width,
+            d.l, d.N, c.list, c.width, c.nbr + (size_t)j * d.N * c.width,
             c.ncnt + (size_t)j * d.N);
 }
 
@@ -1741,7 +1776,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 void k_force_rep(const RepArgs *__restrict__ R, const int2 *__restrict__ map) {
     int2 const m = map[blockIdx.x];
     RepArgs const &r = R[m.x];
-    force_mx_body<KIND, LPA, SAMPLE>(r.a, r.l, r.rt, r.N, r.npair, r.rmin, r.gs, r.pB,
+    force_mx_body<KIND, LPA, SAMPLE>(r.a, r.l, r.rt, r.N, r.npair, r.rmin, (float)r.rc2A, r.gs, r.pB,
                                      Blk{m.y, r.gridB});
 }
 
@@ -2340,8 +2375,12 @@ mdl_status mdl_driver_get_atoms(mdl_driver *p, int32_t what, double *out) {
     if (what == MDL_A_PAIRS_RC || what == MDL_A_PAIRS_MID) {
         if (!d->has_eam)
             return bad("MDL_A_PAIRS_RC / MDL_A_PAIRS_MID: the driver has no density pass");
-        k_int_to_double<<<(d->N + 255) / 256, 256, 0, d->stream>>>(
-            what == MDL_A_PAIRS_RC ? d->l.ccnt : d->l.mcnt, d->N, d->d_stage);
+        if (what == MDL_A_PAIRS_RC && d->precision == MDL_MIXED)
+            k_count_within_rc<<<(d->N + 255) / 256, 256, 0, d->stream>>>(
+                d->a, d->l, d->N, d->gs, (float)(d->rc2 * 1e16), d->d_stage);
+        else
+            k_int_to_double<<<(d->N + 255) / 256, 256, 0, d->stream>>>(
+                what == MDL_A_PAIRS_RC ? d->l.ccnt : d->l.mcnt, d->N, d->d_stage);
         MDL_HIP(hipGetLastError());
         MDL_HIP(hipMemcpyAsync(out, d->d_stage, sizeof(double) * d->N, hipMemcpyDeviceToHost,
                                d->stream));
@@ -2393,16 +2432,18 @@ mdl_status mdl_driver_cv_descriptors(mdl_driver *p, float rc, float mu0, float d
     }
     // the compacted pairs lie within the potential's cut-off rc2 (cm^2); a float
     // rc equal to it rounds up by < 1e-7
-    int const use_full =
-        (d->cycle_host == 0 || (double)rc * rc * 1e-16 > d->rc2 * (1.0 + 1e-6)) ? 1 : 0;
+    int const which =
+        (d->cycle_host == 0 || (double)rc * rc * 1e-16 > d->rc2 * (1.0 + 1e-6))
+            ? 1
+            : (d->precision == MDL_MIXED ? 2 : 0);
     int const waves = (d->N + 1) / 2;
     if (La)
         k_cv_desc_list<true><<<(waves * 64 + 255) / 256, 256, 0, st>>>(
-            d->a, d->l, d->N, d->gs, 1e8, cv_basis(rc, mu0, dmu, width), use_full, Dp, Jp, Lp,
+            d->a, d->l, d->N, d->gs, 1e8, cv_basis(rc, mu0, dmu, width), which, Dp, Jp, Lp,
             La);
     else
         k_cv_desc_list<false><<<(waves * 64 + 255) / 256, 256, 0, st>>>(
-            d->a, d->l, d->N, d->gs, 1e8, cv_basis(rc, mu0, dmu, width), use_full, Dp, Jp, Lp,
+            d->a, d->l, d->N, d->gs, 1e8, cv_basis(rc, mu0, dmu, width), which, Dp, Jp, Lp,
             nullptr);
     MDL_HIP(hipGetLastError());
     unsigned const g = (d->N + 255) / 256;
@@ -2508,7 +2549,9 @@ mdl_status mdl_driver_set_cv_capture(mdl_driver *p, float rc, float mu0, float d
     c.force = force_out;
     c.slots = slots;
     c.next = 0;
-    c.use_full = (double)rc * rc * 1e-16 > d->rc2 * (1.0 + 1e-6) ? 1 : 0;
+    c.list = (double)rc * rc * 1e-16 > d->rc2 * (1.0 + 1e-6)
+                     ? 1
+                     : (d->precision == MDL_MIXED ? 2 : 0);
     c.basis = cv_basis(rc, mu0, dmu, width);
     d->cvc = c;
     return MDL_OK;

@@ -169,6 +169,12 @@ __device__ __forceinline__ int tab_locate_f(float x, float inv_h, float nm1, flo
     return i;
 }
 
+// Squared pair distance from the fixed-point pair vector, in one fixed form
+// (explicit FMAs), so that both passes take the same r_c decision for a pair.
+__device__ __forceinline__ float pair_r2(float x, float y, float z) {
+    return fmaf(z, z, fmaf(y, y, x * x));
+}
+
 // The pair algebra that must be symmetric in (i, j): products and sums in a
 // fixed order with no fused operations.
 __device__ __forceinline__ float sym_dedr(float dphi, float Fi, float drj, float Fj, float dri) {
@@ -179,10 +185,12 @@ __device__ __forceinline__ float sym_dedr(float dphi, float Fi, float drj, float
 }
 
 // ---------------------------------------------------------------------------
-// Pass A, mixed: as k_density (compaction, FP64 rho), with the pair geometry
-// from the fixed-point positions and the rho(r) and F(rho) polynomials and the
-// g/h sums in FP32. The r_c and r_mid decisions are taken on the float r^2, the
-// same for both visits of a pair, so the compacted lists stay symmetric.
+// Pass A, mixed: as k_density (FP64 rho), with the pair geometry from the
+// fixed-point positions and the rho(r) and F(rho) polynomials and the g/h sums
+// in FP32. It scans the mid list (the full list on the steps that refilter it)
+// and, unlike the FP64 pass, writes no compacted r_c list: pass B scans the mid
+// list too and takes the same r_c decision (pair_r2 against rc2f, identical bits
+// in both passes and for both visits of a pair).
 // ---------------------------------------------------------------------------
 template <int KIND, int LPA, bool SAMPLE>
 __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTablesF rt,
@@ -226,9 +234,8 @@ __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTab
         }
         int const ia = active ? i : 0;
         const int *nb = (mid_build ? l.nbr : l.mnbr) + (size_t)ia * l.stride;
-        int *cb = l.cnbr + (size_t)ia * l.stride;
         int *mb = l.mnbr + (size_t)ia * l.stride;
-        int nin = 0, nmid = 0;
+        int nmid = 0;
         // Software pipeline: the position of the next candidate and the index
         // of the one after are loaded while this candidate is processed (the
         // table record of this candidate is requested before them, so waiting
@@ -243,7 +250,7 @@ __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTab
             int4 const cj = c0;
             float fx, fy, fz;
             qmin_image(qi, cj, S, fx, fy, fz);
-            float const r2 = fx * fx + fy * fy + fz * fz;
+            float const r2 = pair_r2(fx, fy, fz);
             bool const in = k < cnt && !(r2 > rc2f);
             if (mid_build) {
                 bool const inm = k < cnt && !(r2 > rm2);
@@ -264,10 +271,6 @@ __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTab
             c0 = a.qpos[j1];
             j0 = j1;
             j1 = k + 2 * LPA < cnt ? nb[k + 2 * LPA] : ia;
-            unsigned long long const bal = __ballot(in) & gmask;
-            if (in)
-                cb[nin + __popcll(bal & below)] = j;
-            nin += __popcll(bal);
             rho += in ? (double)p.v(t) : 0.;
             if (SAMPLE) {
                 float const w = in ? p.d1(t, rt.inv_h) * __builtin_amdgcn_rcpf(r) : 0.f;
@@ -318,7 +321,6 @@ __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTab
                 reinterpret_cast<double *>(a.geam + 4 * (size_t)i)[c] = (double)gh;
         }
         if (active && lane == 0) {
-            l.ccnt[i] = nin;
             if (mid_build) {
                 l.mcnt[i] = nmid;
                 a.mref[i] = a.pos[i];
@@ -374,15 +376,17 @@ __global__ __launch_bounds__(BLOCK) void k_density_mx(AtomArrays a, ListArrays l
 }
 
 // ---------------------------------------------------------------------------
-// Pass B, mixed: pair geometry from the fixed-point positions; forces (FP64
-// sums of FP32 pair terms, symmetric in the pair), energies,
+// Pass B, mixed: over the mid list, the pairs within r_c by pass A's decision;
+// pair geometry from the fixed-point positions; forces (FP64 sums of FP32 pair
+// terms, symmetric in the pair), energies,
 // Born stress, and when sampled the Born monomials and Laplacian (FP32 per
 // lane). The virial is taken as the trace of the Born stress in k_post.
 // Outputs per atom in CGS; block sums in eV and Angstrom (scaled in k_post).
 // ---------------------------------------------------------------------------
 template <int KIND, int LPA, bool SAMPLE>
 __device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTablesF rt, int N,
-                                              int npair, float rmin, DevState *__restrict__ gs,
+                                              int npair, float rmin, float rc2f,
+                                              DevState *__restrict__ gs,
                                               double *__restrict__ part, Blk bk) {
     if (gs->err)
         return;
@@ -428,7 +432,7 @@ __device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTable
         // count is the wave's longest list; lanes past their own list (and
         // lanes of no atom) carry atom ia itself as a harmless dummy
         int const ia = active ? i : 0;
-        int const cnt = active ? l.ccnt[i] : 0;
+        int const cnt = active ? l.mcnt[i] : 0;
         int wmax = cnt;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1)
@@ -437,7 +441,7 @@ __device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTable
             int4 const ci = a.qpos[ia];
             int const ti = ci.w;
             float2 const Fi = a.fprime_f[ia];
-            const int *nb = l.cnbr + (size_t)ia * l.stride;
+            const int *nb = l.mnbr + (size_t)ia * l.stride;
             struct Geo {
                 float x, y, z, r, t;
                 int tj;
@@ -446,10 +450,10 @@ __device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTable
             // pair vector, interval and record of one pair
             auto geometry = [&](const int4 &cj, bool act, Geo &g) -> int {
                 qmin_image(ci, cj, S, g.x, g.y, g.z);
-                float const xx = g.x * g.x, yy = g.y * g.y, zz = g.z * g.z;
-                g.r = __builtin_amdgcn_sqrtf(xx + yy + zz);
+                float const r2 = pair_r2(g.x, g.y, g.z);
+                g.r = __builtin_amdgcn_sqrtf(r2);
                 g.tj = cj.w;
-                g.act = act;
+                g.act = act && !(r2 > rc2f);
                 int const tmax = max(ti, g.tj), tmin = min(ti, g.tj);
                 int const pslot = ((tmax * (tmax + 1)) >> 1) + tmin;
                 int const it = tab_locate_f(g.r, rt.inv_h, nm1, g.t);
@@ -587,8 +591,8 @@ __device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTable
 template <int KIND, int LPA, bool SAMPLE>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 void k_force_mx(AtomArrays a, ListArrays l, RTablesF rt, int N, int npair, float rmin,
-                DevState *__restrict__ gs, double *__restrict__ part) {
-    force_mx_body<KIND, LPA, SAMPLE>(a, l, rt, N, npair, rmin, gs, part, hw_blk());
+                float rc2f, DevState *__restrict__ gs, double *__restrict__ part) {
+    force_mx_body<KIND, LPA, SAMPLE>(a, l, rt, N, npair, rmin, rc2f, gs, part, hw_blk());
 }
 
 }  // namespace mdl

@@ -757,13 +757,15 @@ class MDDriver:
 
     def pairs_within_rc(self) -> np.ndarray:
         """Per atom, the pairs within r_c at the last density pass (both directions;
-        the length of the list the force pass walks), a diagnostic without
-        counterpart in the reference."""
+        the pairs the force pass evaluates), a diagnostic without counterpart in
+        the reference. FP64: the length of the compacted list the force pass walks;
+        mixed: counted over the mid list with the passes' own r_c decision."""
         return self._atoms("pairs_rc").astype(np.int64)
 
     def mid_list_lengths(self) -> np.ndarray:
         """Per atom, the entries of the mid list the density pass scans (pairs within
-        r_c + 0.3 A at its last filtering), a diagnostic."""
+        r_c + 0.3 A at its last filtering; r_c + 0.15 A in mixed precision, where
+        the force pass scans it too), a diagnostic."""
         return self._atoms("pairs_mid").astype(np.int64)
 
     @property

@@ -46,6 +46,33 @@ def test_mixed_static_vs_fp64_oracle(cellname, interp):
     _check(o, g)
 
 
+def test_mixed_pairs_within_rc_match_the_distances():
+    """The mixed passes keep no compacted r_c list: both scan the mid list and
+    decide r <= r_c on the float r^2 of the fixed-point pair vector. Per atom,
+    the pairs that decision keeps (pairs_within_rc) are those within r_c by FP64
+    distances, up to pairs within 1e-5 A of r_c; the FP64 driver's compacted
+    counts agree the same way, and every mid list holds all of them."""
+    from mdelast_amd import units as U
+    cell = cells.nialco_cell()
+    setup = H.md_setup(cell, static=True)
+    pos = cell["pos"] / U.angstrom
+    L = cell["box"] / U.angstrom
+    rc = cell["data"].rcut / U.angstrom
+    d = pos[:, None, :] - pos[None, :, :]
+    d -= L * np.round(d / L)
+    r = np.sqrt((d ** 2).sum(-1))
+    np.fill_diagonal(r, np.inf)
+    inside = (r <= rc).sum(1)
+    band = (np.abs(r - rc) < 1e-5).sum(1)
+    for precision in ("mixed", "fp64"):
+        g = H.gpu_md(cell, setup, precision=precision, neighbors_list=True, r_skin=1.2)
+        n = g.pairs_within_rc()
+        m = g.mid_list_lengths()
+        print(precision, "pairs within r_c per atom", n.mean(), "mid", m.mean(), flush=True)
+        assert np.all(np.abs(n - inside) <= band), precision
+        assert np.all(m >= n), precision
+
+
 def test_mixed_momentum_stays_at_fp64_roundoff():
     # pair forces are bit-symmetric, so 1000 steps keep |P| at FP64 round-off
     # (the reference aborts above |P|^2 / (2 m_0 K) = 1e-14, md_driver.cpp:620-627)
