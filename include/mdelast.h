@@ -227,7 +227,7 @@ enum {
     MDL_S_WARN_VIRIAL = 76,   /* steps whose virial self-check failed (md_driver.cpp:422-429) */
     MDL_S_WARN_RMIN = 77,     /* pair distances below r_min seen (potential.hpp:93-95) */
     MDL_S_NLIST_ENTRIES = 78, /* stored (full) list entries */
-    MDL_S_NHC_GENERAL = 79,   /* chain propagations done in the wide form (any argument) */
+    MDL_S_NHC_GENERAL = 79,   /* chain propagations with a factor in the general form */
     MDL_S_NMID_BUILDS = 80,   /* filterings of the mid list (pairs within r_c + 0.3 A) */
     MDL_S_LANES_PER_ATOM = 81,/* lanes per atom of the pair passes (given or autotuned) */
     MDL_NSCALARS = 82
@@ -257,7 +257,7 @@ mdl_status mdl_driver_kernel_times(mdl_driver *d, double *out);
 /* exp(-x) and (1 - exp(-x))/x at n points on the GPU, as the integrator takes
  * them (expx_operator, numerics.hpp:11-27; thermostat.cpp:40-66). form 0: the
  * barostat's form; 1: the chain sweeps' wide form (any x); 2: their series form
- * (|x| <= 1/8). Host arrays in and out. */
+ * (|x| <= 1/4). Host arrays in and out. */
 mdl_status mdl_expx_eval(int32_t form, int64_t n, const double *x, double *expx, double *op);
 
 #ifdef __cplusplus

@@ -113,8 +113,9 @@ struct DevState {
     int err;
     double err_val[4];
     long long warn_virial, warn_rmin;
-    long long nhc_general;   // chain propagations with an argument beyond the series range
-    int nhc_wide[2];         // thermostat, barostat chain: its last propagation had one
+    long long nhc_general;   // chain propagations with a factor from the general form
+    int nhc_form[2];         // thermostat, barostat chain: the sweep form its last
+                             // propagation needed (SW_SERIES, SW_TOP, SW_WIDE)
     int init_done;
 };
 

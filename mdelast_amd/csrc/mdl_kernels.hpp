@@ -189,24 +189,25 @@ __device__ __forceinline__ void min_image(const double4 &ci, const double4 &cj, 
 // exp(-x) and op(x) = (1 - exp(-x))/x of numerics.hpp:11-18. The reference
 // takes exp from the library and op from a 5th-order Taylor form for
 // |x| <= 1e-3, else from the quotient. The chain arguments are mostly small
-// (|x| ~ dt/tau): for |x| <= 1/8, op comes from its Taylor series to degree 10
-// (truncation below 3e-19 relative) and exp(-x) = 1 - x op(x) from one more
+// (|x| ~ dt/tau): for |x| <= 1/4, op comes from its Taylor series to degree 12
+// (truncation below 2e-19 relative) and exp(-x) = 1 - x op(x) from one more
 // FMA (within rounding of the library exp; the quotient form of op loses up to
 // 1/|x| ulp to cancellation and agrees to that).
-constexpr double EXPX_SMALL = 0.125;
+constexpr double EXPX_SMALL = 0.25;
 
 // Both polynomials below are evaluated by Estrin's scheme: the chain sweeps
 // run them on one lane, one after another, so their cost is the length of the
 // dependent chain (4 FMA levels after the powers instead of 10-12 in Horner form).
 __device__ __forceinline__ double op_series(double x) {
-    // sum_{m=0}^{10} (-x)^m / (m+1)!
+    // sum_{m=0}^{12} (-x)^m / (m+1)!
     double const y = -x, y2 = y * y, y4 = y2 * y2, y8 = y4 * y4;
     double const q0 = fma(y, 0.5, 1.), q1 = fma(y, 1. / 24., 1. / 6.),
                  q2 = fma(y, 1. / 720., 1. / 120.), q3 = fma(y, 1. / 40320., 1. / 5040.),
-                 q4 = fma(y, 1. / 3628800., 1. / 362880.);
-    double const s0 = fma(q1, y2, q0), s1 = fma(q3, y2, q2),
-                 s2 = fma(1. / 39916800., y2, q4);
-    return fma(s2, y8, fma(s1, y4, s0));
+                 q4 = fma(y, 1. / 3628800., 1. / 362880.),
+                 q5 = fma(y, 1. / 479001600., 1. / 39916800.);
+    double const s0 = fma(q1, y2, q0), s1 = fma(q3, y2, q2), s2 = fma(q5, y2, q4);
+    double const t0 = fma(s1, y4, s0), t1 = fma(1. / 6227020800., y4, s2);
+    return fma(t1, y8, t0);
 }
 
 // exp(-x) for any x: y = -x = k ln2 + r with |r| <= ln2/2 (Cody-Waite: ln2
@@ -379,17 +380,25 @@ __device__ __forceinline__ void expx_any(double x, double &expx, double &op) {
 // BARO: the coupling is p_eps^2/(2W) and U3 scales p_eps (held in kin); else
 // the coupling is cw*K + c0 (md_driver.cpp:512-531) and U3 scales K by f^2 and
 // the running velocity factor by f.
-// WIDE = false: every factor from the series form (op_series), which holds for
-// |x| <= EXPX_SMALL only; WIDE = true: every factor from expx_any. Both return
-// whether an argument exceeded EXPX_SMALL; no branch depends on the values.
-template <int M, bool BARO, bool WIDE>
-__device__ __forceinline__ bool nhc_sweeps(double (&p)[M], double (&eta)[M], const double (&iq)[M],
-                                           double Nf, double kT, double dt, double &kin,
-                                           double &scale, double cw, double c0, double inv_W) {
-    bool big = false;
-    auto factor = [&big](double x, double &e, double &o) {
-        big |= !(fabs(x) <= EXPX_SMALL);
-        if (WIDE) {
+// FORM says which factors come from expx_any (valid for any x); the others
+// come from the series form (op_series), valid for |x| <= EXPX_SMALL only:
+// SW_SERIES none, SW_TOP the factor of the top element (argument p_{M-1}, the
+// chain's end, whose momentum a hard-driven chain leaves largest), SW_WIDE all.
+// Returns which arguments exceeded EXPX_SMALL: bit 0 the top element's, bit 1
+// any other's; no branch depends on the values.
+enum SweepForm { SW_SERIES = 0, SW_TOP = 1, SW_WIDE = 2 };
+template <int M, bool BARO, int FORM>
+__device__ __forceinline__ int nhc_sweeps(double (&p)[M], double (&eta)[M], const double (&iq)[M],
+                                          double Nf, double kT, double dt, double &kin,
+                                          double &scale, double cw, double c0, double inv_W) {
+    bool over_top = false, over_rest = false;
+    auto factor = [&over_top, &over_rest](double x, double &e, double &o, bool top) {
+        bool const out = !(fabs(x) <= EXPX_SMALL);
+        if (top)
+            over_top |= out;
+        else
+            over_rest |= out;
+        if (FORM == SW_WIDE || (FORM == SW_TOP && top)) {
             expx_any(x, e, o);
         } else {
             o = op_series(x);
@@ -417,13 +426,13 @@ __device__ __forceinline__ bool nhc_sweeps(double (&p)[M], double (&eta)[M], con
         p[M - 1] += tG[M - 1];
 #pragma unroll
         for (int j = M - 2; j >= 0; --j) {
-            factor(c[j + 1] * p[j + 1], E[j], O[j]);
+            factor(c[j + 1] * p[j + 1], E[j], O[j], j == M - 2);
             p[j] = fma(p[j], E[j], tG[j] * O[j]);
         }
         // U3
         {
             double f, unused;
-            factor((2. * c[0]) * p[0], f, unused);
+            factor((2. * c[0]) * p[0], f, unused, false);
 #pragma unroll
             for (int j = 0; j < M; ++j)
                 eta[j] = fma(2. * c[j], p[j], eta[j]);
@@ -450,21 +459,23 @@ __device__ __forceinline__ bool nhc_sweeps(double (&p)[M], double (&eta)[M], con
                 p[j] += t;
         }
     }
-    return big;
+    return (over_top ? 1 : 0) | (over_rest ? 2 : 0);
 }
 
-// One U_NHC propagation. `wide` says whether the chain's previous propagation
-// had an argument beyond the series range. If not, the series sweeps run
-// first and, should an argument leave the range after all, the chain is
-// restored and the wide sweeps run instead; one uniform branch per
-// propagation, none per factor. `wide` is updated for the next propagation.
-// Returns whether the result came from the wide sweeps.
+// One U_NHC propagation. `form` is the sweep form the chain's previous
+// propagation needed (the series form where its arguments stayed within
+// EXPX_SMALL). That form runs first; should an argument it takes in the series
+// form leave the range after all, the chain is restored and the wide sweeps run
+// instead. One uniform branch per propagation, none per factor. `form` is
+// updated for the next propagation. Returns whether any factor came from
+// expx_any.
 template <int M, bool BARO>
 __device__ __forceinline__ bool nhc_prop(double (&p)[M], double (&eta)[M], const double (&iq)[M],
                                          double Nf, double kT, double dt, double &kin,
                                          double &scale, double cw, double c0, double inv_W,
-                                         int &wide) {
-    if (!wide) {
+                                         int &form) {
+    auto next = [](int over) { return (over & 2) ? SW_WIDE : (over & 1) ? SW_TOP : SW_SERIES; };
+    if (form != SW_WIDE) {
         double p0[M], eta0[M];
 #pragma unroll
         for (int j = 0; j < M; ++j) {
@@ -472,10 +483,16 @@ __device__ __forceinline__ bool nhc_prop(double (&p)[M], double (&eta)[M], const
             eta0[j] = eta[j];
         }
         double const kin0 = kin, scale0 = scale;
-        bool const big = nhc_sweeps<M, BARO, false>(p, eta, iq, Nf, kT, dt, kin, scale, cw, c0,
-                                                    inv_W);
-        if (!__builtin_amdgcn_readfirstlane(big ? 1 : 0))
-            return false;
+        int const tried = form;
+        int const over = __builtin_amdgcn_readfirstlane(
+            tried == SW_SERIES
+                ? nhc_sweeps<M, BARO, SW_SERIES>(p, eta, iq, Nf, kT, dt, kin, scale, cw, c0, inv_W)
+                : nhc_sweeps<M, BARO, SW_TOP>(p, eta, iq, Nf, kT, dt, kin, scale, cw, c0, inv_W));
+        // the series factors were all in range: the result stands
+        if ((over & (tried == SW_SERIES ? 3 : 2)) == 0) {
+            form = next(over);
+            return tried != SW_SERIES;
+        }
 #pragma unroll
         for (int j = 0; j < M; ++j) {
             p[j] = p0[j];
@@ -484,7 +501,8 @@ __device__ __forceinline__ bool nhc_prop(double (&p)[M], double (&eta)[M], const
         kin = kin0;
         scale = scale0;
     }
-    wide = nhc_sweeps<M, BARO, true>(p, eta, iq, Nf, kT, dt, kin, scale, cw, c0, inv_W) ? 1 : 0;
+    form = next(__builtin_amdgcn_readfirstlane(
+        nhc_sweeps<M, BARO, SW_WIDE>(p, eta, iq, Nf, kT, dt, kin, scale, cw, c0, inv_W)));
     return true;
 }
 
@@ -502,7 +520,7 @@ template <int M> __device__ __forceinline__ double nhc_thermo_m(DevState &s, dou
     }
     double kin = s.ekin_tot, scale = 1.;
     if (nhc_prop<M, false>(c.p, c.eta, c.iq, s.th.Nf, kT, dt, kin, scale, cw, c0, 0.,
-                           s.nhc_wide[0]))
+                           s.nhc_form[0]))
         s.nhc_general++;
     s.ekin_tot = kin;
     c.store(s.th);
@@ -530,7 +548,7 @@ template <int M> __device__ __forceinline__ void nhc_baro_m(DevState &s, double 
     double const kT = K_BOLTZ * s.bth.T;
     double peps = s.peps, unused = 1.;
     if (nhc_prop<M, true>(c.p, c.eta, c.iq, s.bth.Nf, kT, dt, peps, unused, 0., 0.,
-                          1. / s.bar_W, s.nhc_wide[1]))
+                          1. / s.bar_W, s.nhc_form[1]))
         s.nhc_general++;
     s.peps = peps;
     c.store(s.bth);

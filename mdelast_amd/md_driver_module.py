@@ -781,9 +781,9 @@ class MDDriver:
 
     @property
     def nhc_general_path_count(self):
-        """Chain propagations done in the wide form (range-reduced exp): one with a
-        chain argument dt p/q beyond 1/8, or following one (diagnostic; no
-        counterpart in the reference)."""
+        """Chain propagations that took at least one factor in the general form
+        (range-reduced exp): those with a chain argument dt p/q beyond 1/4, or
+        following one (diagnostic; no counterpart in the reference)."""
         return int(self._scalars()[nat.S_NHC_GENERAL])
 
     @property

@@ -3,10 +3,10 @@
 numerics.hpp:11-27 of the reference: exp(-x) and op(x) = (1 - exp(-x))/x, which
 every Nose-Hoover chain element and the barostat take (thermostat.cpp:40-66,
 md_driver.cpp:462-499). Three device forms: the barostat's (a Taylor series of
-op for |x| <= 1/8, else a range-reduced exponential and a Newton reciprocal),
+op for |x| <= 1/4, else a range-reduced exponential and a Newton reciprocal),
 the chain sweeps' wide form (one branch-free formula for any x: op from the
 series when the reduction leaves x as it is, else from expm1 = 2^k r p(r) +
-2^k - 1) and their series form (|x| <= 1/8). Each is checked against numpy's
+2^k - 1) and their series form (|x| <= 1/4). Each is checked against numpy's
 80-bit long double (exp, expm1) over the whole range the chains reach, so a
 wrong coefficient, a dropped term or a wrong reduction constant shows as an
 error of many ulp.
@@ -22,15 +22,16 @@ from mdelast_amd import _native as nat
 pytestmark = pytest.mark.gpu
 
 EPS = np.finfo(np.float64).eps
+SERIES = 0.25   # the series form's range
 
 
 def _points():
     rs = np.random.RandomState(7)
-    small = rs.uniform(-0.125, 0.125, 20000)
+    small = rs.uniform(-SERIES, SERIES, 20000)
     tiny = rs.uniform(-1e-3, 1e-3, 2000)
-    mid = rs.uniform(0.125, 4.0, 20000) * rs.choice([-1.0, 1.0], 20000)
+    mid = rs.uniform(SERIES, 4.0, 20000) * rs.choice([-1.0, 1.0], 20000)
     big = np.exp(rs.uniform(np.log(4.0), np.log(700.0), 20000)) * rs.choice([-1.0, 1.0], 20000)
-    edges = np.array([0.125, -0.125, np.nextafter(0.125, 1.0), np.nextafter(-0.125, -1.0),
+    edges = np.array([SERIES, -SERIES, np.nextafter(SERIES, 1.0), np.nextafter(-SERIES, -1.0),
                       0.5 * np.log(2.0), -0.5 * np.log(2.0), np.nextafter(0.5 * np.log(2.0), 1.0),
                       1e-300, -1e-300, 700.0, -700.0])
     return np.concatenate([small, tiny, mid, big, edges])
@@ -49,10 +50,10 @@ def test_expx_against_long_double(form):
     rel_e = np.abs((e - e_true) / e_true).astype(np.float64)
     assert rel_e.max() <= 2.5 * EPS, (x[rel_e.argmax()], rel_e.max())
     err = np.abs(op - op_true).astype(np.float64)
-    small = np.abs(x) <= 0.125
+    small = np.abs(x) <= SERIES
     assert (err[small] <= 4.0 * EPS * np.abs(op_true[small])).all()
     if form == 0:
-        # beyond 1/8 the product (1 - e) * (1/x) carries e's rounding amplified
+        # beyond 1/4 the product (1 - e) * (1/x) carries e's rounding amplified
         # by e / (1 - e) (the reference's quotient carries the same), plus the
         # reciprocal and the product
         bound = EPS * (4.0 * np.abs(op_true) + 2.0 * e_true / np.abs(x.astype(np.longdouble)))
@@ -64,7 +65,7 @@ def test_expx_against_long_double(form):
 
 
 def test_series_form_in_its_range():
-    x = np.concatenate([_points()[:22000], [0.0, 0.125, -0.125]])
+    x = np.concatenate([_points()[:22000], [0.0, SERIES, -SERIES]])
     e, op = nat.expx_eval(x, 2)
     e_true, op_true = _truth(np.where(x == 0.0, 1e-300, x))
     assert (np.abs(e - e_true) <= 2.5 * EPS * e_true).all()
