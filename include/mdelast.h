@@ -241,9 +241,10 @@ enum {
     MDL_A_POSITIONS = 0, MDL_A_COORDINATES, MDL_A_VELOCITIES, MDL_A_FORCES, MDL_A_RHO,
     MDL_A_EPOT, MDL_A_EKIN,
     MDL_A_PAIRS_RC = 7,       /* N: pairs within r_c of each atom at the last density pass
-                                 (the force pass's list length; EAM drivers only) */
-    MDL_A_PAIRS_MID = 8       /* N: entries of each atom's mid list (pairs within r_c + 0.3 A
-                                 at its last filtering, the density pass's scan length) */
+                                 (the pairs the force pass evaluates; EAM drivers only) */
+    MDL_A_PAIRS_MID = 8       /* N: entries of each atom's mid list (pairs within r_c + 0.3 A,
+                                 mixed: 0.15 A, at its last filtering; the density pass's
+                                 scan length, and in mixed precision the force pass's) */
 };
 mdl_status mdl_driver_get_atoms(mdl_driver *d, int32_t what, double *out);
 /* p_eta and eta of the thermostat (which = 0) or barostat chain (which = 1). */

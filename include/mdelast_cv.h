@@ -57,12 +57,14 @@ mdl_status mdl_driver_set_cv_capture(mdl_driver *d, float rc, float mu0, float d
                                      float *D, float *J, float *L, double *pos_out,
                                      double *force_out, int32_t slots);
 
-/* With a capture set, also the pairs of each captured step: nbr_out
+/* With a capture set, also the candidate pairs of each captured step: nbr_out
  * [slots][N][width] and count_out [slots][N] (int32 device) receive, per atom,
- * its pairs within the capture's cut-off in the driver's list order (the
- * compacted pairs within r_c, or the full list when rc exceeds r_c) and their
- * number; a count above width means the row was cut. NULL stops it; a new
- * mdl_driver_set_cv_capture clears it. No counterpart in the reference. */
+ * a list in the driver's order that holds all its pairs within the capture's
+ * cut-off (FP64 drivers: the compacted pairs within r_c; mixed drivers: the mid
+ * list, pairs within r_c + 0.15 A; either: the full list when rc exceeds r_c),
+ * and its length; readers test r. A count above width means the row was cut.
+ * NULL stops it; a new mdl_driver_set_cv_capture clears it. No counterpart in
+ * the reference. */
 mdl_status mdl_driver_set_cv_capture_pairs(mdl_driver *d, int32_t *nbr_out, int32_t *count_out,
                                            int32_t width);
 
