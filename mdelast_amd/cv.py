@@ -1054,9 +1054,10 @@ def capture_frames(drv, nframes, stride, T, basis: Basis = Basis(), device="cuda
     stride-th step: its scalars from the run's records, and its per-atom inputs
     of the controls (D, J, L, positions, forces) captured on the device inside
     the run (mdl_driver_set_cv_capture with one slot per frame); with
-    ``pair_width`` also each atom's pairs within the cut-off, in the driver's
-    list order (mdl_driver_set_cv_capture_pairs), which the stress proxy's
-    kernels then take instead of searching all atoms."""
+    ``pair_width`` also each atom's candidate pairs, a list in the driver's order
+    that holds all its pairs within the cut-off (mdl_driver_set_cv_capture_pairs),
+    which the stress proxy's kernels then take, testing r, instead of searching
+    all atoms."""
     torch = _torch()
     N = drv.N
     store = dict(D=torch.empty((nframes, N, NF), device=device),
