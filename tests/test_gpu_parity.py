@@ -111,9 +111,17 @@ def test_table_eval_vs_oracle(interp):
             assert np.max(np.abs(a - b)) / np.max(np.abs(b)) < tol
 
 
-def _traj_compare(ensemble, nsteps, interp="quintic", neighbors=True, chain_kw=None):
+def _traj_compare(ensemble, nsteps, interp="quintic", neighbors=True, chain_kw=None,
+                  top_x=None):
     cell = cells.zn06_cell(5)
     setup = H.md_setup(cell, T=300.0, M=3, Mb=3, seed=0, **(chain_kw or {}))
+    if top_x is not None:
+        # the chain's last momentum set so that its factor's argument
+        # dt_w/4 p/q reaches about top_x at the largest Suzuki-Yoshida weight
+        T, q, p_eta, Nf = setup["thermostat"]
+        p_eta = np.array(p_eta, dtype=float)
+        p_eta[-1] = top_x * q[-1] / (0.5 * 1.3151863206839063 * 0.5 * 2e-15)
+        setup = dict(setup, thermostat=(T, q, p_eta, Nf))
     kw = dict(interpolation=interp, ensemble=ensemble, dt=2e-15, neighbors_list=neighbors,
               r_skin=1.2)
     o = H.oracle_md(cell, setup, **kw)
@@ -162,12 +170,16 @@ def test_trajectory_vs_oracle(ensemble):
 
 
 @pytest.mark.parametrize("ensemble", ["NVT", "NPT"])
-@pytest.mark.parametrize("path", ["series", "general"])
+@pytest.mark.parametrize("path", ["series", "general", "top"])
 def test_trajectory_chain_paths(ensemble, path):
-    # tau = 500 fs keeps every chain argument dt p/q far below 1/8 (series path
-    # only); tau = 4 fs puts many beyond (range-reduced exp and quotient)
-    tau = 5e-13 if path == "series" else 4e-15
-    worst, g = _traj_compare(ensemble, 20, chain_kw=dict(tau_T=tau, tau_B=tau))
+    # tau = 500 fs keeps every chain argument dt p/q far below 1/4 (series sweeps
+    # only); tau = 4 fs puts many beyond (range-reduced exp and quotient); "top":
+    # tau = 500 fs with the last chain element's momentum raised so that its
+    # argument reaches about 0.6 while the others stay small (the sweeps with
+    # only the top element's factor in the general form)
+    tau = 4e-15 if path == "general" else 5e-13
+    worst, g = _traj_compare(ensemble, 20, chain_kw=dict(tau_T=tau, tau_B=tau),
+                             top_x=0.6 if path == "top" else None)
     print(ensemble, path, worst, g.nhc_general_path_count)
     if path == "series":
         assert g.nhc_general_path_count == 0
