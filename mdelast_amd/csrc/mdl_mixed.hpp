@@ -263,8 +263,8 @@ __device__ __forceinline__ void density_mx_body(AtomArrays a, ListArrays l, RTab
             int const tj = cj.w;
             float t;
             int const it = tab_locate_f(r, rt.inv_h, nm1, t);
-            // about half of the candidates lie beyond r_c: their lanes request
-            // no table record
+            // the candidates beyond r_c (about one in seven of a mid list, half
+            // of the full list when it is refiltered) request no table record
             P p;
             if (in)
                 p.load(rt.rec + (it * rowS + (npair + tj) * P::S));
