@@ -86,11 +86,37 @@ def test_nialco108k_mixed_nvt_properties():
         assert np.max(np.abs(ra[:, k] / rb[:, k] - 1)) < tol, k
     assert np.max(np.abs(ra[:, 7] - rb[:, 7])) < 2e-7
     assert a.neighbor_list_builds >= 10
-    # the density pass scans the mid list (pairs within r_c + 0.3 A), refiltered
-    # at every full build and when an atom may have moved 0.15 A since
+    # both mixed passes scan the mid list (pairs within r_c + 0.15 A), refiltered
+    # at every full build and when an atom may have moved 0.075 A since
     nb, nm = a.neighbor_list_builds, a.mid_list_builds
     print("full builds", nb, "mid filterings", nm)
     assert nb < nm < 200
+
+
+def test_nialco108k_mixed_long_run():
+    """20 000 mixed NVT steps (20 ps) of the 108 000-atom workload, as the bench
+    runs it (graph chunks of 20 steps, the list rebuilt every 20): momentum at
+    FP64 round-off, the conserved energy flat after the lattice start's first
+    steps, the temperature at the thermostat's 600 K after the first 2 ps."""
+    w = bench.build_workload()
+    d, keep = bench.make_driver(w, "mixed")
+    d.graph_steps = 20
+    m0 = w["mass"][0]
+    err0 = None
+    for chunk in range(10):
+        rec = d.run(2000, 100, 20)
+        assert np.all(np.isfinite(rec))
+        ratio = (rec[:, 21:24] ** 2).sum(1) / (2 * m0 * rec[:, 4])
+        err = rec[:, 7].max()
+        err0 = err if err0 is None else err0
+        print(f"steps {2000 * (chunk + 1)}: max |P|^2/(2 m0 K) {ratio.max():.2e}, "
+              f"max E_cons err {err:.3e}, T {rec[-1, 11]:.1f} K", flush=True)
+        assert ratio.max() < 1e-20
+        assert err < 2e-5 and err < 1.05 * err0
+        # the lattice start pours kinetic energy into the potential first (T dips
+        # to about 420 K); after 2 ps the thermostat holds 600 K
+        lo = 200.0 if chunk == 0 else 560.0
+        assert lo < rec[:, 11].min() and rec[:, 11].max() < 640.0
 
 
 def _oracle_from_driver(d, data, T, q, p_eta):
