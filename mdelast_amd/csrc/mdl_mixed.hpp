@@ -419,7 +419,9 @@ __device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTable
     // the global-to-LDS loads that fill it.
     constexpr int CH = Rec::NF / 4;
     __shared__ float4 stage_all[BLOCK / 64][64 * CH];
-    float4 *const stage = stage_all[threadIdx.x / 64];
+    // the wave's stage, its index made wave-uniform (the LDS base of the
+    // global-to-LDS loads is a scalar register)
+    float4 *const stage = stage_all[__builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64))];
     int const l64 = threadIdx.x & 63;
     int const rot = l64 * CH / 16;
     const float4 *const prec4 = reinterpret_cast<const float4 *>(rt.prec);
@@ -466,9 +468,11 @@ __device__ __forceinline__ void force_mx_body(AtomArrays a, ListArrays l, RTable
                     int const src = q * (64 / CH) + l64 / CH;
                     int const rsrc = __shfl(rec, src, 64);
                     int const chunk = (l64 % CH - src * CH / 16) & (CH - 1);
+                    // a 32-bit offset from the table base (the record index
+                    // times CH stays far below 2^32)
                     __builtin_amdgcn_global_load_lds(
-                        (const void __attribute__((address_space(1))) *)(prec4 + (size_t)rsrc * CH +
-                                                                          chunk),
+                        (const void __attribute__((address_space(1))) *)(
+                            prec4 + (unsigned)(rsrc * CH + chunk)),
                         (void __attribute__((address_space(3))) *)(st + q * 64), 16, 0, 0);
                 }
             };
