@@ -1025,8 +1025,10 @@ __global__ __launch_bounds__(256) void k_cv_desc_list(AtomArrays a, ListArrays l
     int const i = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 2 + (lane >> 5);
     bool const active = i < N;
     int const ia = active ? i : 0;
-    double const L[3] = {gs->L[0], gs->L[1], gs->L[2]};
-    double4 const ci = a.pos[ia];
+    // pair vectors from the fixed-point positions (A per unit: L 2^-32 to_A)
+    float const S[3] = {(float)(gs->L[0] * to_A) * 0x1p-32f, (float)(gs->L[1] * to_A) * 0x1p-32f,
+                        (float)(gs->L[2] * to_A) * 0x1p-32f};
+    int4 const ci = a.qpos[ia];
     const int *nb;
     int const cnt0 = pair_list(l, which, ia, nb);
     int const cnt = active ? cnt0 : 0;
@@ -1040,16 +1042,12 @@ __global__ __launch_bounds__(256) void k_cv_desc_list(AtomArrays a, ListArrays l
         float x = 0.f, y = 0.f, z = 0.f, r = 1.f;
         int sp = 0, ok = 0;
         if (q < cnt) {
-            double4 const cj = a.pos[nb[q]];
-            double dx, dy, dz;
-            min_image(ci, cj, L, dx, dy, dz);
-            x = (float)(dx * to_A);
-            y = (float)(dy * to_A);
-            z = (float)(dz * to_A);
+            int4 const cj = a.qpos[nb[q]];
+            qmin_image(ci, cj, S, x, y, z);
             float const r2 = x * x + y * y + z * z;
             ok = r2 < basis.rc2 ? 1 : 0;
             r = ok ? __builtin_amdgcn_sqrtf(r2) : 1.f;
-            sp = (int)cj.w;
+            sp = cj.w;
         }
         int const m = min(32, cnt - q0);
         for (int t = 0; t < m; ++t) {
