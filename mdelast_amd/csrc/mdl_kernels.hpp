@@ -637,8 +637,8 @@ __global__ __launch_bounds__(BLOCK) void k_pre(DevState *__restrict__ gs, int sa
         return;
     __shared__ DevState s;
     state_load(s, gs);
-    // the whole first wave runs the scalar step with identical values (the
-    // chain sweeps spread independent factors over its lanes)
+    // the whole first wave runs the scalar step with identical values (one
+    // active lane instead measured slower)
     if (threadIdx.x < 64) {
         pre_step(s, sample);
     }
